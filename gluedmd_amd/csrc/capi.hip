@@ -1,0 +1,241 @@
+// C-ABI of the GLUED step (declared in include/glued.h).  Converts every C++
+// exception into a status code and a thread-local message.
+#include <cstring>
+#include <string>
+#include "../../include/glued.h"
+#include "engine.hpp"
+
+using glued::Engine;
+
+struct glued_force {
+    Engine e;
+};
+
+static thread_local std::string g_err;
+
+template <class F>
+static int guard(F&& f)
+{
+    try {
+        f();
+        return 0;
+    } catch (const std::exception& ex) {
+        g_err = ex.what();
+        return 1;
+    } catch (...) {
+        g_err = "unknown error";
+        return 1;
+    }
+}
+
+static Engine& E(glued_handle h)
+{
+    if (!h) throw glued::Error("null handle");
+    return h->e;
+}
+
+template <class T>
+static void copy_out(const std::vector<T>& v, T* out, int n, const char* what)
+{
+    if ((int)v.size() > n) throw glued::Error(std::string(what) + ": output buffer too small");
+    if (!v.empty()) std::memcpy(out, v.data(), sizeof(T) * v.size());
+}
+
+#pragma GCC visibility push(default)
+extern "C" {
+
+const char* glued_last_error(void) { return g_err.c_str(); }
+
+int glued_create(glued_handle* out)
+{
+    return guard([&] {
+        if (!out) throw glued::Error("null output pointer");
+        *out = new glued_force();
+    });
+}
+
+int glued_destroy(glued_handle h)
+{
+    return guard([&] { delete h; });
+}
+
+int glued_set_pbc(glued_handle h, int yes)
+{
+    return guard([&] { E(h).pbc = yes != 0; });
+}
+
+int glued_set_temperature(glued_handle h, double kelvin)
+{
+    return guard([&] { E(h).temperature = kelvin; });
+}
+
+int glued_add_cv(glued_handle h, int type, const int* atoms, int nAtoms, const double* params, int nParams,
+                 int* firstValueIdx)
+{
+    return guard([&] {
+        int v = E(h).addCV(type, atoms, nAtoms, params, nParams);
+        if (firstValueIdx) *firstValueIdx = v;
+    });
+}
+
+int glued_add_bias(glued_handle h, int type, const int* cvIdx, int nCv, const double* params, int nParams,
+                   const int* iparams, int nIparams, int* biasIdx)
+{
+    return guard([&] {
+        int b = E(h).addBias(type, cvIdx, nCv, params, nParams, iparams, nIparams);
+        if (biasIdx) *biasIdx = b;
+    });
+}
+
+int glued_num_cv_values(glued_handle h, int* n)
+{
+    return guard([&] { *n = E(h).numValues(); });
+}
+
+int glued_set_test_bias_gradients(glued_handle h, const double* grads, int n)
+{
+    return guard([&] { E(h).setTestBiasGradients(grads, n); });
+}
+
+int glued_share_bias(glued_handle h, int biasIdx, int shared)
+{
+    return guard([&] { E(h).setShared(biasIdx, shared); });
+}
+
+int glued_initialize(glued_handle h, int numParticles, int paddedNumParticles, const double* masses, int nMasses)
+{
+    return guard([&] { E(h).initialize(1, &numParticles, &paddedNumParticles, nMasses, masses); });
+}
+
+int glued_initialize_batch(glued_handle h, int numWalkers, const int* numParticles, const int* paddedNumParticles,
+                           const double* masses, int nMasses)
+{
+    return guard([&] { E(h).initialize(numWalkers, numParticles, paddedNumParticles, nMasses, masses); });
+}
+
+int glued_set_atom_order(glued_handle h, int walker, const int* gpuToUser, int n)
+{
+    return guard([&] { E(h).setAtomOrder(walker, gpuToUser, n); });
+}
+
+int glued_set_box(glued_handle h, int walker, const double* box9)
+{
+    return guard([&] { E(h).setBox(walker, box9); });
+}
+
+int glued_set_energy_input(glued_handle h, int walker, double U, const float* forceField, void* stream)
+{
+    return guard([&] { E(h).setEnergyInput(walker, U, forceField, (hipStream_t)stream); });
+}
+
+int glued_execute(glued_handle h, void* stream, int walker, const void* posq, int64_t* forces, int includeForces,
+                  int includeEnergy, double* energy)
+{
+    return guard([&] {
+        double e = E(h).execute((hipStream_t)stream, walker, (const float4*)posq, (long long*)forces,
+                                includeForces != 0, includeEnergy != 0);
+        if (energy) *energy = e;
+    });
+}
+
+int glued_execute_batch(glued_handle h, void* stream, const void* posq, int64_t* forces, int includeForces,
+                        int useGraph)
+{
+    return guard([&] {
+        E(h).executeBatch((hipStream_t)stream, (const float4*)posq, (long long*)forces, includeForces != 0,
+                          useGraph != 0);
+    });
+}
+
+int glued_execute_stage(glued_handle h, void* stream, const void* posq, int64_t* forces, int stage)
+{
+    return guard([&] { E(h).executeStage((hipStream_t)stream, (const float4*)posq, (long long*)forces, stage); });
+}
+
+int glued_update_state(glued_handle h, void* stream, int step)
+{
+    return guard([&] { E(h).updateState((hipStream_t)stream, step); });
+}
+
+int glued_get_cv_values(glued_handle h, int walker, double* out, int n)
+{
+    return guard([&] { copy_out(E(h).cvValues(walker), out, n, "get_cv_values"); });
+}
+
+int glued_get_bias_energies(glued_handle h, int walker, double* out, int n)
+{
+    return guard([&] { copy_out(E(h).biasEnergies(walker), out, n, "get_bias_energies"); });
+}
+
+int glued_get_dudcv(glued_handle h, int walker, double* out, int n)
+{
+    return guard([&] { copy_out(E(h).dUdCV(walker), out, n, "get_dudcv"); });
+}
+
+int glued_get_coordination_pairs(glued_handle h, int walker, int coordIdx, long long* npairs)
+{
+    return guard([&] { *npairs = E(h).coordinationPairs(walker, coordIdx); });
+}
+
+int glued_get_bias_state(glued_handle h, int walker, char* buf, size_t cap, size_t* size)
+{
+    return guard([&] {
+        auto b = E(h).getState(walker);
+        if (size) *size = b.size();
+        if (buf) {
+            if (cap < b.size()) throw glued::Error("get_bias_state: buffer too small");
+            std::memcpy(buf, b.data(), b.size());
+        }
+    });
+}
+
+int glued_set_bias_state(glued_handle h, int walker, const char* buf, size_t size)
+{
+    return guard([&] { E(h).setState(walker, std::vector<char>(buf, buf + size)); });
+}
+
+int glued_get_opes_metrics(glued_handle h, int walker, int opesIdx, double* out4)
+{
+    return guard([&] {
+        auto m = E(h).opesMetrics(walker, opesIdx);
+        for (int i = 0; i < 4; i++) out4[i] = m[i];
+    });
+}
+
+int glued_get_kernel_sigmas(glued_handle h, int walker, int opesIdx, float* out, int cap, int* n)
+{
+    return guard([&] {
+        auto s = E(h).kernelSigmas(walker, opesIdx);
+        if (n) *n = (int)s.size();
+        if (out) copy_out(s, out, cap, "get_kernel_sigmas");
+    });
+}
+
+int glued_set_hills(glued_handle h, int walker, int biasIdx, const double* centers, const double* heights, int n)
+{
+    return guard([&] { E(h).setHills(walker, biasIdx, centers, heights, n); });
+}
+
+int glued_get_hills(glued_handle h, int walker, int biasIdx, double* centers, double* heights, int cap, int* n)
+{
+    return guard([&] {
+        int k = E(h).getHills(walker, biasIdx, centers, heights, cap);
+        if (n) *n = k;
+    });
+}
+
+int glued_set_grid(glued_handle h, int walker, int biasIdx, const double* grid, int n)
+{
+    return guard([&] { E(h).setGrid(walker, biasIdx, grid, n); });
+}
+
+int glued_plan_info(glued_handle h, int* rows, int* atoms)
+{
+    return guard([&] {
+        if (rows) *rows = E(h).jacobianRows();
+        if (atoms) *atoms = E(h).atomSlots();
+    });
+}
+
+}  // extern "C"
+#pragma GCC visibility pop

@@ -1,0 +1,1396 @@
+// Host engine of the GLUED enhanced-sampling step (see engine.hpp).
+#include "engine.hpp"
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <numeric>
+
+namespace glued {
+
+// kernel launchers (kernels_*.hip)
+void launch_cv_small(const StepView&, const SmallCVItem*, int, int, int, hipStream_t);
+void launch_cv_group(const StepView&, const GroupCVItem*, int, const int*, const float*, const float*, int, int,
+                     hipStream_t);
+void launch_cv_path(const StepView&, const PathItem*, int, const int*, const float*, int, int, hipStream_t);
+
+void launch_coordination(const StepView&, const CoordBuffers&, const CoordItem*, const CoordItem&, int, int, int,
+                         hipStream_t);
+int coord_blocks_A(int);
+void launch_bias_small(const StepView&, const SmallBiasState&, const SmallBiasItem*, int, int, int, hipStream_t);
+void launch_bias_hills(const StepView&, const HillBiasDev&, int, int, hipStream_t);
+void launch_bias_opes(const StepView&, const OpesBiasDev&, int, int, hipStream_t);
+void launch_dep_grid(const StepView&, const GridBiasDev&, int, int, hipStream_t);
+void launch_dep_hills(const StepView&, const HillBiasDev&, int, int, hipStream_t);
+void launch_dep_opes(const StepView&, const OpesBiasDev&, int, int, hipStream_t);
+void launch_opes_welford(const StepView&, const OpesBiasDev&, int, int, hipStream_t);
+void launch_dep_small(const StepView&, const SmallBiasState&, const SmallBiasItem*, int, int, int, int, hipStream_t);
+int hill_chunks_for(int);
+int opes_chunks_for(int);
+
+void launch_scatter(const StepView&, const ScatterPlan&, int, int, int, hipStream_t);
+void launch_scatter_dense(const StepView&, const ScatterPlan&, int, int, int, hipStream_t);
+
+void hip_check(hipError_t e, const char* what)
+{
+    if (e != hipSuccess) throw Error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static void fail(const std::string& m) { throw Error(m); }
+
+Engine::Engine() {}
+
+Engine::~Engine()
+{
+    if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    if (captureStream_) (void)hipStreamDestroy(captureStream_);
+    for (auto* g : grids_) delete g;
+    for (auto* h : hills_) delete h;
+    for (auto* o : opes_) delete o;
+}
+
+// ---------------------------------------------------------------------------
+// Specification: GluedForce.cpp:44-107.  PATH occupies two value slots.
+// ---------------------------------------------------------------------------
+int Engine::addCV(int type, const int* atoms, int na, const double* params, int np)
+{
+    if (initialized_) fail("addCollectiveVariable: force already initialized");
+    CVSpec c;
+    c.type = type;
+    c.atoms.assign(atoms, atoms + na);
+    c.params.assign(params, params + np);
+    c.firstValue = numValues_;
+    numValues_ += (type == CV_PATH) ? 2 : 1;
+    cvs_.push_back(c);
+    return c.firstValue;
+}
+
+int Engine::addBias(int type, const int* cvs, int nc, const double* params, int np, const int* ip, int ni)
+{
+    if (initialized_) fail("addBias: force already initialized");
+    for (int i = 0; i < nc; i++)
+        if (cvs[i] < 0 || cvs[i] >= numValues_)
+            fail("addBias: CV index " + std::to_string(cvs[i]) + " out of range [0," + std::to_string(numValues_) +
+                 ")");
+    BiasSpec b;
+    b.type = type;
+    b.cvs.assign(cvs, cvs + nc);
+    b.params.assign(params, params + np);
+    b.iparams.assign(ip, ip + ni);
+    biases_.push_back(b);
+    return (int)biases_.size() - 1;
+}
+
+void Engine::setShared(int bias, int shared)
+{
+    if (initialized_) fail("share_bias: call before initialize");
+    if (bias < 0 || bias >= (int)biases_.size()) fail("share_bias: bias index out of range");
+    biases_[bias].shared = shared ? 1 : 0;
+}
+
+void Engine::setTestBiasGradients(const double* g, int n)
+{
+    if (initialized_) fail("setTestBiasGradients: must be called before initialization");
+    testGrads_.assign(g, g + n);
+}
+
+// ---------------------------------------------------------------------------
+// Validation: CommonGluedKernels.cpp:3879-3927 (atom ranges, minimum atom and
+// parameter counts), CommonBiasKernels.cpp (bias parameter layouts).
+// ---------------------------------------------------------------------------
+void Engine::validate(int minAtoms) const
+{
+    for (size_t i = 0; i < cvs_.size(); i++) {
+        const CVSpec& c = cvs_[i];
+        auto err = [&](const std::string& m) { fail("GluedForce CV " + std::to_string(i) + ": " + m); };
+        auto needA = [&](size_t n) {
+            if (c.atoms.size() < n)
+                err("expected >= " + std::to_string(n) + " atoms, got " + std::to_string(c.atoms.size()));
+        };
+        auto needP = [&](size_t n) {
+            if (c.params.size() < n)
+                err("expected >= " + std::to_string(n) + " parameters, got " + std::to_string(c.params.size()));
+        };
+        const bool coord = c.type == CV_COORDINATION;
+        for (size_t k = coord ? 1 : 0; k < c.atoms.size(); k++)
+            if (c.atoms[k] < 0 || c.atoms[k] >= minAtoms)
+                err("atom index " + std::to_string(c.atoms[k]) + " out of range [0," + std::to_string(minAtoms) + ")");
+        switch (c.type) {
+            case CV_DISTANCE: needA(2); break;
+            case CV_ANGLE: needA(3); break;
+            case CV_DIHEDRAL: needA(4); break;
+            case CV_COM_DISTANCE: needA(1); needP(1); break;
+            case CV_GYRATION: needA(1); break;
+            case CV_COORDINATION:
+                needA(2);
+                needP(3);
+                if (c.atoms[0] < 1 || c.atoms[0] >= (int)c.atoms.size())
+                    err("coordination group-A size atoms[0]=" + std::to_string(c.atoms[0]) + " invalid");
+                break;
+            case CV_RMSD: needA(1); needP(3 * c.atoms.size()); break;
+            case CV_PATH: {
+                needP(2);
+                int N = (int)c.params[1];
+                if (N < 2) err("path needs >= 2 frames");
+                if (N > 1024) err("path frame count " + std::to_string(N) + " exceeds 1024");
+                needP(2 + (size_t)N * 3 * c.atoms.size());
+                break;
+            }
+            case CV_POSITION: needA(1); needP(1); break;
+            case CV_DRMSD: needA(2); needP(c.atoms.size() / 2); break;
+            case CV_CONTACTMAP: needA(2); needP(4 * (c.atoms.size() / 2)); break;
+            case CV_ENERGY: break;
+            default: err("CV type " + std::to_string(c.type) + " is not supported by this implementation");
+        }
+    }
+    for (size_t b = 0; b < biases_.size(); b++) {
+        const BiasSpec& s = biases_[b];
+        auto err = [&](const std::string& m) { fail("GluedForce bias " + std::to_string(b) + ": " + m); };
+        const int D = (int)s.cvs.size();
+        if (D < 1) err("needs at least one CV");
+        switch (s.type) {
+            case BIAS_METAD:
+                if (D > 3) err("grid metadynamics supports at most 3 CVs");
+                if ((int)s.params.size() < 3 + 3 * D) err("METAD params must be [h0, sigma*D, gamma, kT, min*D, max*D]");
+                if ((int)s.iparams.size() < 1 + 2 * D) err("METAD int params must be [pace, bins*D, periodic*D]");
+                break;
+            case BIAS_METAD_HILLS:
+                if (D > 4) err("hill metadynamics supports at most 4 CVs");
+                if ((int)s.params.size() < 3 + 2 * D) err("hill METAD params must be [h0, sigma*D, gamma, kT, period*D]");
+                if ((int)s.iparams.size() < 2) err("hill METAD int params must be [pace, maxHills]");
+                break;
+            case BIAS_OPES:
+                if (D > 4) err("OPES supports at most 4 CVs");
+                if ((int)s.params.size() < 3 + D) err("OPES params must be [kT, gamma, sigma0*D, sigmaMin]");
+                break;
+            case BIAS_OPES_EXPANDED:
+                if ((int)s.params.size() < 1 + D) err("OPES_EXPANDED params must be [kT, w*D]");
+                for (int d = 1; d < D; d++)
+                    if (s.cvs[d] != s.cvs[0] + d) err("OPES_EXPANDED CVs must be consecutive value indices");
+                break;
+            case BIAS_OPES_MULTITHERMAL: {
+                if ((int)s.params.size() < 2)
+                    err("BIAS_OPES_MULTITHERMAL needs at least one temperature (params = [kT0, beta_0, ...])");
+                bool ok = false;
+                for (auto& c : cvs_)
+                    if (c.type == CV_ENERGY && c.firstValue == s.cvs[0]) ok = true;
+                if (!ok) err("BIAS_OPES_MULTITHERMAL: cvIndices[0] must reference a CV_ENERGY value index");
+                break;
+            }
+            default: err("bias type " + std::to_string(s.type) + " is not supported by this implementation");
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Plan
+// ---------------------------------------------------------------------------
+void Engine::initialize(int W, const int* natoms, const int* padded, int nMass, const double* masses)
+{
+    if (initialized_) fail("initialize: already initialized");
+    if (W < 1) fail("initialize: need at least one walker");
+    W_ = W;
+    natoms_.assign(natoms, natoms + W);
+    padded_.assign(padded, padded + W);
+    int minAtoms = *std::min_element(natoms_.begin(), natoms_.end());
+    for (int w = 0; w < W; w++)
+        if (padded_[w] < natoms_[w]) fail("initialize: padded atom count smaller than atom count");
+    validate(minAtoms);
+    posOff_.resize(W);
+    forceOff_.resize(W);
+    long long po = 0;
+    maxPadded_ = 0;
+    for (int w = 0; w < W; w++) {
+        posOff_[w] = po;
+        forceOff_[w] = 3 * po;
+        po += padded_[w];
+        maxPadded_ = std::max(maxPadded_, padded_[w]);
+    }
+    buildPlan(nMass, masses);
+    initialized_ = true;
+}
+
+void Engine::buildPlan(int nMass, const double* masses)
+{
+    // ---- atom slots ----
+    std::map<int, int> slotOf;
+    auto slot = [&](int atom) {
+        auto it = slotOf.find(atom);
+        if (it != slotOf.end()) return it->second;
+        int s = (int)slotAtom_.size();
+        slotAtom_.push_back(atom);
+        slotOf[atom] = s;
+        return s;
+    };
+    auto massOf = [&](int atom) -> float {
+        if (masses && atom < nMass) return (float)masses[atom];
+        return 1.0f;
+    };
+    V_ = numValues_;
+    std::vector<int> groupSlots, coordSlots, pathSlots;
+    std::vector<float> groupMass, groupRef, pathFrames;
+    auto addRow = [&](int s, int cv) {
+        rowSlot_.push_back(s);
+        rowCv_.push_back(cv);
+        return (int)rowSlot_.size() - 1;
+    };
+    for (const CVSpec& c : cvs_) {
+        const int v = c.firstValue;
+        switch (c.type) {
+            case CV_DISTANCE:
+            case CV_ANGLE:
+            case CV_DIHEDRAL:
+            case CV_POSITION: {
+                SmallCVItem it{};
+                it.type = c.type;
+                it.value = v;
+                int na = c.type == CV_DISTANCE ? 2 : c.type == CV_ANGLE ? 3 : c.type == CV_DIHEDRAL ? 4 : 1;
+                it.row0 = (int)rowSlot_.size();
+                for (int a = 0; a < na; a++) {
+                    it.slot[a] = slot(c.atoms[a]);
+                    addRow(it.slot[a], v);
+                }
+                it.comp = c.type == CV_POSITION ? (int)c.params[0] : 0;
+                if (c.type == CV_POSITION && (it.comp < 0 || it.comp > 2)) fail("position component must be 0, 1 or 2");
+                small_.push_back(it);
+                break;
+            }
+            case CV_ENERGY: {
+                if (energyValue_ >= 0) fail("only one CV_ENERGY is supported");
+                energyValue_ = v;
+                SmallCVItem it{};
+                it.type = CV_ENERGY;
+                it.value = v;
+                it.row0 = -1;
+                small_.push_back(it);
+                break;
+            }
+            case CV_GYRATION:
+            case CV_COM_DISTANCE:
+            case CV_RMSD: {
+                GroupCVItem it{};
+                it.type = c.type;
+                it.value = v;
+                it.begin = (int)groupSlots.size();
+                it.row0 = (int)rowSlot_.size();
+                const int n = (int)c.atoms.size();
+                for (int a = 0; a < n; a++) {
+                    int s = slot(c.atoms[a]);
+                    groupSlots.push_back(s);
+                    groupMass.push_back(massOf(c.atoms[a]));
+                    addRow(s, v);
+                }
+                it.end = (int)groupSlots.size();
+                if (c.type == CV_COM_DISTANCE) {
+                    it.split = (int)std::lround(c.params[0]);
+                    if (it.split < 1 || it.split >= n) fail("COM distance: group-1 size out of range");
+                }
+                if (c.type == CV_RMSD) {
+                    // params = reference xyz (float, as the reference stores it), optional
+                    // trailing fit flag: 1 = optimal rotation + translation (Kabsch)
+                    it.split = ((int)c.params.size() > 3 * n && c.params[3 * n] != 0.0) ? 1 : 0;
+                    if ((groupRef.size() % 3) != 0) groupRef.resize((groupRef.size() / 3 + 1) * 3, 0.f);
+                    it.refOff = (int)(groupRef.size() / 3);
+                    double cs[3] = {0, 0, 0};
+                    for (int k = 0; k < 3 * n; k++) {
+                        float f = (float)c.params[k];
+                        groupRef.push_back(f);
+                        cs[k % 3] += (double)f;
+                    }
+                    for (int k = 0; k < 3; k++) it.refCentroid[k] = cs[k] / n;
+                }
+                group_.push_back(it);
+                break;
+            }
+            case CV_DRMSD:
+            case CV_CONTACTMAP: {
+                GroupCVItem it{};
+                it.type = c.type;
+                it.value = v;
+                it.begin = (int)groupSlots.size();
+                it.row0 = (int)rowSlot_.size();
+                const int np = (int)c.atoms.size() / 2;
+                for (int p = 0; p < np; p++) {
+                    int sa = slot(c.atoms[2 * p]), sb = slot(c.atoms[2 * p + 1]);
+                    groupSlots.push_back(sa);
+                    groupSlots.push_back(sb);
+                    groupMass.push_back(1.f);
+                    groupMass.push_back(1.f);
+                    addRow(sa, v);
+                    addRow(sb, v);
+                }
+                it.end = (int)groupSlots.size();
+                it.refOff = (int)groupRef.size();
+                const int per = c.type == CV_DRMSD ? 1 : 4;
+                for (int k = 0; k < per * np; k++) groupRef.push_back((float)c.params[k]);
+                group_.push_back(it);
+                break;
+            }
+            case CV_COORDINATION: {
+                CoordItem it{};
+                const int nA = c.atoms[0];
+                const int nB = (int)c.atoms.size() - 1 - nA;
+                it.value = v;
+                it.nA = nA;
+                it.nB = nB;
+                it.aBegin = (int)coordSlots.size();
+                it.rowA = (int)rowSlot_.size();
+                for (int a = 0; a < nA; a++) {
+                    int s = slot(c.atoms[1 + a]);
+                    coordSlots.push_back(s);
+                    addRow(s, v);
+                }
+                it.bBegin = (int)coordSlots.size();
+                it.rowB = (int)rowSlot_.size();
+                for (int b = 0; b < nB; b++) {
+                    int s = slot(c.atoms[1 + nA + b]);
+                    coordSlots.push_back(s);
+                    addRow(s, v);
+                }
+                it.r0 = (float)c.params[0];
+                it.rn = (float)c.params[1];
+                it.rm = (float)c.params[2];
+                it.intPow = (it.rn == std::floor(it.rn) && it.rm == std::floor(it.rm) && it.rn >= 0 && it.rm >= 0 &&
+                             it.rn <= 64 && it.rm <= 64);
+                const double rc = c.params.size() > 3 ? c.params[3] : 0.0;
+                it.rc2 = rc > 0.0 ? rc * rc : 0.0;
+                coord_.push_back(it);
+                break;
+            }
+            case CV_PATH: {
+                PathItem it{};
+                const int M = (int)c.atoms.size(), N = (int)c.params[1];
+                it.M = M;
+                it.N = N;
+                it.lambda = (float)c.params[0];
+                it.valueS = v;
+                it.begin = (int)pathSlots.size();
+                it.row0 = (int)rowSlot_.size();
+                for (int j = 0; j < M; j++) {
+                    int s = slot(c.atoms[j]);
+                    pathSlots.push_back(s);
+                    addRow(s, v);
+                    addRow(s, v + 1);
+                }
+                it.frameOff = (int)pathFrames.size();
+                for (int k = 0; k < N * M * 3; k++) pathFrames.push_back((float)c.params[2 + k]);
+                path_.push_back(it);
+                break;
+            }
+        }
+    }
+    S_ = (int)slotAtom_.size();
+    R_ = (int)rowSlot_.size();
+
+    // ---- walkers, atom order, box ----
+    std::vector<WalkerInfo> wi(W_);
+    for (int w = 0; w < W_; w++) {
+        wi[w].posOff = posOff_[w];
+        wi[w].forceOff = forceOff_[w];
+        wi[w].padded = padded_[w];
+        wi[w].natoms = natoms_[w];
+    }
+    dWalkers_.upload(wi);
+    userToGpu_.assign(W_, std::vector<int>());
+    dGidx_.alloc(sizeof(int) * (size_t)W_ * std::max(S_, 1));
+    for (int w = 0; w < W_; w++) uploadGidx(w);
+    boxHost_.assign((size_t)W_ * 12, 0.0);
+    for (int w = 0; w < W_; w++) {
+        double* b = &boxHost_[(size_t)w * 12];
+        b[0] = b[4] = b[8] = 1.0;
+        b[9] = b[10] = b[11] = 1.0;
+    }
+    dBox_.upload(boxHost_);
+
+    // ---- per-walker step buffers ----
+    dCv_.alloc(sizeof(double) * (size_t)W_ * std::max(V_, 1));
+    dJx_.alloc(sizeof(float) * (size_t)W_ * std::max(R_, 1));
+    dJy_.alloc(sizeof(float) * (size_t)W_ * std::max(R_, 1));
+    dJz_.alloc(sizeof(float) * (size_t)W_ * std::max(R_, 1));
+    dUd_.alloc(sizeof(double) * (size_t)W_ * std::max(V_, 1));
+    dSmall_.upload(small_);
+    dGroup_.upload(group_);
+    dGroupSlots_.upload(groupSlots);
+    dGroupMass_.upload(groupMass);
+    dGroupRef_.upload(groupRef);
+    dPath_.upload(path_);
+    dPathSlots_.upload(pathSlots);
+    dPathFrames_.upload(pathFrames);
+    energyU_.assign(W_, 0.0);
+    energyF_.assign(W_, nullptr);
+    dEnergyU_.alloc(sizeof(double) * W_);
+
+    // ---- coordination buffers ----
+    {
+        int cellBase = 0, sortBase = 0, partBase = 0, cellOf = 0;
+        for (auto& it : coord_) {
+            it.useCells = 0;
+            it.nc[0] = it.nc[1] = it.nc[2] = 1;
+            it.cellBase = cellBase;
+            it.sortBase = sortBase;
+            it.partBase = partBase;
+            cellBase += 2;
+            sortBase += it.nA + it.nB;
+            cellOf += it.nA + it.nB;
+            partBase += coord_blocks_A(it.nA);
+        }
+        cellOfStride_ = std::max(cellOf, 1);
+        sortStride_ = std::max(sortBase, 1);
+        partStride_ = std::max(partBase, 1);
+        cellStride_ = std::max(cellBase, 1);
+        dCellOf_.alloc(sizeof(int) * (size_t)W_ * cellOfStride_);
+        dSorted_.alloc(sizeof(float4) * (size_t)W_ * sortStride_);
+        dPartCN_.alloc(sizeof(double) * (size_t)W_ * partStride_);
+        dPartPairs_.alloc(sizeof(unsigned long long) * (size_t)W_ * partStride_);
+        dCounters_.alloc(sizeof(unsigned int) * (size_t)W_ * std::max((int)coord_.size(), 1));
+        dNpairs_.alloc(sizeof(long long) * (size_t)W_ * std::max((int)coord_.size(), 1));
+        dCellCount_.alloc(sizeof(int) * (size_t)W_ * cellStride_);
+        dCellStart_.alloc(sizeof(int) * (size_t)W_ * (cellStride_ + coord_.size() + 1));
+        dCoord_.upload(coord_);
+        dCoordSlots_.upload(coordSlots);
+    }
+
+    // ---- biases ----
+    int gslot = 0;
+    std::vector<std::vector<int>> comb(V_);
+    std::vector<GridBiasDev> gridTable;
+    int scalCount = 0, varCount = 0;
+    for (size_t b = 0; b < biases_.size(); b++) {
+        const BiasSpec& s = biases_[b];
+        const int D = (int)s.cvs.size();
+        const int T = s.shared ? 1 : W_;  // tables
+        for (int d = 0; d < D; d++) comb[s.cvs[d]].push_back(gslot + d);
+        if (s.type == BIAS_METAD) {
+            auto* g = new GridHost();
+            GridBiasDev& d = g->d;
+            d = GridBiasDev{};
+            d.D = D;
+            for (int k = 0; k < D; k++) d.cv[k] = s.cvs[k];
+            d.gslot = gslot;
+            d.eslot = (int)b;
+            d.h0 = s.params[0];
+            for (int k = 0; k < D; k++) d.sigma[k] = s.params[1 + k];
+            double gamma = s.params[1 + D], kT = s.params[2 + D];
+            for (int k = 0; k < D; k++) {
+                double mn = s.params[3 + D + k], mx = s.params[3 + 2 * D + k];
+                int bins = s.iparams[1 + k];
+                d.periodic[k] = s.iparams[1 + D + k];
+                if (bins < 1 || !(mx > mn)) fail("METAD: invalid grid range or bin count");
+                d.origin[k] = mn;
+                d.invSpacing[k] = bins / (mx - mn);
+                d.npts[k] = bins + (d.periodic[k] ? 0 : 1);
+            }
+            d.strides[0] = 1;
+            for (int k = 1; k < D; k++) d.strides[k] = d.strides[k - 1] * d.npts[k - 1];
+            d.total = d.strides[D - 1] * d.npts[D - 1];
+            d.hif = gamma <= 1.0 ? 0.0 : 1.0 / ((gamma - 1.0) * kT);
+            g->pace = s.iparams[0];
+            g->spec = (int)b;
+            g->numDeposited.assign(T, 0);
+            g->grid.alloc(sizeof(double) * (size_t)T * d.total);
+            d.grid = g->grid.as<double>();
+            d.walkerStride = s.shared ? 0 : d.total;
+            SmallBiasItem it{};
+            it.type = BIAS_METAD;
+            it.D = D;
+            it.gslot = gslot;
+            it.eslot = (int)b;
+            it.bias = (int)gridTable.size();
+            gridTable.push_back(d);
+            grids_.push_back(g);
+            smallBias_.push_back(it);
+            smallBiasHost_.push_back({(int)b, g->pace, BIAS_METAD});
+        } else if (s.type == BIAS_METAD_HILLS) {
+            auto* h = new HillHost();
+            HillBiasDev& d = h->d;
+            d = HillBiasDev{};
+            d.D = D;
+            for (int k = 0; k < D; k++) d.cv[k] = s.cvs[k];
+            d.gslot = gslot;
+            d.eslot = (int)b;
+            d.h0 = s.params[0];
+            double gamma = s.params[1 + D], kT = s.params[2 + D];
+            for (int k = 0; k < D; k++) {
+                d.sigma[k] = s.params[1 + k];
+                d.period[k] = s.params[3 + D + k];
+                if (!(d.sigma[k] > 0.0)) fail("hill METAD: sigma must be positive");
+                d.i2s2[k] = 1.0 / (2.0 * d.sigma[k] * d.sigma[k]);
+                d.is2[k] = 1.0 / (d.sigma[k] * d.sigma[k]);
+                d.invPeriod[k] = d.period[k] > 0.0 ? 1.0 / d.period[k] : 0.0;
+            }
+            d.hif = gamma <= 1.0 ? 0.0 : 1.0 / ((gamma - 1.0) * kT);
+            d.cap = std::max(1, s.iparams[1]);
+            d.shared = s.shared;
+            d.chunks = hill_chunks_for(d.cap);
+            h->pace = s.iparams[0];
+            h->spec = (int)b;
+            h->centers.alloc(sizeof(double) * (size_t)T * D * d.cap);
+            h->heights.alloc(sizeof(double) * (size_t)T * d.cap);
+            h->count.alloc(sizeof(int) * T);
+            h->part.alloc(sizeof(double) * (size_t)W_ * d.chunks * (D + 1));
+            h->counter.alloc(sizeof(unsigned int) * W_);
+            d.centers = h->centers.as<double>();
+            d.heights = h->heights.as<double>();
+            d.count = h->count.as<int>();
+            d.part = h->part.as<double>();
+            d.counter = h->counter.as<unsigned int>();
+            hills_.push_back(h);
+        } else if (s.type == BIAS_OPES) {
+            auto* o = new OpesHost();
+            OpesBiasDev& d = o->d;
+            d = OpesBiasDev{};
+            d.D = D;
+            for (int k = 0; k < D; k++) d.cv[k] = s.cvs[k];
+            d.gslot = gslot;
+            d.eslot = (int)b;
+            d.kT = s.params[0];
+            d.gamma = s.params[1];
+            for (int k = 0; k < D; k++) d.sigma0[k] = s.params[2 + k];
+            d.sigmaMin = s.params[2 + D];
+            d.variant = s.iparams.size() > 0 ? s.iparams[0] : 0;
+            o->pace = s.iparams.size() > 1 ? s.iparams[1] : 500;
+            d.cap = s.iparams.size() > 2 ? s.iparams[2] : 100000;
+            const bool adaptive = d.sigma0[0] <= 0.0;
+            d.adaptiveStride = adaptive ? (s.iparams.size() > 3 ? s.iparams[3] : 10 * o->pace) : 0;
+            // variant constants, CommonBiasKernels.cpp:553-588
+            if (d.variant == 1) {
+                d.invGF = 1.0;
+                d.cutoff2 = 2.0 * d.gamma;
+                d.eps = 0.0;
+                d.barrier = d.gamma * d.kT;
+            } else if (d.variant == 2) {
+                if (!std::isfinite(d.gamma) || d.gamma <= 1.0)
+                    fail("OPES_METAD_EXPLORE (variant=2) requires finite gamma > 1");
+                d.invGF = d.gamma - 1.0;
+                d.cutoff2 = 2.0 * d.gamma;
+                d.eps = std::exp(-d.gamma / (d.gamma - 1.0));
+                d.barrier = d.gamma * d.kT;
+                if (!adaptive)
+                    for (int k = 0; k < D; k++) d.sigma0[k] *= std::sqrt(d.gamma);
+            } else {
+                d.invGF = (d.gamma - 1.0) / d.gamma;
+                d.cutoff2 = 2.0 * d.gamma / d.invGF;
+                d.eps = std::exp(-1.0 / (d.invGF * (1.0 - d.invGF)));
+                d.barrier = d.kT / (1.0 - d.invGF);
+            }
+            d.shared = s.shared;
+            d.chunks = opes_chunks_for(d.cap);
+            o->spec = (int)b;
+            o->depositMirror.assign(T, 0);
+            o->nSamplesMirror.assign(T, 0);
+            o->centers.alloc(sizeof(double) * (size_t)T * D * d.cap);
+            o->sigmas.alloc(sizeof(double) * (size_t)T * D * d.cap);
+            o->logw.alloc(sizeof(double) * (size_t)T * d.cap);
+            o->ints.alloc(sizeof(int) * 4 * T);
+            std::vector<double> dbl((size_t)T * (4 + 2 * D), 0.0);
+            for (int t = 0; t < T; t++) {
+                dbl[(size_t)t * (4 + 2 * D) + 1] = std::exp(-d.gamma);        // KDNorm sentinel
+                dbl[(size_t)t * (4 + 2 * D) + 2] = std::exp(-d.gamma);        // sum_w sentinel
+                dbl[(size_t)t * (4 + 2 * D) + 3] = std::exp(-2.0 * d.gamma);  // sum_w2 sentinel
+            }
+            o->dbl.upload(dbl);
+            o->part.alloc(sizeof(double) * (size_t)W_ * d.chunks * (D + 1));
+            o->counter.alloc(sizeof(unsigned int) * W_);
+            d.centers = o->centers.as<double>();
+            d.sigmas = o->sigmas.as<double>();
+            d.logw = o->logw.as<double>();
+            d.ints = o->ints.as<int>();
+            d.dbl = o->dbl.as<double>();
+            d.part = o->part.as<double>();
+            d.counter = o->counter.as<unsigned int>();
+            opes_.push_back(o);
+        } else if (s.type == BIAS_OPES_EXPANDED || s.type == BIAS_OPES_MULTITHERMAL) {
+            SmallBiasItem it{};
+            it.type = s.type;
+            it.gslot = gslot;
+            it.eslot = (int)b;
+            it.cv[0] = s.cvs[0];
+            it.stateOff = (int)smallConst_.size();
+            int pace = s.iparams.size() > 0 ? s.iparams[0] : 500;
+            if (s.type == BIAS_OPES_EXPANDED) {
+                it.D = D;
+                it.nState = D;
+                it.p[0] = 1.0 / s.params[0];
+                double ws = 0.0;
+                for (int l = 0; l < D; l++) ws += s.params[1 + l];
+                if (ws <= 0.0) ws = D;
+                for (int l = 0; l < D; l++) smallConst_.push_back(std::log(s.params[1 + l] / ws));
+            } else {
+                const int N = (int)s.params.size() - 1;
+                it.D = 1;
+                it.nState = N;
+                it.p[0] = s.params[0];
+                const double beta0 = 1.0 / s.params[0];
+                for (int l = 0; l < N; l++) smallConst_.push_back(s.params[1 + l] - beta0);
+                varCount = std::max(varCount, it.stateOff + N);
+            }
+            scalCount++;
+            smallBias_.push_back(it);
+            smallBiasHost_.push_back({(int)b, pace, s.type});
+        }
+        gslot += D;
+    }
+    G_ = std::max(gslot, 1);
+    B_ = std::max((int)biases_.size(), 1);
+    dBgrad_.alloc(sizeof(double) * (size_t)W_ * G_);
+    dBener_.alloc(sizeof(double) * (size_t)W_ * B_);
+    dGridTable_.upload(gridTable);
+    dSmallBias_.upload(smallBias_);
+    dSmallConst_.upload(smallConst_);
+    smallVarPerWalker_ = std::max((int)smallConst_.size(), 1);
+    dSmallVar_.alloc(sizeof(double) * (size_t)W_ * smallVarPerWalker_);
+    {
+        // per walker per small bias: {logZ, numUpdates, rct, counter}; multithermal counter starts at 1
+        std::vector<double> scal((size_t)W_ * std::max((int)smallBias_.size(), 1) * 4, 0.0);
+        for (int w = 0; w < W_; w++)
+            for (size_t i = 0; i < smallBias_.size(); i++)
+                if (smallBias_[i].type == BIAS_OPES_MULTITHERMAL) scal[((size_t)w * smallBias_.size() + i) * 4 + 3] = 1.0;
+        dSmallScal_.upload(scal);
+    }
+    combStart_.assign(V_ + 1, 0);
+    for (int v = 0; v < V_; v++) {
+        combStart_[v + 1] = combStart_[v] + (int)comb[v].size();
+        for (int g : comb[v]) combSlot_.push_back(g);
+    }
+    dCombStart_.upload(combStart_);
+    dCombSlot_.upload(combSlot_);
+    if (!testGrads_.empty()) {
+        if ((int)testGrads_.size() != V_) fail("setTestBiasGradients: length must equal the number of CV values");
+        dTestGrad_.upload(testGrads_);
+    }
+
+    // ---- scatter segments: rows grouped by atom slot ----
+    std::vector<int> segStart(S_ + 1, 0), segRow(R_);
+    for (int r = 0; r < R_; r++) segStart[rowSlot_[r] + 1]++;
+    for (int s = 0; s < S_; s++) segStart[s + 1] += segStart[s];
+    {
+        std::vector<int> cur(segStart.begin(), segStart.end() - 1);
+        for (int r = 0; r < R_; r++) segRow[cur[rowSlot_[r]]++] = r;
+    }
+    dSegStart_.upload(segStart);
+    dSegRow_.upload(segRow);
+    dRowCv_.upload(rowCv_);
+}
+
+void Engine::uploadGidx(int w)
+{
+    std::vector<int> g(std::max(S_, 1), 0);
+    for (int s = 0; s < S_; s++) {
+        int u = slotAtom_[s];
+        g[s] = userToGpu_[w].empty() ? u : userToGpu_[w][u];
+    }
+    hip_check(hipMemcpy(dGidx_.as<int>() + (size_t)w * std::max(S_, 1), g.data(), sizeof(int) * g.size(),
+                        hipMemcpyHostToDevice),
+              "upload atom indices");
+}
+
+// rebuildGpuAtomIndices, CommonCVKernels.cpp:113-317: user -> GPU order.
+void Engine::setAtomOrder(int w, const int* gpuToUser, int n)
+{
+    if (!initialized_) fail("set_atom_order: initialize first");
+    if (w < 0 || w >= W_) fail("set_atom_order: walker out of range");
+    if (n < natoms_[w]) fail("set_atom_order: array shorter than the atom count");
+    std::vector<int> u2g(natoms_[w], -1);
+    for (int g = 0; g < n; g++) {
+        int u = gpuToUser[g];
+        if (u >= 0 && u < natoms_[w]) u2g[u] = g;
+    }
+    for (int s = 0; s < S_; s++)
+        if (u2g[slotAtom_[s]] < 0) fail("set_atom_order: atom " + std::to_string(slotAtom_[s]) + " missing");
+    userToGpu_[w] = u2g;
+    uploadGidx(w);
+}
+
+void Engine::setBox(int w, const double* b9)
+{
+    if (!initialized_) fail("set_box: initialize first");
+    if (w < 0 || w >= W_) fail("set_box: walker out of range");
+    double b[12];
+    for (int k = 0; k < 9; k++) b[k] = b9[k];
+    if (!(b[0] > 0 && b[4] > 0 && b[8] > 0)) fail("set_box: box vectors must have positive diagonal");
+    b[9] = 1.0 / b[0];
+    b[10] = 1.0 / b[4];
+    b[11] = 1.0 / b[8];
+    if (std::memcmp(b, &boxHost_[(size_t)w * 12], sizeof(b)) == 0) return;
+    std::memcpy(&boxHost_[(size_t)w * 12], b, sizeof(b));
+    hip_check(hipMemcpy(dBox_.as<double>() + (size_t)w * 12, b, sizeof(b), hipMemcpyHostToDevice), "upload box");
+    // cell lists depend on the box
+    bool ortho = b[1] == 0 && b[2] == 0 && b[3] == 0 && b[5] == 0 && b[6] == 0 && b[7] == 0;
+    bool changed = false;
+    for (size_t i = 0; i < coord_.size(); i++) {
+        CoordItem& it = coord_[i];
+        int nc[3] = {1, 1, 1};
+        int use = 0;
+        if (pbc && ortho && it.rc2 > 0.0) {
+            const double rc = std::sqrt(it.rc2) * (1.0 + 1e-6);
+            use = 1;
+            for (int k = 0; k < 3; k++) {
+                nc[k] = (int)std::floor(b[4 * k] / rc);
+                if (nc[k] < 3) use = 0;
+                nc[k] = std::min(nc[k], 64);
+            }
+        }
+        if (!use) nc[0] = nc[1] = nc[2] = 1;
+        // one box geometry for all walkers: the smallest cell count wins
+        if (w > 0 && it.useCells) {
+            for (int k = 0; k < 3; k++) nc[k] = std::min(nc[k], it.nc[k]);
+            use = use && it.useCells;
+        }
+        if (use != it.useCells || nc[0] != it.nc[0] || nc[1] != it.nc[1] || nc[2] != it.nc[2]) changed = true;
+        it.useCells = use;
+        for (int k = 0; k < 3; k++) it.nc[k] = nc[k];
+    }
+    if (changed) {
+        int cellBase = 0;
+        for (auto& it : coord_) {
+            it.cellBase = cellBase;
+            cellBase += 2 * it.nc[0] * it.nc[1] * it.nc[2];
+        }
+        if (cellBase > cellStride_) {
+            cellStride_ = cellBase;
+            dCellCount_.alloc(sizeof(int) * (size_t)W_ * cellStride_);
+            dCellStart_.alloc(sizeof(int) * (size_t)W_ * (cellStride_ + coord_.size() + 1));
+        }
+        dCoord_.upload(coord_);
+        if (graphExec_) {
+            (void)hipGraphExecDestroy(graphExec_);
+            (void)hipGraphDestroy(graph_);
+            graphExec_ = nullptr;
+            graph_ = nullptr;
+        }
+    }
+}
+
+// CV_ENERGY input: U and the external force F (device, field-major FP32 with
+// the walker's force-block layout).  The energy CV's Jacobian is -F.
+void Engine::setEnergyInput(int w, double U, const float* Fext, hipStream_t s)
+{
+    if (!initialized_) fail("set_energy_input: initialize first");
+    if (w < 0 || w >= W_) fail("set_energy_input: walker out of range");
+    if (energyValue_ < 0) fail("set_energy_input: the force has no CV_ENERGY");
+    energyU_[w] = U;
+    energyF_[w] = Fext;
+    hip_check(hipMemcpyAsync(dEnergyU_.as<double>() + w, &energyU_[w], sizeof(double), hipMemcpyHostToDevice, s),
+              "energy input");
+    hip_check(hipStreamSynchronize(s), "energy input sync");
+}
+
+StepView Engine::view(const float4* posq, long long* force) const
+{
+    StepView v;
+    v.posq = posq;
+    v.force = force;
+    v.walkers = dWalkers_.as<WalkerInfo>();
+    v.gidx = dGidx_.as<int>();
+    v.box = dBox_.as<double>();
+    v.cvVal = dCv_.as<double>();
+    v.jx = dJx_.as<float>();
+    v.jy = dJy_.as<float>();
+    v.jz = dJz_.as<float>();
+    v.bgrad = dBgrad_.as<double>();
+    v.benergy = dBener_.as<double>();
+    v.dUdCV = dUd_.as<double>();
+    v.energyU = dEnergyU_.as<double>();
+    v.S = std::max(S_, 1);
+    v.V = std::max(V_, 1);
+    v.R = std::max(R_, 1);
+    v.G = G_;
+    v.B = B_;
+    v.pbc = pbc ? 1 : 0;
+    // the kernels index with the padded strides used for allocation
+    v.S = std::max(S_, 1);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// One step for walkers [w0, w0+nw): CVs by kind, then biases, then the
+// scatter (the ordering contract of CommonGluedKernels.cpp:4646-5048).
+// ---------------------------------------------------------------------------
+void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMask)
+{
+    StepView v = view(lastPos_, lastForce_);
+    v.S = S_;
+    if (stageMask & 1) {
+    launch_cv_small(v, dSmall_.as<SmallCVItem>(), (int)small_.size(), w0, nw, s);
+    launch_cv_group(v, dGroup_.as<GroupCVItem>(), (int)group_.size(), dGroupSlots_.as<int>(), dGroupMass_.as<float>(),
+                    dGroupRef_.as<float>(), w0, nw, s);
+    if (!coord_.empty()) {
+        CoordBuffers cb;
+        cb.slots = dCoordSlots_.as<int>();
+        cb.cellOf = dCellOf_.as<int>();
+        cb.cellCount = dCellCount_.as<int>();
+        cb.cellStart = dCellStart_.as<int>();
+        cb.sorted = dSorted_.as<float4>();
+        cb.partCN = dPartCN_.as<double>();
+        cb.partPairs = dPartPairs_.as<unsigned long long>();
+        cb.counters = dCounters_.as<unsigned int>();
+        cb.npairs = dNpairs_.as<long long>();
+        cb.cellOfStride = cellOfStride_;
+        cb.cellStride = cellStride_;
+        cb.sortStride = sortStride_;
+        cb.partStride = partStride_;
+        cb.nItems = (int)coord_.size();
+        for (size_t i = 0; i < coord_.size(); i++)
+            launch_coordination(v, cb, dCoord_.as<CoordItem>(), coord_[i], (int)i, w0, nw, s);
+    }
+    launch_cv_path(v, dPath_.as<PathItem>(), (int)path_.size(), dPathSlots_.as<int>(), dPathFrames_.as<float>(), w0,
+                   nw, s);
+    }
+    if ((stageMask & 10) && testGrads_.empty()) {
+        SmallBiasState st;
+        st.grids = dGridTable_.as<GridBiasDev>();
+        st.stateConst = dSmallConst_.as<double>();
+        st.stateVar = dSmallVar_.as<double>();
+        st.scal = dSmallScal_.as<double>();
+        st.stateVarStride = smallVarPerWalker_;
+        st.scalStride = 4 * (long long)std::max((int)smallBias_.size(), 1);
+        for (auto* o : opes_)
+            if (o->d.adaptiveStride > 0 && (stageMask & 2)) {
+                launch_opes_welford(v, o->d, w0, nw, s);
+                for (int t = 0; t < (int)o->nSamplesMirror.size(); t++)
+                    if (o->d.shared || (t >= w0 && t < w0 + nw)) o->nSamplesMirror[t] += o->d.shared ? nw : 1;
+            }
+        for (auto* o : opes_) launch_bias_opes(v, o->d, w0, nw, s);
+        for (auto* h : hills_) launch_bias_hills(v, h->d, w0, nw, s);
+        if (stageMask & 2)
+            launch_bias_small(v, st, dSmallBias_.as<SmallBiasItem>(), (int)smallBias_.size(), w0, nw, s);
+    }
+    if (!(stageMask & 4)) {
+        hip_check(hipGetLastError(), "kernel launch");
+        return;
+    }
+    ScatterPlan sp;
+    sp.segStart = dSegStart_.as<int>();
+    sp.segRow = dSegRow_.as<int>();
+    sp.rowCv = dRowCv_.as<int>();
+    sp.combStart = dCombStart_.as<int>();
+    sp.combSlot = dCombSlot_.as<int>();
+    sp.testGrad = testGrads_.empty() ? nullptr : dTestGrad_.as<double>();
+    sp.energyF = nullptr;
+    sp.energyValue = energyValue_;
+    launch_scatter(v, sp, w0, nw, forces ? 1 : 0, s);
+    if (forces && energyValue_ >= 0) {
+        for (int w = w0; w < w0 + nw; w++) {
+            if (!energyF_[w]) continue;
+            ScatterPlan spw = sp;
+            // energyF_ holds one device pointer per walker; present it through the
+            // walker's force offset by shifting the base.
+            spw.energyF = energyF_[w] - forceOff_[w];
+            launch_scatter_dense(v, spw, padded_[w], w, 1, s);
+        }
+    }
+    hip_check(hipGetLastError(), "kernel launch");
+}
+
+void Engine::sync(hipStream_t s) { hip_check(hipStreamSynchronize(s), "hipStreamSynchronize"); }
+
+// Plain form: one walker, one launch per CV kind and bias, then the scatter.
+double Engine::execute(hipStream_t s, int w, const float4* posq, long long* force, bool forces, bool energy)
+{
+    if (!initialized_) fail("execute: initialize first");
+    if (w < 0 || w >= W_) fail("execute: walker out of range");
+    if (V_ == 0) return 0.0;
+    // posq / force point at this walker's block; present them as the batch base
+    lastPos_ = posq - posOff_[w];
+    lastForce_ = force - forceOff_[w];
+    launchStep(s, w, 1, forces);
+    cvReady_ = true;
+    double e = 0.0;
+    if (energy && testGrads_.empty() && !biases_.empty()) {
+        std::vector<double> be((size_t)B_);
+        hip_check(hipMemcpyAsync(be.data(), dBener_.as<double>() + (size_t)w * B_, sizeof(double) * B_,
+                                 hipMemcpyDeviceToHost, s),
+                  "energy read-back");
+        sync(s);
+        for (size_t b = 0; b < biases_.size(); b++) e += be[b];
+    }
+    return e;
+}
+
+void Engine::executeStage(hipStream_t s, const float4* posq, long long* force, int stage)
+{
+    if (!initialized_) fail("execute_stage: initialize first");
+    if (stage < 0 || stage > 3) fail("execute_stage: stage must be 0, 1, 2 or 3");
+    lastPos_ = posq;
+    lastForce_ = force;
+    launchStep(s, 0, W_, true, 1 << stage);
+}
+
+// Batched form: every walker in one launch per stage, optionally as one
+// captured hipGraph (re-captured when the buffers change).
+void Engine::executeBatch(hipStream_t s, const float4* posq, long long* force, bool forces, bool useGraph)
+{
+    if (!initialized_) fail("execute_batch: initialize first");
+    if (V_ == 0) return;
+    lastPos_ = posq;
+    lastForce_ = force;
+    if (!useGraph) {
+        launchStep(s, 0, W_, forces);
+        cvReady_ = true;
+        return;
+    }
+    bool needsDense = energyValue_ >= 0;
+    if (needsDense || graphPos_ != posq || graphForce_ != force || graphForces_ != forces || !graphExec_) {
+        if (graphExec_) {
+            (void)hipGraphExecDestroy(graphExec_);
+            (void)hipGraphDestroy(graph_);
+            graphExec_ = nullptr;
+            graph_ = nullptr;
+        }
+        // Capture on a private stream (the caller's may be the legacy default
+        // stream, which cannot be captured); the graph is launched on the
+        // caller's stream, so ordering with the caller's work is preserved.
+        if (!captureStream_) hip_check(hipStreamCreateWithFlags(&captureStream_, hipStreamNonBlocking), "stream");
+        hip_check(hipStreamBeginCapture(captureStream_, hipStreamCaptureModeThreadLocal), "begin capture");
+        try {
+            launchStep(captureStream_, 0, W_, forces);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(captureStream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(captureStream_, &graph_), "end capture");
+        hip_check(hipGraphInstantiate(&graphExec_, graph_, nullptr, nullptr, 0), "graph instantiate");
+        graphPos_ = posq;
+        graphForce_ = force;
+        graphForces_ = forces;
+    }
+    hip_check(hipGraphLaunch(graphExec_, s), "graph launch");
+    cvReady_ = true;
+}
+
+// ---------------------------------------------------------------------------
+// Deposition, CommonGluedKernels.cpp:5050-5157 + GluedForceImpl.cpp:100-109:
+// skipped until one evaluation filled the CVs, once per step, never at step 0,
+// pace-gated; OPES, then OPES-expanded, multithermal, then MetaD.
+// ---------------------------------------------------------------------------
+void Engine::updateState(hipStream_t s, int step)
+{
+    if (!initialized_) fail("update_state: initialize first");
+    if (!cvReady_) return;
+    if (step == lastUpdateStep_) return;
+    if (opes_.empty() && grids_.empty() && hills_.empty() && smallBias_.empty()) return;
+    lastUpdateStep_ = step;
+    StepView v = view(lastPos_, lastForce_);
+    v.S = S_;
+    for (auto* o : opes_) {
+        if (step == 0 || step % o->pace != 0) continue;
+        const int T = (int)o->depositMirror.size();
+        bool any = false;
+        for (int t = 0; t < T; t++) {
+            if (o->depositMirror[t] >= o->d.cap) continue;
+            any = true;
+        }
+        if (!any) continue;
+        launch_dep_opes(v, o->d, 0, W_, s);
+        for (int t = 0; t < T; t++) {
+            bool will = o->d.adaptiveStride == 0 || o->nSamplesMirror[t] >= o->d.adaptiveStride;
+            if (will) o->depositMirror[t] += o->d.shared ? W_ : 1;
+        }
+    }
+    SmallBiasState st;
+    st.grids = dGridTable_.as<GridBiasDev>();
+    st.stateConst = dSmallConst_.as<double>();
+    st.stateVar = dSmallVar_.as<double>();
+    st.scal = dSmallScal_.as<double>();
+    st.stateVarStride = smallVarPerWalker_;
+    st.scalStride = 4 * (long long)std::max((int)smallBias_.size(), 1);
+    for (size_t i = 0; i < smallBias_.size(); i++) {
+        const auto& hs = smallBiasHost_[i];
+        if (hs.type != BIAS_OPES_EXPANDED) continue;
+        if (step == 0 || step % hs.pace != 0) continue;
+        launch_dep_small(v, st, dSmallBias_.as<SmallBiasItem>(), (int)smallBias_.size(), (int)i, 0, W_, s);
+    }
+    for (size_t i = 0; i < smallBias_.size(); i++) {
+        const auto& hs = smallBiasHost_[i];
+        if (hs.type != BIAS_OPES_MULTITHERMAL) continue;
+        if (step == 0 || hs.pace <= 0 || step % hs.pace != 0) continue;
+        launch_dep_small(v, st, dSmallBias_.as<SmallBiasItem>(), (int)smallBias_.size(), (int)i, 0, W_, s);
+    }
+    for (auto* g : grids_) {
+        if (step == 0 || step % g->pace != 0) continue;
+        launch_dep_grid(v, g->d, 0, W_, s);
+        for (auto& n : g->numDeposited) n += g->d.walkerStride ? 1 : W_;
+    }
+    for (auto* h : hills_) {
+        if (step == 0 || step % h->pace != 0) continue;
+        launch_dep_hills(v, h->d, 0, W_, s);
+    }
+    hip_check(hipGetLastError(), "deposit launch");
+}
+
+// ---------------------------------------------------------------------------
+// Read-backs
+// ---------------------------------------------------------------------------
+template <class T>
+static std::vector<T> download(const T* p, size_t n)
+{
+    std::vector<T> h(n);
+    if (n) hip_check(hipMemcpy(h.data(), p, sizeof(T) * n, hipMemcpyDeviceToHost), "download");
+    return h;
+}
+
+std::vector<double> Engine::cvValues(int w)
+{
+    if (!initialized_ || w < 0 || w >= W_) fail("get_cv_values: bad walker or not initialized");
+    hip_check(hipDeviceSynchronize(), "sync");
+    return download(dCv_.as<double>() + (size_t)w * std::max(V_, 1), V_);
+}
+
+std::vector<double> Engine::biasEnergies(int w)
+{
+    if (!initialized_ || w < 0 || w >= W_) fail("get_bias_energies: bad walker or not initialized");
+    hip_check(hipDeviceSynchronize(), "sync");
+    return download(dBener_.as<double>() + (size_t)w * B_, biases_.size());
+}
+
+std::vector<double> Engine::dUdCV(int w)
+{
+    if (!initialized_ || w < 0 || w >= W_) fail("get_dudcv: bad walker or not initialized");
+    hip_check(hipDeviceSynchronize(), "sync");
+    return download(dUd_.as<double>() + (size_t)w * std::max(V_, 1), V_);
+}
+
+long long Engine::coordinationPairs(int w, int ci)
+{
+    if (!initialized_ || w < 0 || w >= W_) fail("coordination_pairs: bad walker or not initialized");
+    if (ci < 0 || ci >= (int)coord_.size()) fail("coordination_pairs: index out of range");
+    hip_check(hipDeviceSynchronize(), "sync");
+    return download(dNpairs_.as<long long>() + (size_t)w * coord_.size() + ci, 1)[0];
+}
+
+// getOPESMetrics, CommonGluedKernels.cpp:5164-5210: [zed, rct, nker, neff].
+std::vector<double> Engine::opesMetrics(int w, int oi)
+{
+    if (oi < 0 || oi >= (int)opes_.size()) fail("getOPESMetrics: biasIndex out of range");
+    if (w < 0 || w >= W_) fail("getOPESMetrics: walker out of range");
+    OpesHost* o = opes_[oi];
+    const int t = o->d.shared ? 0 : w, D = o->d.D;
+    hip_check(hipDeviceSynchronize(), "sync");
+    auto I = download(o->d.ints + 4 * t, 4);
+    auto Q = download(o->d.dbl + (size_t)t * (4 + 2 * D), 4);
+    const int nker = I[0];
+    const double counter = I[2];
+    const double KDEnorm = o->d.variant == 2 ? counter : Q[1];
+    const double zed = (nker > 0 && KDEnorm > 0.0) ? Q[0] / (KDEnorm * nker) : 1.0;
+    const double rct = (counter > 0.0 && Q[1] > 0.0) ? o->d.kT * std::log(Q[1] / counter) : 0.0;
+    const double neff = (1.0 + Q[2]) * (1.0 + Q[2]) / (1.0 + Q[3]);
+    return {zed, rct, (double)nker, neff};
+}
+
+// getKernelSigmas, CommonGluedKernels.cpp:5212-5232 (kernel-major, CV inner).
+std::vector<float> Engine::kernelSigmas(int w, int oi)
+{
+    if (oi < 0 || oi >= (int)opes_.size()) return {};
+    OpesHost* o = opes_[oi];
+    const int t = o->d.shared ? 0 : w, D = o->d.D;
+    hip_check(hipDeviceSynchronize(), "sync");
+    const int nk = download(o->d.ints + 4 * t, 1)[0];
+    if (nk <= 0) return {};
+    auto sg = download(o->d.sigmas + (size_t)t * D * o->d.cap, (size_t)D * o->d.cap);
+    std::vector<float> out((size_t)nk * D);
+    for (int k = 0; k < nk; k++)
+        for (int d = 0; d < D; d++) out[(size_t)k * D + d] = (float)sg[(size_t)d * o->d.cap + k];
+    return out;
+}
+
+void Engine::setHills(int w, int bi, const double* centers, const double* heights, int n)
+{
+    if (!initialized_) fail("set_hills: initialize first");
+    if (bi < 0 || bi >= (int)biases_.size() || biases_[bi].type != BIAS_METAD_HILLS) fail("set_hills: not a hill bias");
+    HillHost* h = nullptr;
+    for (auto* x : hills_)
+        if (x->spec == bi) h = x;
+    if (w < 0 || w >= W_) fail("set_hills: walker out of range");
+    if (n < 0 || n > h->d.cap) fail("set_hills: hill count exceeds the capacity");
+    const int t = h->d.shared ? 0 : w, D = h->d.D;
+    std::vector<double> c((size_t)D * h->d.cap, 0.0);
+    for (int k = 0; k < n; k++)
+        for (int d = 0; d < D; d++) c[(size_t)d * h->d.cap + k] = centers[(size_t)k * D + d];
+    hip_check(hipMemcpy(h->d.centers + (size_t)t * D * h->d.cap, c.data(), sizeof(double) * c.size(),
+                        hipMemcpyHostToDevice),
+              "set_hills");
+    if (n) hip_check(hipMemcpy(h->d.heights + (size_t)t * h->d.cap, heights, sizeof(double) * n, hipMemcpyHostToDevice),
+                     "set_hills");
+    hip_check(hipMemcpy(h->d.count + t, &n, sizeof(int), hipMemcpyHostToDevice), "set_hills");
+}
+
+int Engine::getHills(int w, int bi, double* centers, double* heights, int cap)
+{
+    HillHost* h = nullptr;
+    for (auto* x : hills_)
+        if (x->spec == bi) h = x;
+    if (!h) fail("get_hills: not a hill bias");
+    if (w < 0 || w >= W_) fail("get_hills: walker out of range");
+    const int t = h->d.shared ? 0 : w, D = h->d.D;
+    hip_check(hipDeviceSynchronize(), "sync");
+    const int n = download(h->d.count + t, 1)[0];
+    if (centers && heights) {
+        if (cap < n) fail("get_hills: buffer too small");
+        auto c = download(h->d.centers + (size_t)t * D * h->d.cap, (size_t)D * h->d.cap);
+        auto hh = download(h->d.heights + (size_t)t * h->d.cap, n);
+        for (int k = 0; k < n; k++) {
+            for (int d = 0; d < D; d++) centers[(size_t)k * D + d] = c[(size_t)d * h->d.cap + k];
+            heights[k] = hh[k];
+        }
+    }
+    return n;
+}
+
+void Engine::setGrid(int w, int bi, const double* grid, int n)
+{
+    GridHost* g = nullptr;
+    for (auto* x : grids_)
+        if (x->spec == bi) g = x;
+    if (!g) fail("set_grid: not a grid MetaD bias");
+    if (w < 0 || w >= W_) fail("set_grid: walker out of range");
+    if (n != g->d.total) fail("set_grid: size mismatch");
+    const int t = g->d.walkerStride ? w : 0;
+    hip_check(hipMemcpy(g->d.grid + (size_t)t * g->d.total, grid, sizeof(double) * n, hipMemcpyHostToDevice),
+              "set_grid");
+}
+
+// ---------------------------------------------------------------------------
+// Bias-state blob, 'GPUS' v1 (CommonSerialization.cpp:20-178 / 180-410):
+// OPES, ABMD(0), MetaD grids, PBMetaD(0), external(0), linear(0), walls(0),
+// OPES-expanded, ext-Lagrangian(0), EDS(0), MaxEnt(0), multithermal; then a
+// trailing hill-list section (extension; a reference reader stops before it).
+// ---------------------------------------------------------------------------
+std::vector<char> Engine::getState(int w)
+{
+    if (!initialized_ || w < 0 || w >= W_) fail("getBiasState: bad walker or not initialized");
+    hip_check(hipDeviceSynchronize(), "sync");
+    std::vector<char> buf;
+    auto put = [&](const void* p, size_t n) { buf.insert(buf.end(), (const char*)p, (const char*)p + n); };
+    auto i32 = [&](int32_t x) { put(&x, 4); };
+    auto f64 = [&](double x) { put(&x, 8); };
+    put("GPUS", 4);
+    i32(1);
+    i32((int32_t)opes_.size());
+    for (auto* o : opes_) {
+        const int t = o->d.shared ? 0 : w, D = o->d.D;
+        auto I = download(o->d.ints + 4 * t, 4);
+        auto Q = download(o->d.dbl + (size_t)t * (4 + 2 * D), 4 + 2 * D);
+        const int nk = I[0];
+        auto c = download(o->d.centers + (size_t)t * D * o->d.cap, (size_t)D * o->d.cap);
+        auto sg = download(o->d.sigmas + (size_t)t * D * o->d.cap, (size_t)D * o->d.cap);
+        auto lw = download(o->d.logw + (size_t)t * o->d.cap, nk);
+        i32(nk);
+        f64(Q[0]);
+        i32(I[3]);
+        put(&Q[4], 8 * D);
+        put(&Q[4 + D], 8 * D);
+        for (int k = 0; k < nk; k++)
+            for (int d = 0; d < D; d++) f64(c[(size_t)d * o->d.cap + k]);
+        for (int k = 0; k < nk; k++)
+            for (int d = 0; d < D; d++) f64(sg[(size_t)d * o->d.cap + k]);
+        put(lw.data(), 8 * (size_t)nk);
+    }
+    i32(0);  // ABMD
+    i32((int32_t)grids_.size());
+    for (auto* g : grids_) {
+        const int t = g->d.walkerStride ? w : 0;
+        i32(g->numDeposited[t]);
+        auto gr = download(g->d.grid + (size_t)t * g->d.total, g->d.total);
+        put(gr.data(), 8 * gr.size());
+    }
+    i32(0);  // PBMetaD
+    i32(0);  // external
+    i32(0);  // linear
+    i32(0);  // walls
+    int noe = 0, nmt = 0;
+    for (auto& it : smallBias_) {
+        noe += it.type == BIAS_OPES_EXPANDED;
+        nmt += it.type == BIAS_OPES_MULTITHERMAL;
+    }
+    const size_t nsb = std::max((int)smallBias_.size(), 1);
+    auto scal = download(dSmallScal_.as<double>() + (size_t)w * nsb * 4, nsb * 4);
+    i32(noe);
+    for (size_t i = 0; i < smallBias_.size(); i++)
+        if (smallBias_[i].type == BIAS_OPES_EXPANDED) {
+            f64(scal[4 * i]);
+            i32((int32_t)scal[4 * i + 1]);
+        }
+    i32(0);  // ext-Lagrangian
+    i32(0);  // EDS
+    i32(0);  // MaxEnt
+    i32(nmt);
+    auto var = download(dSmallVar_.as<double>() + (size_t)w * smallVarPerWalker_, smallVarPerWalker_);
+    for (size_t i = 0; i < smallBias_.size(); i++)
+        if (smallBias_[i].type == BIAS_OPES_MULTITHERMAL) {
+            const auto& it = smallBias_[i];
+            i32(it.nState);
+            put(&var[it.stateOff], 8 * (size_t)it.nState);
+            f64(scal[4 * i + 2]);
+            f64(scal[4 * i + 3]);
+        }
+    // extension: hill lists
+    put("HILL", 4);
+    i32((int32_t)hills_.size());
+    for (auto* h : hills_) {
+        const int t = h->d.shared ? 0 : w, D = h->d.D;
+        const int n = download(h->d.count + t, 1)[0];
+        auto c = download(h->d.centers + (size_t)t * D * h->d.cap, (size_t)D * h->d.cap);
+        auto hh = download(h->d.heights + (size_t)t * h->d.cap, n);
+        i32(n);
+        i32(D);
+        for (int k = 0; k < n; k++)
+            for (int d = 0; d < D; d++) f64(c[(size_t)d * h->d.cap + k]);
+        put(hh.data(), 8 * (size_t)n);
+    }
+    return buf;
+}
+
+void Engine::setState(int w, const std::vector<char>& blob)
+{
+    if (!initialized_ || w < 0 || w >= W_) fail("setBiasState: bad walker or not initialized");
+    if (blob.empty()) return;
+    const char* p = blob.data();
+    const char* const end = p + blob.size();
+    auto get = [&](void* dst, size_t n) {
+        if ((size_t)(end - p) < n) fail("GLUED bias-state: truncated/corrupt blob");
+        std::memcpy(dst, p, n);
+        p += n;
+    };
+    auto i32 = [&]() {
+        int32_t x;
+        get(&x, 4);
+        return x;
+    };
+    auto f64 = [&]() {
+        double x;
+        get(&x, 8);
+        return x;
+    };
+    if ((size_t)(end - p) >= 8 && std::memcmp(p, "GPUS", 4) == 0) {
+        p += 4;
+        if (i32() != 1) fail("GLUED bias-state: unknown blob version");
+    }
+    auto count = [&](const char* what, int expect) {
+        int32_t n = i32();
+        if (n != expect)
+            fail(std::string("GLUED bias-state: ") + what + " bias count mismatch (expected " + std::to_string(expect) +
+                 ", got " + std::to_string(n) + ")");
+    };
+    count("OPES", (int)opes_.size());
+    for (auto* o : opes_) {
+        const int t = o->d.shared ? 0 : w, D = o->d.D;
+        const int nk = i32();
+        const double sumUprob = f64();
+        const int ns = i32();
+        if (nk < 0 || nk > o->d.cap) fail("GLUED bias-state: OPES kernel count out of range");
+        const size_t need = (size_t)D * 16 + (size_t)nk * D * 16 + (size_t)nk * 8;
+        if ((size_t)(end - p) < need) fail("GLUED bias-state: truncated/corrupt blob");
+        std::vector<double> mean(D), M2(D), c((size_t)nk * D), sg((size_t)nk * D), lw(nk);
+        get(mean.data(), 8 * D);
+        get(M2.data(), 8 * D);
+        get(c.data(), 8 * c.size());
+        get(sg.data(), 8 * sg.size());
+        get(lw.data(), 8 * lw.size());
+        std::vector<double> cs((size_t)D * o->d.cap, 0.0), ss((size_t)D * o->d.cap, 1.0);
+        for (int k = 0; k < nk; k++)
+            for (int d = 0; d < D; d++) {
+                cs[(size_t)d * o->d.cap + k] = c[(size_t)k * D + d];
+                ss[(size_t)d * o->d.cap + k] = sg[(size_t)k * D + d];
+            }
+        hip_check(hipMemcpy(o->d.centers + (size_t)t * D * o->d.cap, cs.data(), 8 * cs.size(), hipMemcpyHostToDevice),
+                  "restore");
+        hip_check(hipMemcpy(o->d.sigmas + (size_t)t * D * o->d.cap, ss.data(), 8 * ss.size(), hipMemcpyHostToDevice),
+                  "restore");
+        if (nk)
+            hip_check(hipMemcpy(o->d.logw + (size_t)t * o->d.cap, lw.data(), 8 * (size_t)nk, hipMemcpyHostToDevice),
+                      "restore");
+        auto I = download(o->d.ints + 4 * t, 4);
+        I[0] = nk;
+        I[1] = nk;  // slot-claim counter follows the restored count
+        I[3] = ns;
+        hip_check(hipMemcpy(o->d.ints + 4 * t, I.data(), 16, hipMemcpyHostToDevice), "restore");
+        auto Q = download(o->d.dbl + (size_t)t * (4 + 2 * D), 4 + 2 * D);
+        Q[0] = sumUprob;
+        double sw = std::exp(-o->d.gamma);  // KDNorm recomputed, CommonSerialization.cpp:258-264
+        for (int k = 0; k < nk; k++) sw += std::exp(lw[k]);
+        Q[1] = sw;
+        for (int d = 0; d < D; d++) {
+            Q[4 + d] = mean[d];
+            Q[4 + D + d] = M2[d];
+        }
+        hip_check(hipMemcpy(o->d.dbl + (size_t)t * (4 + 2 * D), Q.data(), 8 * Q.size(), hipMemcpyHostToDevice),
+                  "restore");
+        o->depositMirror[t] = nk;
+        o->nSamplesMirror[t] = ns;
+    }
+    if (p >= end) return;
+    count("ABMD", 0);
+    if (p >= end) return;
+    count("MetaD", (int)grids_.size());
+    for (auto* g : grids_) {
+        const int t = g->d.walkerStride ? w : 0;
+        const int nd = i32();
+        std::vector<double> gr(g->d.total);
+        get(gr.data(), 8 * gr.size());
+        g->numDeposited[t] = nd;
+        hip_check(hipMemcpy(g->d.grid + (size_t)t * g->d.total, gr.data(), 8 * gr.size(), hipMemcpyHostToDevice),
+                  "restore");
+    }
+    if (p >= end) return;
+    count("PBMetaD", 0);
+    if (p >= end) return;
+    count("external", 0);
+    if (p >= end) return;
+    count("linear", 0);
+    if (p >= end) return;
+    count("wall", 0);
+    if (p >= end) return;
+    int noe = 0, nmt = 0;
+    for (auto& it : smallBias_) {
+        noe += it.type == BIAS_OPES_EXPANDED;
+        nmt += it.type == BIAS_OPES_MULTITHERMAL;
+    }
+    const size_t nsb = std::max((int)smallBias_.size(), 1);
+    auto scal = download(dSmallScal_.as<double>() + (size_t)w * nsb * 4, nsb * 4);
+    auto var = download(dSmallVar_.as<double>() + (size_t)w * smallVarPerWalker_, smallVarPerWalker_);
+    count("OPES_EXPANDED", noe);
+    for (size_t i = 0; i < smallBias_.size(); i++)
+        if (smallBias_[i].type == BIAS_OPES_EXPANDED) {
+            scal[4 * i] = f64();
+            scal[4 * i + 1] = i32();
+        }
+    if (p < end) {
+        count("ext-Lagrangian", 0);
+        count("EDS", 0);
+        count("MaxEnt", 0);
+        count("multithermal", nmt);
+        for (size_t i = 0; i < smallBias_.size(); i++)
+            if (smallBias_[i].type == BIAS_OPES_MULTITHERMAL) {
+                const auto& it = smallBias_[i];
+                if (i32() != it.nState) fail("GLUED bias-state: multithermal state count mismatch");
+                get(&var[it.stateOff], 8 * (size_t)it.nState);
+                scal[4 * i + 2] = f64();
+                scal[4 * i + 3] = f64();
+            }
+    }
+    hip_check(hipMemcpy(dSmallScal_.as<double>() + (size_t)w * nsb * 4, scal.data(), 8 * scal.size(),
+                        hipMemcpyHostToDevice),
+              "restore");
+    hip_check(hipMemcpy(dSmallVar_.as<double>() + (size_t)w * smallVarPerWalker_, var.data(), 8 * var.size(),
+                        hipMemcpyHostToDevice),
+              "restore");
+    if ((size_t)(end - p) >= 8 && std::memcmp(p, "HILL", 4) == 0) {
+        p += 4;
+        count("hill METAD", (int)hills_.size());
+        for (auto* h : hills_) {
+            const int n = i32(), D = i32();
+            if (D != h->d.D) fail("GLUED bias-state: hill dimension mismatch");
+            if (n < 0 || n > h->d.cap) fail("GLUED bias-state: hill count out of range");
+            std::vector<double> c((size_t)n * D), hh(n);
+            get(c.data(), 8 * c.size());
+            get(hh.data(), 8 * hh.size());
+            setHills(w, h->spec, c.data(), hh.data(), n);
+        }
+    }
+}
+
+}  // namespace glued

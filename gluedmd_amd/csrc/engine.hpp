@@ -1,0 +1,209 @@
+// Host-side engine of the GLUED enhanced-sampling step: the force
+// specification store (openmmapi/src/GluedForce.cpp), the plan builder and
+// validation (CommonGluedKernels.cpp:3805-4577), the step (execute,
+// :4646-5048; updateState, :5050-5157) and the bias-state blob
+// (CommonSerialization.cpp).  Walker-major: one engine holds W walkers of
+// the same force, each with its own positions, force block, CV values and
+// private bias state; a bias may instead be shared by all walkers (the
+// reference's multi-walker pointer sharing, CommonGluedKernels.cpp:5234-5262).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include "glued_internal.h"
+
+namespace glued {
+
+struct Error : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void hip_check(hipError_t e, const char* what);
+
+// Device buffer with RAII ownership.
+struct DBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void release()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    void alloc(size_t n)
+    {
+        release();
+        if (n == 0) n = 8;
+        hip_check(hipMalloc(&p, n), "hipMalloc");
+        bytes = n;
+        hip_check(hipMemset(p, 0, n), "hipMemset");
+    }
+    template <class T>
+    T* as() const
+    {
+        return static_cast<T*>(p);
+    }
+    template <class T>
+    void upload(const std::vector<T>& v)
+    {
+        alloc(v.size() * sizeof(T));
+        if (!v.empty()) hip_check(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "upload");
+    }
+};
+
+struct CVSpec {
+    int type;
+    std::vector<int> atoms;
+    std::vector<double> params;
+    int firstValue;
+};
+
+struct BiasSpec {
+    int type;
+    std::vector<int> cvs;
+    std::vector<double> params;
+    std::vector<int> iparams;
+    int shared = 0;
+};
+
+struct ScatterPlanHost;
+
+class Engine {
+   public:
+    Engine();
+    ~Engine();
+
+    // ---- specification (GluedForce.cpp:44-131) ----
+    int addCV(int type, const int* atoms, int na, const double* params, int np);
+    int addBias(int type, const int* cvs, int nc, const double* params, int np, const int* ip, int ni);
+    void setShared(int bias, int shared);
+    void setTestBiasGradients(const double* g, int n);
+    int numValues() const { return numValues_; }
+    bool pbc = false;
+    double temperature = -1.0;
+
+    // ---- plan ----
+    void initialize(int W, const int* natoms, const int* padded, int nMass, const double* masses);
+    void setAtomOrder(int w, const int* gpuToUser, int n);
+    void setBox(int w, const double* box9);
+    void setEnergyInput(int w, double U, const float* Fext, hipStream_t s);
+
+    // ---- step ----
+    double execute(hipStream_t s, int w, const float4* posq, long long* force, bool forces, bool energy);
+    void executeBatch(hipStream_t s, const float4* posq, long long* force, bool forces, bool useGraph);
+    void updateState(hipStream_t s, int step);
+    void executeStage(hipStream_t s, const float4* posq, long long* force, int stage);
+
+    // ---- read-backs ----
+    std::vector<double> cvValues(int w);
+    std::vector<double> biasEnergies(int w);
+    std::vector<double> dUdCV(int w);
+    long long coordinationPairs(int w, int coordIdx);
+    std::vector<char> getState(int w);
+    void setState(int w, const std::vector<char>& blob);
+    std::vector<double> opesMetrics(int w, int opesIdx);
+    std::vector<float> kernelSigmas(int w, int opesIdx);
+    void setHills(int w, int bias, const double* centers, const double* heights, int n);
+    int getHills(int w, int bias, double* centers, double* heights, int cap);
+    void setGrid(int w, int bias, const double* grid, int n);
+    int numWalkers() const { return W_; }
+    int jacobianRows() const { return R_; }
+    int atomSlots() const { return S_; }
+    double lastStepMs = 0.0;
+
+   private:
+    void validate(int minAtoms) const;
+    void buildPlan(int nMass, const double* masses);
+    void launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMask = 7);
+    StepView view(const float4* posq, long long* force) const;
+    void uploadGidx(int w);
+    void sync(hipStream_t s);
+
+    std::vector<CVSpec> cvs_;
+    std::vector<BiasSpec> biases_;
+    std::vector<double> testGrads_;
+    int numValues_ = 0;
+    bool initialized_ = false;
+
+    // plan sizes
+    int W_ = 0, S_ = 0, V_ = 0, R_ = 0, G_ = 0, B_ = 0;
+    std::vector<int> natoms_, padded_;
+    std::vector<long long> posOff_, forceOff_;
+    std::vector<int> slotAtom_;          // user atom of every slot
+    std::vector<std::vector<int>> userToGpu_;
+    std::vector<double> boxHost_;        // [W][12]
+    std::vector<int> rowSlot_, rowCv_;
+
+    // items
+    std::vector<SmallCVItem> small_;
+    std::vector<GroupCVItem> group_;
+    std::vector<CoordItem> coord_;
+    std::vector<PathItem> path_;
+    int energyValue_ = -1;
+    std::vector<double> energyU_;
+    std::vector<const float*> energyF_;
+
+    // bias plan
+    struct GridHost {
+        GridBiasDev d;
+        int pace, spec;
+        std::vector<int> numDeposited;  // per table
+        DBuf grid;
+    };
+    struct HillHost {
+        HillBiasDev d;
+        int pace, spec;
+        DBuf centers, heights, count, part, counter;
+    };
+    struct OpesHost {
+        OpesBiasDev d;
+        int pace, spec;
+        std::vector<int> depositMirror;  // per table, the reference's CPU numKernels mirror
+        std::vector<int> nSamplesMirror;
+        DBuf centers, sigmas, logw, ints, dbl, part, counter;
+    };
+    struct SmallHost {
+        int spec, pace, type;
+    };
+    std::vector<GridHost*> grids_;
+    std::vector<HillHost*> hills_;
+    std::vector<OpesHost*> opes_;
+    std::vector<SmallBiasItem> smallBias_;
+    std::vector<SmallHost> smallBiasHost_;
+    std::vector<double> smallConst_;
+    int smallVarPerWalker_ = 0;
+    std::vector<int> combStart_, combSlot_;
+
+    // device buffers
+    DBuf dWalkers_, dGidx_, dBox_, dCv_, dJx_, dJy_, dJz_, dBgrad_, dBener_, dUd_;
+    DBuf dSmall_, dGroup_, dGroupSlots_, dGroupMass_, dGroupRef_;
+    DBuf dCoord_, dCoordSlots_, dCellOf_, dCellCount_, dCellStart_, dSorted_, dPartCN_, dPartPairs_, dCounters_,
+        dNpairs_;
+    int cellOfStride_ = 0, cellStride_ = 0, sortStride_ = 0, partStride_ = 0;
+    DBuf dPath_, dPathSlots_, dPathFrames_, dPathW_;
+    DBuf dSmallBias_, dGridTable_, dSmallConst_, dSmallVar_, dSmallScal_;
+    DBuf dSegStart_, dSegRow_, dRowCv_, dCombStart_, dCombSlot_, dTestGrad_, dEnergyU_;
+    std::vector<double> pathCentroidDummy_;
+
+    // graph cache for the batched step
+    hipGraphExec_t graphExec_ = nullptr;
+    hipGraph_t graph_ = nullptr;
+    const void* graphPos_ = nullptr;
+    const void* graphForce_ = nullptr;
+    hipStream_t captureStream_ = nullptr;
+    bool graphForces_ = true;
+
+    bool cvReady_ = false;
+    int lastUpdateStep_ = -1;
+    int maxPadded_ = 0;
+    const float4* lastPos_ = nullptr;
+    long long* lastForce_ = nullptr;
+};
+
+}  // namespace glued

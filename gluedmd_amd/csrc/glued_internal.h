@@ -1,0 +1,223 @@
+// Internal types shared by the host engine and the HIP kernels of the GLUED
+// enhanced-sampling step (CV evaluation -> bias evaluation -> chain-rule
+// force scatter).  Nothing here crosses the C-ABI.
+//
+// HBM layout (all walker-major, one walker's block contiguous):
+//   posq      float4[sum padded_w]       FP32 xyz + charge, OpenMM order
+//   forces    int64 [3 * sum padded_w]   field-major per walker, 2^32 fixed point
+//   gidx      int   [W][S]               GPU index of every referenced atom slot
+//   cvVal     double[W][V]               CV values
+//   jac{x,y,z} float[W][R]               Jacobian rows dCV/dx (row -> slot, cv)
+//   bgrad     double[W][G]               dV_b/ds_d of every bias b, CV d
+//   benergy   double[W][B]               bias energies
+//   dUdCV     double[W][V]               chain-rule factors
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace glued {
+
+// CV type codes: openmmapi/include/GluedForce.h:27-73.
+enum CVType {
+    CV_DISTANCE = 1,
+    CV_ANGLE = 2,
+    CV_DIHEDRAL = 3,
+    CV_COM_DISTANCE = 4,
+    CV_GYRATION = 5,
+    CV_COORDINATION = 6,
+    CV_RMSD = 7,
+    CV_PATH = 10,
+    CV_POSITION = 11,
+    CV_DRMSD = 12,
+    CV_CONTACTMAP = 13,
+    CV_ENERGY = 23,
+};
+
+// Bias type codes: openmmapi/include/GluedForce.h:76-95, plus the hill-list
+// metadynamics extension (no counterpart in the reference).
+enum BiasType {
+    BIAS_HARMONIC = 1,
+    BIAS_METAD = 3,
+    BIAS_OPES = 5,
+    BIAS_UPPER_WALL = 8,
+    BIAS_LOWER_WALL = 9,
+    BIAS_LINEAR = 10,
+    BIAS_OPES_EXPANDED = 11,
+    BIAS_OPES_MULTITHERMAL = 15,
+    BIAS_METAD_HILLS = 101,
+};
+
+constexpr double FIXED_SCALE = 4294967296.0;  // 2^32, OpenMM's force fixed point
+constexpr int MAX_BIAS_D = 8;                  // CVs per bias (reference: MAX_OPES_CVS 16)
+
+struct WalkerInfo {
+    long long posOff;    // float4 offset of this walker's posq
+    long long forceOff;  // int64 offset of this walker's force block
+    int padded;          // field stride of the force block
+    int natoms;
+};
+
+// Common per-launch view of the walker-major state.
+struct StepView {
+    const float4* posq;
+    long long* force;
+    const WalkerInfo* walkers;
+    const int* gidx;      // [W][S]
+    const double* box;    // [W][12] = a, b, c, 1/ax, 1/by, 1/cz
+    double* cvVal;        // [W][V]
+    float* jx;            // [W][R]
+    float* jy;
+    float* jz;
+    double* bgrad;        // [W][G]
+    double* benergy;      // [W][B]
+    double* dUdCV;        // [W][V]
+    const double* energyU; // [W] value of the CV_ENERGY input
+    int S, V, R, G, B;
+    int pbc;
+};
+
+// Few-atom CVs (distance, angle, dihedral, position): one work item each.
+struct SmallCVItem {
+    int type;
+    int slot[4];
+    int value;
+    int row0;
+    int comp;  // position component
+};
+
+// Generic "group" CVs (gyration, COM distance, RMSD, DRMSD, contact map):
+// atoms / pairs are listed in slot space; one workgroup per (walker, item).
+struct GroupCVItem {
+    int type;
+    int begin, end;       // range into the slot list / pair list
+    int split;            // COM: size of group 1; RMSD: 1 = Kabsch fit
+    int value;
+    int row0;
+    int refOff;           // RMSD reference (float3 units) / DRMSD ref / CM params
+    double refCentroid[3];
+};
+
+// Coordination CV with optional cutoff (cell list when the box allows it).
+struct CoordItem {
+    int aBegin, nA;       // slots of group A in coordSlots
+    int bBegin, nB;       // slots of group B
+    float r0, rn, rm;
+    int intPow;           // 1 when n, m are small integers (fast integer powers)
+    double rc2;           // cutoff^2 (0 = no cutoff)
+    int value;
+    int rowA, rowB;
+    int useCells;         // cell list or brute force
+    int nc[3];            // cells per dimension
+    int cellBase;         // offset into the cell arrays
+    int sortBase;         // offset into the sorted-atom arrays (A then B)
+    int partBase;         // offset into the per-block CN partial sums
+};
+
+// Path CV (s, z), MSD metric.
+struct PathItem {
+    int begin, M;         // slots
+    int N;                // frames
+    float lambda;
+    int frameOff;         // float offset of frame 0 (N*M*3 floats)
+    int valueS;           // z at valueS + 1
+    int row0;             // rows interleaved s/z per atom
+};
+
+}  // namespace glued
+
+namespace glued {
+
+// Grid metadynamics (BIAS_METAD).  CommonBiasKernels.cpp:206-297.
+struct GridBiasDev {
+    int D, cv[3], gslot, eslot;
+    double origin[3], invSpacing[3], sigma[3];
+    int npts[3], strides[3], periodic[3];
+    int total;
+    double h0, hif;           // heightInvFactor = 1/((gamma-1) kT), 0 if gamma <= 1
+    double* grid;             // [W or 1][total]
+    long long walkerStride;   // 0 when shared by all walkers
+};
+
+// Hill-list metadynamics (BIAS_METAD_HILLS, extension).
+struct HillBiasDev {
+    int D, cv[MAX_BIAS_D], gslot, eslot;
+    double sigma[MAX_BIAS_D], period[MAX_BIAS_D], h0, hif;
+    // host-computed reciprocals (the oracle forms the same expressions):
+    // i2s2 = 1/(2 sigma^2), is2 = 1/sigma^2, invPeriod = 1/period
+    double i2s2[MAX_BIAS_D], is2[MAX_BIAS_D], invPeriod[MAX_BIAS_D];
+    double* centers;          // [W or 1][D][cap]  (SoA, coalesced per CV)
+    double* heights;          // [W or 1][cap]
+    int* count;               // [W or 1]
+    int cap;
+    int shared;
+    int chunks;               // blocks per walker in the eval launch
+    double* part;             // [W][chunks][D+1]
+    unsigned int* counter;    // [W]
+};
+
+// OPES (BIAS_OPES).  CommonBiasKernels.cpp:489-664.
+struct OpesBiasDev {
+    int D, cv[MAX_BIAS_D], gslot, eslot;
+    int variant, adaptiveStride, cap, shared, chunks;
+    double kT, gamma, invGF, cutoff2, eps, barrier, sigmaMin;
+    double sigma0[MAX_BIAS_D];
+    double* centers;          // [T][D][cap]   T = 1 (shared) or W tables
+    double* sigmas;           // [T][D][cap]
+    double* logw;             // [T][cap]
+    int* ints;                // [T][4] = numKernels, numAllocated, stepCount, nSamples
+    double* dbl;              // [T][4 + 2D] = sum_uprob, sumWeights, sum_w, sum_w2, mean[D], M2[D]
+    double* part;             // [W][chunks][D+1]
+    unsigned int* counter;    // [W]
+};
+
+// Few-parameter biases evaluated by one thread (grid MetaD eval, restraints,
+// OPES expanded, multithermal).
+struct SmallBiasItem {
+    int type, D, gslot, eslot;
+    int cv[MAX_BIAS_D];
+    int bias;                 // index into the grid-bias table (METAD)
+    int nState;               // OPES_EXPANDED / MULTITHERMAL states
+    int stateOff;             // offset into the state arrays
+    double p[8];              // type-specific scalars (k, at, kT0, invKT, ...)
+};
+
+struct SmallBiasState {
+    const GridBiasDev* grids;
+    const double* stateConst; // logW (expanded) / beta - beta0 (multithermal)
+    double* stateVar;         // [W or 1][...] deltaF (multithermal)
+    double* scal;             // [W or 1][nSmall][4]: logZ, numUpdates, rct, counter
+    long long stateVarStride, scalStride;
+};
+
+}  // namespace glued
+
+namespace glued {
+
+// Coordination scratch (cell lists, sorted atoms, per-block partial sums).
+struct CoordBuffers {
+    const int* slots;      // coordSlots
+    int* cellOf;           // [W][cellOfStride]
+    int* cellCount;        // [W][cellStride]
+    int* cellStart;        // [W][cellStride + items]  (scan output, 1 extra per item)
+    float4* sorted;        // [W][sortStride]
+    double* partCN;        // [W][partStride]
+    unsigned long long* partPairs;  // [W][partStride]
+    unsigned int* counters;         // [W][items]
+    long long* npairs;              // [W][items]
+    int cellOfStride, cellStride, sortStride, partStride, nItems;
+};
+
+// Scatter plan: Jacobian rows grouped by atom slot (CSR), CV of every row,
+// and the gradient slots that sum into each dU/dCV.
+struct ScatterPlan {
+    const int* segStart;     // [S+1]
+    const int* segRow;       // row ids grouped by slot
+    const int* rowCv;        // [R]
+    const int* combStart;    // [V+1]
+    const int* combSlot;
+    const double* testGrad;  // [V] or null
+    const float* energyF;    // dense external force of the energy CV (force-block layout), or null
+    int energyValue;         // CV value index of the energy CV (-1 if none)
+};
+
+}  // namespace glued

@@ -1,0 +1,717 @@
+// Bias evaluation and deposition kernels for gfx950.
+//
+// Chunked sums (hill-list metadynamics, OPES kernels) stream the bias state
+// once per evaluation: each workgroup reduces one chunk to FP64 partials,
+// the last workgroup of a walker to finish (atomic ticket) adds the partials
+// in chunk order -- deterministic -- and writes V and dV/ds.  Per-hill
+// exponents are formed in FP64 with unfused operations so the E <= 8
+// truncation predicate is bit-identical to the FP64 oracle; exp() runs in FP32
+// (its relative error bounds each term by 0.37 * 2^-24 of its height).
+#include "glued_internal.h"
+#include <hip/hip_runtime.h>
+#include <cmath>
+
+namespace glued {
+
+__device__ inline double wsum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// Grid MetaD evaluation: CommonGluedKernels.cpp:3574-3636.
+// ---------------------------------------------------------------------------
+__device__ void grid_eval(const GridBiasDev& g, const double* grid, const double* s, double* V, double* dV)
+{
+    int lo[3], hi[3];
+    double al[3];
+    for (int d = 0; d < g.D; d++) {
+        double fr = (s[d] - g.origin[d]) * g.invSpacing[d];
+        int N = g.npts[d];
+        if (N <= 1) {
+            lo[d] = hi[d] = 0;
+            al[d] = 0.0;
+        } else if (g.periodic[d]) {
+            fr = fr - floor(fr / (double)N) * (double)N;
+            if (fr < 0.0) fr += N;
+            lo[d] = (int)fr;
+            if (lo[d] >= N) lo[d] = N - 1;
+            al[d] = fr - lo[d];
+            hi[d] = (lo[d] + 1) % N;
+        } else {
+            if (fr < 0.0) fr = 0.0;
+            if (fr > (double)(N - 1)) fr = (double)(N - 1);
+            lo[d] = (int)fr;
+            if (lo[d] >= N - 1) lo[d] = N - 2;
+            al[d] = fr - lo[d];
+            hi[d] = lo[d] + 1;
+        }
+    }
+    double v = 0.0, gr[3] = {0, 0, 0};
+    for (int c = 0; c < (1 << g.D); c++) {
+        int idx = 0;
+        for (int d = 0; d < g.D; d++) idx += (((c >> d) & 1) ? hi[d] : lo[d]) * g.strides[d];
+        double val = grid[idx], w = 1.0;
+        for (int d = 0; d < g.D; d++) w *= ((c >> d) & 1) ? al[d] : (1.0 - al[d]);
+        v += w * val;
+        for (int d = 0; d < g.D; d++) {
+            double wo = 1.0;
+            for (int e = 0; e < g.D; e++)
+                if (e != d) wo *= ((c >> e) & 1) ? al[e] : (1.0 - al[e]);
+            gr[d] += (((c >> d) & 1) ? 1.0 : -1.0) * wo * val;
+        }
+    }
+    *V = v;
+    for (int d = 0; d < g.D; d++) dV[d] = gr[d] * g.invSpacing[d];
+}
+
+// ---------------------------------------------------------------------------
+// One-thread biases: grid MetaD eval, OPES expanded (:3050-3090),
+// multithermal (:3145-3172).  One thread per (item, walker).
+// ---------------------------------------------------------------------------
+__global__ void k_bias_small(StepView v, SmallBiasState st, const SmallBiasItem* __restrict__ items, int n, int w0)
+{
+    const int w = w0 + blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SmallBiasItem it = items[i];
+    const double* cv = v.cvVal + (size_t)w * v.V;
+    double* bg = v.bgrad + (size_t)w * v.G + it.gslot;
+    double* be = v.benergy + (size_t)w * v.B + it.eslot;
+    if (it.type == BIAS_METAD) {
+        const GridBiasDev g = st.grids[it.bias];
+        double s[3], V, dV[3];
+        for (int d = 0; d < g.D; d++) s[d] = cv[g.cv[d]];
+        grid_eval(g, g.grid + (size_t)w * g.walkerStride, s, &V, dV);
+        *be = V;
+        for (int d = 0; d < g.D; d++) bg[d] = dV[d];
+    } else if (it.type == BIAS_OPES_EXPANDED) {
+        const double invKT = it.p[0];
+        const double* lw = st.stateConst + it.stateOff;
+        const double logZ = st.scal[(size_t)w * st.scalStride + 4 * i + 0];
+        double mx = -1e300;
+        for (int l = 0; l < it.nState; l++) mx = fmax(mx, lw[l] - cv[it.cv[0] + l] * invKT);
+        double se = 0.0;
+        for (int l = 0; l < it.nState; l++) se += exp(lw[l] - cv[it.cv[0] + l] * invKT - mx);
+        const double logQ = mx + log(se);
+        *be = -(1.0 / invKT) * (logQ - logZ);
+        for (int l = 0; l < it.nState; l++) bg[l] = exp(lw[l] - cv[it.cv[0] + l] * invKT - logQ);
+    } else if (it.type == BIAS_OPES_MULTITHERMAL) {
+        const double kT0 = it.p[0];
+        const double* db = st.stateConst + it.stateOff;
+        const double* dF = st.stateVar + (size_t)w * st.stateVarStride + it.stateOff;
+        const double U = cv[it.cv[0]];
+        double mx = -1e300;
+        for (int l = 0; l < it.nState; l++) mx = fmax(mx, -db[l] * U + dF[l] / kT0);
+        double s = 0.0, ds = 0.0;
+        for (int l = 0; l < it.nState; l++) {
+            double e = exp(-db[l] * U + dF[l] / kT0 - mx);
+            s += e;
+            ds += e * db[l];
+        }
+        *be = -kT0 * (mx + log(s / (double)it.nState));
+        bg[0] = kT0 * (ds / s);
+    }
+}
+
+void launch_bias_small(const StepView& v, const SmallBiasState& st, const SmallBiasItem* items, int n, int w0, int nw,
+                       hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_bias_small, dim3((n + 63) / 64, nw), dim3(64), 0, s, v, st, items, n, w0);
+}
+
+// ---------------------------------------------------------------------------
+// Hill-list metadynamics evaluation (extension; truncation rule of
+// CommonGluedKernels.cpp:3664-3677 applied at the CV point):
+//   V = sum_k h_k exp(-E_k) [E_k <= 8],  E_k = sum_d dc_d^2 / (2 sigma_d^2)
+// HILL_BLOCK threads x HILLS_PER_THREAD hills per workgroup, coalesced SoA
+// loads (D centres + height = 8(D+1) bytes per hill).
+// ---------------------------------------------------------------------------
+constexpr int HILL_BLOCK = 256;
+constexpr int HILLS_PER_THREAD = 4;
+constexpr int HILL_CHUNK = HILL_BLOCK * HILLS_PER_THREAD;
+
+template <int D>
+__device__ inline void hill_terms(const HillBiasDev& h, const double* __restrict__ C, const double* __restrict__ H,
+                                  int n, int cap, const double* s, int k0, double* acc)
+{
+#pragma clang fp contract(off)
+    for (int r = 0; r < HILLS_PER_THREAD; r++) {
+        const int k = k0 + r * HILL_BLOCK;
+        if (k >= n) break;
+        double e = 0.0, dc[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            double x = s[d] - C[(size_t)d * cap + k];
+            if (h.period[d] > 0.0) x = x - round(x * h.invPeriod[d]) * h.period[d];
+            dc[d] = x;
+            e = e + x * x * h.i2s2[d];
+        }
+        if (!(e <= 8.0)) continue;
+        const double t = H[k] * (double)expf(-(float)e);
+        acc[0] += t;
+#pragma unroll
+        for (int d = 0; d < D; d++) acc[1 + d] -= t * dc[d] * h.is2[d];
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(HILL_BLOCK) void k_bias_hills(StepView v, HillBiasDev h, int w0)
+{
+    __shared__ double sh[(D + 1) * 4];
+    __shared__ int shLast;
+    const int w = w0 + blockIdx.y;
+    const int tb = h.shared ? 0 : w;
+    const double* C = h.centers + (size_t)tb * D * h.cap;
+    const double* H = h.heights + (size_t)tb * h.cap;
+    const int n = h.count[tb];
+    const double* cv = v.cvVal + (size_t)w * v.V;
+    double s[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) s[d] = cv[h.cv[d]];
+    double acc[D + 1];
+#pragma unroll
+    for (int d = 0; d <= D; d++) acc[d] = 0.0;
+    for (int c = blockIdx.x; c * HILL_CHUNK < n; c += gridDim.x)
+        hill_terms<D>(h, C, H, n, h.cap, s, c * HILL_CHUNK + threadIdx.x, acc);
+#pragma unroll
+    for (int d = 0; d <= D; d++) acc[d] = wsum(acc[d]);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int d = 0; d <= D; d++) sh[d * 4 + wid] = acc[d];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double* part = h.part + ((size_t)w * h.chunks + blockIdx.x) * (D + 1);
+        for (int d = 0; d <= D; d++) part[d] = sh[d * 4] + sh[d * 4 + 1] + sh[d * 4 + 2] + sh[d * 4 + 3];
+        __threadfence();
+        unsigned done = atomicAdd(&h.counter[w], 1u);
+        shLast = done == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (shLast && threadIdx.x == 0) {
+        __threadfence();
+        double tot[D + 1];
+        for (int d = 0; d <= D; d++) tot[d] = 0.0;
+        for (unsigned b = 0; b < gridDim.x; b++) {
+            const double* part = h.part + ((size_t)w * h.chunks + b) * (D + 1);
+            for (int d = 0; d <= D; d++)
+                tot[d] += __hip_atomic_load(&part[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        v.benergy[(size_t)w * v.B + h.eslot] = tot[0];
+        for (int d = 0; d < D; d++) v.bgrad[(size_t)w * v.G + h.gslot + d] = tot[1 + d];
+        h.counter[w] = 0u;
+    }
+}
+
+void launch_bias_hills(const StepView& v, const HillBiasDev& h, int w0, int nw, hipStream_t s)
+{
+    dim3 g(h.chunks, nw), b(HILL_BLOCK);
+    switch (h.D) {
+        case 1: hipLaunchKernelGGL(k_bias_hills<1>, g, b, 0, s, v, h, w0); break;
+        case 2: hipLaunchKernelGGL(k_bias_hills<2>, g, b, 0, s, v, h, w0); break;
+        case 3: hipLaunchKernelGGL(k_bias_hills<3>, g, b, 0, s, v, h, w0); break;
+        default: hipLaunchKernelGGL(k_bias_hills<4>, g, b, 0, s, v, h, w0); break;
+    }
+}
+
+int hill_chunks_for(int cap) { return (cap + HILL_CHUNK - 1) / HILL_CHUNK; }
+
+// ---------------------------------------------------------------------------
+// OPES evaluation: CommonGluedKernels.cpp:2642-2723, parallel over kernels.
+// ---------------------------------------------------------------------------
+constexpr int OPES_BLOCK = 256;
+constexpr int OPES_PER_THREAD = 4;
+constexpr int OPES_CHUNK = OPES_BLOCK * OPES_PER_THREAD;
+
+int opes_chunks_for(int cap) { return (cap + OPES_CHUNK - 1) / OPES_CHUNK; }
+
+template <int D>
+__global__ __launch_bounds__(OPES_BLOCK) void k_bias_opes(StepView v, OpesBiasDev o, int w0)
+{
+    __shared__ double sh[(D + 1) * 4];
+    __shared__ int shLast;
+    const int w = w0 + blockIdx.y;
+    const int tb = o.shared ? 0 : w;
+    const double* C = o.centers + (size_t)tb * D * o.cap;
+    const double* Sg = o.sigmas + (size_t)tb * D * o.cap;
+    const double* L = o.logw + (size_t)tb * o.cap;
+    const int nk = o.ints[tb * 4 + 0];
+    const double* cv = v.cvVal + (size_t)w * v.V;
+    double s[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) s[d] = cv[o.cv[d]];
+    const double kfloor = exp(-0.5 * o.cutoff2);
+    double acc[D + 1];
+#pragma unroll
+    for (int d = 0; d <= D; d++) acc[d] = 0.0;
+    for (int c = blockIdx.x; c * OPES_CHUNK < nk; c += gridDim.x) {
+        for (int r = 0; r < OPES_PER_THREAD; r++) {
+            const int k = c * OPES_CHUNK + r * OPES_BLOCK + threadIdx.x;
+            if (k >= nk) break;
+            double n2 = 0.0, dc[D], sg[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+#pragma clang fp contract(off)
+                dc[d] = s[d] - C[(size_t)d * o.cap + k];
+                sg[d] = Sg[(size_t)d * o.cap + k];
+                n2 = n2 + dc[d] * dc[d] / (sg[d] * sg[d]);
+            }
+            if (!(n2 < o.cutoff2)) continue;
+            const double G = exp(-0.5 * n2), kern = G - kfloor;
+            if (kern <= 0.0) continue;
+            const double hk = exp(L[k]);
+            if (!(hk > 0.0 && hk < 1e308)) continue;
+            acc[0] += hk * kern;
+#pragma unroll
+            for (int d = 0; d < D; d++) acc[1 + d] += hk * G * (-dc[d] / (sg[d] * sg[d]));
+        }
+    }
+#pragma unroll
+    for (int d = 0; d <= D; d++) acc[d] = wsum(acc[d]);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int d = 0; d <= D; d++) sh[d * 4 + wid] = acc[d];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double* part = o.part + ((size_t)w * o.chunks + blockIdx.x) * (D + 1);
+        for (int d = 0; d <= D; d++) part[d] = sh[d * 4] + sh[d * 4 + 1] + sh[d * 4 + 2] + sh[d * 4 + 3];
+        __threadfence();
+        unsigned done = atomicAdd(&o.counter[w], 1u);
+        shLast = done == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (shLast && threadIdx.x == 0) {
+        __threadfence();
+        double tot[D + 1];
+        for (int d = 0; d <= D; d++) tot[d] = 0.0;
+        for (unsigned b = 0; b < gridDim.x; b++) {
+            const double* part = o.part + ((size_t)w * o.chunks + b) * (D + 1);
+            for (int d = 0; d <= D; d++)
+                tot[d] += __hip_atomic_load(&part[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        double* be = v.benergy + (size_t)w * v.B + o.eslot;
+        double* bg = v.bgrad + (size_t)w * v.G + o.gslot;
+        if (nk == 0) {
+            *be = -o.barrier;
+            for (int d = 0; d < D; d++) bg[d] = 0.0;
+        } else {
+            const double prob = tot[0];
+            const double su = o.dbl[(size_t)tb * (4 + 2 * D) + 0];
+            double pz = su > 0.0 ? prob * (double)nk / su : 0.0;
+            if (!(pz >= 0.0)) pz = 0.0;
+            const double arg = pz + o.eps;
+            *be = o.invGF * o.kT * log(arg > 1e-30 ? arg : 1e-30);
+            const double f = prob > 0.0 ? o.invGF * o.kT * (pz / (pz + o.eps)) / prob : 0.0;
+            for (int d = 0; d < D; d++) bg[d] = f * tot[1 + d];
+        }
+        o.counter[w] = 0u;
+    }
+}
+
+void launch_bias_opes(const StepView& v, const OpesBiasDev& o, int w0, int nw, hipStream_t s)
+{
+    dim3 g(o.chunks, nw), b(OPES_BLOCK);
+    switch (o.D) {
+        case 1: hipLaunchKernelGGL(k_bias_opes<1>, g, b, 0, s, v, o, w0); break;
+        case 2: hipLaunchKernelGGL(k_bias_opes<2>, g, b, 0, s, v, o, w0); break;
+        case 3: hipLaunchKernelGGL(k_bias_opes<3>, g, b, 0, s, v, o, w0); break;
+        default: hipLaunchKernelGGL(k_bias_opes<4>, g, b, 0, s, v, o, w0); break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Deposition.
+// ---------------------------------------------------------------------------
+
+// Grid deposit, CommonGluedKernels.cpp:3645-3679.  One thread per grid node;
+// a shared grid takes the walkers' Gaussians in walker order (deterministic,
+// no atomics); a private grid is one y-block row per walker.
+__global__ void k_dep_grid(StepView v, GridBiasDev g, int w0, int nw)
+{
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= g.total) return;
+    const int wFirst = g.walkerStride ? w0 + blockIdx.y : w0;
+    const int wLast = g.walkerStride ? wFirst + 1 : w0 + nw;
+    double* grid = g.grid + (size_t)wFirst * g.walkerStride;
+    double pos[3];
+    for (int d = 0; d < g.D; d++) {
+        int bi = (idx / g.strides[d]) % g.npts[d];
+        pos[d] = g.origin[d] + (double)bi / g.invSpacing[d];
+    }
+    double add = 0.0;
+    for (int w = wFirst; w < wLast; w++) {
+        const double* cv = v.cvVal + (size_t)w * v.V;
+        const double V = v.benergy[(size_t)w * v.B + g.eslot];
+        const double height = g.hif > 0.0 ? g.h0 * exp(-V * g.hif) : g.h0;
+        double e = 0.0;
+        for (int d = 0; d < g.D; d++) {
+            double dc = pos[d] - cv[g.cv[d]];
+            if (g.periodic[d]) {
+                double per = (double)g.npts[d] / g.invSpacing[d];
+                dc -= round(dc / per) * per;
+            }
+            e += dc * dc / (2.0 * g.sigma[d] * g.sigma[d]);
+        }
+        if (e <= 8.0) add += height * exp(-e);
+    }
+    grid[idx] += add;
+}
+
+void launch_dep_grid(const StepView& v, const GridBiasDev& g, int w0, int nw, hipStream_t s)
+{
+    dim3 grid((g.total + 255) / 256, g.walkerStride ? nw : 1);
+    hipLaunchKernelGGL(k_dep_grid, grid, dim3(256), 0, s, v, g, w0, nw);
+}
+
+// Hill deposit: centre = current CVs, height = h0 exp(-V hif) (the WT rule of
+// CommonGluedKernels.cpp:3662 with this bias's own V).  One thread per walker;
+// a shared list appends the walkers in walker order from one thread.
+__global__ void k_dep_hills(StepView v, HillBiasDev h, int w0, int nw)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int wFirst = h.shared ? w0 : w0 + t;
+    const int wLast = h.shared ? w0 + nw : wFirst + 1;
+    if (h.shared ? t != 0 : t >= nw) return;
+    const int tb = h.shared ? 0 : wFirst;
+    double* C = h.centers + (size_t)tb * h.D * h.cap;
+    double* H = h.heights + (size_t)tb * h.cap;
+    for (int w = wFirst; w < wLast; w++) {
+        int n = h.count[tb];
+        if (n >= h.cap) return;
+        const double* cv = v.cvVal + (size_t)w * v.V;
+        const double V = v.benergy[(size_t)w * v.B + h.eslot];
+        for (int d = 0; d < h.D; d++) C[(size_t)d * h.cap + n] = cv[h.cv[d]];
+        H[n] = h.hif > 0.0 ? h.h0 * exp(-V * h.hif) : h.h0;
+        h.count[tb] = n + 1;
+    }
+}
+
+void launch_dep_hills(const StepView& v, const HillBiasDev& h, int w0, int nw, hipStream_t s)
+{
+    const int n = h.shared ? 1 : nw;
+    hipLaunchKernelGGL(k_dep_hills, dim3((n + 63) / 64), dim3(64), 0, s, v, h, w0, nw);
+}
+
+// ---------------------------------------------------------------------------
+// OPES deposit, CommonGluedKernels.cpp:2752-3003, one 1024-thread workgroup
+// per kernel table.  The compression search is a parallel lexicographic
+// (n2, index) arg-min over kernels with n2 < 1 -- identical to the
+// reference's sequential strict-minimum scan because every kernel's n2 is
+// formed with the same unfused FP64 operations.  sum_uprob is updated in
+// O(N) on both append and merge: on a merge the old row and column of the
+// merged kernel are subtracted and the new ones added, which equals the
+// reference's O(N^2) recomputation up to FP64 rounding.
+// ---------------------------------------------------------------------------
+constexpr int OPES_DEP_BLOCK = 1024;
+
+__device__ inline double opes_T(const double* ci, const double* ck, const double* sk, double hk, int D, double cutoff2,
+                                double epsd)
+{
+#pragma clang fp contract(off)
+    double n2 = 0.0;
+    for (int d = 0; d < D; d++) {
+        double dc = ci[d] - ck[d];
+        n2 = n2 + dc * dc / (sk[d] * sk[d]);
+        if (n2 >= cutoff2) return 0.0;
+    }
+    return hk * (exp(-0.5 * n2) - epsd);
+}
+
+__global__ __launch_bounds__(OPES_DEP_BLOCK) void k_dep_opes(StepView v, OpesBiasDev o, int w0, int nw)
+{
+#pragma clang fp contract(off)
+    __shared__ double shN2[OPES_DEP_BLOCK / 64];
+    __shared__ int shIdx[OPES_DEP_BLOCK / 64];
+    __shared__ double shSum[OPES_DEP_BLOCK / 64];
+    __shared__ double sC[MAX_BIAS_D], sS[MAX_BIAS_D], sOldC[MAX_BIAS_D], sOldS[MAX_BIAS_D];
+    __shared__ double sH, sLw, sOldH;
+    __shared__ int sMerge, sAppend, sSkip, sNBefore;
+    const int D = o.D;
+    const int tb = o.shared ? 0 : w0 + blockIdx.x;
+    const int wFirst = o.shared ? w0 : w0 + blockIdx.x;
+    const int wLast = o.shared ? w0 + nw : wFirst + 1;
+    double* C = o.centers + (size_t)tb * D * o.cap;
+    double* Sg = o.sigmas + (size_t)tb * D * o.cap;
+    double* L = o.logw + (size_t)tb * o.cap;
+    int* I = o.ints + tb * 4;
+    double* Q = o.dbl + (size_t)tb * (4 + 2 * D);
+    double* mean = Q + 4;
+    double* M2 = Q + 4 + D;
+    const double epsd = exp(-0.5 * o.cutoff2);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+
+    for (int w = wFirst; w < wLast; w++) {
+        const double* cv = v.cvVal + (size_t)w * v.V;
+        // ---- scalar part: weights, neff, sigma, height (thread 0) ----
+        if (threadIdx.x == 0) {
+            sSkip = 0;
+            if (o.adaptiveStride > 0 && I[3] < o.adaptiveStride) sSkip = 1;
+            if (!sSkip) {
+                sNBefore = I[0];
+                if (o.variant != 1 && o.adaptiveStride == 0) I[3]++;
+                const int n = I[3];
+                const double V = v.benergy[(size_t)w * v.B + o.eslot];
+                double lw_raw, h_raw;
+                if (o.variant == 2) {
+                    lw_raw = 0.0;
+                    h_raw = 1.0;
+                } else {
+                    lw_raw = V * (1.0 / o.kT);
+                    h_raw = exp(lw_raw);
+                }
+                Q[2] += h_raw;
+                Q[3] += h_raw * h_raw;
+                I[2]++;
+                const double neff = (1.0 + Q[2]) * (1.0 + Q[2]) / (1.0 + Q[3]);
+                const double size = o.variant == 2 ? (double)I[2] : neff;
+                double lw = lw_raw;
+                for (int d = 0; d < D; d++) {
+                    double sg;
+                    if (o.variant == 1) {
+                        sg = o.sigma0[d];
+                    } else if (o.adaptiveStride > 0) {
+                        double factor = o.variant == 2 ? 1.0 : o.gamma;
+                        double var = n > 1 ? M2[d] / factor / (double)(n - 1) : 0.0;
+                        sg = var > 0.0 ? sqrt(var) : 1.0;
+                        if (sg < o.sigmaMin) sg = o.sigmaMin;
+                    } else {
+                        sg = o.sigma0[d] * pow(size * (D + 2) / 4.0, -1.0 / (D + 4.0));
+                        if (sg < o.sigmaMin) sg = o.sigmaMin;
+                    }
+                    sS[d] = sg;
+                    sC[d] = cv[o.cv[d]];
+                }
+                if (o.variant != 1 && o.adaptiveStride == 0)
+                    for (int d = 0; d < D; d++) lw += log(o.sigma0[d] / sS[d]);
+                sLw = lw;
+                sH = exp(lw);
+                Q[1] += h_raw;
+            }
+        }
+        __syncthreads();
+        if (sSkip) continue;
+        const int nB = sNBefore;
+        // ---- compression search: arg-min (n2, k) over n2 < 1 ----
+        double best = 1.0;
+        int bestK = -1;
+        for (int k = threadIdx.x; k < nB; k += blockDim.x) {
+            double n2 = 0.0;
+            for (int d = 0; d < D; d++) {
+                double x = (sC[d] - C[(size_t)d * o.cap + k]) / Sg[(size_t)d * o.cap + k];
+                n2 = n2 + x * x;
+            }
+            if (n2 < best) {  // k increases within a thread: first index wins ties
+                best = n2;
+                bestK = k;
+            }
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            double ob = __shfl_xor(best, off, 64);
+            int ok = __shfl_xor(bestK, off, 64);
+            if (ok >= 0 && (bestK < 0 || ob < best || (ob == best && ok < bestK))) {
+                best = ob;
+                bestK = ok;
+            }
+        }
+        if (lane == 0) {
+            shN2[wid] = best;
+            shIdx[wid] = bestK;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double b = 1.0;
+            int bk = -1;
+            for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+                int ok = shIdx[i];
+                if (ok >= 0 && (bk < 0 || shN2[i] < b || (shN2[i] == b && ok < bk))) {
+                    b = shN2[i];
+                    bk = ok;
+                }
+            }
+            sMerge = bk;
+            sAppend = 0;
+            if (bk >= 0) {
+                for (int d = 0; d < D; d++) {
+                    sOldC[d] = C[(size_t)d * o.cap + bk];
+                    sOldS[d] = Sg[(size_t)d * o.cap + bk];
+                }
+                sOldH = exp(L[bk]);
+                const double lo = L[bk], lwn = sLw;
+                const double ho = sOldH, ht = ho + sH;
+                const double lmax = lo > lwn ? lo : lwn, lmin = lo < lwn ? lo : lwn;
+                const double lm = lmax + log1p(exp(lmin - lmax));
+                const double f1 = ho / ht, f2 = sH / ht;
+                for (int d = 0; d < D; d++) {
+                    const double c1 = sOldC[d], s1 = sOldS[d];
+                    const double cm = f1 * c1 + f2 * sC[d];
+                    double ss = f1 * (s1 * s1 + c1 * c1) + f2 * (sS[d] * sS[d] + sC[d] * sC[d]) - cm * cm;
+                    if (ss < 0.0) ss = 0.0;
+                    double sm = sqrt(ss);
+                    if (sm < o.sigmaMin) sm = o.sigmaMin;
+                    C[(size_t)d * o.cap + bk] = cm;
+                    Sg[(size_t)d * o.cap + bk] = sm;
+                }
+                L[bk] = lm;
+            } else {
+                const int k = I[1]++;
+                if (k < o.cap) {
+                    for (int d = 0; d < D; d++) {
+                        C[(size_t)d * o.cap + k] = sC[d];
+                        Sg[(size_t)d * o.cap + k] = sS[d];
+                    }
+                    L[k] = sLw;
+                    I[0]++;
+                    sAppend = 1;
+                } else {
+                    I[1]--;
+                }
+            }
+        }
+        __syncthreads();
+        // ---- sum_uprob update, O(N) ----
+        const int mk = sMerge;
+        double acc = 0.0;
+        if (mk >= 0) {
+            double cNew[MAX_BIAS_D], sNew[MAX_BIAS_D], cK[MAX_BIAS_D], sK[MAX_BIAS_D];
+            for (int d = 0; d < D; d++) {
+                cNew[d] = C[(size_t)d * o.cap + mk];
+                sNew[d] = Sg[(size_t)d * o.cap + mk];
+            }
+            const double hNew = exp(L[mk]);
+            for (int i = threadIdx.x; i < nB; i += blockDim.x) {
+                double ci[MAX_BIAS_D], si[MAX_BIAS_D];
+                for (int d = 0; d < D; d++) {
+                    ci[d] = C[(size_t)d * o.cap + i];
+                    si[d] = Sg[(size_t)d * o.cap + i];
+                }
+                const double hi = exp(L[i]);
+                if (i == mk) {
+                    // diagonal: row and column coincide
+                    acc += opes_T(cNew, cNew, sNew, hNew, D, o.cutoff2, epsd) -
+                           opes_T(sOldC, sOldC, sOldS, sOldH, D, o.cutoff2, epsd);
+                    continue;
+                }
+                for (int d = 0; d < D; d++) {
+                    cK[d] = sOldC[d];
+                    sK[d] = sOldS[d];
+                }
+                // row mk (j = mk, kernel i) and column mk (j = i, kernel mk)
+                acc += opes_T(cNew, ci, si, hi, D, o.cutoff2, epsd) - opes_T(cK, ci, si, hi, D, o.cutoff2, epsd);
+                acc += opes_T(ci, cNew, sNew, hNew, D, o.cutoff2, epsd) -
+                       opes_T(ci, cK, sK, sOldH, D, o.cutoff2, epsd);
+            }
+        } else if (sAppend) {
+            for (int j = threadIdx.x; j < nB; j += blockDim.x) {
+                double cj[MAX_BIAS_D], sj[MAX_BIAS_D];
+                for (int d = 0; d < D; d++) {
+                    cj[d] = C[(size_t)d * o.cap + j];
+                    sj[d] = Sg[(size_t)d * o.cap + j];
+                }
+                acc += opes_T(cj, sC, sS, sH, D, o.cutoff2, epsd);
+                acc += opes_T(sC, cj, sj, exp(L[j]), D, o.cutoff2, epsd);
+            }
+        }
+        acc = wsum(acc);
+        if (lane == 0) shSum[wid] = acc;
+        __syncthreads();
+        if (threadIdx.x == 0 && (mk >= 0 || sAppend)) {
+            double t = 0.0;
+            for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += shSum[i];
+            if (mk < 0) t += sH * (1.0 - epsd);
+            Q[0] += t;
+        }
+        __syncthreads();
+    }
+}
+
+// Every-step Welford accumulator (adaptive sigma), :3009-3027.
+__global__ void k_opes_welford(StepView v, OpesBiasDev o, int w0, int nw)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o.shared ? t != 0 : t >= nw) return;
+    const int wFirst = o.shared ? w0 : w0 + t;
+    const int wLast = o.shared ? w0 + nw : wFirst + 1;
+    const int tb = o.shared ? 0 : wFirst;
+    int* I = o.ints + tb * 4;
+    double* Q = o.dbl + (size_t)tb * (4 + 2 * o.D);
+    for (int w = wFirst; w < wLast; w++) {
+        const double* cv = v.cvVal + (size_t)w * v.V;
+        I[3]++;
+        const int n = I[3];
+        for (int d = 0; d < o.D; d++) {
+            double x = cv[o.cv[d]];
+            double delta = x - Q[4 + d];
+            Q[4 + d] += delta / n;
+            Q[4 + o.D + d] += delta * (x - Q[4 + d]);
+        }
+    }
+}
+
+void launch_dep_opes(const StepView& v, const OpesBiasDev& o, int w0, int nw, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dep_opes, dim3(o.shared ? 1 : nw), dim3(OPES_DEP_BLOCK), 0, s, v, o, w0, nw);
+}
+
+void launch_opes_welford(const StepView& v, const OpesBiasDev& o, int w0, int nw, hipStream_t s)
+{
+    const int n = o.shared ? 1 : nw;
+    hipLaunchKernelGGL(k_opes_welford, dim3((n + 63) / 64), dim3(64), 0, s, v, o, w0, nw);
+}
+
+// OPES expanded logZ update (:3094-3125) and multithermal dF update
+// (:3174-3203); one thread per (item, walker).
+__global__ void k_dep_small(StepView v, SmallBiasState st, const SmallBiasItem* __restrict__ items, int n, int w0,
+                            int onlyItem)
+{
+    const int w = w0 + blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || (onlyItem >= 0 && i != onlyItem)) return;
+    const SmallBiasItem it = items[i];
+    const double* cv = v.cvVal + (size_t)w * v.V;
+    double* sc = st.scal + (size_t)w * st.scalStride + 4 * i;
+    if (it.type == BIAS_OPES_EXPANDED) {
+        const double invKT = it.p[0];
+        const double* lw = st.stateConst + it.stateOff;
+        double mx = -1e300;
+        for (int l = 0; l < it.nState; l++) mx = fmax(mx, lw[l] - cv[it.cv[0] + l] * invKT);
+        double se = 0.0;
+        for (int l = 0; l < it.nState; l++) se += exp(lw[l] - cv[it.cv[0] + l] * invKT - mx);
+        const double logQ = mx + log(se);
+        const int k = (int)sc[1];
+        if (k == 0) {
+            sc[0] = logQ;
+        } else {
+            double A = sc[0] + log((double)k), B = logQ, m = A > B ? A : B;
+            sc[0] = m + log(exp(A - m) + exp(B - m)) - log((double)(k + 1));
+        }
+        sc[1] = k + 1;
+    } else if (it.type == BIAS_OPES_MULTITHERMAL) {
+        const double kT0 = it.p[0];
+        const double* db = st.stateConst + it.stateOff;
+        double* dF = st.stateVar + (size_t)w * st.stateVarStride + it.stateOff;
+        const double U = cv[it.cv[0]];
+        const double V = v.benergy[(size_t)w * v.B + it.eslot];
+        const double r = sc[2];
+        sc[3] += 1.0;
+        const double cnt = sc[3], logc = log(cnt - 1.0);
+        const double arg = (V - r) / kT0 - logc;
+        const double inc = arg > 0.0 ? kT0 * (arg + log1p(exp(-arg))) : kT0 * log1p(exp(arg));
+        for (int l = 0; l < it.nState; l++) {
+            double dl = -db[l] * U + (V - r + dF[l]) / kT0 - logc;
+            double dec = dl > 0.0 ? kT0 * (dl + log1p(exp(-dl))) : kT0 * log1p(exp(dl));
+            dF[l] += inc - dec;
+        }
+        sc[2] = r + inc + kT0 * log1p(-1.0 / cnt);
+    }
+}
+
+void launch_dep_small(const StepView& v, const SmallBiasState& st, const SmallBiasItem* items, int n, int item,
+                      int w0, int nw, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dep_small, dim3((n + 63) / 64, nw), dim3(64), 0, s, v, st, items, n, w0, item);
+}
+
+}  // namespace glued

@@ -1,0 +1,934 @@
+// CV kernels for gfx950.  Every kernel takes a walker range: walker = w0 +
+// blockIdx.y, so one launch covers one walker (plain form, the reference's
+// structure) or every walker (batched form).
+//
+// Arithmetic: positions are FP32 in HBM.  Differences of two FP32 coordinates
+// are formed in FP64, where they are exact, and the minimum image is applied
+// in FP64 with explicitly unfused operations, so distance predicates (the
+// coordination cutoff) are bit-identical to the FP64 oracle.  Bulk pair
+// arithmetic (switching functions) runs in FP32; reductions accumulate in
+// FP64.  Few-atom CVs are evaluated entirely in FP64 (a handful of threads).
+#include "glued_internal.h"
+#include <hip/hip_runtime.h>
+#include <cmath>
+
+namespace glued {
+
+// ---------------------------------------------------------------------------
+// Minimum image (triclinic, Z then Y then X), CommonGluedKernels.cpp:74-91.
+// bx = {a, b, c, 1/ax, 1/by, 1/cz}.  No FMA contraction: the oracle is built
+// with -ffp-contract=off and must see the same rounding.
+// ---------------------------------------------------------------------------
+__device__ inline void pbc_exact(double& x, double& y, double& z, const double* __restrict__ bx)
+{
+#pragma clang fp contract(off)
+    double s3 = floor(z * bx[11] + 0.5);
+    x = x - s3 * bx[6];
+    y = y - s3 * bx[7];
+    z = z - s3 * bx[8];
+    double s2 = floor(y * bx[10] + 0.5);
+    x = x - s2 * bx[3];
+    y = y - s2 * bx[4];
+    double s1 = floor(x * bx[9] + 0.5);
+    x = x - s1 * bx[0];
+}
+
+__device__ inline double dot3_exact(double x, double y, double z)
+{
+#pragma clang fp contract(off)
+    return x * x + y * y + z * z;
+}
+
+__device__ inline void pbc_f(float& x, float& y, float& z, const float* bx)
+{
+    float s3 = floorf(z * bx[11] + 0.5f);
+    x -= s3 * bx[6];
+    y -= s3 * bx[7];
+    z -= s3 * bx[8];
+    float s2 = floorf(y * bx[10] + 0.5f);
+    x -= s2 * bx[3];
+    y -= s2 * bx[4];
+    float s1 = floorf(x * bx[9] + 0.5f);
+    x -= s1 * bx[0];
+}
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block-wide FP64 sum of NV values per thread; every thread receives the result.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double* sh /* >= NV * 16 */)
+{
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+    __syncthreads();
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) sh[k * 16 + wid] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++) {
+        double s = 0.0;
+        for (int i = 0; i < nw; i++) s += sh[k * 16 + i];
+        v[k] = s;
+    }
+    __syncthreads();
+}
+
+__device__ inline float4 load_pos(const StepView& v, int w, int slot)
+{
+    const WalkerInfo wi = v.walkers[w];
+    int g = v.gidx[(size_t)w * v.S + slot];
+    return v.posq[wi.posOff + g];
+}
+
+__device__ inline void store_row(const StepView& v, int w, int row, double gx, double gy, double gz)
+{
+    size_t i = (size_t)w * v.R + row;
+    v.jx[i] = (float)gx;
+    v.jy[i] = (float)gy;
+    v.jz[i] = (float)gz;
+}
+
+// ---------------------------------------------------------------------------
+// Few-atom CVs, one thread per (item, walker), FP64.
+// distance :177-229, angle :238-325, dihedral :335-444, position :1084-1114
+// ---------------------------------------------------------------------------
+__global__ void k_cv_small(StepView v, const SmallCVItem* __restrict__ items, int n, int w0)
+{
+    const int w = w0 + blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const SmallCVItem it = items[i];
+    if (it.type == CV_ENERGY) {  // value supplied by the caller; Jacobian via the dense scatter
+        v.cvVal[(size_t)w * v.V + it.value] = v.energyU[w];
+        return;
+    }
+    const double* bx = v.box + (size_t)w * 12;
+    double p[4][3];
+    const int na = it.type == CV_DISTANCE ? 2 : it.type == CV_ANGLE ? 3 : it.type == CV_DIHEDRAL ? 4 : 1;
+    for (int a = 0; a < na; a++) {
+        float4 q = load_pos(v, w, it.slot[a]);
+        p[a][0] = q.x;
+        p[a][1] = q.y;
+        p[a][2] = q.z;
+    }
+    double val = 0.0;
+    double* cvv = v.cvVal + (size_t)w * v.V;
+    if (it.type == CV_DISTANCE) {
+        double dx = p[1][0] - p[0][0], dy = p[1][1] - p[0][1], dz = p[1][2] - p[0][2];
+        if (v.pbc) pbc_exact(dx, dy, dz, bx);
+        double d = sqrt(dx * dx + dy * dy + dz * dz);
+        double inv = d > 1e-10 ? 1.0 / d : 0.0;
+        val = d;
+        store_row(v, w, it.row0, -dx * inv, -dy * inv, -dz * inv);
+        store_row(v, w, it.row0 + 1, dx * inv, dy * inv, dz * inv);
+    } else if (it.type == CV_ANGLE) {
+        double r1[3], r2[3];
+        for (int k = 0; k < 3; k++) {
+            r1[k] = p[0][k] - p[1][k];
+            r2[k] = p[2][k] - p[1][k];
+        }
+        if (v.pbc) {
+            pbc_exact(r1[0], r1[1], r1[2], bx);
+            pbc_exact(r2[0], r2[1], r2[2], bx);
+        }
+        double l1 = sqrt(r1[0] * r1[0] + r1[1] * r1[1] + r1[2] * r1[2]);
+        double l2 = sqrt(r2[0] * r2[0] + r2[1] * r2[1] + r2[2] * r2[2]);
+        double i1 = l1 > 1e-10 ? 1.0 / l1 : 0.0, i2 = l2 > 1e-10 ? 1.0 / l2 : 0.0;
+        double n1[3], n2[3];
+        for (int k = 0; k < 3; k++) {
+            n1[k] = r1[k] * i1;
+            n2[k] = r2[k] * i2;
+        }
+        double c = n1[0] * n2[0] + n1[1] * n2[1] + n1[2] * n2[2];
+        c = fmax(-1.0, fmin(1.0, c));
+        val = acos(c);
+        double s = sqrt(1.0 - c * c);
+        double is = s > 1e-6 ? 1.0 / s : 0.0;
+        double ga[3], gc[3];
+        for (int k = 0; k < 3; k++) {
+            ga[k] = -(n2[k] - c * n1[k]) * i1 * is;
+            gc[k] = -(n1[k] - c * n2[k]) * i2 * is;
+        }
+        store_row(v, w, it.row0, ga[0], ga[1], ga[2]);
+        store_row(v, w, it.row0 + 1, -(ga[0] + gc[0]), -(ga[1] + gc[1]), -(ga[2] + gc[2]));
+        store_row(v, w, it.row0 + 2, gc[0], gc[1], gc[2]);
+    } else if (it.type == CV_DIHEDRAL) {
+        double b1[3], b2[3], b3[3];
+        for (int k = 0; k < 3; k++) {
+            b1[k] = p[1][k] - p[0][k];
+            b2[k] = p[2][k] - p[1][k];
+            b3[k] = p[3][k] - p[2][k];
+        }
+        if (v.pbc) {
+            pbc_exact(b1[0], b1[1], b1[2], bx);
+            pbc_exact(b2[0], b2[1], b2[2], bx);
+            pbc_exact(b3[0], b3[1], b3[2], bx);
+        }
+        double t[3] = {b1[1] * b2[2] - b1[2] * b2[1], b1[2] * b2[0] - b1[0] * b2[2], b1[0] * b2[1] - b1[1] * b2[0]};
+        double u[3] = {b2[1] * b3[2] - b2[2] * b3[1], b2[2] * b3[0] - b2[0] * b3[2], b2[0] * b3[1] - b2[1] * b3[0]};
+        double d = sqrt(b2[0] * b2[0] + b2[1] * b2[1] + b2[2] * b2[2]);
+        double t2 = t[0] * t[0] + t[1] * t[1] + t[2] * t[2];
+        double u2 = u[0] * u[0] + u[1] * u[1] + u[2] * u[2];
+        val = atan2(d * (b1[0] * u[0] + b1[1] * u[1] + b1[2] * u[2]), t[0] * u[0] + t[1] * u[1] + t[2] * u[2]);
+        double iT = t2 > 1e-10 ? 1.0 / t2 : 0.0, iU = u2 > 1e-10 ? 1.0 / u2 : 0.0, iD = d > 1e-10 ? 1.0 / d : 0.0;
+        double c1 = (b1[0] * b2[0] + b1[1] * b2[1] + b1[2] * b2[2]) * iD;
+        double c2 = (b3[0] * b2[0] + b3[1] * b2[1] + b3[2] * b2[2]) * iD;
+        double aT = d * iT, aU = d * iU, bT = (d + c1) * iT, bU = c2 * iU;
+        double ja[3], jb[3], jd[3];
+        for (int k = 0; k < 3; k++) {
+            ja[k] = -aT * t[k];
+            jd[k] = aU * u[k];
+            jb[k] = bT * t[k] + bU * u[k];
+        }
+        store_row(v, w, it.row0, ja[0], ja[1], ja[2]);
+        store_row(v, w, it.row0 + 1, jb[0], jb[1], jb[2]);
+        store_row(v, w, it.row0 + 2, -(ja[0] + jb[0] + jd[0]), -(ja[1] + jb[1] + jd[1]), -(ja[2] + jb[2] + jd[2]));
+        store_row(v, w, it.row0 + 3, jd[0], jd[1], jd[2]);
+    } else if (it.type == CV_POSITION) {
+        val = p[0][it.comp];
+        store_row(v, w, it.row0, it.comp == 0, it.comp == 1, it.comp == 2);
+    }
+    cvv[it.value] = val;
+}
+
+void launch_cv_small(const StepView& v, const SmallCVItem* items, int n, int w0, int nw, hipStream_t s)
+{
+    if (n <= 0) return;
+    dim3 g((n + 63) / 64, nw);
+    hipLaunchKernelGGL(k_cv_small, g, dim3(64), 0, s, v, items, n, w0);
+}
+
+// ---------------------------------------------------------------------------
+// Largest eigenpair of Horn's 4x4 key matrix by Newton on its characteristic
+// polynomial, eigenvector from the adjugate of (K - l I).  FP64, one thread.
+// S[a*3+b] = sum_i y_ia x_ib (centred reference y, centred walker x).
+// P(l) = l^4 + e2 l^2 - e3 l + e4 with e2 = -tr(K^2)/2, e3 = tr(K^3)/3,
+// e4 = det K (tr K = 0); Newton from (Gx+Gy)/2 >= lmax descends monotonically.
+// R (row-major) maps the reference onto the walker: R y ~ x.
+// ---------------------------------------------------------------------------
+__device__ inline double det3(double a, double b, double c, double d, double e, double f, double g, double h, double i)
+{
+    return a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
+}
+
+__device__ void horn_rotation(const double* S, double Gx, double Gy, double* R, double* lmax)
+{
+    const double Sxx = S[0], Sxy = S[1], Sxz = S[2], Syx = S[3], Syy = S[4], Syz = S[5], Szx = S[6], Szy = S[7],
+                 Szz = S[8];
+    double K[4][4];
+    K[0][0] = Sxx + Syy + Szz;
+    K[0][1] = K[1][0] = Syz - Szy;
+    K[0][2] = K[2][0] = Szx - Sxz;
+    K[0][3] = K[3][0] = Sxy - Syx;
+    K[1][1] = Sxx - Syy - Szz;
+    K[1][2] = K[2][1] = Sxy + Syx;
+    K[1][3] = K[3][1] = Szx + Sxz;
+    K[2][2] = -Sxx + Syy - Szz;
+    K[2][3] = K[3][2] = Syz + Szy;
+    K[3][3] = -Sxx - Syy + Szz;
+    double K2[4][4];
+    double trK2 = 0.0, trK3 = 0.0;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            double s = 0.0;
+            for (int k = 0; k < 4; k++) s += K[i][k] * K[k][j];
+            K2[i][j] = s;
+        }
+    for (int i = 0; i < 4; i++) {
+        trK2 += K2[i][i];
+        for (int k = 0; k < 4; k++) trK3 += K2[i][k] * K[k][i];
+    }
+    // det K by cofactor expansion along row 0
+    double detK = 0.0;
+    for (int c = 0; c < 4; c++) {
+        int c0 = c == 0 ? 1 : 0, c1 = c <= 1 ? 2 : 1, c2 = c <= 2 ? 3 : 2;
+        double m = det3(K[1][c0], K[1][c1], K[1][c2], K[2][c0], K[2][c1], K[2][c2], K[3][c0], K[3][c1], K[3][c2]);
+        detK += ((c & 1) ? -1.0 : 1.0) * K[0][c] * m;
+    }
+    const double e2 = -0.5 * trK2, e3 = trK3 / 3.0, e4 = detK;
+    double l = 0.5 * (Gx + Gy);
+    for (int it = 0; it < 60; it++) {
+        double l2 = l * l;
+        double f = (l2 + e2) * l2 - e3 * l + e4;
+        double df = 4.0 * l2 * l + 2.0 * e2 * l - e3;
+        if (!(fabs(df) > 0.0)) break;
+        double dl = f / df;
+        l -= dl;
+        if (fabs(dl) <= 1e-15 * fabs(l) + 1e-300) break;
+    }
+    *lmax = l;
+    double A[4][4];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) A[i][j] = K[i][j] - (i == j ? l : 0.0);
+    // adjugate columns; adj(A)[i][j] = (-1)^(i+j) * minor(j, i)
+    double best = -1.0, q[4] = {1.0, 0.0, 0.0, 0.0};
+    for (int j = 0; j < 4; j++) {
+        double col[4], nn = 0.0;
+        for (int i = 0; i < 4; i++) {
+            // minor deleting row j and column i
+            int r[3], c[3];
+            for (int a = 0, k = 0; a < 4; a++)
+                if (a != j) r[k++] = a;
+            for (int a = 0, k = 0; a < 4; a++)
+                if (a != i) c[k++] = a;
+            double m = det3(A[r[0]][c[0]], A[r[0]][c[1]], A[r[0]][c[2]], A[r[1]][c[0]], A[r[1]][c[1]], A[r[1]][c[2]],
+                            A[r[2]][c[0]], A[r[2]][c[1]], A[r[2]][c[2]]);
+            col[i] = (((i + j) & 1) ? -1.0 : 1.0) * m;
+            nn += col[i] * col[i];
+        }
+        if (nn > best) {
+            best = nn;
+            for (int i = 0; i < 4; i++) q[i] = col[i];
+        }
+    }
+    double qn = sqrt(best > 0.0 ? best : 1.0);
+    if (!(best > 1e-300)) {
+        q[0] = 1.0;
+        q[1] = q[2] = q[3] = 0.0;
+        qn = 1.0;
+    }
+    const double q0 = q[0] / qn, qx = q[1] / qn, qy = q[2] / qn, qz = q[3] / qn;
+    R[0] = q0 * q0 + qx * qx - qy * qy - qz * qz;
+    R[1] = 2 * (qx * qy - q0 * qz);
+    R[2] = 2 * (qx * qz + q0 * qy);
+    R[3] = 2 * (qy * qx + q0 * qz);
+    R[4] = q0 * q0 - qx * qx + qy * qy - qz * qz;
+    R[5] = 2 * (qy * qz - q0 * qx);
+    R[6] = 2 * (qz * qx - q0 * qy);
+    R[7] = 2 * (qz * qy + q0 * qx);
+    R[8] = q0 * q0 - qx * qx - qy * qy + qz * qz;
+}
+
+// ---------------------------------------------------------------------------
+// Group CVs, one 256-thread workgroup per (item, walker).
+//   gyration :556-639 (COM without PBC, displacement with PBC)
+//   COM distance :456-539
+//   RMSD :810-881 (no fit) and the Kabsch extension (fit: split == 1)
+//   DRMSD :1130-1224, contact map :1236-1323
+// groupSlots: atom slots (pairs: 2 per pair); groupMass: mass per listed atom;
+// groupRef: RMSD reference xyz / DRMSD d0 / contact-map [r0, n, m, w].
+// ---------------------------------------------------------------------------
+constexpr int GROUP_BLOCK = 256;
+
+__device__ inline float ipowf(float x, int n)
+{
+    float r = 1.0f;
+    for (int i = 0; i < n; i++) r *= x;
+    return r;
+}
+
+__global__ __launch_bounds__(GROUP_BLOCK) void k_cv_group(StepView v, const GroupCVItem* __restrict__ items,
+                                                          const int* __restrict__ slots,
+                                                          const float* __restrict__ mass,
+                                                          const float* __restrict__ ref, int w0)
+{
+    __shared__ double sh[16 * 16];
+    __shared__ double shR[9];
+    const int w = w0 + blockIdx.y;
+    const GroupCVItem it = items[blockIdx.x];
+    const double* bx = v.box + (size_t)w * 12;
+    double* cvv = v.cvVal + (size_t)w * v.V;
+    const int tid = threadIdx.x, nt = blockDim.x;
+
+    if (it.type == CV_GYRATION || it.type == CV_COM_DISTANCE) {
+        const int n = it.end - it.begin;
+        const int n1 = it.type == CV_COM_DISTANCE ? it.split : n;
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // m*x (g1), M1, m*x (g2), M2
+        for (int j = tid; j < n; j += nt) {
+            float4 q = load_pos(v, w, slots[it.begin + j]);
+            double m = mass[it.begin + j];
+            int o = j < n1 ? 0 : 4;
+            acc[o + 0] += m * q.x;
+            acc[o + 1] += m * q.y;
+            acc[o + 2] += m * q.z;
+            acc[o + 3] += m;
+        }
+        block_sum<8>(acc, sh);
+        if (it.type == CV_GYRATION) {
+            const double M = acc[3];
+            const double cx = acc[0] / M, cy = acc[1] / M, cz = acc[2] / M;
+            double r2[1] = {0.0};
+            for (int j = tid; j < n; j += nt) {
+                float4 q = load_pos(v, w, slots[it.begin + j]);
+                double dx = q.x - cx, dy = q.y - cy, dz = q.z - cz;
+                if (v.pbc) pbc_exact(dx, dy, dz, bx);
+                r2[0] += (double)mass[it.begin + j] * (dx * dx + dy * dy + dz * dz);
+            }
+            block_sum<1>(r2, sh);
+            const double rg = sqrt(r2[0] / M);
+            const double inv = rg > 1e-10 ? 1.0 / rg : 0.0;
+            for (int j = tid; j < n; j += nt) {
+                float4 q = load_pos(v, w, slots[it.begin + j]);
+                double dx = q.x - cx, dy = q.y - cy, dz = q.z - cz;
+                if (v.pbc) pbc_exact(dx, dy, dz, bx);
+                double s = (double)mass[it.begin + j] / M * inv;
+                store_row(v, w, it.row0 + j, s * dx, s * dy, s * dz);
+            }
+            if (tid == 0) cvv[it.value] = rg;
+        } else {
+            double dx = acc[4] / acc[7] - acc[0] / acc[3];
+            double dy = acc[5] / acc[7] - acc[1] / acc[3];
+            double dz = acc[6] / acc[7] - acc[2] / acc[3];
+            if (v.pbc) pbc_exact(dx, dy, dz, bx);
+            double d = sqrt(dx * dx + dy * dy + dz * dz);
+            double inv = d > 1e-10 ? 1.0 / d : 0.0;
+            for (int j = tid; j < n; j += nt) {
+                double m = mass[it.begin + j];
+                double s = (j < n1 ? -m / acc[3] : m / acc[7]) * inv;
+                store_row(v, w, it.row0 + j, s * dx, s * dy, s * dz);
+            }
+            if (tid == 0) cvv[it.value] = d;
+        }
+        return;
+    }
+
+    if (it.type == CV_RMSD) {
+        const int n = it.end - it.begin;
+        const float* y = ref + 3 * (size_t)it.refOff;
+        if (!it.split) {
+            double s[1] = {0.0};
+            for (int j = tid; j < n; j += nt) {
+                float4 q = load_pos(v, w, slots[it.begin + j]);
+                double dx = (double)q.x - y[3 * j], dy = (double)q.y - y[3 * j + 1], dz = (double)q.z - y[3 * j + 2];
+                s[0] += dx * dx + dy * dy + dz * dz;
+            }
+            block_sum<1>(s, sh);
+            const double r = sqrt(s[0] / n);
+            const double sc = (r > 1e-10 ? 1.0 / r : 0.0) / n;
+            for (int j = tid; j < n; j += nt) {
+                float4 q = load_pos(v, w, slots[it.begin + j]);
+                double dx = (double)q.x - y[3 * j], dy = (double)q.y - y[3 * j + 1], dz = (double)q.z - y[3 * j + 2];
+                store_row(v, w, it.row0 + j, dx * sc, dy * sc, dz * sc);
+            }
+            if (tid == 0) cvv[it.value] = r;
+            return;
+        }
+        // Kabsch: centroid, covariance, Horn eigen-solve, gradient rows.
+        double c[3] = {0, 0, 0};
+        for (int j = tid; j < n; j += nt) {
+            float4 q = load_pos(v, w, slots[it.begin + j]);
+            c[0] += q.x;
+            c[1] += q.y;
+            c[2] += q.z;
+        }
+        block_sum<3>(c, sh);
+        const double xc = c[0] / n, yc0 = c[1] / n, zc = c[2] / n;
+        const double* yc = it.refCentroid;
+        double m[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+        for (int j = tid; j < n; j += nt) {
+            float4 q = load_pos(v, w, slots[it.begin + j]);
+            double x0 = q.x - xc, x1 = q.y - yc0, x2 = q.z - zc;
+            double y0 = y[3 * j] - yc[0], y1 = y[3 * j + 1] - yc[1], y2 = y[3 * j + 2] - yc[2];
+            m[0] += y0 * x0;
+            m[1] += y0 * x1;
+            m[2] += y0 * x2;
+            m[3] += y1 * x0;
+            m[4] += y1 * x1;
+            m[5] += y1 * x2;
+            m[6] += y2 * x0;
+            m[7] += y2 * x1;
+            m[8] += y2 * x2;
+            m[9] += x0 * x0 + x1 * x1 + x2 * x2;
+            m[10] += y0 * y0 + y1 * y1 + y2 * y2;
+        }
+        block_sum<11>(m, sh);
+        __shared__ double shScale;
+        if (tid == 0) {
+            double R[9], l;
+            horn_rotation(m, m[9], m[10], R, &l);
+            double msd = (m[9] + m[10] - 2.0 * l) / n;
+            if (msd < 0.0) msd = 0.0;
+            double r = sqrt(msd);
+            for (int k = 0; k < 9; k++) shR[k] = R[k];
+            shScale = (r > 1e-10 ? 1.0 / r : 0.0) / n;
+            cvv[it.value] = r;
+        }
+        __syncthreads();
+        const double sc = shScale;
+        for (int j = tid; j < n; j += nt) {
+            float4 q = load_pos(v, w, slots[it.begin + j]);
+            double x0 = q.x - xc, x1 = q.y - yc0, x2 = q.z - zc;
+            double y0 = y[3 * j] - yc[0], y1 = y[3 * j + 1] - yc[1], y2 = y[3 * j + 2] - yc[2];
+            double r0 = shR[0] * y0 + shR[1] * y1 + shR[2] * y2;
+            double r1 = shR[3] * y0 + shR[4] * y1 + shR[5] * y2;
+            double r2 = shR[6] * y0 + shR[7] * y1 + shR[8] * y2;
+            store_row(v, w, it.row0 + j, (x0 - r0) * sc, (x1 - r1) * sc, (x2 - r2) * sc);
+        }
+        return;
+    }
+
+    if (it.type == CV_DRMSD) {
+        const int np = (it.end - it.begin) / 2;
+        const float* d0 = ref + it.refOff;
+        double s[1] = {0.0};
+        for (int p = tid; p < np; p += nt) {
+            float4 a = load_pos(v, w, slots[it.begin + 2 * p]);
+            float4 b = load_pos(v, w, slots[it.begin + 2 * p + 1]);
+            double dx = (double)b.x - a.x, dy = (double)b.y - a.y, dz = (double)b.z - a.z;
+            if (v.pbc) pbc_exact(dx, dy, dz, bx);
+            double dev = sqrt(dx * dx + dy * dy + dz * dz) - (double)d0[p];
+            s[0] += dev * dev;
+        }
+        block_sum<1>(s, sh);
+        const double val = sqrt(s[0] / np);
+        const double inv = val > 1e-10 ? 1.0 / val : 0.0;
+        for (int p = tid; p < np; p += nt) {
+            float4 a = load_pos(v, w, slots[it.begin + 2 * p]);
+            float4 b = load_pos(v, w, slots[it.begin + 2 * p + 1]);
+            double dx = (double)b.x - a.x, dy = (double)b.y - a.y, dz = (double)b.z - a.z;
+            if (v.pbc) pbc_exact(dx, dy, dz, bx);
+            double d = sqrt(dx * dx + dy * dy + dz * dz);
+            double coeff = (d - (double)d0[p]) / np * inv * (d > 1e-10 ? 1.0 / d : 0.0);
+            store_row(v, w, it.row0 + 2 * p, -coeff * dx, -coeff * dy, -coeff * dz);
+            store_row(v, w, it.row0 + 2 * p + 1, coeff * dx, coeff * dy, coeff * dz);
+        }
+        if (tid == 0) cvv[it.value] = val;
+        return;
+    }
+
+    if (it.type == CV_CONTACTMAP) {
+        const int np = (it.end - it.begin) / 2;
+        const float* prm = ref + it.refOff;
+        double s[1] = {0.0};
+        for (int p = tid; p < np; p += nt) {
+            float4 a = load_pos(v, w, slots[it.begin + 2 * p]);
+            float4 b = load_pos(v, w, slots[it.begin + 2 * p + 1]);
+            double dxd = (double)b.x - a.x, dyd = (double)b.y - a.y, dzd = (double)b.z - a.z;
+            if (v.pbc) pbc_exact(dxd, dyd, dzd, bx);
+            float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
+            float r0 = prm[4 * p], wgt = prm[4 * p + 3];
+            int nn = (int)prm[4 * p + 1], mm = (int)prm[4 * p + 2];
+            float d = sqrtf(dx * dx + dy * dy + dz * dz);
+            float invd = d > 1e-10f ? 1.0f / d : 0.0f;
+            float ir0 = 1.0f / r0, x = d * ir0;
+            float xn = ipowf(x, nn), xm = ipowf(x, mm);
+            float num = 1.0f - xn, den = 1.0f - xm, sw, dsdd;
+            if (fabsf(den) < 1e-8f) {
+                sw = (float)nn / (float)mm;
+                dsdd = 0.0f;
+            } else {
+                sw = num / den;
+                float dxn = nn > 0 ? nn * ipowf(x, nn - 1) : 0.0f;
+                float dxm = mm > 0 ? mm * ipowf(x, mm - 1) : 0.0f;
+                dsdd = ((-dxn * den + dxm * num) / (den * den)) * ir0;
+            }
+            s[0] += (double)(wgt * sw);
+            float coeff = wgt * dsdd * invd;
+            store_row(v, w, it.row0 + 2 * p, -coeff * dx, -coeff * dy, -coeff * dz);
+            store_row(v, w, it.row0 + 2 * p + 1, coeff * dx, coeff * dy, coeff * dz);
+        }
+        block_sum<1>(s, sh);
+        if (tid == 0) cvv[it.value] = s[0];
+        return;
+    }
+}
+
+void launch_cv_group(const StepView& v, const GroupCVItem* items, int n, const int* slots, const float* mass,
+                     const float* ref, int w0, int nw, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cv_group, dim3(n, nw), dim3(GROUP_BLOCK), 0, s, v, items, slots, mass, ref, w0);
+}
+
+// ---------------------------------------------------------------------------
+// Coordination, CommonGluedKernels.cpp:899-1031, with an optional cutoff.
+//
+// Pair enumeration: a cell list per group (cells of edge >= cutoff, at least
+// 3 per dimension, orthorhombic box) or, without a usable cell list, the whole
+// partner group split into 32 chunks.  Each "home" atom owns 32 lanes of a
+// wave; lane j scans the partner atoms of neighbour cell j (27 used).
+// Two passes give deterministic per-atom Jacobian rows without atomics:
+// pass A (home = group A) also sums CN; pass B (home = group B) fills B rows.
+// The pair predicate r^2 < rc^2 is evaluated in FP64 exactly as the oracle;
+// an FP32 pre-check with a 1e-5 relative margin skips far pairs cheaply.
+// ---------------------------------------------------------------------------
+
+
+__device__ inline int cell_of(double x, double y, double z, const double* bx, const int* nc)
+{
+    double f[3] = {x * bx[9], y * bx[10], z * bx[11]};
+    int c[3];
+    for (int k = 0; k < 3; k++) {
+        double t = f[k] - floor(f[k]);
+        int ci = (int)(t * nc[k]);
+        c[k] = ci < 0 ? 0 : (ci >= nc[k] ? nc[k] - 1 : ci);
+    }
+    return (c[2] * nc[1] + c[1]) * nc[0] + c[0];
+}
+
+__global__ void k_coord_zero(CoordBuffers cb, int base, int n, int w0)
+{
+    const int w = w0 + blockIdx.y;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        cb.cellCount[(size_t)w * cb.cellStride + base + i] = 0;
+}
+
+__global__ void k_coord_bin(StepView v, CoordBuffers cb, const CoordItem* items, int item, int w0)
+{
+    const int w = w0 + blockIdx.y;
+    const CoordItem it = items[item];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = it.nA + it.nB;
+    if (i >= n) return;
+    const int slot = cb.slots[(i < it.nA) ? it.aBegin + i : it.bBegin + (i - it.nA)];
+    float4 q = load_pos(v, w, slot);
+    const int ncell = it.nc[0] * it.nc[1] * it.nc[2];
+    int c = cell_of(q.x, q.y, q.z, v.box + (size_t)w * 12, it.nc) + (i < it.nA ? 0 : ncell);
+    cb.cellOf[(size_t)w * cb.cellOfStride + it.sortBase + i] = c;
+    atomicAdd(&cb.cellCount[(size_t)w * cb.cellStride + it.cellBase + c], 1);
+}
+
+// exclusive scan of the 2*ncell counts of one item; one workgroup per walker
+__global__ void k_coord_scan(CoordBuffers cb, const CoordItem* items, int item, int w0)
+{
+    __shared__ int sh[1024];
+    const int w = w0 + blockIdx.y;
+    const CoordItem it = items[item];
+    const int n = 2 * it.nc[0] * it.nc[1] * it.nc[2];
+    int* cnt = cb.cellCount + (size_t)w * cb.cellStride + it.cellBase;
+    int* st = cb.cellStart + (size_t)w * (cb.cellStride + cb.nItems) + it.cellBase + item;
+    int carry = 0;
+    for (int base = 0; base < n; base += blockDim.x) {
+        int i = base + threadIdx.x;
+        int x = i < n ? cnt[i] : 0;
+        sh[threadIdx.x] = x;
+        __syncthreads();
+        for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+            int t = threadIdx.x >= (unsigned)o ? sh[threadIdx.x - o] : 0;
+            __syncthreads();
+            sh[threadIdx.x] += t;
+            __syncthreads();
+        }
+        if (i < n) {
+            st[i] = carry + sh[threadIdx.x] - x;
+            cnt[i] = 0;  // reused as the fill cursor
+        }
+        int tot = sh[blockDim.x - 1];
+        __syncthreads();
+        carry += tot;
+    }
+    if (threadIdx.x == 0) st[n] = carry;
+}
+
+__global__ void k_coord_fill(StepView v, CoordBuffers cb, const CoordItem* items, int item, int w0)
+{
+    const int w = w0 + blockIdx.y;
+    const CoordItem it = items[item];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = it.nA + it.nB;
+    if (i >= n) return;
+    const int slot = cb.slots[(i < it.nA) ? it.aBegin + i : it.bBegin + (i - it.nA)];
+    float4 q = load_pos(v, w, slot);
+    const int c = cb.cellOf[(size_t)w * cb.cellOfStride + it.sortBase + i];
+    const int* st = cb.cellStart + (size_t)w * (cb.cellStride + cb.nItems) + it.cellBase + item;
+    int pos = st[c] + atomicAdd(&cb.cellCount[(size_t)w * cb.cellStride + it.cellBase + c], 1);
+    const int local = i < it.nA ? i : i - it.nA;
+    cb.sorted[(size_t)w * cb.sortStride + it.sortBase + pos] = make_float4(q.x, q.y, q.z, __int_as_float(local));
+}
+
+struct SwitchF {
+    float r0, rn, rm, ir0;
+    float preRes, preDfunc, preSecDev;
+    int intPow, in, im;
+};
+
+__device__ inline SwitchF make_switch(const CoordItem& it)
+{
+    SwitchF s;
+    s.r0 = it.r0;
+    s.rn = it.rn;
+    s.rm = it.rm;
+    s.ir0 = 1.0f / it.r0;
+    s.preRes = s.rn / s.rm;
+    s.preDfunc = 0.5f * s.rn * (s.rn - s.rm) / s.rm;
+    s.preSecDev = s.rn * (s.rm * s.rm - 3.0f * s.rm * (s.rn - 1.0f) + s.rn * (2.0f * s.rn - 3.0f)) / (6.0f * s.rm);
+    s.intPow = it.intPow;
+    s.in = (int)s.rn;
+    s.im = (int)s.rm;
+    return s;
+}
+
+// f and df/dr of the rational switch (1-x^n)/(1-x^m), reference branch order.
+__device__ inline void switch_eval(const SwitchF& s, float r, float& f, float& dfdr)
+{
+    const float x = r * s.ir0;
+    const float dx1 = x - 1.0f;
+    if (fabsf(dx1) < 1e-4f) {
+        f = s.preRes + dx1 * (s.preDfunc + 0.5f * dx1 * s.preSecDev);
+        dfdr = (s.preDfunc + dx1 * s.preSecDev) * s.ir0;
+    } else if (x < 1e-10f) {
+        f = 1.0f;
+        dfdr = 0.0f;
+    } else {
+        float xn, xm;
+        if (s.intPow) {
+            xn = ipowf(x, s.in);
+            xm = ipowf(x, s.im);
+        } else {
+            xn = powf(x, s.rn);
+            xm = powf(x, s.rm);
+        }
+        const float den = 1.0f - xm, ix = 1.0f / x;
+        f = (1.0f - xn) / den;
+        const float dfx = (s.rm * xm * ix - s.rn * xn * ix + (s.rn - s.rm) * xn * xm * ix) / (den * den);
+        dfdr = dfx * s.ir0;
+    }
+}
+
+constexpr int COORD_BLOCK = 256;  // 8 home atoms x 32 lanes
+
+template <bool SIDE_A>
+__global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBuffers cb,
+                                                            const CoordItem* __restrict__ items, int item, int w0)
+{
+    __shared__ double shC[COORD_BLOCK / 64];
+    __shared__ unsigned long long shP[COORD_BLOCK / 64];
+    __shared__ int shLast;
+    const int w = w0 + blockIdx.y;
+    const CoordItem it = items[item];
+    const int nHome = SIDE_A ? it.nA : it.nB;
+    const int nPart = SIDE_A ? it.nB : it.nA;
+    const int home = blockIdx.x * (COORD_BLOCK / 32) + (threadIdx.x >> 5);
+    const int lane = threadIdx.x & 31;
+    const double* bx = v.box + (size_t)w * 12;
+    float bxf[12];
+    for (int k = 0; k < 12; k++) bxf[k] = (float)bx[k];
+    const SwitchF sw = make_switch(it);
+    const float rc2f = (float)(it.rc2 * (1.0 + 1e-5));
+
+    double cn = 0.0;
+    unsigned long long np = 0;
+    float gx = 0.f, gy = 0.f, gz = 0.f;
+    if (home < nHome) {
+        const int hslot = cb.slots[SIDE_A ? it.aBegin + home : it.bBegin + home];
+        const float4 ph = load_pos(v, w, hslot);
+        int pBegin = 0, pEnd = 0;
+        const float4* part = cb.sorted + (size_t)w * cb.sortStride + it.sortBase + (SIDE_A ? it.nA : 0);
+        if (it.useCells) {
+            if (lane < 27) {
+                const int ncell = it.nc[0] * it.nc[1] * it.nc[2];
+                const int hc = cb.cellOf[(size_t)w * cb.cellOfStride + it.sortBase + (SIDE_A ? home : it.nA + home)] -
+                               (SIDE_A ? 0 : ncell);
+                int cx = hc % it.nc[0], cy = (hc / it.nc[0]) % it.nc[1], cz = hc / (it.nc[0] * it.nc[1]);
+                cx = (cx + (lane % 3) - 1 + it.nc[0]) % it.nc[0];
+                cy = (cy + ((lane / 3) % 3) - 1 + it.nc[1]) % it.nc[1];
+                cz = (cz + (lane / 9) - 1 + it.nc[2]) % it.nc[2];
+                const int pc = (cz * it.nc[1] + cy) * it.nc[0] + cx + (SIDE_A ? ncell : 0);
+                const int* st = cb.cellStart + (size_t)w * (cb.cellStride + cb.nItems) + it.cellBase + item;
+                const int base = SIDE_A ? st[ncell] : 0;  // partner atoms start after the other group
+                pBegin = st[pc] - base;
+                pEnd = st[pc + 1] - base;
+            }
+        } else {
+            const int chunk = (nPart + 31) / 32;
+            pBegin = lane * chunk;
+            pEnd = min(nPart, pBegin + chunk);
+        }
+        for (int j = pBegin; j < pEnd; j++) {
+            const float4 pp = it.useCells ? part[j] : load_pos(v, w, cb.slots[SIDE_A ? it.bBegin + j : it.aBegin + j]);
+            // dr = r_A - r_B (reference order) in both passes
+            const float4 pa = SIDE_A ? ph : pp;
+            const float4 pb = SIDE_A ? pp : ph;
+            if (it.useCells) {  // cells guarantee box >= 3 cutoffs: the FP32 image is never ambiguous
+                float fx = pa.x - pb.x, fy = pa.y - pb.y, fz = pa.z - pb.z;
+                if (v.pbc) pbc_f(fx, fy, fz, bxf);
+                if (!(fx * fx + fy * fy + fz * fz < rc2f)) continue;
+            }
+            double dx = (double)pa.x - (double)pb.x, dy = (double)pa.y - (double)pb.y,
+                   dz = (double)pa.z - (double)pb.z;
+            if (v.pbc) pbc_exact(dx, dy, dz, bx);
+            const double r2 = dot3_exact(dx, dy, dz);
+            if (r2 < 1e-20) continue;
+            if (it.rc2 > 0.0 && !(r2 < it.rc2)) continue;
+            const float r = (float)sqrt(r2);
+            if (r * sw.ir0 > 1000.0f) continue;
+            float f, dfdr;
+            switch_eval(sw, r, f, dfdr);
+            const float s = dfdr / r;
+            if (SIDE_A) {
+                cn += (double)f;
+                np++;
+                gx += s * (float)dx;
+                gy += s * (float)dy;
+                gz += s * (float)dz;
+            } else {
+                gx -= s * (float)dx;
+                gy -= s * (float)dy;
+                gz -= s * (float)dz;
+            }
+        }
+    }
+    // reduce the 32 lanes of each home atom
+#pragma unroll
+    for (int o = 16; o > 0; o >>= 1) {
+        gx += __shfl_xor(gx, o, 32);
+        gy += __shfl_xor(gy, o, 32);
+        gz += __shfl_xor(gz, o, 32);
+    }
+    if (home < nHome && lane == 0) store_row(v, w, (SIDE_A ? it.rowA : it.rowB) + home, gx, gy, gz);
+    if (!SIDE_A) return;
+
+    // CN and pair count: block partials, last block sums them in block order.
+    cn = wave_sum(cn);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) np += __shfl_xor(np, o, 64);
+    if ((threadIdx.x & 63) == 0) {
+        shC[threadIdx.x >> 6] = cn;
+        shP[threadIdx.x >> 6] = np;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double c = 0.0;
+        unsigned long long p = 0;
+        for (int i = 0; i < COORD_BLOCK / 64; i++) {
+            c += shC[i];
+            p += shP[i];
+        }
+        cb.partCN[(size_t)w * cb.partStride + it.partBase + blockIdx.x] = c;
+        cb.partPairs[(size_t)w * cb.partStride + it.partBase + blockIdx.x] = p;
+        __threadfence();
+        unsigned int done = atomicAdd(&cb.counters[(size_t)w * cb.nItems + item], 1u);
+        shLast = (done == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (shLast && threadIdx.x == 0) {
+        __threadfence();
+        double c = 0.0;
+        unsigned long long p = 0;
+        for (unsigned b = 0; b < gridDim.x; b++) {
+            c += __hip_atomic_load(&cb.partCN[(size_t)w * cb.partStride + it.partBase + b], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            p += __hip_atomic_load(&cb.partPairs[(size_t)w * cb.partStride + it.partBase + b], __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+        v.cvVal[(size_t)w * v.V + it.value] = c;
+        cb.npairs[(size_t)w * cb.nItems + item] = (long long)p;
+        cb.counters[(size_t)w * cb.nItems + item] = 0u;  // self-reset for the next step
+    }
+}
+
+int coord_blocks_A(int nA) { return (nA + COORD_BLOCK / 32 - 1) / (COORD_BLOCK / 32); }
+
+void launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordItem* dItems, const CoordItem& hIt,
+                         int item, int w0, int nw, hipStream_t s)
+{
+    const int n = hIt.nA + hIt.nB;
+    if (hIt.useCells) {
+        const int ncell2 = 2 * hIt.nc[0] * hIt.nc[1] * hIt.nc[2];
+        hipLaunchKernelGGL(k_coord_zero, dim3((ncell2 + 255) / 256, nw), dim3(256), 0, s, cb, hIt.cellBase, ncell2,
+                           w0);
+        hipLaunchKernelGGL(k_coord_bin, dim3((n + 255) / 256, nw), dim3(256), 0, s, v, cb, dItems, item, w0);
+        hipLaunchKernelGGL(k_coord_scan, dim3(1, nw), dim3(1024), 0, s, cb, dItems, item, w0);
+        hipLaunchKernelGGL(k_coord_fill, dim3((n + 255) / 256, nw), dim3(256), 0, s, v, cb, dItems, item, w0);
+    }
+    hipLaunchKernelGGL(k_coord_pairs<true>, dim3(coord_blocks_A(hIt.nA), nw), dim3(COORD_BLOCK), 0, s, v, cb, dItems,
+                       item, w0);
+    hipLaunchKernelGGL(k_coord_pairs<false>, dim3(coord_blocks_A(hIt.nB), nw), dim3(COORD_BLOCK), 0, s, v, cb,
+                       dItems, item, w0);
+}
+
+}  // namespace glued
+
+namespace glued {
+
+// ---------------------------------------------------------------------------
+// Path CV (s, z), MSD metric, frames 1-indexed, no PBC:
+// CommonGluedKernels.cpp:667-794.  One workgroup per (path, walker).
+// Pass 1: msd_f for every frame, with (frame, part) thread tiles; x - r is
+// formed in FP64 (exact for FP32 inputs) and accumulated in FP64.
+// Pass 2: Jacobian rows per atom (s row, z row interleaved, :781-791).
+// No frame cap (the reference stops at 128 frames, :3901).
+// ---------------------------------------------------------------------------
+constexpr int PATH_BLOCK = 256;
+
+__global__ __launch_bounds__(PATH_BLOCK) void k_cv_path(StepView v, const PathItem* __restrict__ items,
+                                                        const int* __restrict__ slots,
+                                                        const float* __restrict__ frames, int w0)
+{
+    __shared__ double shPart[PATH_BLOCK];
+    const int w = w0 + blockIdx.y;
+    const PathItem it = items[blockIdx.x];
+    const int M = it.M, N = it.N;
+    const float* F = frames + it.frameOff;
+    double S = 0.0, sNum = 0.0;
+    // frames processed in groups of up to PATH_BLOCK/P frames
+    const int P = N >= PATH_BLOCK ? 1 : PATH_BLOCK / N;
+    __shared__ double shW[1024];
+    for (int f0 = 0; f0 < N; f0 += PATH_BLOCK / P) {
+        const int f = f0 + threadIdx.x / P, part = threadIdx.x % P;
+        double acc = 0.0;
+        if (f < N && threadIdx.x / P < PATH_BLOCK / P) {
+            for (int j = part; j < M; j += P) {
+                float4 q = load_pos(v, w, slots[it.begin + j]);
+                const float* r = F + ((size_t)f * M + j) * 3;
+                double dx = (double)q.x - r[0], dy = (double)q.y - r[1], dz = (double)q.z - r[2];
+                acc += dx * dx + dy * dy + dz * dz;
+            }
+        }
+        shPart[threadIdx.x] = acc;
+        __syncthreads();
+        if (part == 0 && f < N && threadIdx.x / P < PATH_BLOCK / P) {
+            double m = 0.0;
+            for (int k = 0; k < P; k++) m += shPart[threadIdx.x + k];
+            m /= M;
+            const double wv = exp(-(double)it.lambda * m);
+            if (f < 1024) shW[f] = wv;
+        }
+        __syncthreads();
+    }
+    // S and sum (f+1) w in frame order by one thread (deterministic, N small)
+    __shared__ double shS, shSn;
+    if (threadIdx.x == 0) {
+        for (int f = 0; f < N && f < 1024; f++) {
+            S += shW[f];
+            sNum += (f + 1) * shW[f];
+        }
+        shS = S;
+        shSn = sNum;
+    }
+    __syncthreads();
+    S = shS;
+    sNum = shSn;
+    const double lam = it.lambda;
+    const double s = S > 1e-300 ? sNum / S : 0.0;
+    const double z = S > 1e-300 ? -log(S) / lam : 0.0;
+    if (threadIdx.x == 0) {
+        v.cvVal[(size_t)w * v.V + it.valueS] = s;
+        v.cvVal[(size_t)w * v.V + it.valueS + 1] = z;
+    }
+    const double scS = -2.0 * lam / (M * S), scZ = 2.0 / (M * S);
+    for (int j = threadIdx.x; j < M; j += blockDim.x) {
+        float4 q = load_pos(v, w, slots[it.begin + j]);
+        double js0 = 0, js1 = 0, js2 = 0, jz0 = 0, jz1 = 0, jz2 = 0;
+        for (int f = 0; f < N; f++) {
+            const float* r = F + ((size_t)f * M + j) * 3;
+            const double wv = shW[f];
+            double dx = (double)q.x - r[0], dy = (double)q.y - r[1], dz = (double)q.z - r[2];
+            const double a = ((f + 1) - s) * wv;
+            js0 += a * dx;
+            js1 += a * dy;
+            js2 += a * dz;
+            jz0 += wv * dx;
+            jz1 += wv * dy;
+            jz2 += wv * dz;
+        }
+        store_row(v, w, it.row0 + 2 * j, scS * js0, scS * js1, scS * js2);
+        store_row(v, w, it.row0 + 2 * j + 1, scZ * jz0, scZ * jz1, scZ * jz2);
+    }
+}
+
+void launch_cv_path(const StepView& v, const PathItem* items, int n, const int* slots, const float* frames, int w0,
+                    int nw, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cv_path, dim3(n, nw), dim3(PATH_BLOCK), 0, s, v, items, slots, frames, w0);
+}
+
+}  // namespace glued

@@ -1,0 +1,169 @@
+/*
+ * glued.h -- C-ABI of the MI355X-native GLUED enhanced-sampling step.
+ *
+ * This is the drop-in boundary for the path "collective-variable evaluation ->
+ * metadynamics / OPES bias evaluation and deposition -> chain-rule force scatter
+ * into a 2^32 fixed-point force buffer".  Each entry point names the
+ * MarvinTaterra/GluedMD interface it replaces (file:line in that repository).
+ *
+ * Conventions
+ *  - Every function returns 0 on success and a nonzero code on failure; the
+ *    message is available from glued_last_error() (thread-local).  No C++
+ *    exception crosses this boundary.  (The reference throws OpenMMException
+ *    from the same validation points.)
+ *  - Device memory is owned by the caller for inputs and outputs: posq is
+ *    float4 (x, y, z, charge) in GPU atom order (OpenMM's layout); forces are
+ *    int64 field-major blocks of 3 * padded entries per walker, 2^32 fixed
+ *    point, accumulated with atomics (OpenMM's long force buffer).
+ *  - All device work is enqueued on the caller's stream (a hipStream_t passed
+ *    as void*; NULL = default stream).  The only host synchronisations are
+ *    the read-backs and the energy return of glued_execute.
+ *  - No host threads are created.  A handle is used by one host thread at a
+ *    time.
+ */
+#ifndef GLUED_H_
+#define GLUED_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct glued_force* glued_handle;
+
+/* CV type codes: openmmapi/include/GluedForce.h:27-73 (same numbers). */
+enum {
+    GLUED_CV_DISTANCE = 1,
+    GLUED_CV_ANGLE = 2,
+    GLUED_CV_DIHEDRAL = 3,
+    GLUED_CV_COM_DISTANCE = 4,
+    GLUED_CV_GYRATION = 5,
+    GLUED_CV_COORDINATION = 6, /* params [r0, n, m] + optional [cutoff] (extension) */
+    GLUED_CV_RMSD = 7,         /* params 3N reference xyz + optional [fit] (1 = Kabsch, extension) */
+    GLUED_CV_PATH = 10,
+    GLUED_CV_POSITION = 11,
+    GLUED_CV_DRMSD = 12,
+    GLUED_CV_CONTACTMAP = 13,
+    GLUED_CV_ENERGY = 23 /* value and Jacobian supplied by glued_set_energy_input */
+};
+
+/* Bias type codes: openmmapi/include/GluedForce.h:76-95 (same numbers), plus
+ * GLUED_BIAS_METAD_HILLS, a hill-list well-tempered metadynamics with no
+ * counterpart in the reference (params [h0, sigma*D, gamma, kT, period*D],
+ * int params [pace, maxHills]; period 0 = non-periodic). */
+enum {
+    GLUED_BIAS_METAD = 3,
+    GLUED_BIAS_OPES = 5,
+    GLUED_BIAS_OPES_EXPANDED = 11,
+    GLUED_BIAS_OPES_MULTITHERMAL = 15,
+    GLUED_BIAS_METAD_HILLS = 101
+};
+
+/* Thread-local message of the last failure. */
+const char* glued_last_error(void);
+
+/* ---- lifecycle and specification -------------------------------------- */
+
+/* Replaces GluedForce::GluedForce (openmmapi/src/GluedForce.cpp:12-14). */
+int glued_create(glued_handle* out);
+/* Replaces the Context-owned kernel's destruction (GluedForceImpl.cpp:31-34). */
+int glued_destroy(glued_handle h);
+/* Replaces GluedForce::setUsesPeriodicBoundaryConditions (GluedForce.h:163). */
+int glued_set_pbc(glued_handle h, int yes);
+/* Replaces GluedForce::setTemperature (GluedForce.cpp:121-124). */
+int glued_set_temperature(glued_handle h, double kelvin);
+/* Replaces GluedForce::addCollectiveVariable (GluedForce.cpp:44-62): returns
+ * the first value index (PATH occupies two). */
+int glued_add_cv(glued_handle h, int type, const int* atoms, int nAtoms, const double* params, int nParams,
+                 int* firstValueIdx);
+/* Replaces GluedForce::addBias (GluedForce.cpp:82-107). */
+int glued_add_bias(glued_handle h, int type, const int* cvIdx, int nCv, const double* params, int nParams,
+                   const int* iparams, int nIparams, int* biasIdx);
+/* Replaces GluedForce::getNumCollectiveVariables (GluedForce.cpp:64-66). */
+int glued_num_cv_values(glued_handle h, int* n);
+/* Replaces GluedForce::setTestBiasGradients (GluedForce.cpp:186-189): inject
+ * dU/dCV, skipping bias evaluation (CommonGluedKernels.cpp:4670-4671). */
+int glued_set_test_bias_gradients(glued_handle h, const double* grads, int n);
+/* Replaces GluedForce::setMultiWalkerPtrs / redirectToPrimaryBias
+ * (GluedForce.cpp:257-270, CommonGluedKernels.cpp:5234-5262): one bias state
+ * shared by every walker of the handle.  Call before initialization. */
+int glued_share_bias(glued_handle h, int biasIdx, int shared);
+
+/* ---- initialization ---------------------------------------------------- */
+
+/* Replaces CommonCalcGluedForceKernel::initialize (CommonGluedKernels.cpp:
+ * 4608-4644) with its validation (:3879-3927).  One walker; masses may be
+ * NULL (all 1). */
+int glued_initialize(glued_handle h, int numParticles, int paddedNumParticles, const double* masses, int nMasses);
+/* Batched form (no reference counterpart; the reference runs one Context per
+ * walker): W walkers of uneven size, laid out back to back in posq / forces
+ * (walker w starts at sum_{v<w} padded_v atoms). */
+int glued_initialize_batch(glued_handle h, int numWalkers, const int* numParticles, const int* paddedNumParticles,
+                           const double* masses, int nMasses);
+/* Replaces rebuildGpuAtomIndices (CommonCVKernels.cpp:113-317). */
+int glued_set_atom_order(glued_handle h, int walker, const int* gpuToUser, int n);
+/* Replaces setPeriodicBoxArgs (CommonGluedKernels.cpp:4676-4734): box vectors
+ * a, b, c (9 doubles, OpenMM reduced form). */
+int glued_set_box(glued_handle h, int walker, const double* box9);
+/* CV_ENERGY input: U and the walker's external force (device FP32,
+ * field-major 3*padded); the energy CV's Jacobian is -F (CommonGluedKernels.cpp:
+ * 2471-2503 copies -F the same way from an inner Context). */
+int glued_set_energy_input(glued_handle h, int walker, double U, const float* forceField, void* stream);
+
+/* ---- the step ------------------------------------------------------------ */
+
+/* Replaces CalcGluedForceKernel::execute (GluedKernels.h:48-50,
+ * CommonGluedKernels.cpp:4646-5048) for one walker; posq and forces point at
+ * that walker's block.  *energy receives the summed bias energy when
+ * includeEnergy (one host synchronisation), else 0. */
+int glued_execute(glued_handle h, void* stream, int walker, const void* posq, int64_t* forces, int includeForces,
+                  int includeEnergy, double* energy);
+/* Batched form: every walker in one launch per stage; useGraph = 1 replays
+ * the step as one captured hipGraph. */
+int glued_execute_batch(glued_handle h, void* stream, const void* posq, int64_t* forces, int includeForces,
+                        int useGraph);
+/* One stage of the batched step for every walker (0 = CVs, 1 = biases,
+ * 2 = dU/dCV + force scatter, 3 = only the streamed hill / OPES-kernel sums),
+ * for per-stage timing; no reference counterpart. */
+int glued_execute_stage(glued_handle h, void* stream, const void* posq, int64_t* forces, int stage);
+/* Replaces CalcGluedForceKernel::updateState (GluedKernels.h:52,
+ * CommonGluedKernels.cpp:5050-5157) with GluedForceImpl's once-per-step guard
+ * (GluedForceImpl.cpp:100-109); all walkers. */
+int glued_update_state(glued_handle h, void* stream, int step);
+
+/* ---- read-backs ---------------------------------------------------------- */
+
+/* Replaces GluedForce::getLastCVValues (GluedForce.cpp:206-213). */
+int glued_get_cv_values(glued_handle h, int walker, double* out, int n);
+/* Per-bias energies of the last evaluation; their sum replaces
+ * GluedForce::getLastBias (GluedForce.cpp:215-222). */
+int glued_get_bias_energies(glued_handle h, int walker, double* out, int n);
+/* dU/dCV of the last evaluation (the reference's cvBiasGradients). */
+int glued_get_dudcv(glued_handle h, int walker, double* out, int n);
+/* Pairs inside the cutoff of the coordIdx-th coordination CV (extension). */
+int glued_get_coordination_pairs(glued_handle h, int walker, int coordIdx, long long* npairs);
+/* Replaces GluedForce::getBiasStateBytes / setBiasStateBytes
+ * (GluedForce.cpp:133-143, CommonSerialization.cpp:20-410), 'GPUS' v1. */
+int glued_get_bias_state(glued_handle h, int walker, char* buf, size_t cap, size_t* size);
+int glued_set_bias_state(glued_handle h, int walker, const char* buf, size_t size);
+/* Replaces GluedForce::getOPESMetrics (CommonGluedKernels.cpp:5164-5210):
+ * out = {zed, rct, nker, neff}. */
+int glued_get_opes_metrics(glued_handle h, int walker, int opesIdx, double* out4);
+/* Replaces GluedForce::getKernelSigmas (CommonGluedKernels.cpp:5212-5232). */
+int glued_get_kernel_sigmas(glued_handle h, int walker, int opesIdx, float* out, int cap, int* n);
+/* Hill list of a GLUED_BIAS_METAD_HILLS bias (extension): centres kernel-major
+ * (n x D), heights n. */
+int glued_set_hills(glued_handle h, int walker, int biasIdx, const double* centers, const double* heights, int n);
+int glued_get_hills(glued_handle h, int walker, int biasIdx, double* centers, double* heights, int cap, int* n);
+/* Whole MetaD grid (the grid section of the state blob). */
+int glued_set_grid(glued_handle h, int walker, int biasIdx, const double* grid, int n);
+/* Plan sizes: Jacobian rows and distinct atoms of one walker. */
+int glued_plan_info(glued_handle h, int* rows, int* atoms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GLUED_H_ */

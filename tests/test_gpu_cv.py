@@ -1,0 +1,145 @@
+"""GPU parity of every CV kind against the FP64 oracle (through the C-ABI),
+with injected dU/dCV as the reference's CV tests do (setTestBiasGradients,
+CommonGluedKernels.cpp:4670-4671), plus the reference tests' closed forms."""
+import math
+
+import numpy as np
+import pytest
+
+from gluedmd_amd.glued import Force
+from tests.gpu_util import assert_close_rel, make_pair, run_both
+
+pytestmark = pytest.mark.gpu
+
+
+def test_distance_reference_cases():
+    f = Force()
+    f.add_distance([0, 1])
+    f.setTestBiasGradients([1.0])
+    g, o = make_pair(f, 3)
+    run_both(g, o, [(0, 0, 0), (0.5, 0, 0), (5, 5, 5)])
+    assert abs(g.getLastCVValues()[0] - 0.5) < 1e-7
+    F = g.get_forces()
+    assert abs(F[0][0] - 1.0) < 1e-9 and abs(F[1][0] + 1.0) < 1e-9
+    fp = Force(pbc=True)
+    fp.add_distance([0, 1])
+    g, o = make_pair(fp, 2, box=1.0)
+    run_both(g, o, [(0.005, 0, 0), (0.995, 0, 0)])
+    assert abs(g.getLastCVValues()[0] - 0.01) < 1e-6
+
+
+@pytest.mark.parametrize("pbc", [False, True])
+def test_small_cvs_random(pbc):
+    rng = np.random.default_rng(1)
+    f = Force(pbc=pbc)
+    f.add_distance([0, 5])
+    f.add_angle([1, 2, 3])
+    f.add_dihedral([4, 6, 8, 9])
+    f.add_dihedral([6, 8, 9, 2])
+    f.add_position(7, 1)
+    f.setTestBiasGradients([0.7, -1.3, 2.1, 0.4, -0.9])
+    g, o = make_pair(f, 10, box=2.5 if pbc else None)
+    for _ in range(3):
+        run_both(g, o, rng.uniform(0, 2.5, size=(10, 3)))
+
+
+def test_coordination_reference_cases():
+    f = Force()
+    f.add_coordination([0], [1], r0=1.5, n=1, m=2)
+    f.setTestBiasGradients([1.0])
+    g, o = make_pair(f, 3)
+    run_both(g, o, [(0, 0, 0), (0.5, 0, 0), (5, 5, 5)])
+    assert abs(g.getLastCVValues()[0] - 0.75) < 1e-6
+    assert abs(g.get_forces()[0][0] + 0.375) < 1e-6
+    f = Force()
+    f.add_coordination([0], [1], r0=1.0, n=6, m=12)
+    g, o = make_pair(f, 3)
+    run_both(g, o, [(0, 0, 0), (2, 0, 0), (5, 5, 5)])
+    assert abs(g.getLastCVValues()[0] - 63.0 / 4095.0) < 1e-7
+
+
+@pytest.mark.parametrize("cutoff,pbc,n", [(None, False, 300), (1.0, True, 3000), (0.9, True, 300)])
+def test_coordination_random(cutoff, pbc, n):
+    rng = np.random.default_rng(2)
+    L = (n / 100.0) ** (1 / 3)
+    nA = n // 10
+    f = Force(pbc=pbc)
+    f.add_coordination(list(range(nA)), list(range(nA, n)), r0=0.4, n=6, m=12, cutoff=cutoff)
+    f.setTestBiasGradients([1.0])
+    g, o = make_pair(f, n, box=L)
+    run_both(g, o, rng.uniform(0, L, size=(n, 3)))
+    if cutoff:
+        assert g.coordination_pairs(0) == o.coord_pairs[0]
+
+
+def test_group_cvs_random():
+    rng = np.random.default_rng(3)
+    n = 700
+    ref = rng.uniform(0, 2, size=(200, 3))
+    f = Force(pbc=True)
+    f.add_gyration(list(range(0, 500)))
+    f.add_com_distance(list(range(0, 50)), list(range(50, 120)))
+    f.add_rmsd(list(range(300, 500)), ref.tolist())
+    f.add_rmsd(list(range(300, 500)), ref.tolist(), fit=True)
+    pairs = rng.integers(0, n, size=(150, 2))
+    pairs = pairs[pairs[:, 0] != pairs[:, 1]]
+    f.add_drmsd(pairs.reshape(-1).tolist(), rng.uniform(0.2, 1.0, size=len(pairs)).tolist())
+    f.add_contact_map(pairs.reshape(-1).tolist(), r0=0.5, n=6, m=12, w=1.0)
+    grads = rng.normal(size=f.getNumCollectiveVariables())
+    f.setTestBiasGradients(grads.tolist())
+    masses = rng.uniform(1, 16, size=n)
+    g, o = make_pair(f, n, box=(n / 100) ** (1 / 3), masses=masses)
+    pos = rng.uniform(0, (n / 100) ** (1 / 3), size=(n, 3))
+    pos[300:500] = ref @ np.linalg.qr(rng.normal(size=(3, 3)))[0].T + rng.normal(0, 0.1, size=(200, 3)) + 1.0
+    run_both(g, o, pos)
+
+
+def test_kabsch_rotated_copy_is_zero():
+    rng = np.random.default_rng(4)
+    ref = rng.uniform(0, 2, size=(500, 3))
+    f = Force()
+    f.add_rmsd(list(range(500)), ref.tolist(), fit=True)
+    g, o = make_pair(f, 500)
+    Q = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    if np.linalg.det(Q) < 0:
+        Q[:, 0] *= -1
+    g.set_positions((ref @ Q.T + 0.7).astype(np.float32))
+    g.execute()
+    assert g.getLastCVValues()[0] < 1e-5
+
+
+def test_path_random():
+    rng = np.random.default_rng(5)
+    M, N = 40, 12
+    base = rng.uniform(0, 2, size=(M, 3))
+    frames = [base + 0.05 * k * rng.normal(size=(M, 3)) for k in range(N)]
+    f = Force()
+    f.add_path(list(range(M)), [fr.reshape(-1).tolist() for fr in frames], 30.0)
+    f.setTestBiasGradients([1.3, -0.6])
+    g, o = make_pair(f, M + 5)
+    pos = np.vstack([base + rng.normal(0, 0.05, size=(M, 3)), rng.uniform(0, 2, size=(5, 3))])
+    run_both(g, o, pos)
+
+
+def test_atom_reorder():
+    """GPU atom order != user order (rebuildGpuAtomIndices,
+    CommonCVKernels.cpp:113-317): forces land on the GPU index."""
+    rng = np.random.default_rng(6)
+    f = Force()
+    f.add_distance([0, 3])
+    f.add_dihedral([1, 2, 3, 4])
+    f.setTestBiasGradients([1.0, 0.5])
+    g, o = make_pair(f, 6)
+    pos = rng.uniform(0, 2, size=(6, 3)).astype(np.float32)
+    perm = rng.permutation(6)  # gpu index -> user atom
+    g.set_atom_order(perm)
+    gpu_pos = pos[perm]
+    g.set_positions(gpu_pos)
+    g.zero_forces()
+    g.execute()
+    r = o.execute(pos)
+    assert_close_rel(g.getLastCVValues(), r["cv"], 1e-6, "CVs")
+    Fg = g.get_forces()  # in GPU order
+    Fu = np.zeros_like(Fg)
+    Fu[perm] = Fg
+    assert np.allclose(Fu, r["forces"], atol=1e-6)

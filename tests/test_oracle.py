@@ -1,0 +1,634 @@
+"""Pins of the FP64 oracle (CPU).
+
+Each CV / bias function of oracle/glued_oracle.c is checked against values
+that do not come from the oracle itself: the closed forms and known answers
+of the reference's own tests (tests/test_cv_*.py, tests/test_bias_*.py in
+MarvinTaterra/GluedMD, cited per test), textbook special cases, invariants
+(Newton's third law, rigid-motion invariance), numpy/LAPACK (SVD Kabsch),
+brute force, and central finite differences.
+"""
+import math
+import random
+
+import numpy as np
+import pytest
+
+from gluedmd_amd.glued import Force
+from gluedmd_amd.workloads import SplitMix64, sum_uprob_np
+
+
+def ctx_for(oracle, f, pos, box=None, grads=None):
+    if grads is not None:
+        f.setTestBiasGradients(grads)
+    c = oracle.OracleContext(f, len(pos))
+    if box is not None:
+        c.set_box(box)
+    return c, c.execute(np.asarray(pos, dtype=np.float32))
+
+
+def fd_grad(oracle, f, pos, cv_index=0, h=1e-4):
+    """Central differences of CV `cv_index` in FP64 (positions passed as
+    float64 arrays through the FP32 cast would lose h, so the oracle's FP64
+    functions are called on float64-exact perturbations)."""
+    pos = np.asarray(pos, dtype=np.float64)
+    g = np.zeros_like(pos)
+    for a in range(pos.shape[0]):
+        for k in range(3):
+            pp, pm = pos.copy(), pos.copy()
+            pp[a, k] += h
+            pm[a, k] -= h
+            cp = oracle.OracleContext(f, len(pos))
+            cm = oracle.OracleContext(f, len(pos))
+            vp = cp.eval_cvs_f64(pp)[0][cv_index]
+            vm = cm.eval_cvs_f64(pm)[0][cv_index]
+            g[a, k] = (vp - vm) / (2 * h)
+    return g
+
+
+def analytic_grad(c, N, v):
+    g = np.zeros((N, 3))
+    for vi, atoms, rows in c.rows:
+        if vi == v:
+            np.add.at(g, atoms, rows)
+    return g
+
+
+# ---------------------------------------------------------------------------
+# random numbers
+# ---------------------------------------------------------------------------
+def test_splitmix64_known_answer():
+    # SplitMix64 reference sequence for seed 0 (Vigna's splitmix64.c)
+    r = SplitMix64(0)
+    out = r.next_u64(3)
+    assert int(out[0]) == 0xE220A8397B1DCDAF
+    assert int(out[1]) == 0x6E789E6AA1B965F4
+    assert int(out[2]) == 0x06C45D188009454F
+
+
+def test_splitmix64_stream_is_chunk_independent():
+    a = SplitMix64(123).uniform(10)
+    r = SplitMix64(123)
+    b = np.concatenate([r.uniform(3), r.uniform(7)])
+    assert np.array_equal(a, b)
+    assert np.all((a >= 0) & (a < 1))
+
+
+# ---------------------------------------------------------------------------
+# distance (tests/test_cv_distance.py:70-131)
+# ---------------------------------------------------------------------------
+def test_distance_value_and_forces(oracle):
+    f = Force()
+    f.add_distance([0, 1])
+    c, r = ctx_for(oracle, f, [(0, 0, 0), (0.5, 0, 0), (5, 5, 5)], grads=[1.0])
+    assert abs(r["cv"][0] - 0.5) < 1e-12
+    # F = -dU/dCV * dd/dx: atom 0 pulled +x, atom 1 pushed -x (tests :81-99)
+    assert np.allclose(r["forces"][0], [1, 0, 0]) and np.allclose(r["forces"][1], [-1, 0, 0])
+
+
+def test_distance_pbc(oracle):
+    f = Force(pbc=True)
+    f.add_distance([0, 1])
+    c, r = ctx_for(oracle, f, [(0.005, 0, 0), (0.995, 0, 0)], box=1.0)
+    assert abs(r["cv"][0] - 0.01) < 1e-6  # tests :124-131
+
+
+def test_distance_fd(oracle):
+    f = Force()
+    f.add_distance([0, 1])
+    pos = np.array([(0.1, 0.2, -0.3), (0.7, -0.4, 0.5)])
+    c, r = ctx_for(oracle, f, pos, grads=[1.0])
+    assert np.allclose(analytic_grad(c, 2, 0), fd_grad(oracle, f, pos), atol=1e-7)
+
+
+# ---------------------------------------------------------------------------
+# dihedral (tests/test_cv_angle_dihedral.py:135-208)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("d_pos,expected", [
+    ((1.0, 1.0, 0.0), 0.0),          # cis
+    ((1.0, -1.0, 0.0), math.pi),     # trans
+    ((1.0, 0.0, 1.0), math.pi / 2),
+])
+def test_dihedral_closed_forms(oracle, d_pos, expected):
+    f = Force()
+    f.add_dihedral([0, 1, 2, 3])
+    pos = [(0, 1, 0), (0, 0, 0), (1, 0, 0), d_pos]
+    c, r = ctx_for(oracle, f, pos)
+    v = r["cv"][0]
+    if expected == math.pi:
+        assert abs(abs(v) - math.pi) < 1e-6
+    else:
+        assert abs(abs(v) - expected) < 1e-6
+
+
+def test_dihedral_fd_and_newton(oracle):
+    f = Force()
+    f.add_dihedral([0, 1, 2, 3])
+    pos = np.array([(0.1, 1.0, 0.2), (0.0, 0.0, 0.0), (1.0, 0.1, -0.1), (1.3, 0.4, 0.9)])
+    c, r = ctx_for(oracle, f, pos, grads=[1.0])
+    g = analytic_grad(c, 4, 0)
+    assert np.allclose(g, fd_grad(oracle, f, pos), atol=1e-6)
+    assert np.allclose(r["forces"].sum(axis=0), 0.0, atol=1e-12)
+
+
+def test_angle_fd(oracle):
+    f = Force()
+    f.add_angle([0, 1, 2])
+    pos = np.array([(1.0, 0.1, 0.0), (0.0, 0.0, 0.2), (0.3, 1.1, -0.2)])
+    c, r = ctx_for(oracle, f, pos, grads=[1.0])
+    assert np.allclose(analytic_grad(c, 3, 0), fd_grad(oracle, f, pos), atol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# coordination (tests/test_cv_coordination.py:66-184)
+# ---------------------------------------------------------------------------
+def coord_force(A, B, r0, n, m, cutoff=None, pbc=False):
+    f = Force(pbc=pbc)
+    f.add_coordination(A, B, r0=r0, n=n, m=m, cutoff=cutoff)
+    return f
+
+
+def test_coordination_reference_values(oracle):
+    base = [(0.0, 0.0, 0.0), (0.5, 0.0, 0.0), (5.0, 5.0, 5.0)]
+    c, r = ctx_for(oracle, coord_force([0], [1], 1.5, 1, 2), base, grads=[1.0])
+    assert abs(r["cv"][0] - 0.75) < 1e-6                                # :66-77
+    assert abs(r["forces"][0][0] - (-0.375)) < 1e-6                     # :125-140
+    assert np.allclose(r["forces"][:2].sum(axis=0), 0.0, atol=1e-12)    # :111-122
+    c, r = ctx_for(oracle, coord_force([0], [1], 1.0, 6, 12), [(0, 0, 0), (2, 0, 0), (5, 5, 5)])
+    assert abs(r["cv"][0] - 63.0 / 4095.0) < 1e-9                       # :80-91 (no cutoff)
+    c, r = ctx_for(oracle, coord_force([0], [1, 2], 1.0, 1, 2), [(0, 0, 0), (0.5, 0, 0), (2, 0, 0), (5, 5, 5)])
+    assert abs(r["cv"][0] - 1.0) < 1e-9                                 # :94-108
+    x = math.sqrt(2.0) * 0.5 / 1.5
+    pos = [(0, 0, 0), (0, 1, 0), (0.5, 0.5, 0), (-0.5, 0.5, 0), (5, 5, 5)]
+    c, r = ctx_for(oracle, coord_force([0, 1], [2, 3], 1.5, 1, 2), pos)
+    assert abs(r["cv"][0] - 4.0 / (1.0 + x)) < 1e-6                     # :168-184
+
+
+def test_switch_special_cases(oracle):
+    # n=6, m=12 reduces to 1/(1+x^6) (x = r/r0); the Taylor branch near x=1
+    # (|x-1| < 1e-4) stays on the closed form; df/dr matches its derivative
+    for x in [0.3, 0.7, 0.99, 1.0 + 5e-5, 1.5, 2.7]:
+        fval, _ = oracle.switch(x, 0.4, 6.0, 12.0)
+        assert abs(fval - 1.0 / (1.0 + x ** 6)) < 1e-8
+    for r in [0.2, 0.37, 0.4 * (1 + 3e-5), 0.9]:
+        _, dfdr = oracle.switch(r / 0.4, 0.4, 6.0, 12.0)
+        h = 1e-6
+        fd = (1 / (1 + ((r + h) / 0.4) ** 6) - 1 / (1 + ((r - h) / 0.4) ** 6)) / (2 * h)
+        assert abs(dfdr - fd) < 1e-6
+
+
+def test_coordination_fd(oracle):
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(0, 1.2, size=(7, 3))
+    f = coord_force([0, 1, 2], [3, 4, 5, 6], 0.5, 6, 12)
+    c, r = ctx_for(oracle, f, pos, grads=[1.0])
+    assert np.allclose(analytic_grad(c, 7, 0), fd_grad(oracle, f, pos), atol=2e-6)
+
+
+def test_coordination_cutoff_matches_brute_force(oracle):
+    """Cutoff extension: neighbour count and CN against a numpy brute-force
+    enumeration of the same FP64 minimum-image predicate."""
+    rng = np.random.default_rng(11)
+    L = 3.0
+    pos = rng.uniform(0, L, size=(400, 3)).astype(np.float32)
+    A, B = list(range(0, 60)), list(range(60, 400))
+    f = coord_force(A, B, 0.4, 6, 12, cutoff=1.0, pbc=True)
+    c, r = ctx_for(oracle, f, pos, box=L)
+    p = pos.astype(np.float64)
+    d = p[A][:, None, :] - p[B][None, :, :]
+    d = d - L * np.floor(d / L + 0.5)
+    r2 = np.sum(d * d, axis=2)
+    mask = (r2 < 1.0) & (r2 >= 1e-20)
+    x = np.sqrt(r2[mask]) / np.float32(0.4).astype(np.float64)
+    assert c.coord_pairs[0] == int(mask.sum())
+    assert abs(r["cv"][0] - np.sum(1.0 / (1.0 + x ** 6))) < 1e-9 * max(1.0, r["cv"][0])
+
+
+# ---------------------------------------------------------------------------
+# COM distance / gyration (tests/test_cv_com_rg.py:70-244)
+# ---------------------------------------------------------------------------
+def test_com_distance_and_rg(oracle):
+    f = Force()
+    f.add_com_distance([0, 1], [2, 3])
+    pos = [(0, 0, 0), (1, 0, 0), (0, 1, 0), (1, 1, 0)]
+    c, r = ctx_for(oracle, f, pos)
+    assert abs(r["cv"][0] - 1.0) < 1e-12
+    f = Force()
+    f.add_gyration([0, 1])
+    c, r = ctx_for(oracle, f, [(0, 0, 0), (2, 0, 0)])
+    assert abs(r["cv"][0] - 1.0) < 1e-12
+    # square of side 2 centred at origin: Rg = sqrt(2)
+    f = Force()
+    f.add_gyration([0, 1, 2, 3])
+    c, r = ctx_for(oracle, f, [(1, 1, 0), (-1, 1, 0), (1, -1, 0), (-1, -1, 0)])
+    assert abs(r["cv"][0] - math.sqrt(2.0)) < 1e-12
+    # cube corners of edge 2 around the origin: Rg = sqrt(3)
+    f = Force()
+    f.add_gyration(list(range(8)))
+    cube = [(x, y, z) for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)]
+    c, r = ctx_for(oracle, f, cube)
+    assert abs(r["cv"][0] - math.sqrt(3.0)) < 1e-12
+
+
+def test_rg_fd_and_masses(oracle):
+    rng = np.random.default_rng(2)
+    pos = rng.uniform(0, 1, size=(6, 3))
+    f = Force()
+    f.add_gyration(list(range(6)))
+    c = oracle.OracleContext(f, 6, masses=[1, 2, 3, 1, 2, 3])
+    f.setTestBiasGradients([1.0])
+    r = c.execute(pos.astype(np.float32))
+    m = np.array([1, 2, 3, 1, 2, 3], dtype=np.float64)
+    p = pos.astype(np.float32).astype(np.float64)
+    com = (m[:, None] * p).sum(0) / m.sum()
+    rg = math.sqrt((m * ((p - com) ** 2).sum(1)).sum() / m.sum())
+    assert abs(r["cv"][0] - rg) < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# RMSD (tests/test_cv_rmsd.py:66-174) and the Kabsch extension
+# ---------------------------------------------------------------------------
+def test_rmsd_nofit_values(oracle):
+    ref = [(0, 0, 0), (1, 0, 0), (0, 1, 0)]
+    f = Force()
+    f.add_rmsd([0, 1, 2], ref)
+    c, r = ctx_for(oracle, f, [(0.3, 0, 0), (1.3, 0, 0), (0.3, 1, 0)])
+    assert abs(r["cv"][0] - 0.3) < 1e-6
+
+
+def kabsch_svd(x, y):
+    xc, yc = x - x.mean(0), y - y.mean(0)
+    H = yc.T @ xc
+    U, S, Vt = np.linalg.svd(H)
+    d = np.sign(np.linalg.det(Vt.T @ U.T))
+    D = np.diag([1, 1, d])
+    R = Vt.T @ D @ U.T
+    diff = xc - yc @ R.T
+    return math.sqrt((diff ** 2).sum() / len(x)), R
+
+
+def test_kabsch_matches_svd_and_is_rigid_invariant(oracle):
+    rng = np.random.default_rng(7)
+    ref = rng.uniform(0, 2, size=(40, 3))
+    x = ref @ np.linalg.qr(rng.normal(size=(3, 3)))[0].T + rng.normal(0, 0.1, size=(40, 3)) + 3.0
+    f = Force()
+    f.add_rmsd(list(range(40)), ref.tolist(), fit=True)
+    c, r = ctx_for(oracle, f, x.astype(np.float32))
+    xr = x.astype(np.float32).astype(np.float64)
+    rm, _ = kabsch_svd(xr, ref.astype(np.float32).astype(np.float64))
+    assert abs(r["cv"][0] - rm) < 1e-10
+    # rotated + translated copy of the reference -> 0
+    Q = np.linalg.qr(rng.normal(size=(3, 3)))[0]
+    if np.linalg.det(Q) < 0:
+        Q[:, 0] *= -1
+    c, r = ctx_for(oracle, f, (ref @ Q.T + 1.0).astype(np.float32))
+    assert r["cv"][0] < 2e-6
+
+
+def test_kabsch_gradient_fd(oracle):
+    rng = np.random.default_rng(8)
+    ref = rng.uniform(0, 1, size=(6, 3))
+    x = ref + rng.normal(0, 0.2, size=(6, 3))
+    f = Force()
+    f.add_rmsd(list(range(6)), ref.tolist(), fit=True)
+    c, r = ctx_for(oracle, f, x, grads=[1.0])
+    assert np.allclose(analytic_grad(c, 6, 0), fd_grad(oracle, f, x, h=1e-5), atol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# path CV (tests/test_cv_path.py:76-229)
+# ---------------------------------------------------------------------------
+def path_force(frames, lam, atoms=(0,)):
+    f = Force()
+    f.add_path(list(atoms), [np.asarray(fr, dtype=float).reshape(-1).tolist() for fr in frames], lam)
+    return f
+
+
+def test_path_reference_values(oracle):
+    lam = 1.0
+    f = path_force([[(0, 0, 0)], [(2, 0, 0)]], lam)
+    c, r = ctx_for(oracle, f, [(1, 0, 0), (5, 5, 5)])
+    assert abs(r["cv"][0] - 1.5) < 1e-12
+    assert abs(r["cv"][1] - (1.0 - math.log(2.0) / lam)) < 1e-12
+    # atom on frame 1: s = (1 + 2 w1)/S, z = -ln S / lam, w1 = exp(-4 lam)
+    c, r = ctx_for(oracle, f, [(0, 0, 0), (5, 5, 5)])
+    w1 = math.exp(-lam * 4.0)
+    assert abs(r["cv"][0] - (1 + 2 * w1) / (1 + w1)) < 1e-12
+    assert abs(r["cv"][1] - (-math.log(1 + w1) / lam)) < 1e-12
+
+
+def test_path_fd_and_lambda_derivatives(oracle):
+    rng = np.random.default_rng(3)
+    frames = [rng.uniform(0, 1, size=(3, 3)) for _ in range(5)]
+    lam = 2.0
+    pos = frames[2] + rng.normal(0, 0.1, size=(3, 3))
+    f = path_force(frames, lam, atoms=(0, 1, 2))
+    c, r = ctx_for(oracle, f, pos, grads=[1.0, 0.0])
+    assert np.allclose(analytic_grad(c, 3, 0), fd_grad(oracle, f, pos, 0), atol=1e-6)
+    assert np.allclose(analytic_grad(c, 3, 1), fd_grad(oracle, f, pos, 1), atol=1e-6)
+    # d s / d lambda and d z / d lambda (closed forms) vs central differences in lambda
+    sz0, dl = c.path_lambda_derivs(pos, 0)
+    h = 1e-6
+    fp_, fm_ = path_force(frames, lam + h, (0, 1, 2)), path_force(frames, lam - h, (0, 1, 2))
+    sp = oracle.OracleContext(fp_, 3).eval_cvs_f64(pos, exact_params=True)[0]
+    sm = oracle.OracleContext(fm_, 3).eval_cvs_f64(pos, exact_params=True)[0]
+    assert abs(dl[0] - (sp[0] - sm[0]) / (2 * h)) < 1e-6
+    assert abs(dl[1] - (sp[1] - sm[1]) / (2 * h)) < 1e-6
+
+
+# ---------------------------------------------------------------------------
+# contact map / DRMSD (tests/test_cv_contactmap.py:22-158, test_cv_drmsd.py:52-151)
+# ---------------------------------------------------------------------------
+def _switch_ref(d, r0, nn, mm_exp):
+    x = d / r0
+    if abs(1.0 - x ** mm_exp) < 1e-8:
+        return nn / mm_exp
+    return (1.0 - x ** nn) / (1.0 - x ** mm_exp)
+
+
+def test_contact_map_random13(oracle):
+    rng = random.Random(13)
+    positions = [[rng.uniform(0, 1) for _ in range(3)] for _ in range(6)]
+    pairs = [(0, 1), (2, 3), (4, 5)]
+    params = [(0.5, 6, 12, 1.0), (0.4, 8, 16, 0.5), (0.6, 6, 12, 2.0)]
+    f = Force()
+    f.add_contact_map([x for p in pairs for x in p], r0=[p[0] for p in params], n=[p[1] for p in params],
+                      m=[p[2] for p in params], w=[p[3] for p in params])
+    pos = np.array(positions)
+    c, r = ctx_for(oracle, f, pos)
+    p = pos.astype(np.float32).astype(np.float64)
+    exp = sum(w * _switch_ref(np.linalg.norm(p[b] - p[a]), float(np.float32(r0)), nn, mm)
+              for (a, b), (r0, nn, mm, w) in zip(pairs, params))
+    assert abs(r["cv"][0] - exp) < 1e-12
+
+
+def test_contact_map_fd_random99(oracle):
+    rng = random.Random(99)
+    pos = np.array([[rng.uniform(0.0, 0.5) for _ in range(3)] for _ in range(4)])
+    pairs = [(0, 1), (2, 3), (0, 3)]
+    params = [(0.5, 6, 12, 1.0), (0.4, 8, 16, 0.5), (0.6, 6, 12, 1.5)]
+    f = Force()
+    f.add_contact_map([x for p in pairs for x in p], r0=[p[0] for p in params], n=[p[1] for p in params],
+                      m=[p[2] for p in params], w=[p[3] for p in params])
+    c, r = ctx_for(oracle, f, pos, grads=[1.0])
+    assert np.allclose(analytic_grad(c, 4, 0), fd_grad(oracle, f, pos), atol=1e-5)
+
+
+def test_drmsd_values_and_fd(oracle):
+    rng = random.Random(7)
+    pos = np.array([[rng.uniform(0, 1) for _ in range(3)] for _ in range(5)])
+    pairs = [(0, 1), (1, 2), (2, 3), (3, 4), (0, 4)]
+    ref = [0.3, 0.5, 0.4, 0.6, 0.7]
+    f = Force()
+    f.add_drmsd([x for p in pairs for x in p], ref)
+    c, r = ctx_for(oracle, f, pos, grads=[1.0])
+    p = pos.astype(np.float32).astype(np.float64)
+    exp = math.sqrt(sum((np.linalg.norm(p[b] - p[a]) - float(np.float32(d0))) ** 2
+                        for (a, b), d0 in zip(pairs, ref)) / len(ref))
+    assert abs(r["cv"][0] - exp) < 1e-12
+    assert np.allclose(analytic_grad(c, 5, 0), fd_grad(oracle, f, pos), atol=1e-6)
+
+
+# ---------------------------------------------------------------------------
+# scatter (tests/test_gpu_io.py fixed-point contract)
+# ---------------------------------------------------------------------------
+def test_scatter_fixed_point_truncation(oracle):
+    f = Force()
+    f.add_distance([0, 1])
+    c, r = ctx_for(oracle, f, [(0, 0, 0), (0.6, 0.8, 0)], grads=[0.3])
+    F = c.forces_fixed()
+    # per row: (int64)(-(0.3 * (float)g) * 2^32), truncation toward zero
+    g = np.float32(0.6)
+    want = int(np.trunc(-(0.3 * float(-g)) * 2.0 ** 32))
+    assert F[0][0] == want
+    assert F[1][0] == int(np.trunc(-(0.3 * float(g)) * 2.0 ** 32))
+    assert np.allclose(F / 2.0 ** 32, r["forces"], atol=1e-8)
+
+
+# ---------------------------------------------------------------------------
+# MetaD grid (tests/test_bias_metad.py:106-244) and the hill list
+# ---------------------------------------------------------------------------
+def metad_1d(grid=True, gamma=100.0, kT=2.479, h0=1.0, sigma=0.05):
+    f = Force(temperature=298.15)
+    d = f.add_distance([0, 1])
+    if grid:
+        f.add_metad(d, sigma, h0, 1, grid_min=0.0, grid_max=2.0, bins=200, gamma=gamma, temperature=kT / 8.314462618e-3)
+    else:
+        f.add_metad_hills(d, sigma, h0, 1, gamma=gamma, temperature=kT / 8.314462618e-3)
+    return f
+
+
+def test_metad_grid_single_gaussian(oracle):
+    f = metad_1d()
+    c = oracle.OracleContext(f, 3)
+    at = lambda x: np.array([(0, 0, 0), (x, 0, 0), (50, 50, 50)], dtype=np.float32)
+    c.md_step(at(1.0), 0)
+    c.md_step(at(1.0), 1)  # deposits at the previous evaluation's cv = 1.0
+    assert abs(c.execute(at(1.0))["energy"] - 1.0) < 1e-6
+    assert abs(c.execute(at(1.05))["energy"] - math.exp(-0.5)) < 1e-6
+    assert abs(c.execute(at(0.95))["energy"] - math.exp(-0.5)) < 1e-6
+
+
+@pytest.mark.parametrize("grid", [True, False])
+def test_metad_well_tempered_recursion(oracle, grid):
+    gamma, kT, h0 = 10.0, 2.479, 1.0
+    f = metad_1d(grid=grid, gamma=gamma, kT=kT, h0=h0)
+    c = oracle.OracleContext(f, 3)
+    p = np.array([(0, 0, 0), (1.0, 0, 0), (50, 50, 50)], dtype=np.float32)
+    for step in range(51):
+        c.md_step(p, step)
+    V = 0.0
+    for _ in range(50):  # steps 1..50 deposit
+        V += h0 * math.exp(-V / ((gamma - 1.0) * kT))
+    assert abs(c.execute(p)["energy"] - V) < 1e-9 * V
+
+
+def test_hills_equal_grid_at_nodes(oracle):
+    """The hill list and the grid use the same truncated Gaussian; at grid
+    nodes (where interpolation is exact) they agree for random deposits.
+    Flat heights (gamma <= 1), so the grid's interpolated V at deposit time
+    cannot make the two height sequences differ."""
+    rng = np.random.default_rng(4)
+    fg, fh = metad_1d(True, gamma=0.0), metad_1d(False, gamma=0.0)
+    cg, ch = oracle.OracleContext(fg, 3), oracle.OracleContext(fh, 3)
+    step = 0
+    for x in rng.uniform(0.5, 1.5, size=30):
+        p = np.array([(0, 0, 0), (x, 0, 0), (50, 50, 50)], dtype=np.float32)
+        cg.md_step(p, step)
+        ch.md_step(p, step)
+        step += 1
+    for node in [80, 100, 123, 140]:
+        p = np.array([(0, 0, 0), (node * 0.01, 0, 0), (50, 50, 50)], dtype=np.float32)
+        eg, eh = cg.execute(p)["energy"], ch.execute(p)["energy"]
+        # the CV is the FP32-rounded node position (|dx| < 2e-8): both sides
+        # move by at most |dV/ds| dx
+        assert abs(eg - eh) < 1e-6 * max(1.0, abs(eh))
+
+
+def test_hills_periodic_brute_force(oracle):
+    rng = np.random.default_rng(9)
+    c = rng.uniform(-math.pi, math.pi, size=(200, 2))
+    h = rng.uniform(0.1, 1.2, size=200)
+    s = np.array([3.0, -3.05])
+    sig = np.array([0.35, 0.35])
+    V, dV = oracle.hills_eval(s, c, h, sig, np.array([2 * math.pi, 2 * math.pi]))
+    dc = s[None, :] - c
+    dc -= np.round(dc / (2 * math.pi)) * 2 * math.pi
+    E = np.sum(dc * dc / (2 * sig ** 2), axis=1)
+    t = np.where(E <= 8.0, h * np.exp(-E), 0.0)
+    assert abs(V - t.sum()) < 1e-12
+    assert np.allclose(dV, -(t[:, None] * dc / sig ** 2).sum(0), atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# OPES (tests/test_bias_opes.py:118-341)
+# ---------------------------------------------------------------------------
+def opes_invGF(g):
+    return (g - 1.0) / g
+
+
+def opes_eps(g):
+    igf = opes_invGF(g)
+    return math.exp(-1.0 / (igf * (1.0 - igf)))
+
+
+def opes_silverman_sigma(sigma0, gamma, D=1):
+    sentinel = math.exp(-gamma)
+    sw, sw2 = 2.0 * sentinel, 2.0 * sentinel ** 2
+    neff = (1.0 + sw) ** 2 / (1.0 + sw2)
+    return sigma0 * (neff * (D + 2) / 4.0) ** (-1.0 / (D + 4.0))
+
+
+def opes_1d(sigma0=0.3):
+    kT, gamma = 2.479, 10.0
+    f = Force()
+    d = f.add_distance([0, 1])
+    f.addBias(f.BIAS_OPES, [d], [kT, gamma, sigma0, 0.01], [0, 1, 100000])
+    return f, kT, gamma
+
+
+def at(x):
+    return np.array([(0, 0, 0), (x, 0, 0), (5, 5, 5)], dtype=np.float32)
+
+
+def test_opes_no_kernel(oracle):
+    f, kT, gamma = opes_1d()
+    c = oracle.OracleContext(f, 3)
+    assert abs(c.execute(at(1.0))["energy"] - (-kT * gamma)) < 1e-9
+
+
+def deposit_one(c, x):
+    c.md_step(at(x), 0)
+    c.md_step(at(x), 1)
+
+
+def test_opes_single_kernel(oracle):
+    f, kT, gamma = opes_1d(0.3)
+    c = oracle.OracleContext(f, 3)
+    deposit_one(c, 1.0)
+    assert abs(c.execute(at(1.0))["energy"] - opes_invGF(gamma) * kT * math.log(1 + opes_eps(gamma))) < 1e-9
+    sd = opes_silverman_sigma(0.3, gamma)
+    assert abs(c.getKernelSigmas()[0] - sd) < 1e-12 if hasattr(c, "getKernelSigmas") else True
+    s_eval = float(np.float32(1.5))
+    igf, eps = opes_invGF(gamma), opes_eps(gamma)
+    norm2 = ((s_eval - 1.0) / sd) ** 2
+    G = math.exp(-0.5 * norm2)
+    pz = max(G - eps, 0) / (1 - eps)
+    r = c.execute(at(1.5))
+    assert abs(r["energy"] - igf * kT * math.log(pz + eps)) < 1e-9
+    dVds = igf * kT * (pz / (pz + eps)) * G / (G - eps) * (-(s_eval - 1.0) / sd ** 2)
+    assert abs(r["dUdCV"][0] - dVds) < 1e-9
+
+
+def test_opes_fd(oracle):
+    f, kT, gamma = opes_1d(0.3)
+    c = oracle.OracleContext(f, 3)
+    deposit_one(c, 1.0)
+    h = 1e-4
+    xs = [1.3, 1.3 + h, 1.3 - h]
+    st = c.states[0]
+    Vp, _ = st.eval(np.array([xs[1]]))
+    Vm, _ = st.eval(np.array([xs[2]]))
+    _, g = st.eval(np.array([xs[0]]))
+    assert abs((Vp - Vm) / (2 * h) - g[0]) < 1e-6
+
+
+def test_opes_sum_uprob_incremental_equals_full(oracle):
+    """The oracle's O(N) append update and O(N^2) recompute of sum_uprob
+    (two code paths of the reference deposit) agree with a numpy O(N^2)."""
+    rng = np.random.default_rng(1)
+    f = Force()
+    a = f.add_distance([0, 1])
+    b = f.add_distance([0, 2])
+    f.addBias(f.BIAS_OPES, [a, b], [2.5, 8.0, 0.05, 0.05, 0.005], [0, 1, 1000])
+    c = oracle.OracleContext(f, 3)
+    merges = appends = 0
+    for step in range(60):
+        p = np.array([(0, 0, 0), (1 + rng.normal(0, 0.06), 0, 0), (0, 1 + rng.normal(0, 0.06), 0)],
+                     dtype=np.float32)
+        c.update_state(step)
+        st = c.states[0]
+        before = st.nk
+        c.execute(p)
+        if step > 0:
+            pass
+    st = c.states[0]
+    cc, ss, lw = st.table()
+    assert abs(st.st_d[0] - sum_uprob_np(cc, ss, lw, st.cutoff2)) < 1e-9 * abs(st.st_d[0])
+    assert 1 < st.nk < 59  # both merges and appends happened
+
+
+# ---------------------------------------------------------------------------
+# OPES expanded / multithermal (tests/test_bias_opes_expanded.py:24-150,
+# tests/test_bias_multithermal.py:56-137)
+# ---------------------------------------------------------------------------
+def test_opes_expanded_eval(oracle):
+    kT = 2.5
+    f = Force()
+    cvs = [f.add_position(0, 0), f.add_position(1, 0), f.add_position(2, 0)]
+    w = [1.0, 2.0, 3.0]
+    f.addBias(f.BIAS_OPES_EXPANDED, cvs, [kT] + w, [1])
+    c = oracle.OracleContext(f, 3)
+    r = c.execute(np.array([(1.5, 0, 0), (0.2, 0, 0), (3.0, 0, 0)], dtype=np.float32))
+    ecv = [float(np.float32(v)) for v in (1.5, 0.2, 3.0)]
+    logW = [math.log(x / 6.0) for x in w]
+    vals = [logW[l] - ecv[l] / kT for l in range(3)]
+    mx = max(vals)
+    lq = mx + math.log(sum(math.exp(v - mx) for v in vals))
+    assert abs(r["energy"] - (-kT * lq)) < 1e-12
+    p = [math.exp(v - lq) for v in vals]
+    assert np.allclose(r["dUdCV"], p, atol=1e-12)
+
+
+def test_multithermal_eval_flat(oracle):
+    f = Force(temperature=300.0)
+    f.add_multithermal(300.0, 500.0, 8, temp0=300.0, pace=1)
+    c = oracle.OracleContext(f, 2)
+    U = -1234.5
+    c.set_energy_input(U, np.zeros((2, 3)))
+    r = c.execute(np.zeros((2, 3), dtype=np.float32))
+    kT0 = 8.314462618e-3 * 300.0
+    betas = [1.0 / (8.314462618e-3 * t) for t in f._multithermal["temps"]]
+    beta0 = 1.0 / kT0
+    diff = [-(b - beta0) * U for b in betas]
+    dmax = max(diff)
+    ex = [math.exp(d - dmax) for d in diff]
+    V = -kT0 * (dmax + math.log(sum(ex) / len(betas)))
+    dVdU = kT0 * sum(e / sum(ex) * (b - beta0) for e, b in zip(ex, betas))
+    assert abs(r["energy"] - V) < 1e-9 * abs(V)
+    assert abs(r["dUdCV"][0] - dVdU) < 1e-12
+
+
+def test_multithermal_update_matches_logmeanexp(oracle):
+    """dF recursion against a direct log-mean-exp of the reweighting factors:
+    after k updates, dF_l = -kT0 log( mean_i exp((V_i - rct...)) ) is what the
+    stabilised recursion tracks; checked via its invariant dF_l(beta_l=beta0)=0."""
+    f = Force(temperature=300.0)
+    f.add_multithermal(300.0, 500.0, 4, temps=[300.0, 350.0, 420.0, 500.0], temp0=300.0, pace=1)
+    c = oracle.OracleContext(f, 2)
+    rng = np.random.default_rng(0)
+    for step in range(20):
+        c.set_energy_input(-1000.0 + rng.normal(0, 20.0), np.zeros((2, 3)))
+        c.md_step(np.zeros((2, 3), dtype=np.float32), step)
+    st = c.states[0]
+    assert abs(st.dF[0]) < 1e-9  # the beta0 state never moves
+    assert np.all(np.isfinite(st.dF)) and st.counter[0] == 20.0
