@@ -164,6 +164,9 @@ void Engine::validate(int minAtoms) const
             case BIAS_OPES:
                 if (D > 4) err("OPES supports at most 4 CVs");
                 if ((int)s.params.size() < 3 + D) err("OPES params must be [kT, gamma, sigma0*D, sigmaMin]");
+                // CommonBiasKernels.cpp:568-570
+                if (!s.iparams.empty() && s.iparams[0] == 2 && (!std::isfinite(s.params[1]) || s.params[1] <= 1.0))
+                    err("OPES_METAD_EXPLORE (variant=2) requires finite gamma > 1");
                 break;
             case BIAS_OPES_EXPANDED:
                 if ((int)s.params.size() < 1 + D) err("OPES_EXPANDED params must be [kT, w*D]");

@@ -547,7 +547,11 @@ __global__ __launch_bounds__(OPES_DEP_BLOCK) void k_dep_opes(StepView v, OpesBia
                 for (int d = 0; d < D; d++) {
                     const double c1 = sOldC[d], s1 = sOldS[d];
                     const double cm = f1 * c1 + f2 * sC[d];
-                    double ss = f1 * (s1 * s1 + c1 * c1) + f2 * (sS[d] * sS[d] + sC[d] * sC[d]) - cm * cm;
+                    // the reference's f1(s1^2+c1^2) + f2(s2^2+c2^2) - cm^2
+                    // (CommonGluedKernels.cpp:2898-2905) in the algebraically
+                    // identical form without its cancellation at large |c|
+                    const double dcm = c1 - sC[d];
+                    double ss = f1 * s1 * s1 + f2 * sS[d] * sS[d] + f1 * f2 * dcm * dcm;
                     if (ss < 0.0) ss = 0.0;
                     double sm = sqrt(ss);
                     if (sm < o.sigmaMin) sm = o.sigmaMin;

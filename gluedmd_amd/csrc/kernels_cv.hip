@@ -317,9 +317,9 @@ __device__ void horn_rotation(const double* S, double Gx, double Gy, double* R, 
 // ---------------------------------------------------------------------------
 constexpr int GROUP_BLOCK = 256;
 
-__device__ inline float ipowf(float x, int n)
+__device__ inline double ipowd(double x, int n)
 {
-    float r = 1.0f;
+    double r = 1.0;
     for (int i = 0; i < n; i++) r *= x;
     return r;
 }
@@ -494,33 +494,34 @@ __global__ __launch_bounds__(GROUP_BLOCK) void k_cv_group(StepView v, const Grou
     }
 
     if (it.type == CV_CONTACTMAP) {
+        // FP64: a few thousand listed pairs; FP32 loses digits in 1 - x^m near x = 1
         const int np = (it.end - it.begin) / 2;
         const float* prm = ref + it.refOff;
         double s[1] = {0.0};
         for (int p = tid; p < np; p += nt) {
             float4 a = load_pos(v, w, slots[it.begin + 2 * p]);
             float4 b = load_pos(v, w, slots[it.begin + 2 * p + 1]);
-            double dxd = (double)b.x - a.x, dyd = (double)b.y - a.y, dzd = (double)b.z - a.z;
-            if (v.pbc) pbc_exact(dxd, dyd, dzd, bx);
-            float dx = (float)dxd, dy = (float)dyd, dz = (float)dzd;
-            float r0 = prm[4 * p], wgt = prm[4 * p + 3];
-            int nn = (int)prm[4 * p + 1], mm = (int)prm[4 * p + 2];
-            float d = sqrtf(dx * dx + dy * dy + dz * dz);
-            float invd = d > 1e-10f ? 1.0f / d : 0.0f;
-            float ir0 = 1.0f / r0, x = d * ir0;
-            float xn = ipowf(x, nn), xm = ipowf(x, mm);
-            float num = 1.0f - xn, den = 1.0f - xm, sw, dsdd;
-            if (fabsf(den) < 1e-8f) {
-                sw = (float)nn / (float)mm;
-                dsdd = 0.0f;
+            double dx = (double)b.x - a.x, dy = (double)b.y - a.y, dz = (double)b.z - a.z;
+            if (v.pbc) pbc_exact(dx, dy, dz, bx);
+            const double r0 = prm[4 * p], wgt = prm[4 * p + 3];
+            const int nn = (int)prm[4 * p + 1], mm = (int)prm[4 * p + 2];
+            const double d = sqrt(dx * dx + dy * dy + dz * dz);
+            const double invd = d > 1e-10 ? 1.0 / d : 0.0;
+            const double ir0 = 1.0 / r0, x = d * ir0;
+            const double xn = ipowd(x, nn), xm = ipowd(x, mm);
+            const double num = 1.0 - xn, den = 1.0 - xm;
+            double sw, dsdd;
+            if (fabs(den) < 1e-8) {
+                sw = (double)nn / (double)mm;
+                dsdd = 0.0;
             } else {
                 sw = num / den;
-                float dxn = nn > 0 ? nn * ipowf(x, nn - 1) : 0.0f;
-                float dxm = mm > 0 ? mm * ipowf(x, mm - 1) : 0.0f;
+                const double dxn = nn > 0 ? nn * ipowd(x, nn - 1) : 0.0;
+                const double dxm = mm > 0 ? mm * ipowd(x, mm - 1) : 0.0;
                 dsdd = ((-dxn * den + dxm * num) / (den * den)) * ir0;
             }
-            s[0] += (double)(wgt * sw);
-            float coeff = wgt * dsdd * invd;
+            s[0] += wgt * sw;
+            const double coeff = wgt * dsdd * invd;
             store_row(v, w, it.row0 + 2 * p, -coeff * dx, -coeff * dy, -coeff * dz);
             store_row(v, w, it.row0 + 2 * p + 1, coeff * dx, coeff * dy, coeff * dz);
         }
@@ -633,51 +634,54 @@ __global__ void k_coord_fill(StepView v, CoordBuffers cb, const CoordItem* items
     cb.sorted[(size_t)w * cb.sortStride + it.sortBase + pos] = make_float4(q.x, q.y, q.z, __int_as_float(local));
 }
 
-struct SwitchF {
-    float r0, rn, rm, ir0;
-    float preRes, preDfunc, preSecDev;
+struct SwitchD {
+    double r0, rn, rm, ir0;
+    double preRes, preDfunc, preSecDev;
     int intPow, in, im;
 };
 
-__device__ inline SwitchF make_switch(const CoordItem& it)
+__device__ inline SwitchD make_switch(const CoordItem& it)
 {
-    SwitchF s;
+    SwitchD s;
     s.r0 = it.r0;
     s.rn = it.rn;
     s.rm = it.rm;
-    s.ir0 = 1.0f / it.r0;
+    s.ir0 = 1.0 / s.r0;
     s.preRes = s.rn / s.rm;
-    s.preDfunc = 0.5f * s.rn * (s.rn - s.rm) / s.rm;
-    s.preSecDev = s.rn * (s.rm * s.rm - 3.0f * s.rm * (s.rn - 1.0f) + s.rn * (2.0f * s.rn - 3.0f)) / (6.0f * s.rm);
+    s.preDfunc = 0.5 * s.rn * (s.rn - s.rm) / s.rm;
+    s.preSecDev = s.rn * (s.rm * s.rm - 3.0 * s.rm * (s.rn - 1.0) + s.rn * (2.0 * s.rn - 3.0)) / (6.0 * s.rm);
     s.intPow = it.intPow;
     s.in = (int)s.rn;
     s.im = (int)s.rm;
     return s;
 }
 
-// f and df/dr of the rational switch (1-x^n)/(1-x^m), reference branch order.
-__device__ inline void switch_eval(const SwitchF& s, float r, float& f, float& dfdr)
+// f and df/dr of the rational switch (1-x^n)/(1-x^m), the reference's branch
+// order (CommonGluedKernels.cpp:979-1007).  FP64: in FP32 the numerator and
+// denominator cancel near x = 1 (1 - x^12 loses ~4 digits at |x-1| = 1e-4),
+// which breaks the force tolerance.
+__device__ inline void switch_eval(const SwitchD& s, double r, double& f, double& dfdr)
 {
-    const float x = r * s.ir0;
-    const float dx1 = x - 1.0f;
-    if (fabsf(dx1) < 1e-4f) {
-        f = s.preRes + dx1 * (s.preDfunc + 0.5f * dx1 * s.preSecDev);
+    const double x = r * s.ir0;
+    const double dx1 = x - 1.0;
+    if (fabs(dx1) < 1e-4) {
+        f = s.preRes + dx1 * (s.preDfunc + 0.5 * dx1 * s.preSecDev);
         dfdr = (s.preDfunc + dx1 * s.preSecDev) * s.ir0;
-    } else if (x < 1e-10f) {
-        f = 1.0f;
-        dfdr = 0.0f;
+    } else if (x < 1e-10) {
+        f = 1.0;
+        dfdr = 0.0;
     } else {
-        float xn, xm;
+        double xn, xm;
         if (s.intPow) {
-            xn = ipowf(x, s.in);
-            xm = ipowf(x, s.im);
+            xn = ipowd(x, s.in);
+            xm = ipowd(x, s.im);
         } else {
-            xn = powf(x, s.rn);
-            xm = powf(x, s.rm);
+            xn = pow(x, s.rn);
+            xm = pow(x, s.rm);
         }
-        const float den = 1.0f - xm, ix = 1.0f / x;
-        f = (1.0f - xn) / den;
-        const float dfx = (s.rm * xm * ix - s.rn * xn * ix + (s.rn - s.rm) * xn * xm * ix) / (den * den);
+        const double den = 1.0 - xm, ix = 1.0 / x;
+        f = (1.0 - xn) / den;
+        const double dfx = (s.rm * xm * ix - s.rn * xn * ix + (s.rn - s.rm) * xn * xm * ix) / (den * den);
         dfdr = dfx * s.ir0;
     }
 }
@@ -700,12 +704,12 @@ __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBu
     const double* bx = v.box + (size_t)w * 12;
     float bxf[12];
     for (int k = 0; k < 12; k++) bxf[k] = (float)bx[k];
-    const SwitchF sw = make_switch(it);
+    const SwitchD sw = make_switch(it);
     const float rc2f = (float)(it.rc2 * (1.0 + 1e-5));
 
     double cn = 0.0;
     unsigned long long np = 0;
-    float gx = 0.f, gy = 0.f, gz = 0.f;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
     if (home < nHome) {
         const int hslot = cb.slots[SIDE_A ? it.aBegin + home : it.bBegin + home];
         const float4 ph = load_pos(v, w, hslot);
@@ -747,21 +751,21 @@ __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBu
             const double r2 = dot3_exact(dx, dy, dz);
             if (r2 < 1e-20) continue;
             if (it.rc2 > 0.0 && !(r2 < it.rc2)) continue;
-            const float r = (float)sqrt(r2);
-            if (r * sw.ir0 > 1000.0f) continue;
-            float f, dfdr;
+            const double r = sqrt(r2);
+            if (r * sw.ir0 > 1000.0) continue;
+            double f, dfdr;
             switch_eval(sw, r, f, dfdr);
-            const float s = dfdr / r;
+            const double s = dfdr / r;
             if (SIDE_A) {
-                cn += (double)f;
+                cn += f;
                 np++;
-                gx += s * (float)dx;
-                gy += s * (float)dy;
-                gz += s * (float)dz;
+                gx += s * dx;
+                gy += s * dy;
+                gz += s * dz;
             } else {
-                gx -= s * (float)dx;
-                gy -= s * (float)dy;
-                gz -= s * (float)dz;
+                gx -= s * dx;
+                gy -= s * dy;
+                gz -= s * dz;
             }
         }
     }

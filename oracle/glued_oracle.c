@@ -796,6 +796,9 @@ EXPORT void orc_opes_welford(const double* cv, int D, double* mean, double* M2, 
 /* weight accumulators and neff, Silverman / adaptive sigma, height          */
 /* correction, strict-min Mahalanobis compression with the existing kernel's */
 /* sigma (first index wins ties), GMM merge or append, sum_uprob updated     */
+/* (merge variance f1 s1^2 + f2 s2^2 + f1 f2 (c1-c2)^2: equal to the         */
+/* reference's f1(s1^2+c1^2)+f2(s2^2+c2^2)-cm^2, whose cancellation loses    */
+/* ~11 digits when |c| ~ 3e4 and sigma ~ 0.05)                               */
 /* O(N) on append and recomputed O(N^2) on merge -- the reference's own       */
 /* algorithm, kept here as the oracle.                                       */
 /* Returns the merged index, -1 for an append, -2 for no deposit, -3 when    */
@@ -875,7 +878,10 @@ EXPORT int orc_opes_deposit(const double* cvnew, int D, double V, double invKT, 
         for (int d = 0; d < D; d++) {
             double c1 = centers[(size_t)mk * D + d], s1 = sigmas[(size_t)mk * D + d];
             double cm = f1 * c1 + f2 * c[d];
-            double ss = f1 * (s1 * s1 + c1 * c1) + f2 * (sg[d] * sg[d] + c[d] * c[d]) - cm * cm;
+            /* f1 (s1^2 + c1^2) + f2 (s2^2 + c2^2) - cm^2 of the reference, in
+               the algebraically identical form without its cancellation */
+            double dcm = c1 - c[d];
+            double ss = f1 * s1 * s1 + f2 * sg[d] * sg[d] + f1 * f2 * dcm * dcm;
             if (ss < 0.0) ss = 0.0;
             double sm = sqrt(ss);
             if (sm < sigmaMin) sm = sigmaMin;

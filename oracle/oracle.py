@@ -534,7 +534,7 @@ class OracleContext:
         return cv, rows
 
     # -- biases --
-    def eval_biases(self, cv):
+    def eval_biases(self, cv, welford=True):
         nB = len(self.force.biases)
         be = np.zeros(nB)
         dU = np.zeros(self.nV)
@@ -544,7 +544,7 @@ class OracleContext:
             if t == BIAS_OPES_MULTITHERMAL:
                 V, g = st.eval(s[0])
             else:
-                if t == BIAS_OPES and st.adaptiveStride > 0:
+                if t == BIAS_OPES and st.adaptiveStride > 0 and welford:
                     st.welford(s)
                 V, g = st.eval(s)
             be[b] = V

@@ -54,8 +54,14 @@ def check(g, o, r, eg, walker=0, f_rms=F_RMS):
     cv = g.getLastCVValues(walker)
     assert_close_rel(cv, r["cv"], CV_RTOL, "CV values")
     if o.force.getNumBiases() and o.force._test_grads is None:
+        # end to end (oracle CVs from the same positions) ...
         assert_close_rel(g.getBiasEnergies(walker), r["bias_energies"], E_RTOL, "bias energies", atol=1e-10)
         assert_close_rel(eg, r["energy"], E_RTOL, "total bias energy", atol=1e-10)
+        # ... and the bias arithmetic alone, at the GPU's own CV values
+        be, dU = o.eval_biases(cv, welford=False)
+        assert_close_rel(g.getBiasEnergies(walker), be, E_RTOL, "bias energies at GPU CVs", atol=1e-10)
+        scale = max(np.abs(dU).max(), 1e-300)
+        assert np.abs(g.getDUDCV(walker) - dU).max() <= 1e-6 * scale, "dU/dCV at GPU CVs"
     F = g.get_forces(walker)
     if np.linalg.norm(r["forces"]) > 0:
         e = rel_rms(F, r["forces"])

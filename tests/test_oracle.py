@@ -632,3 +632,27 @@ def test_multithermal_update_matches_logmeanexp(oracle):
     st = c.states[0]
     assert abs(st.dF[0]) < 1e-9  # the beta0 state never moves
     assert np.all(np.isfinite(st.dF)) and st.counter[0] == 20.0
+
+
+def test_opes_merge_variance_forms_agree():
+    """The merge variance f1 s1^2 + f2 s2^2 + f1 f2 (c1-c2)^2 used here equals
+    the reference's f1 (s1^2+c1^2) + f2 (s2^2+c2^2) - cm^2
+    (CommonGluedKernels.cpp:2898-2905) where the latter is well conditioned,
+    and stays accurate where the latter cancels (|c| >> sigma)."""
+    from fractions import Fraction as Fr
+    rng = np.random.default_rng(3)
+    for scale in (1.0, 1e2, 3.3e4):
+        for _ in range(20):
+            c1, c2 = rng.normal(0, 1) * scale, rng.normal(0, 1) * scale + rng.normal(0, 0.05)
+            s1, s2 = rng.uniform(0.01, 0.1), rng.uniform(0.01, 0.1)
+            f1 = rng.uniform(0.05, 0.95)
+            f2 = 1.0 - f1
+            cm = f1 * c1 + f2 * c2
+            ref = f1 * (s1 * s1 + c1 * c1) + f2 * (s2 * s2 + c2 * c2) - cm * cm
+            new = f1 * s1 * s1 + f2 * s2 * s2 + f1 * f2 * (c1 - c2) ** 2
+            F1, F2, C1, C2, S1, S2 = (Fr(x) for x in (f1, f2, c1, c2, s1, s2))
+            CM = F1 * C1 + F2 * C2
+            exact = float(F1 * (S1 * S1 + C1 * C1) + F2 * (S2 * S2 + C2 * C2) - CM * CM)
+            assert abs(new - exact) <= 1e-14 * exact
+            if scale == 1.0:
+                assert abs(ref - exact) <= 1e-12 * exact
