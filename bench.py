@@ -42,6 +42,8 @@ def main():
     ap.add_argument("--hills", type=int, default=20000)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="time only the batched step (for profiling its kernels)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -77,6 +79,11 @@ def main():
     torch.cuda.synchronize()
     t_batched = (time.perf_counter() - t0) / args.steps
     print(f"bench: batched {t_batched * 1e6:.1f} us/step", file=sys.stderr, flush=True)
+
+    if args.batched_only:
+        print(json.dumps({"metric": "batched step only", "ms_per_step": round(t_batched * 1e3, 4),
+                          "value": round(W / t_batched, 1), "unit": "evals/s"}))
+        return 0
 
     # ---- plain: one walker per launch sequence ----
     for _ in range(2):

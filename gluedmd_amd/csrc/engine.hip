@@ -31,6 +31,8 @@ int opes_chunks_for(int);
 
 void launch_scatter(const StepView&, const ScatterPlan&, int, int, int, hipStream_t);
 void launch_scatter_dense(const StepView&, const ScatterPlan&, int, int, int, hipStream_t);
+void launch_step_bias(const StepView&, const FusedBiasPlan&, int, hipStream_t);
+void launch_cv_fused(const StepView&, const CVFusedPlan&, int, hipStream_t);
 
 void hip_check(hipError_t e, const char* what)
 {
@@ -675,6 +677,38 @@ void Engine::buildPlan(int nMass, const double* masses)
     dSegStart_.upload(segStart);
     dSegRow_.upload(segRow);
     dRowCv_.upload(rowCv_);
+
+    // ---- the batched bias + scatter stage ----
+    {
+        std::vector<HillBiasDev> ht;
+        std::vector<OpesBiasDev> ot;
+        for (auto* h : hills_) ht.push_back(h->d);
+        for (auto* o : opes_) ot.push_back(o->d);
+        if (ht.size() + ot.size() > (size_t)MAX_CHUNKED) fail("too many hill / OPES biases (max 16)");
+        dHillTable_.upload(ht);
+        dOpesTable_.upload(ot);
+        fplan_ = FusedBiasPlan{};
+        fplan_.hills = dHillTable_.as<HillBiasDev>();
+        fplan_.opes = dOpesTable_.as<OpesBiasDev>();
+        fplan_.nHill = (int)ht.size();
+        fplan_.nOpes = (int)ot.size();
+        fplan_.nChunked = fplan_.nHill + fplan_.nOpes;
+        int c = 0;
+        for (int k = 0; k < fplan_.nChunked; k++) {
+            fplan_.chunkStart[k] = c;
+            c += k < fplan_.nHill ? ht[k].chunks : ot[k - fplan_.nHill].chunks;
+        }
+        fplan_.chunkStart[fplan_.nChunked] = c;
+        fplan_.totalChunks = c;
+        dFusedPart_.alloc(sizeof(double) * (size_t)W_ * std::max(c, 1) * (MAX_BIAS_D + 1));
+        dFusedCounter_.alloc(sizeof(unsigned int) * W_);
+        fplan_.part = dFusedPart_.as<double>();
+        fplan_.counter = dFusedCounter_.as<unsigned int>();
+        fplan_.small = dSmallBias_.as<SmallBiasItem>();
+        fplan_.nSmall = (int)smallBias_.size();
+        // one workgroup scatters a walker when its atom list is short
+        fplan_.scatter = (S_ <= 4096 && V_ <= 1024) ? 1 : 0;
+    }
 }
 
 void Engine::uploadGidx(int w)
@@ -811,14 +845,32 @@ StepView Engine::view(const float4* posq, long long* force) const
 // One step for walkers [w0, w0+nw): CVs by kind, then biases, then the
 // scatter (the ordering contract of CommonGluedKernels.cpp:4646-5048).
 // ---------------------------------------------------------------------------
-void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMask)
+void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMask, bool fused)
 {
     StepView v = view(lastPos_, lastForce_);
     v.S = S_;
     if (stageMask & 1) {
-    launch_cv_small(v, dSmall_.as<SmallCVItem>(), (int)small_.size(), w0, nw, s);
-    launch_cv_group(v, dGroup_.as<GroupCVItem>(), (int)group_.size(), dGroupSlots_.as<int>(), dGroupMass_.as<float>(),
-                    dGroupRef_.as<float>(), w0, nw, s);
+    if (fused && w0 == 0 && nw == W_) {
+        CVFusedPlan cp;
+        cp.small = dSmall_.as<SmallCVItem>();
+        cp.nSmall = (int)small_.size();
+        cp.group = dGroup_.as<GroupCVItem>();
+        cp.nGroup = (int)group_.size();
+        cp.gslots = dGroupSlots_.as<int>();
+        cp.gmass = dGroupMass_.as<float>();
+        cp.gref = dGroupRef_.as<float>();
+        cp.path = dPath_.as<PathItem>();
+        cp.nPath = (int)path_.size();
+        cp.pslots = dPathSlots_.as<int>();
+        cp.frames = dPathFrames_.as<float>();
+        launch_cv_fused(v, cp, nw, s);
+    } else {
+        launch_cv_small(v, dSmall_.as<SmallCVItem>(), (int)small_.size(), w0, nw, s);
+        launch_cv_group(v, dGroup_.as<GroupCVItem>(), (int)group_.size(), dGroupSlots_.as<int>(),
+                        dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
+        launch_cv_path(v, dPath_.as<PathItem>(), (int)path_.size(), dPathSlots_.as<int>(), dPathFrames_.as<float>(),
+                       w0, nw, s);
+    }
     if (!coord_.empty()) {
         CoordBuffers cb;
         cb.slots = dCoordSlots_.as<int>();
@@ -838,32 +890,14 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         for (size_t i = 0; i < coord_.size(); i++)
             launch_coordination(v, cb, dCoord_.as<CoordItem>(), coord_[i], (int)i, w0, nw, s);
     }
-    launch_cv_path(v, dPath_.as<PathItem>(), (int)path_.size(), dPathSlots_.as<int>(), dPathFrames_.as<float>(), w0,
-                   nw, s);
     }
-    if ((stageMask & 10) && testGrads_.empty()) {
-        SmallBiasState st;
-        st.grids = dGridTable_.as<GridBiasDev>();
-        st.stateConst = dSmallConst_.as<double>();
-        st.stateVar = dSmallVar_.as<double>();
-        st.scal = dSmallScal_.as<double>();
-        st.stateVarStride = smallVarPerWalker_;
-        st.scalStride = 4 * (long long)std::max((int)smallBias_.size(), 1);
-        for (auto* o : opes_)
-            if (o->d.adaptiveStride > 0 && (stageMask & 2)) {
-                launch_opes_welford(v, o->d, w0, nw, s);
-                for (int t = 0; t < (int)o->nSamplesMirror.size(); t++)
-                    if (o->d.shared || (t >= w0 && t < w0 + nw)) o->nSamplesMirror[t] += o->d.shared ? nw : 1;
-            }
-        for (auto* o : opes_) launch_bias_opes(v, o->d, w0, nw, s);
-        for (auto* h : hills_) launch_bias_hills(v, h->d, w0, nw, s);
-        if (stageMask & 2)
-            launch_bias_small(v, st, dSmallBias_.as<SmallBiasItem>(), (int)smallBias_.size(), w0, nw, s);
-    }
-    if (!(stageMask & 4)) {
-        hip_check(hipGetLastError(), "kernel launch");
-        return;
-    }
+    SmallBiasState st;
+    st.grids = dGridTable_.as<GridBiasDev>();
+    st.stateConst = dSmallConst_.as<double>();
+    st.stateVar = dSmallVar_.as<double>();
+    st.scal = dSmallScal_.as<double>();
+    st.stateVarStride = smallVarPerWalker_;
+    st.scalStride = 4 * (long long)std::max((int)smallBias_.size(), 1);
     ScatterPlan sp;
     sp.segStart = dSegStart_.as<int>();
     sp.segRow = dSegRow_.as<int>();
@@ -873,7 +907,41 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
     sp.testGrad = testGrads_.empty() ? nullptr : dTestGrad_.as<double>();
     sp.energyF = nullptr;
     sp.energyValue = energyValue_;
-    launch_scatter(v, sp, w0, nw, forces ? 1 : 0, s);
+    const bool biases = (stageMask & 10) && testGrads_.empty();
+    if (biases || (stageMask & 4)) {
+        for (auto* o : opes_)
+            if (biases && o->d.adaptiveStride > 0 && (stageMask & 2)) {
+                launch_opes_welford(v, o->d, w0, nw, s);
+                if (!capturing_) bumpWelfordMirrors(o, w0, nw);
+            }
+    }
+    bool scattered = false;
+    if (fused && (stageMask & 6) == 6 && w0 == 0 && nw == W_) {
+        // batched: chunked sums + finalize + one-thread biases + dU/dCV (+ scatter) in one launch
+        FusedBiasPlan fp = fplan_;
+        fp.st = st;
+        fp.sp = sp;
+        if (!biases) {
+            fp.nChunked = 0;
+            fp.totalChunks = 0;
+            fp.nSmall = 0;
+        }
+        fp.scatter = fplan_.scatter && forces;
+        launch_step_bias(v, fp, nw, s);
+        scattered = fp.scatter != 0;
+        if (forces && !scattered) launch_scatter(v, sp, w0, nw, 1, s);
+        scattered = true;
+    } else if (biases) {
+        for (auto* o : opes_) launch_bias_opes(v, o->d, w0, nw, s);
+        for (auto* h : hills_) launch_bias_hills(v, h->d, w0, nw, s);
+        if (stageMask & 2)
+            launch_bias_small(v, st, dSmallBias_.as<SmallBiasItem>(), (int)smallBias_.size(), w0, nw, s);
+    }
+    if (!(stageMask & 4)) {
+        hip_check(hipGetLastError(), "kernel launch");
+        return;
+    }
+    if (!scattered) launch_scatter(v, sp, w0, nw, forces ? 1 : 0, s);
     if (forces && energyValue_ >= 0) {
         for (int w = w0; w < w0 + nw; w++) {
             if (!energyF_[w]) continue;
@@ -885,6 +953,14 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         }
     }
     hip_check(hipGetLastError(), "kernel launch");
+}
+
+// The host mirror of the adaptive-sigma sample count (the reference keeps
+// the same mirror, CommonGluedKernels.cpp:4972-4976 and :5097-5102).
+void Engine::bumpWelfordMirrors(OpesHost* o, int w0, int nw)
+{
+    for (int t = 0; t < (int)o->nSamplesMirror.size(); t++)
+        if (o->d.shared || (t >= w0 && t < w0 + nw)) o->nSamplesMirror[t] += o->d.shared ? nw : 1;
 }
 
 void Engine::sync(hipStream_t s) { hip_check(hipStreamSynchronize(s), "hipStreamSynchronize"); }
@@ -930,7 +1006,7 @@ void Engine::executeBatch(hipStream_t s, const float4* posq, long long* force, b
     lastPos_ = posq;
     lastForce_ = force;
     if (!useGraph) {
-        launchStep(s, 0, W_, forces);
+        launchStep(s, 0, W_, forces, 7, true);
         cvReady_ = true;
         return;
     }
@@ -947,9 +1023,12 @@ void Engine::executeBatch(hipStream_t s, const float4* posq, long long* force, b
         // caller's stream, so ordering with the caller's work is preserved.
         if (!captureStream_) hip_check(hipStreamCreateWithFlags(&captureStream_, hipStreamNonBlocking), "stream");
         hip_check(hipStreamBeginCapture(captureStream_, hipStreamCaptureModeThreadLocal), "begin capture");
+        capturing_ = true;
         try {
-            launchStep(captureStream_, 0, W_, forces);
+            launchStep(captureStream_, 0, W_, forces, 7, true);
+            capturing_ = false;
         } catch (...) {
+            capturing_ = false;
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(captureStream_, &g);
             if (g) (void)hipGraphDestroy(g);
@@ -962,6 +1041,9 @@ void Engine::executeBatch(hipStream_t s, const float4* posq, long long* force, b
         graphForces_ = forces;
     }
     hip_check(hipGraphLaunch(graphExec_, s), "graph launch");
+    if (testGrads_.empty())
+        for (auto* o : opes_)
+            if (o->d.adaptiveStride > 0) bumpWelfordMirrors(o, 0, W_);
     cvReady_ = true;
 }
 

@@ -120,7 +120,7 @@ class Engine {
    private:
     void validate(int minAtoms) const;
     void buildPlan(int nMass, const double* masses);
-    void launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMask = 7);
+    void launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMask = 7, bool fused = false);
     StepView view(const float4* posq, long long* force) const;
     void uploadGidx(int w);
     void sync(hipStream_t s);
@@ -174,6 +174,8 @@ class Engine {
     std::vector<GridHost*> grids_;
     std::vector<HillHost*> hills_;
     std::vector<OpesHost*> opes_;
+    void bumpWelfordMirrors(OpesHost* o, int w0, int nw);
+    bool capturing_ = false;
     std::vector<SmallBiasItem> smallBias_;
     std::vector<SmallHost> smallBiasHost_;
     std::vector<double> smallConst_;
@@ -189,7 +191,8 @@ class Engine {
     DBuf dPath_, dPathSlots_, dPathFrames_, dPathW_;
     DBuf dSmallBias_, dGridTable_, dSmallConst_, dSmallVar_, dSmallScal_;
     DBuf dSegStart_, dSegRow_, dRowCv_, dCombStart_, dCombSlot_, dTestGrad_, dEnergyU_;
-    std::vector<double> pathCentroidDummy_;
+    DBuf dHillTable_, dOpesTable_, dFusedPart_, dFusedCounter_;
+    FusedBiasPlan fplan_{};
 
     // graph cache for the batched step
     hipGraphExec_t graphExec_ = nullptr;

@@ -221,3 +221,46 @@ struct ScatterPlan {
 };
 
 }  // namespace glued
+
+namespace glued {
+
+constexpr int MAX_CHUNKED = 16;
+
+// The batched bias + scatter stage (one launch): chunk ranges of every
+// chunked bias (hill lists first, then OPES tables), the one-thread biases,
+// and the scatter plan for the fused scatter.
+struct FusedBiasPlan {
+    const HillBiasDev* hills;
+    const OpesBiasDev* opes;
+    int nHill, nOpes, nChunked;
+    int chunkStart[MAX_CHUNKED + 1];
+    int totalChunks;
+    double* part;            // [W][totalChunks][MAX_BIAS_D + 1]
+    unsigned int* counter;   // [W]
+    const SmallBiasItem* small;
+    int nSmall;
+    SmallBiasState st;
+    ScatterPlan sp;
+    int scatter;             // 1: the finalizing workgroup also scatters
+};
+
+}  // namespace glued
+
+namespace glued {
+
+// The batched CV stage (one launch for the group, path and few-atom CVs).
+struct CVFusedPlan {
+    const SmallCVItem* small;
+    int nSmall;
+    const GroupCVItem* group;
+    int nGroup;
+    const int* gslots;
+    const float* gmass;
+    const float* gref;
+    const PathItem* path;
+    int nPath;
+    const int* pslots;
+    const float* frames;
+};
+
+}  // namespace glued

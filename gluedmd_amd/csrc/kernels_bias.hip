@@ -9,6 +9,7 @@
 // (its relative error bounds each term by 0.37 * 2^-24 of its height).
 #include "glued_internal.h"
 #include <hip/hip_runtime.h>
+#include "scatter_dev.h"
 #include <cmath>
 
 namespace glued {
@@ -23,13 +24,15 @@ __device__ inline double wsum(double v)
 // ---------------------------------------------------------------------------
 // Grid MetaD evaluation: CommonGluedKernels.cpp:3574-3636.
 // ---------------------------------------------------------------------------
-__device__ void grid_eval(const GridBiasDev& g, const double* grid, const double* s, double* V, double* dV)
+template <int D>
+__device__ void grid_eval_t(const GridBiasDev& g, const double* grid, const double* s, double* V, double* dV)
 {
-    int lo[3], hi[3];
-    double al[3];
-    for (int d = 0; d < g.D; d++) {
+    int lo[D], hi[D];
+    double al[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
         double fr = (s[d] - g.origin[d]) * g.invSpacing[d];
-        int N = g.npts[d];
+        const int N = g.npts[d];
         if (N <= 1) {
             lo[d] = hi[d] = 0;
             al[d] = 0.0;
@@ -49,44 +52,71 @@ __device__ void grid_eval(const GridBiasDev& g, const double* grid, const double
             hi[d] = lo[d] + 1;
         }
     }
-    double v = 0.0, gr[3] = {0, 0, 0};
-    for (int c = 0; c < (1 << g.D); c++) {
+    double v = 0.0, gr[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) gr[d] = 0.0;
+#pragma unroll
+    for (int c = 0; c < (1 << D); c++) {
         int idx = 0;
-        for (int d = 0; d < g.D; d++) idx += (((c >> d) & 1) ? hi[d] : lo[d]) * g.strides[d];
-        double val = grid[idx], w = 1.0;
-        for (int d = 0; d < g.D; d++) w *= ((c >> d) & 1) ? al[d] : (1.0 - al[d]);
+#pragma unroll
+        for (int d = 0; d < D; d++) idx += (((c >> d) & 1) ? hi[d] : lo[d]) * g.strides[d];
+        const double val = grid[idx];
+        double w = 1.0;
+#pragma unroll
+        for (int d = 0; d < D; d++) w *= ((c >> d) & 1) ? al[d] : (1.0 - al[d]);
         v += w * val;
-        for (int d = 0; d < g.D; d++) {
+#pragma unroll
+        for (int d = 0; d < D; d++) {
             double wo = 1.0;
-            for (int e = 0; e < g.D; e++)
+#pragma unroll
+            for (int e = 0; e < D; e++)
                 if (e != d) wo *= ((c >> e) & 1) ? al[e] : (1.0 - al[e]);
             gr[d] += (((c >> d) & 1) ? 1.0 : -1.0) * wo * val;
         }
     }
     *V = v;
-    for (int d = 0; d < g.D; d++) dV[d] = gr[d] * g.invSpacing[d];
+#pragma unroll
+    for (int d = 0; d < D; d++) dV[d] = gr[d] * g.invSpacing[d];
+}
+
+// Grid MetaD evaluation: CommonGluedKernels.cpp:3574-3636 (D <= 3).
+__device__ inline void grid_eval(const GridBiasDev& g, const double* grid, const double* cv, double* be, double* bg)
+{
+    double s[3], dV[3], V;
+    if (g.D == 1) {
+        s[0] = cv[g.cv[0]];
+        grid_eval_t<1>(g, grid, s, &V, dV);
+        bg[0] = dV[0];
+    } else if (g.D == 2) {
+        s[0] = cv[g.cv[0]];
+        s[1] = cv[g.cv[1]];
+        grid_eval_t<2>(g, grid, s, &V, dV);
+        bg[0] = dV[0];
+        bg[1] = dV[1];
+    } else {
+        s[0] = cv[g.cv[0]];
+        s[1] = cv[g.cv[1]];
+        s[2] = cv[g.cv[2]];
+        grid_eval_t<3>(g, grid, s, &V, dV);
+        bg[0] = dV[0];
+        bg[1] = dV[1];
+        bg[2] = dV[2];
+    }
+    *be = V;
 }
 
 // ---------------------------------------------------------------------------
 // One-thread biases: grid MetaD eval, OPES expanded (:3050-3090),
 // multithermal (:3145-3172).  One thread per (item, walker).
 // ---------------------------------------------------------------------------
-__global__ void k_bias_small(StepView v, SmallBiasState st, const SmallBiasItem* __restrict__ items, int n, int w0)
+__device__ void small_bias_eval(const StepView& v, const SmallBiasState& st, const SmallBiasItem& it, int i, int w)
 {
-    const int w = w0 + blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const SmallBiasItem it = items[i];
     const double* cv = v.cvVal + (size_t)w * v.V;
     double* bg = v.bgrad + (size_t)w * v.G + it.gslot;
     double* be = v.benergy + (size_t)w * v.B + it.eslot;
     if (it.type == BIAS_METAD) {
-        const GridBiasDev g = st.grids[it.bias];
-        double s[3], V, dV[3];
-        for (int d = 0; d < g.D; d++) s[d] = cv[g.cv[d]];
-        grid_eval(g, g.grid + (size_t)w * g.walkerStride, s, &V, dV);
-        *be = V;
-        for (int d = 0; d < g.D; d++) bg[d] = dV[d];
+        const GridBiasDev& g = st.grids[it.bias];
+        grid_eval(g, g.grid + (size_t)w * g.walkerStride, cv, be, bg);
     } else if (it.type == BIAS_OPES_EXPANDED) {
         const double invKT = it.p[0];
         const double* lw = st.stateConst + it.stateOff;
@@ -116,6 +146,14 @@ __global__ void k_bias_small(StepView v, SmallBiasState st, const SmallBiasItem*
     }
 }
 
+__global__ void k_bias_small(StepView v, SmallBiasState st, const SmallBiasItem* __restrict__ items, int n, int w0)
+{
+    const int w = w0 + blockIdx.y;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    small_bias_eval(v, st, items[i], i, w);
+}
+
 void launch_bias_small(const StepView& v, const SmallBiasState& st, const SmallBiasItem* items, int n, int w0, int nw,
                        hipStream_t s)
 {
@@ -124,49 +162,51 @@ void launch_bias_small(const StepView& v, const SmallBiasState& st, const SmallB
 }
 
 // ---------------------------------------------------------------------------
-// Hill-list metadynamics evaluation (extension; truncation rule of
-// CommonGluedKernels.cpp:3664-3677 applied at the CV point):
-//   V = sum_k h_k exp(-E_k) [E_k <= 8],  E_k = sum_d dc_d^2 / (2 sigma_d^2)
-// HILL_BLOCK threads x HILLS_PER_THREAD hills per workgroup, coalesced SoA
-// loads (D centres + height = 8(D+1) bytes per hill).
+// Chunked sums: hill-list metadynamics (extension; truncation rule of
+// CommonGluedKernels.cpp:3664-3677 applied at the CV point) and OPES
+// (CommonGluedKernels.cpp:2642-2723).
+//   hills: V = sum_k h_k exp(-E_k) [E_k <= 8], E_k = sum_d dc_d^2/(2 sigma_d^2)
+//   OPES : prob = sum_k h_k max(G_k - kfloor, 0), acc_d = sum_k h_k G_k (-dc_d/s_kd^2)
+// One 256-thread workgroup reduces one chunk of 1,024 entries (4 per thread,
+// coalesced SoA loads) to D+1 FP64 partials; the workgroup that finishes
+// last for a walker (atomic ticket) gathers the partials in parallel into LDS
+// and adds them in chunk order (deterministic), then finalizes.
 // ---------------------------------------------------------------------------
-constexpr int HILL_BLOCK = 256;
-constexpr int HILLS_PER_THREAD = 4;
-constexpr int HILL_CHUNK = HILL_BLOCK * HILLS_PER_THREAD;
+constexpr int BIAS_BLOCK = 256;
+constexpr int PER_THREAD = 4;
+constexpr int CHUNK = BIAS_BLOCK * PER_THREAD;
+constexpr int HILL_BLOCK = BIAS_BLOCK;
+constexpr int OPES_BLOCK = BIAS_BLOCK;
+constexpr int PART_LDS = 1024;   // doubles of LDS for gathering partials
+constexpr int PW = MAX_BIAS_D + 1;  // partial width in the fused buffer
 
-template <int D>
-__device__ inline void hill_terms(const HillBiasDev& h, const double* __restrict__ C, const double* __restrict__ H,
-                                  int n, int cap, const double* s, int k0, double* acc)
+int hill_chunks_for(int cap) { return (cap + CHUNK - 1) / CHUNK; }
+int opes_chunks_for(int cap) { return (cap + CHUNK - 1) / CHUNK; }
+
+// Block-wide sum of NV values, written by thread 0 to dst.  sh >= NV*4.
+template <int NV>
+__device__ inline void block_partial(double* acc, double* sh, double* dst)
 {
-#pragma clang fp contract(off)
-    for (int r = 0; r < HILLS_PER_THREAD; r++) {
-        const int k = k0 + r * HILL_BLOCK;
-        if (k >= n) break;
-        double e = 0.0, dc[D];
 #pragma unroll
-        for (int d = 0; d < D; d++) {
-            double x = s[d] - C[(size_t)d * cap + k];
-            if (h.period[d] > 0.0) x = x - round(x * h.invPeriod[d]) * h.period[d];
-            dc[d] = x;
-            e = e + x * x * h.i2s2[d];
-        }
-        if (!(e <= 8.0)) continue;
-        const double t = H[k] * (double)expf(-(float)e);
-        acc[0] += t;
+    for (int d = 0; d < NV; d++) acc[d] = wsum(acc[d]);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
 #pragma unroll
-        for (int d = 0; d < D; d++) acc[1 + d] -= t * dc[d] * h.is2[d];
-    }
+        for (int d = 0; d < NV; d++) sh[d * 4 + wid] = acc[d];
+    __syncthreads();
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int d = 0; d < NV; d++) dst[d] = sh[d * 4] + sh[d * 4 + 1] + sh[d * 4 + 2] + sh[d * 4 + 3];
+    __syncthreads();
 }
 
 template <int D>
-__global__ __launch_bounds__(HILL_BLOCK) void k_bias_hills(StepView v, HillBiasDev h, int w0)
+__device__ void hill_chunk(const StepView& v, const HillBiasDev& h, int w, int chunk, double* sh, double* dst)
 {
-    __shared__ double sh[(D + 1) * 4];
-    __shared__ int shLast;
-    const int w = w0 + blockIdx.y;
+#pragma clang fp contract(off)
     const int tb = h.shared ? 0 : w;
-    const double* C = h.centers + (size_t)tb * D * h.cap;
-    const double* H = h.heights + (size_t)tb * h.cap;
+    const double* __restrict__ C = h.centers + (size_t)tb * D * h.cap;
+    const double* __restrict__ H = h.heights + (size_t)tb * h.cap;
     const int n = h.count[tb];
     const double* cv = v.cvVal + (size_t)w * v.V;
     double s[D];
@@ -175,69 +215,43 @@ __global__ __launch_bounds__(HILL_BLOCK) void k_bias_hills(StepView v, HillBiasD
     double acc[D + 1];
 #pragma unroll
     for (int d = 0; d <= D; d++) acc[d] = 0.0;
-    for (int c = blockIdx.x; c * HILL_CHUNK < n; c += gridDim.x)
-        hill_terms<D>(h, C, H, n, h.cap, s, c * HILL_CHUNK + threadIdx.x, acc);
+    const int k0 = chunk * CHUNK + threadIdx.x;
+    // issue every load of the chunk before the arithmetic
+    double cx[PER_THREAD][D], hk[PER_THREAD];
 #pragma unroll
-    for (int d = 0; d <= D; d++) acc[d] = wsum(acc[d]);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0)
-        for (int d = 0; d <= D; d++) sh[d * 4 + wid] = acc[d];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double* part = h.part + ((size_t)w * h.chunks + blockIdx.x) * (D + 1);
-        for (int d = 0; d <= D; d++) part[d] = sh[d * 4] + sh[d * 4 + 1] + sh[d * 4 + 2] + sh[d * 4 + 3];
-        __threadfence();
-        unsigned done = atomicAdd(&h.counter[w], 1u);
-        shLast = done == gridDim.x - 1;
+    for (int r = 0; r < PER_THREAD; r++) {
+        const int k = k0 + r * BIAS_BLOCK;
+        const bool ok = k < n;
+#pragma unroll
+        for (int d = 0; d < D; d++) cx[r][d] = ok ? C[(size_t)d * h.cap + k] : 0.0;
+        hk[r] = ok ? H[k] : 0.0;
     }
-    __syncthreads();
-    if (shLast && threadIdx.x == 0) {
-        __threadfence();
-        double tot[D + 1];
-        for (int d = 0; d <= D; d++) tot[d] = 0.0;
-        for (unsigned b = 0; b < gridDim.x; b++) {
-            const double* part = h.part + ((size_t)w * h.chunks + b) * (D + 1);
-            for (int d = 0; d <= D; d++)
-                tot[d] += __hip_atomic_load(&part[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+    for (int r = 0; r < PER_THREAD; r++) {
+        double e = 0.0, dc[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            double x = s[d] - cx[r][d];
+            if (h.period[d] > 0.0) x = x - rint(x * h.invPeriod[d]) * h.period[d];
+            dc[d] = x;
+            e = e + x * x * h.i2s2[d];
         }
-        v.benergy[(size_t)w * v.B + h.eslot] = tot[0];
-        for (int d = 0; d < D; d++) v.bgrad[(size_t)w * v.G + h.gslot + d] = tot[1 + d];
-        h.counter[w] = 0u;
+        // hk = 0 for entries past the end: they add exactly zero
+        const double t = (e <= 8.0) ? hk[r] * (double)expf(-(float)e) : 0.0;
+        acc[0] += t;
+#pragma unroll
+        for (int d = 0; d < D; d++) acc[1 + d] -= t * dc[d] * h.is2[d];
     }
+    block_partial<D + 1>(acc, sh, dst);
 }
-
-void launch_bias_hills(const StepView& v, const HillBiasDev& h, int w0, int nw, hipStream_t s)
-{
-    dim3 g(h.chunks, nw), b(HILL_BLOCK);
-    switch (h.D) {
-        case 1: hipLaunchKernelGGL(k_bias_hills<1>, g, b, 0, s, v, h, w0); break;
-        case 2: hipLaunchKernelGGL(k_bias_hills<2>, g, b, 0, s, v, h, w0); break;
-        case 3: hipLaunchKernelGGL(k_bias_hills<3>, g, b, 0, s, v, h, w0); break;
-        default: hipLaunchKernelGGL(k_bias_hills<4>, g, b, 0, s, v, h, w0); break;
-    }
-}
-
-int hill_chunks_for(int cap) { return (cap + HILL_CHUNK - 1) / HILL_CHUNK; }
-
-// ---------------------------------------------------------------------------
-// OPES evaluation: CommonGluedKernels.cpp:2642-2723, parallel over kernels.
-// ---------------------------------------------------------------------------
-constexpr int OPES_BLOCK = 256;
-constexpr int OPES_PER_THREAD = 4;
-constexpr int OPES_CHUNK = OPES_BLOCK * OPES_PER_THREAD;
-
-int opes_chunks_for(int cap) { return (cap + OPES_CHUNK - 1) / OPES_CHUNK; }
 
 template <int D>
-__global__ __launch_bounds__(OPES_BLOCK) void k_bias_opes(StepView v, OpesBiasDev o, int w0)
+__device__ void opes_chunk(const StepView& v, const OpesBiasDev& o, int w, int chunk, double* sh, double* dst)
 {
-    __shared__ double sh[(D + 1) * 4];
-    __shared__ int shLast;
-    const int w = w0 + blockIdx.y;
     const int tb = o.shared ? 0 : w;
-    const double* C = o.centers + (size_t)tb * D * o.cap;
-    const double* Sg = o.sigmas + (size_t)tb * D * o.cap;
-    const double* L = o.logw + (size_t)tb * o.cap;
+    const double* __restrict__ C = o.centers + (size_t)tb * D * o.cap;
+    const double* __restrict__ Sg = o.sigmas + (size_t)tb * D * o.cap;
+    const double* __restrict__ L = o.logw + (size_t)tb * o.cap;
     const int nk = o.ints[tb * 4 + 0];
     const double* cv = v.cvVal + (size_t)w * v.V;
     double s[D];
@@ -247,79 +261,207 @@ __global__ __launch_bounds__(OPES_BLOCK) void k_bias_opes(StepView v, OpesBiasDe
     double acc[D + 1];
 #pragma unroll
     for (int d = 0; d <= D; d++) acc[d] = 0.0;
-    for (int c = blockIdx.x; c * OPES_CHUNK < nk; c += gridDim.x) {
-        for (int r = 0; r < OPES_PER_THREAD; r++) {
-            const int k = c * OPES_CHUNK + r * OPES_BLOCK + threadIdx.x;
-            if (k >= nk) break;
-            double n2 = 0.0, dc[D], sg[D];
+    for (int r = 0; r < PER_THREAD; r++) {
+        const int k = chunk * CHUNK + r * BIAS_BLOCK + threadIdx.x;
+        if (k >= nk) break;
+        double n2 = 0.0, dc[D], sg[D];
 #pragma unroll
-            for (int d = 0; d < D; d++) {
+        for (int d = 0; d < D; d++) {
 #pragma clang fp contract(off)
-                dc[d] = s[d] - C[(size_t)d * o.cap + k];
-                sg[d] = Sg[(size_t)d * o.cap + k];
-                n2 = n2 + dc[d] * dc[d] / (sg[d] * sg[d]);
-            }
-            if (!(n2 < o.cutoff2)) continue;
-            const double G = exp(-0.5 * n2), kern = G - kfloor;
-            if (kern <= 0.0) continue;
-            const double hk = exp(L[k]);
-            if (!(hk > 0.0 && hk < 1e308)) continue;
-            acc[0] += hk * kern;
-#pragma unroll
-            for (int d = 0; d < D; d++) acc[1 + d] += hk * G * (-dc[d] / (sg[d] * sg[d]));
+            dc[d] = s[d] - C[(size_t)d * o.cap + k];
+            sg[d] = Sg[(size_t)d * o.cap + k];
+            n2 = n2 + dc[d] * dc[d] / (sg[d] * sg[d]);
         }
-    }
+        if (!(n2 < o.cutoff2)) continue;
+        const double G = exp(-0.5 * n2), kern = G - kfloor;
+        if (kern <= 0.0) continue;
+        const double hk = exp(L[k]);
+        if (!(hk > 0.0 && hk < 1e308)) continue;
+        acc[0] += hk * kern;
 #pragma unroll
-    for (int d = 0; d <= D; d++) acc[d] = wsum(acc[d]);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0)
-        for (int d = 0; d <= D; d++) sh[d * 4 + wid] = acc[d];
+        for (int d = 0; d < D; d++) acc[1 + d] += hk * G * (-dc[d] / (sg[d] * sg[d]));
+    }
+    block_partial<D + 1>(acc, sh, dst);
+}
+
+// Sum n partial vectors of width wd (written by other workgroups), in index
+// order, into tot (thread 0).  Loads run in parallel through LDS.
+__device__ void sum_partials(const double* src, int n, int wd, double* tot, double* shbuf)
+{
+    if (threadIdx.x == 0)
+        for (int d = 0; d < wd; d++) tot[d] = 0.0;
+    const int per = PART_LDS / wd;
+    for (int base = 0; base < n; base += per) {
+        const int m = min(n - base, per);
+        for (int i = threadIdx.x; i < m * wd; i += blockDim.x)
+            shbuf[i] = __hip_atomic_load(src + (size_t)base * wd + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (int j = 0; j < m; j++)
+                for (int d = 0; d < wd; d++) tot[d] += shbuf[j * wd + d];
+        __syncthreads();
+    }
+}
+
+__device__ inline void hill_finalize(const StepView& v, const HillBiasDev& h, int w, const double* tot)
+{
+    v.benergy[(size_t)w * v.B + h.eslot] = tot[0];
+    for (int d = 0; d < h.D; d++) v.bgrad[(size_t)w * v.G + h.gslot + d] = tot[1 + d];
+}
+
+__device__ inline void opes_finalize(const StepView& v, const OpesBiasDev& o, int w, const double* tot)
+{
+    const int tb = o.shared ? 0 : w;
+    const int nk = o.ints[tb * 4 + 0];
+    double* be = v.benergy + (size_t)w * v.B + o.eslot;
+    double* bg = v.bgrad + (size_t)w * v.G + o.gslot;
+    if (nk == 0) {
+        *be = -o.barrier;
+        for (int d = 0; d < o.D; d++) bg[d] = 0.0;
+        return;
+    }
+    const double prob = tot[0];
+    const double su = o.dbl[(size_t)tb * (4 + 2 * o.D) + 0];
+    double pz = su > 0.0 ? prob * (double)nk / su : 0.0;
+    if (!(pz >= 0.0)) pz = 0.0;
+    const double arg = pz + o.eps;
+    *be = o.invGF * o.kT * log(arg > 1e-30 ? arg : 1e-30);
+    const double f = prob > 0.0 ? o.invGF * o.kT * (pz / (pz + o.eps)) / prob : 0.0;
+    for (int d = 0; d < o.D; d++) bg[d] = f * tot[1 + d];
+}
+
+// Ticket: true in every thread of the workgroup that finished last.
+__device__ inline bool last_block(unsigned int* counter, unsigned int total)
+{
+    __shared__ int shLast;
+    __threadfence();
     __syncthreads();
+    if (threadIdx.x == 0) shLast = atomicAdd(counter, 1u) == total - 1;
+    __syncthreads();
+    if (shLast) __threadfence();
+    return shLast != 0;
+}
+
+// ---- plain form: one launch per chunked bias ----
+template <int D>
+__global__ __launch_bounds__(BIAS_BLOCK) void k_bias_hills(StepView v, HillBiasDev h, int w0)
+{
+    __shared__ double sh[(D + 1) * 4];
+    __shared__ double shbuf[PART_LDS];
+    __shared__ double tot[D + 1];
+    const int w = w0 + blockIdx.y;
+    hill_chunk<D>(v, h, w, blockIdx.x, sh, h.part + ((size_t)w * h.chunks + blockIdx.x) * (D + 1));
+    if (!last_block(&h.counter[w], gridDim.x)) return;
+    sum_partials(h.part + (size_t)w * h.chunks * (D + 1), gridDim.x, D + 1, tot, shbuf);
     if (threadIdx.x == 0) {
-        double* part = o.part + ((size_t)w * o.chunks + blockIdx.x) * (D + 1);
-        for (int d = 0; d <= D; d++) part[d] = sh[d * 4] + sh[d * 4 + 1] + sh[d * 4 + 2] + sh[d * 4 + 3];
-        __threadfence();
-        unsigned done = atomicAdd(&o.counter[w], 1u);
-        shLast = done == gridDim.x - 1;
+        hill_finalize(v, h, w, tot);
+        h.counter[w] = 0u;
     }
-    __syncthreads();
-    if (shLast && threadIdx.x == 0) {
-        __threadfence();
-        double tot[D + 1];
-        for (int d = 0; d <= D; d++) tot[d] = 0.0;
-        for (unsigned b = 0; b < gridDim.x; b++) {
-            const double* part = o.part + ((size_t)w * o.chunks + b) * (D + 1);
-            for (int d = 0; d <= D; d++)
-                tot[d] += __hip_atomic_load(&part[d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        double* be = v.benergy + (size_t)w * v.B + o.eslot;
-        double* bg = v.bgrad + (size_t)w * v.G + o.gslot;
-        if (nk == 0) {
-            *be = -o.barrier;
-            for (int d = 0; d < D; d++) bg[d] = 0.0;
-        } else {
-            const double prob = tot[0];
-            const double su = o.dbl[(size_t)tb * (4 + 2 * D) + 0];
-            double pz = su > 0.0 ? prob * (double)nk / su : 0.0;
-            if (!(pz >= 0.0)) pz = 0.0;
-            const double arg = pz + o.eps;
-            *be = o.invGF * o.kT * log(arg > 1e-30 ? arg : 1e-30);
-            const double f = prob > 0.0 ? o.invGF * o.kT * (pz / (pz + o.eps)) / prob : 0.0;
-            for (int d = 0; d < D; d++) bg[d] = f * tot[1 + d];
-        }
+}
+
+template <int D>
+__global__ __launch_bounds__(BIAS_BLOCK) void k_bias_opes(StepView v, OpesBiasDev o, int w0)
+{
+    __shared__ double sh[(D + 1) * 4];
+    __shared__ double shbuf[PART_LDS];
+    __shared__ double tot[D + 1];
+    const int w = w0 + blockIdx.y;
+    opes_chunk<D>(v, o, w, blockIdx.x, sh, o.part + ((size_t)w * o.chunks + blockIdx.x) * (D + 1));
+    if (!last_block(&o.counter[w], gridDim.x)) return;
+    sum_partials(o.part + (size_t)w * o.chunks * (D + 1), gridDim.x, D + 1, tot, shbuf);
+    if (threadIdx.x == 0) {
+        opes_finalize(v, o, w, tot);
         o.counter[w] = 0u;
+    }
+}
+
+void launch_bias_hills(const StepView& v, const HillBiasDev& h, int w0, int nw, hipStream_t s)
+{
+    dim3 g(h.chunks, nw), b(BIAS_BLOCK);
+    switch (h.D) {
+        case 1: hipLaunchKernelGGL(k_bias_hills<1>, g, b, 0, s, v, h, w0); break;
+        case 2: hipLaunchKernelGGL(k_bias_hills<2>, g, b, 0, s, v, h, w0); break;
+        case 3: hipLaunchKernelGGL(k_bias_hills<3>, g, b, 0, s, v, h, w0); break;
+        default: hipLaunchKernelGGL(k_bias_hills<4>, g, b, 0, s, v, h, w0); break;
     }
 }
 
 void launch_bias_opes(const StepView& v, const OpesBiasDev& o, int w0, int nw, hipStream_t s)
 {
-    dim3 g(o.chunks, nw), b(OPES_BLOCK);
+    dim3 g(o.chunks, nw), b(BIAS_BLOCK);
     switch (o.D) {
         case 1: hipLaunchKernelGGL(k_bias_opes<1>, g, b, 0, s, v, o, w0); break;
         case 2: hipLaunchKernelGGL(k_bias_opes<2>, g, b, 0, s, v, o, w0); break;
         case 3: hipLaunchKernelGGL(k_bias_opes<3>, g, b, 0, s, v, o, w0); break;
         default: hipLaunchKernelGGL(k_bias_opes<4>, g, b, 0, s, v, o, w0); break;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Batched form: every chunk of every chunked bias of every walker in one
+// launch; the last workgroup of a walker finalizes the chunked biases,
+// evaluates the one-thread biases, forms dU/dCV and (when the walker's atom
+// count is small) performs the force scatter -- one launch for the whole
+// bias + scatter stage.  Chunking and summation order are those of the plain
+// kernels, so both forms give bit-identical results.
+// ---------------------------------------------------------------------------
+constexpr int MAX_LDS_VALUES = 1024;
+
+__global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias(StepView v, FusedBiasPlan fp)
+{
+    __shared__ double sh[PW * 4];
+    __shared__ double shbuf[PART_LDS];
+    __shared__ double sU[MAX_LDS_VALUES];
+    const int w = blockIdx.y;
+    const int b = blockIdx.x;
+    if (b < fp.totalChunks) {
+        int k = 0;
+        while (k + 1 < fp.nChunked && b >= fp.chunkStart[k + 1]) k++;
+        const int chunk = b - fp.chunkStart[k];
+        double* dst = fp.part + ((size_t)w * fp.totalChunks + b) * PW;
+        if (k < fp.nHill) {
+            const HillBiasDev& h = fp.hills[k];
+            switch (h.D) {
+                case 1: hill_chunk<1>(v, h, w, chunk, sh, dst); break;
+                case 2: hill_chunk<2>(v, h, w, chunk, sh, dst); break;
+                case 3: hill_chunk<3>(v, h, w, chunk, sh, dst); break;
+                default: hill_chunk<4>(v, h, w, chunk, sh, dst); break;
+            }
+        } else {
+            const OpesBiasDev& o = fp.opes[k - fp.nHill];
+            switch (o.D) {
+                case 1: opes_chunk<1>(v, o, w, chunk, sh, dst); break;
+                case 2: opes_chunk<2>(v, o, w, chunk, sh, dst); break;
+                case 3: opes_chunk<3>(v, o, w, chunk, sh, dst); break;
+                default: opes_chunk<4>(v, o, w, chunk, sh, dst); break;
+            }
+        }
+    }
+    if (!last_block(&fp.counter[w], gridDim.x)) return;
+    // ---- finalize the walker ----
+    __shared__ double tot[PW];
+    for (int k = 0; k < fp.nChunked; k++) {
+        const int c0 = fp.chunkStart[k], nc = fp.chunkStart[k + 1] - c0;
+        sum_partials(fp.part + ((size_t)w * fp.totalChunks + c0) * PW, nc, PW, tot, shbuf);
+        if (threadIdx.x == 0) {
+            if (k < fp.nHill)
+                hill_finalize(v, fp.hills[k], w, tot);
+            else
+                opes_finalize(v, fp.opes[k - fp.nHill], w, tot);
+        }
+    }
+    for (int i = threadIdx.x; i < fp.nSmall; i += blockDim.x) small_bias_eval(v, fp.st, fp.small[i], i, w);
+    __syncthreads();
+    if (threadIdx.x == 0) fp.counter[w] = 0u;
+    if (fp.scatter)
+        scatter_walker_block(v, fp.sp, w, sU, MAX_LDS_VALUES);
+    else
+        combine_dudcv(v, fp.sp, w, sU, MAX_LDS_VALUES, true);
+}
+
+void launch_step_bias(const StepView& v, const FusedBiasPlan& fp, int nw, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_step_bias, dim3(fp.totalChunks > 0 ? fp.totalChunks : 1, nw), dim3(BIAS_BLOCK), 0, s, v, fp);
 }
 
 // ---------------------------------------------------------------------------

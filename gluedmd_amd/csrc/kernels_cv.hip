@@ -99,12 +99,8 @@ __device__ inline void store_row(const StepView& v, int w, int row, double gx, d
 // Few-atom CVs, one thread per (item, walker), FP64.
 // distance :177-229, angle :238-325, dihedral :335-444, position :1084-1114
 // ---------------------------------------------------------------------------
-__global__ void k_cv_small(StepView v, const SmallCVItem* __restrict__ items, int n, int w0)
+__device__ void small_item(const StepView& v, const SmallCVItem& it, int w)
 {
-    const int w = w0 + blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const SmallCVItem it = items[i];
     if (it.type == CV_ENERGY) {  // value supplied by the caller; Jacobian via the dense scatter
         v.cvVal[(size_t)w * v.V + it.value] = v.energyU[w];
         return;
@@ -196,6 +192,13 @@ __global__ void k_cv_small(StepView v, const SmallCVItem* __restrict__ items, in
         store_row(v, w, it.row0, it.comp == 0, it.comp == 1, it.comp == 2);
     }
     cvv[it.value] = val;
+}
+
+__global__ void k_cv_small(StepView v, const SmallCVItem* __restrict__ items, int n, int w0)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    small_item(v, items[i], w0 + blockIdx.y);
 }
 
 void launch_cv_small(const StepView& v, const SmallCVItem* items, int n, int w0, int nw, hipStream_t s)
@@ -324,15 +327,11 @@ __device__ inline double ipowd(double x, int n)
     return r;
 }
 
-__global__ __launch_bounds__(GROUP_BLOCK) void k_cv_group(StepView v, const GroupCVItem* __restrict__ items,
-                                                          const int* __restrict__ slots,
-                                                          const float* __restrict__ mass,
-                                                          const float* __restrict__ ref, int w0)
+__device__ void group_item(const StepView& v, const GroupCVItem& it, const int* __restrict__ slots,
+                           const float* __restrict__ mass, const float* __restrict__ ref, int w)
 {
     __shared__ double sh[16 * 16];
     __shared__ double shR[9];
-    const int w = w0 + blockIdx.y;
-    const GroupCVItem it = items[blockIdx.x];
     const double* bx = v.box + (size_t)w * 12;
     double* cvv = v.cvVal + (size_t)w * v.V;
     const int tid = threadIdx.x, nt = blockDim.x;
@@ -529,6 +528,14 @@ __global__ __launch_bounds__(GROUP_BLOCK) void k_cv_group(StepView v, const Grou
         if (tid == 0) cvv[it.value] = s[0];
         return;
     }
+}
+
+__global__ __launch_bounds__(GROUP_BLOCK) void k_cv_group(StepView v, const GroupCVItem* __restrict__ items,
+                                                          const int* __restrict__ slots,
+                                                          const float* __restrict__ mass,
+                                                          const float* __restrict__ ref, int w0)
+{
+    group_item(v, items[blockIdx.x], slots, mass, ref, w0 + blockIdx.y);
 }
 
 void launch_cv_group(const StepView& v, const GroupCVItem* items, int n, const int* slots, const float* mass,
@@ -802,16 +809,30 @@ __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBu
         shLast = (done == gridDim.x - 1);
     }
     __syncthreads();
-    if (shLast && threadIdx.x == 0) {
-        __threadfence();
-        double c = 0.0;
-        unsigned long long p = 0;
-        for (unsigned b = 0; b < gridDim.x; b++) {
-            c += __hip_atomic_load(&cb.partCN[(size_t)w * cb.partStride + it.partBase + b], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
-            p += __hip_atomic_load(&cb.partPairs[(size_t)w * cb.partStride + it.partBase + b], __ATOMIC_RELAXED,
-                                   __HIP_MEMORY_SCOPE_AGENT);
+    if (!shLast) return;
+    __threadfence();
+    // gather the block partials in parallel, add them in block order
+    __shared__ double shc[COORD_BLOCK];
+    __shared__ unsigned long long shp[COORD_BLOCK];
+    double c = 0.0;
+    unsigned long long p = 0;
+    const double* pc = cb.partCN + (size_t)w * cb.partStride + it.partBase;
+    const unsigned long long* pp = cb.partPairs + (size_t)w * cb.partStride + it.partBase;
+    for (unsigned base = 0; base < gridDim.x; base += COORD_BLOCK) {
+        const unsigned m = min((unsigned)COORD_BLOCK, gridDim.x - base);
+        if (threadIdx.x < m) {
+            shc[threadIdx.x] = __hip_atomic_load(pc + base + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            shp[threadIdx.x] = __hip_atomic_load(pp + base + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
+        __syncthreads();
+        if (threadIdx.x == 0)
+            for (unsigned j = 0; j < m; j++) {
+                c += shc[j];
+                p += shp[j];
+            }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
         v.cvVal[(size_t)w * v.V + it.value] = c;
         cb.npairs[(size_t)w * cb.nItems + item] = (long long)p;
         cb.counters[(size_t)w * cb.nItems + item] = 0u;  // self-reset for the next step
@@ -852,13 +873,10 @@ namespace glued {
 // ---------------------------------------------------------------------------
 constexpr int PATH_BLOCK = 256;
 
-__global__ __launch_bounds__(PATH_BLOCK) void k_cv_path(StepView v, const PathItem* __restrict__ items,
-                                                        const int* __restrict__ slots,
-                                                        const float* __restrict__ frames, int w0)
+__device__ void path_item(const StepView& v, const PathItem& it, const int* __restrict__ slots,
+                          const float* __restrict__ frames, int w)
 {
     __shared__ double shPart[PATH_BLOCK];
-    const int w = w0 + blockIdx.y;
-    const PathItem it = items[blockIdx.x];
     const int M = it.M, N = it.N;
     const float* F = frames + it.frameOff;
     double S = 0.0, sNum = 0.0;
@@ -928,11 +946,46 @@ __global__ __launch_bounds__(PATH_BLOCK) void k_cv_path(StepView v, const PathIt
     }
 }
 
+__global__ __launch_bounds__(PATH_BLOCK) void k_cv_path(StepView v, const PathItem* __restrict__ items,
+                                                        const int* __restrict__ slots,
+                                                        const float* __restrict__ frames, int w0)
+{
+    path_item(v, items[blockIdx.x], slots, frames, w0 + blockIdx.y);
+}
+
 void launch_cv_path(const StepView& v, const PathItem* items, int n, const int* slots, const float* frames, int w0,
                     int nw, hipStream_t s)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_cv_path, dim3(n, nw), dim3(PATH_BLOCK), 0, s, v, items, slots, frames, w0);
+}
+
+// ---------------------------------------------------------------------------
+// Batched CV stage: every group CV, path CV and few-atom CV of every walker
+// in one launch (blocks [0, nGroup) group items, then path items, then
+// blocks of 256 few-atom items).  Same device code as the per-kind kernels.
+// ---------------------------------------------------------------------------
+static_assert(GROUP_BLOCK == PATH_BLOCK, "fused CV stage uses one block size");
+
+__global__ __launch_bounds__(GROUP_BLOCK) void k_cv_fused(StepView v, CVFusedPlan p)
+{
+    const int w = blockIdx.y;
+    const int b = blockIdx.x;
+    if (b < p.nGroup) {
+        group_item(v, p.group[b], p.gslots, p.gmass, p.gref, w);
+    } else if (b < p.nGroup + p.nPath) {
+        path_item(v, p.path[b - p.nGroup], p.pslots, p.frames, w);
+    } else {
+        const int i = (b - p.nGroup - p.nPath) * GROUP_BLOCK + threadIdx.x;
+        if (i < p.nSmall) small_item(v, p.small[i], w);
+    }
+}
+
+void launch_cv_fused(const StepView& v, const CVFusedPlan& p, int nw, hipStream_t s)
+{
+    const int nb = p.nGroup + p.nPath + (p.nSmall + GROUP_BLOCK - 1) / GROUP_BLOCK;
+    if (nb <= 0) return;
+    hipLaunchKernelGGL(k_cv_fused, dim3(nb, nw), dim3(GROUP_BLOCK), 0, s, v, p);
 }
 
 }  // namespace glued

@@ -12,6 +12,7 @@
 // into an auto-cleared buffer), or copy the injected test gradients.
 #include "glued_internal.h"
 #include <hip/hip_runtime.h>
+#include "scatter_dev.h"
 
 namespace glued {
 
@@ -20,48 +21,15 @@ namespace glued {
 constexpr int SCATTER_BLOCK = 256;
 constexpr int MAX_LDS_VALUES = 1024;
 
-__device__ inline void atomic_add_fixed(long long* p, double f)
-{
-    const long long q = (long long)(f * FIXED_SCALE);
-    atomicAdd((unsigned long long*)p, (unsigned long long)q);
-}
-
 __global__ __launch_bounds__(SCATTER_BLOCK) void k_scatter(StepView v, ScatterPlan sp, int w0, int includeForces)
 {
     __shared__ double sU[MAX_LDS_VALUES];
     const int w = w0 + blockIdx.y;
-    for (int c = threadIdx.x; c < v.V; c += blockDim.x) {
-        double u = 0.0;
-        if (sp.testGrad) {
-            u = sp.testGrad[c];
-        } else {
-            for (int k = sp.combStart[c]; k < sp.combStart[c + 1]; k++) u += v.bgrad[(size_t)w * v.G + sp.combSlot[k]];
-        }
-        if (c < MAX_LDS_VALUES) sU[c] = u;
-        if (blockIdx.x == 0) v.dUdCV[(size_t)w * v.V + c] = u;
-    }
-    __syncthreads();
+    combine_dudcv(v, sp, w, sU, MAX_LDS_VALUES, blockIdx.x == 0);
     if (!includeForces) return;
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= v.S) return;
-    const int b = sp.segStart[s], e = sp.segStart[s + 1];
-    if (b == e) return;
-    double fx = 0.0, fy = 0.0, fz = 0.0;
-    const size_t base = (size_t)w * v.R;
-    for (int k = b; k < e; k++) {
-        const int r = sp.segRow[k];
-        const int c = sp.rowCv[r];
-        const double u = c < MAX_LDS_VALUES ? sU[c] : v.dUdCV[(size_t)w * v.V + c];
-        fx -= u * (double)v.jx[base + r];
-        fy -= u * (double)v.jy[base + r];
-        fz -= u * (double)v.jz[base + r];
-    }
-    const WalkerInfo wi = v.walkers[w];
-    const int g = v.gidx[(size_t)w * v.S + s];
-    long long* F = v.force + wi.forceOff;
-    atomic_add_fixed(F + g, fx);
-    atomic_add_fixed(F + g + wi.padded, fy);
-    atomic_add_fixed(F + g + 2 * (size_t)wi.padded, fz);
+    scatter_slot(v, sp, w, s, sU, MAX_LDS_VALUES);
 }
 
 // Dense scatter of an energy CV whose Jacobian is -F_ext over every atom:

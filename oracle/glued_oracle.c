@@ -660,7 +660,8 @@ EXPORT void orc_metad_grid_deposit(double* grid, int D, int total, const double*
 /* only).  Defined as the grid deposit rule of CommonGluedKernels.cpp:       */
 /* 3664-3677 evaluated at the CV point itself instead of at grid nodes:      */
 /*   E_k = sum_d dc_d^2 * (1/(2 sigma_d^2)), dc periodic min-image when      */
-/*   period_d > 0 (dc -= round(dc * (1/P)) P); V = sum_k h_k e^-E_k [E_k<=8]; */
+/*   period_d > 0 (dc -= rint(dc * (1/P)) P, ties to even);                   */
+/*   V = sum_k h_k e^-E_k [E_k<=8];                                          */
 /*   dV/ds_d = sum_k h_k exp(-E_k) [E_k<=8] (-dc_d / sigma_d^2).             */
 /* centers[k*D+d], heights[k].  Pinned by equality with grid MetaD at grid   */
 /* nodes (tests) and brute force.                                            */
@@ -678,7 +679,7 @@ EXPORT double orc_hills_eval(const double* cv, int D, int nh, const double* cent
         double e = 0.0, dc[8];
         for (int d = 0; d < D; d++) {
             double x = cv[d] - centers[(size_t)k * D + d];
-            if (period[d] > 0.0) x -= round(x * ip[d]) * period[d];
+            if (period[d] > 0.0) x -= rint(x * ip[d]) * period[d];
             dc[d] = x;
             e += x * x * i2s2[d];
         }
@@ -706,7 +707,7 @@ EXPORT double orc_hill_min_margin(const double* cv, int D, int nh, const double*
         double e = 0.0;
         for (int d = 0; d < D; d++) {
             double x = cv[d] - centers[(size_t)k * D + d];
-            if (period[d] > 0.0) x -= round(x * ip[d]) * period[d];
+            if (period[d] > 0.0) x -= rint(x * ip[d]) * period[d];
             e += x * x * i2s2[d];
         }
         double a = fabs(e - 8.0);
