@@ -8,12 +8,13 @@ torsions, well-tempered MetaD on (RMSD, phi) with a shared 200x200 grid and
 dV/ds, and the fixed-point force scatter, for one walker.
 
   value        batched evals/s: all 64 walkers per step as one captured
-               hipGraph (CV stage, bias stage, scatter stage)
+               hipGraph (CV kernel, hill-chunk kernel, finalize + scatter
+               kernel)
   plain        the reference's structure: one walker per launch sequence
                (one launch per CV kind, per bias, and the scatter), host loop
                over walkers
-  roofline     the streamed hill-sum kernel (HBM-bound), timed live with
-               HIP events on the launch stream
+  roofline     the batched hill-sum chunk kernel (HBM-bound), timed live
+               with HIP events on the launch stream
   cpu_baseline the FP64 oracle on a bounded sample of the same walkers, one
                host core; its forces also give the force error vs FP64
 
@@ -106,7 +107,7 @@ def main():
     torch.cuda.synchronize()
     t_dep = (time.perf_counter() - t0) / dep_steps
 
-    # ---- roofline: the streamed hill-sum kernel, HIP events on the launch stream ----
+    # ---- roofline: the batched hill-sum chunk kernel, HIP events on the launch stream ----
     stream = torch.cuda.current_stream(dev)
     reps = 200
     for _ in range(10):
@@ -150,7 +151,7 @@ def main():
                         "WT MetaD shared 200x200 grid + 20k private hills/walker",
             "walkers": W,
             "hills_per_walker": args.hills,
-            "parallelism": "batched walkers, one hipGraph per step",
+            "parallelism": "batched walkers, one hipGraph per step (3 kernels)",
         },
         "plain_evals_per_s": round(W / t_plain, 1),
         "batched_over_plain": round(t_plain / t_batched, 2),
@@ -158,7 +159,7 @@ def main():
         "stage_us": stage_t,
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_bias_hills<2>",
+            "kernel": "k_step_bias_chunks (hill sums, D=2)",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -32,6 +32,7 @@ int opes_chunks_for(int);
 void launch_scatter(const StepView&, const ScatterPlan&, int, int, int, hipStream_t);
 void launch_scatter_dense(const StepView&, const ScatterPlan&, int, int, int, hipStream_t);
 void launch_step_bias(const StepView&, const FusedBiasPlan&, int, hipStream_t);
+void launch_step_bias_chunks(const StepView&, const FusedBiasPlan&, int, hipStream_t);
 void launch_cv_fused(const StepView&, const CVFusedPlan&, int, hipStream_t);
 
 void hip_check(hipError_t e, const char* what)
@@ -994,6 +995,16 @@ void Engine::executeStage(hipStream_t s, const float4* posq, long long* force, i
     if (stage < 0 || stage > 3) fail("execute_stage: stage must be 0, 1, 2 or 3");
     lastPos_ = posq;
     lastForce_ = force;
+    if (stage == 3) {
+        // the batched chunk kernel alone (every hill / OPES chunk of every walker)
+        if (fplan_.totalChunks > 0 && testGrads_.empty()) {
+            StepView v = view(lastPos_, lastForce_);
+            v.S = S_;
+            launch_step_bias_chunks(v, fplan_, W_, s);
+        }
+        hip_check(hipGetLastError(), "kernel launch");
+        return;
+    }
     launchStep(s, 0, W_, true, 1 << stage);
 }
 

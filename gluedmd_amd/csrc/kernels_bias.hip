@@ -330,15 +330,20 @@ __device__ inline void opes_finalize(const StepView& v, const OpesBiasDev& o, in
     for (int d = 0; d < o.D; d++) bg[d] = f * tot[1 + d];
 }
 
-// Ticket: true in every thread of the workgroup that finished last.
+// Ticket: true in every thread of the workgroup that finished last.  Only
+// thread 0 wrote this workgroup's partial, so only it fences (an agent-scope
+// fence writes back the XCD's L2 and is expensive on this part: one per
+// workgroup, never one per wave).
 __device__ inline bool last_block(unsigned int* counter, unsigned int total)
 {
     __shared__ int shLast;
-    __threadfence();
     __syncthreads();
-    if (threadIdx.x == 0) shLast = atomicAdd(counter, 1u) == total - 1;
+    if (threadIdx.x == 0) {
+        __threadfence();
+        shLast = atomicAdd(counter, 1u) == total - 1;
+        if (shLast) __threadfence();
+    }
     __syncthreads();
-    if (shLast) __threadfence();
     return shLast != 0;
 }
 
@@ -398,70 +403,93 @@ void launch_bias_opes(const StepView& v, const OpesBiasDev& o, int w0, int nw, h
 }
 
 // ---------------------------------------------------------------------------
-// Batched form: every chunk of every chunked bias of every walker in one
-// launch; the last workgroup of a walker finalizes the chunked biases,
-// evaluates the one-thread biases, forms dU/dCV and (when the walker's atom
-// count is small) performs the force scatter -- one launch for the whole
-// bias + scatter stage.  Chunking and summation order are those of the plain
-// kernels, so both forms give bit-identical results.
+// Batched form, two launches for the whole bias + scatter stage: every chunk
+// of every chunked bias of every walker, then one workgroup per walker that
+// finalizes the chunked biases, evaluates the one-thread biases, forms
+// dU/dCV and (when the walker's atom list is short) performs the force
+// scatter.  The launch boundary orders the partials (no cross-workgroup
+// fences).  Chunking and summation order are those of the plain kernels, so
+// both forms give bit-identical results.
 // ---------------------------------------------------------------------------
 constexpr int MAX_LDS_VALUES = 1024;
 
-__global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias(StepView v, FusedBiasPlan fp)
+__global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_chunks(StepView v, FusedBiasPlan fp)
 {
     __shared__ double sh[PW * 4];
-    __shared__ double shbuf[PART_LDS];
-    __shared__ double sU[MAX_LDS_VALUES];
     const int w = blockIdx.y;
     const int b = blockIdx.x;
-    if (b < fp.totalChunks) {
-        int k = 0;
-        while (k + 1 < fp.nChunked && b >= fp.chunkStart[k + 1]) k++;
-        const int chunk = b - fp.chunkStart[k];
-        double* dst = fp.part + ((size_t)w * fp.totalChunks + b) * PW;
-        if (k < fp.nHill) {
-            const HillBiasDev& h = fp.hills[k];
-            switch (h.D) {
-                case 1: hill_chunk<1>(v, h, w, chunk, sh, dst); break;
-                case 2: hill_chunk<2>(v, h, w, chunk, sh, dst); break;
-                case 3: hill_chunk<3>(v, h, w, chunk, sh, dst); break;
-                default: hill_chunk<4>(v, h, w, chunk, sh, dst); break;
-            }
-        } else {
-            const OpesBiasDev& o = fp.opes[k - fp.nHill];
-            switch (o.D) {
-                case 1: opes_chunk<1>(v, o, w, chunk, sh, dst); break;
-                case 2: opes_chunk<2>(v, o, w, chunk, sh, dst); break;
-                case 3: opes_chunk<3>(v, o, w, chunk, sh, dst); break;
-                default: opes_chunk<4>(v, o, w, chunk, sh, dst); break;
-            }
+    int k = 0;
+    while (k + 1 < fp.nChunked && b >= fp.chunkStart[k + 1]) k++;
+    const int chunk = b - fp.chunkStart[k];
+    double* dst = fp.part + ((size_t)w * fp.totalChunks + b) * PW;
+    if (k < fp.nHill) {
+        const HillBiasDev& h = fp.hills[k];
+        switch (h.D) {
+            case 1: hill_chunk<1>(v, h, w, chunk, sh, dst); break;
+            case 2: hill_chunk<2>(v, h, w, chunk, sh, dst); break;
+            case 3: hill_chunk<3>(v, h, w, chunk, sh, dst); break;
+            default: hill_chunk<4>(v, h, w, chunk, sh, dst); break;
+        }
+    } else {
+        const OpesBiasDev& o = fp.opes[k - fp.nHill];
+        switch (o.D) {
+            case 1: opes_chunk<1>(v, o, w, chunk, sh, dst); break;
+            case 2: opes_chunk<2>(v, o, w, chunk, sh, dst); break;
+            case 3: opes_chunk<3>(v, o, w, chunk, sh, dst); break;
+            default: opes_chunk<4>(v, o, w, chunk, sh, dst); break;
         }
     }
-    if (!last_block(&fp.counter[w], gridDim.x)) return;
-    // ---- finalize the walker ----
+}
+
+// One workgroup per walker: partial sums (written by the previous launch)
+// in chunk order, finalize, one-thread biases, dU/dCV, and the scatter.
+__global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, FusedBiasPlan fp)
+{
+    __shared__ double shbuf[PART_LDS];
+    __shared__ double sU[MAX_LDS_VALUES];
     __shared__ double tot[PW];
+    const int w = blockIdx.x;
     for (int k = 0; k < fp.nChunked; k++) {
         const int c0 = fp.chunkStart[k], nc = fp.chunkStart[k + 1] - c0;
-        sum_partials(fp.part + ((size_t)w * fp.totalChunks + c0) * PW, nc, PW, tot, shbuf);
+        const double* src = fp.part + ((size_t)w * fp.totalChunks + c0) * PW;
+        if (threadIdx.x == 0)
+            for (int d = 0; d < PW; d++) tot[d] = 0.0;
+        const int per = PART_LDS / PW;
+        for (int base = 0; base < nc; base += per) {
+            const int m = min(nc - base, per);
+            for (int i = threadIdx.x; i < m * PW; i += blockDim.x) shbuf[i] = src[(size_t)base * PW + i];
+            __syncthreads();
+            if (threadIdx.x == 0)
+                for (int j = 0; j < m; j++)
+                    for (int d = 0; d < PW; d++) tot[d] += shbuf[j * PW + d];
+            __syncthreads();
+        }
         if (threadIdx.x == 0) {
             if (k < fp.nHill)
                 hill_finalize(v, fp.hills[k], w, tot);
             else
                 opes_finalize(v, fp.opes[k - fp.nHill], w, tot);
         }
+        __syncthreads();
     }
     for (int i = threadIdx.x; i < fp.nSmall; i += blockDim.x) small_bias_eval(v, fp.st, fp.small[i], i, w);
     __syncthreads();
-    if (threadIdx.x == 0) fp.counter[w] = 0u;
     if (fp.scatter)
         scatter_walker_block(v, fp.sp, w, sU, MAX_LDS_VALUES);
     else
         combine_dudcv(v, fp.sp, w, sU, MAX_LDS_VALUES, true);
 }
 
+void launch_step_bias_chunks(const StepView& v, const FusedBiasPlan& fp, int nw, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_step_bias_chunks, dim3(fp.totalChunks, nw), dim3(BIAS_BLOCK), 0, s, v, fp);
+}
+
 void launch_step_bias(const StepView& v, const FusedBiasPlan& fp, int nw, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_bias, dim3(fp.totalChunks > 0 ? fp.totalChunks : 1, nw), dim3(BIAS_BLOCK), 0, s, v, fp);
+    if (fp.totalChunks > 0)
+        hipLaunchKernelGGL(k_step_bias_chunks, dim3(fp.totalChunks, nw), dim3(BIAS_BLOCK), 0, s, v, fp);
+    hipLaunchKernelGGL(k_step_bias_finalize, dim3(nw), dim3(BIAS_BLOCK), 0, s, v, fp);
 }
 
 // ---------------------------------------------------------------------------
