@@ -50,6 +50,11 @@ SIGNATURES = {
     "glued_get_hills": [vp, i, i, c_double_p, c_double_p, i, c_int_p],
     "glued_set_grid": [vp, i, i, c_double_p, i],
     "glued_plan_info": [vp, c_int_p, c_int_p],
+    "glued_copy_buffer": [vp, i, vp, sz, vp],
+    "glued_set_hill_heights": [vp, i, i, vp, i, vp],
+    "glued_set_path_lambda": [vp, i, d],
+    "glued_set_hill_sigma": [vp, i, c_double_p, i],
+    "glued_hill_adjoint": [vp, vp, i, vp, vp, vp],
 }
 
 _lib = None

@@ -1,0 +1,138 @@
+"""Differentiable CV -> bias pipeline on the HIP engine.
+
+``PathMetadPipeline`` evaluates, for a batch of B configurations of M atoms,
+the path CV (s, z) over N reference frames (MSD metric, CommonGluedKernels.cpp:
+667-794) and a hill-list metadynamics bias V(s, z) with a hill set shared by
+the batch.  ``PathMetadFunction`` is its ``torch.autograd.Function``:
+
+  forward   one batched evaluation on the GPU (the configurations are the
+            walkers of one engine, one captured hipGraph);
+  backward  hand-written adjoints, no autograd tape:
+            dL/dx       = g_b dV_b/dx_b, from the chain-rule force scatter
+                          (the force buffer holds -dV/dx in 2^32 fixed point);
+            dL/dh_k     = sum_b g_b exp(-E_bk)[E_bk <= 8]  (k_hill_adjoint);
+            dL/dsigma_d = sum_b g_b sum_k h_k exp(-E_bk) dc_bkd^2 / sigma_d^3;
+            dL/dlambda  = sum_b g_b (dV/ds ds/dlambda + dV/dz dz/dlambda), with
+                          ds/dlambda = -(1/S) sum_f (f+1-s) msd_f w_f and
+                          dz/dlambda = (<msd>_w - z)/lambda from the path kernel.
+If the engine was used for another evaluation between forward and backward,
+backward recomputes the forward first (nothing per pair is stored).
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from ._lib import call
+from .context import Context
+from .glued import Force
+
+FIXED_SCALE = 4294967296.0
+
+
+def _p(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class PathMetadPipeline:
+    def __init__(self, frames, lam, hill_centers, hill_heights, sigma, batch, *, temperature=300.0,
+                 device="cuda:0"):
+        frames = np.asarray(frames, dtype=np.float64)
+        self.N, self.M = frames.shape[0], frames.shape[1]
+        self.B = int(batch)
+        self.H = len(hill_heights)
+        f = Force(temperature=temperature)
+        self.s_index = f.add_path(list(range(self.M)), [fr.reshape(-1).tolist() for fr in frames], float(lam))
+        self.bias = f.add_metad_hills([self.s_index, self.s_index + 1], sigma=[float(x) for x in sigma], height=1.0,
+                                      pace=1, gamma=None, max_hills=self.H)
+        f.share_bias(self.bias)
+        self.force = f
+        self.ctx = Context(f, walkers=[self.M] * self.B, device=device)
+        self.ctx.set_hills(self.bias, hill_centers, hill_heights)
+        self.device = self.ctx.device
+        self._lam = float(lam)
+        self._sigma = [float(x) for x in sigma]
+        self.generation = 0
+        self.grad_sigma = None
+
+    def __call__(self, x, heights, lam, sigma=None):
+        if sigma is None:
+            sigma = torch.tensor(self._sigma, dtype=torch.float64, device=self.device)
+        return PathMetadFunction.apply(x, heights, lam, sigma, self)
+
+    # ---- engine plumbing ----
+    def _evaluate(self, x, heights, lam, sigma):
+        e = self.ctx
+        P = e.padded[0]
+        posq = e.posq.view(self.B, P, 4)
+        posq[:, :self.M, :3].copy_(x.detach().to(torch.float32))
+        h = heights.detach().to(torch.float64).contiguous()
+        call("glued_set_hill_heights", e.h, 0, self.bias, _p(h), self.H, e.stream)
+        lamv = float(lam)
+        if lamv != self._lam:
+            call("glued_set_path_lambda", e.h, 0, lamv)
+            self._lam = lamv
+        sig = [float(v) for v in sigma.detach().cpu()]
+        if sig != self._sigma:
+            arr = (ctypes.c_double * len(sig))(*sig)
+            call("glued_set_hill_sigma", e.h, self.bias, arr, len(sig))
+            self._sigma = sig
+        e.zero_forces()
+        e.execute_batch(use_graph=True)
+        self.generation += 1
+        self._keep = h  # the heights must outlive the asynchronous copy
+        return self.generation
+
+    def _buffer(self, which, n):
+        out = torch.empty(n, dtype=torch.float64, device=self.device)
+        call("glued_copy_buffer", self.ctx.h, int(which), _p(out), out.numel() * 8, self.ctx.stream)
+        return out
+
+    def energies(self):
+        nb = max(self.force.getNumBiases(), 1)
+        return self._buffer(2, self.B * nb).view(self.B, nb)[:, self.bias]
+
+
+class PathMetadFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, heights, lam, sigma, pipe):
+        gen = pipe._evaluate(x, heights, lam, sigma)
+        V = pipe.energies().clone()
+        ctx.pipe = pipe
+        ctx.gen = gen
+        ctx.lam_shape = lam.shape if torch.is_tensor(lam) else None
+        ctx.lam_device = lam.device if torch.is_tensor(lam) else None
+        ctx.save_for_backward(x.detach(), heights.detach(), torch.as_tensor(float(lam), dtype=torch.float64),
+                              sigma.detach())
+        return V
+
+    @staticmethod
+    def backward(ctx, gV):
+        pipe = ctx.pipe
+        x, heights, lam, sigma = ctx.saved_tensors
+        if pipe.generation != ctx.gen:  # the engine moved on: recompute this forward
+            pipe._evaluate(x, heights, lam, sigma)
+        e = pipe.ctx
+        B, M, P = pipe.B, pipe.M, e.padded[0]
+        g = gV.detach().to(torch.float64).contiguous()
+        grad_x = grad_h = grad_lam = grad_sig = None
+        if ctx.needs_input_grad[0]:
+            F = e.forces.view(B, 3, P)[:, :, :M].to(torch.float64) / FIXED_SCALE  # = -dV/dx
+            grad_x = (-F.transpose(1, 2) * g[:, None, None]).to(x.dtype)
+        if ctx.needs_input_grad[1] or ctx.needs_input_grad[3]:
+            gh = torch.zeros(pipe.H, dtype=torch.float64, device=pipe.device)
+            gs = torch.zeros(2, dtype=torch.float64, device=pipe.device)
+            call("glued_hill_adjoint", e.h, e.stream, pipe.bias, _p(g), _p(gh), _p(gs))
+            grad_h = gh.to(heights.dtype) if ctx.needs_input_grad[1] else None
+            grad_sig = gs.to(sigma.dtype) if ctx.needs_input_grad[3] else None
+            pipe.grad_sigma = gs
+        if ctx.needs_input_grad[2]:
+            ng = sum(len(b[1]) for b in pipe.force.biases)
+            bg = pipe._buffer(1, B * ng).view(B, ng)
+            gslot = sum(len(b[1]) for b in pipe.force.biases[:pipe.bias])
+            aux = pipe._buffer(4, B * 2).view(B, 2)
+            dl = bg[:, gslot] * aux[:, 0] + bg[:, gslot + 1] * aux[:, 1]
+            grad_lam = (g * dl).sum()
+            if ctx.lam_shape is not None:
+                grad_lam = grad_lam.reshape(ctx.lam_shape).to(ctx.lam_device)
+        return grad_x, grad_h, grad_lam, grad_sig, None

@@ -229,6 +229,32 @@ int glued_set_grid(glued_handle h, int walker, int biasIdx, const double* grid, 
     return guard([&] { E(h).setGrid(walker, biasIdx, grid, n); });
 }
 
+int glued_copy_buffer(glued_handle h, int which, void* dst, size_t bytes, void* stream)
+{
+    return guard([&] { E(h).copyBuffer(which, dst, bytes, (hipStream_t)stream); });
+}
+
+int glued_set_hill_heights(glued_handle h, int walker, int biasIdx, const double* dHeights, int n, void* stream)
+{
+    return guard([&] { E(h).setHillHeightsDevice(walker, biasIdx, dHeights, n, (hipStream_t)stream); });
+}
+
+int glued_set_path_lambda(glued_handle h, int pathIdx, double lambda)
+{
+    return guard([&] { E(h).setPathLambda(pathIdx, lambda); });
+}
+
+int glued_set_hill_sigma(glued_handle h, int biasIdx, const double* sigma, int D)
+{
+    return guard([&] { E(h).setHillSigma(biasIdx, sigma, D); });
+}
+
+int glued_hill_adjoint(glued_handle h, void* stream, int biasIdx, const double* dGradV, double* dGradHeights,
+                       double* dGradSigma)
+{
+    return guard([&] { E(h).hillAdjoint((hipStream_t)stream, biasIdx, dGradV, dGradHeights, dGradSigma); });
+}
+
 int glued_plan_info(glued_handle h, int* rows, int* atoms)
 {
     return guard([&] {

@@ -34,6 +34,9 @@ void launch_scatter_dense(const StepView&, const ScatterPlan&, int, int, int, hi
 void launch_step_bias(const StepView&, const FusedBiasPlan&, int, hipStream_t);
 void launch_step_bias_chunks(const StepView&, const FusedBiasPlan&, int, hipStream_t);
 void launch_cv_fused(const StepView&, const CVFusedPlan&, int, hipStream_t);
+void launch_hill_adjoint(const StepView&, const HillBiasDev&, const double*, double*, double*, double*, int,
+                         hipStream_t);
+int hill_adjoint_blocks(int);
 
 void hip_check(hipError_t e, const char* what)
 {
@@ -370,7 +373,8 @@ void Engine::buildPlan(int nMass, const double* masses)
                 const int M = (int)c.atoms.size(), N = (int)c.params[1];
                 it.M = M;
                 it.N = N;
-                it.lambda = (float)c.params[0];
+                it.lambda = c.params[0];
+                it.aux = (int)path_.size();
                 it.valueS = v;
                 it.begin = (int)pathSlots.size();
                 it.row0 = (int)rowSlot_.size();
@@ -423,6 +427,7 @@ void Engine::buildPlan(int nMass, const double* masses)
     dGroupRef_.upload(groupRef);
     dPath_.upload(path_);
     dPathSlots_.upload(pathSlots);
+    dPathAux_.alloc(sizeof(double) * (size_t)W_ * std::max((int)path_.size(), 1) * 2);
     dPathFrames_.upload(pathFrames);
     energyU_.assign(W_, 0.0);
     energyF_.assign(W_, nullptr);
@@ -831,6 +836,8 @@ StepView Engine::view(const float4* posq, long long* force) const
     v.benergy = dBener_.as<double>();
     v.dUdCV = dUd_.as<double>();
     v.energyU = dEnergyU_.as<double>();
+    v.pathAux = dPathAux_.as<double>();
+    v.nPath = std::max((int)path_.size(), 1);
     v.S = std::max(S_, 1);
     v.V = std::max(V_, 1);
     v.R = std::max(R_, 1);
@@ -1116,6 +1123,88 @@ void Engine::updateState(hipStream_t s, int step)
         launch_dep_hills(v, h->d, 0, W_, s);
     }
     hip_check(hipGetLastError(), "deposit launch");
+}
+
+// ---------------------------------------------------------------------------
+// Autograd support: device-to-device state copies, differentiable
+// parameters, and the hill adjoint.
+// ---------------------------------------------------------------------------
+void Engine::copyBuffer(int which, void* dst, size_t bytes, hipStream_t s)
+{
+    if (!initialized_) fail("copy_buffer: initialize first");
+    const void* src = nullptr;
+    size_t n = 0;
+    switch (which) {
+        case 0: src = dCv_.p; n = sizeof(double) * (size_t)W_ * std::max(V_, 1); break;
+        case 1: src = dBgrad_.p; n = sizeof(double) * (size_t)W_ * G_; break;
+        case 2: src = dBener_.p; n = sizeof(double) * (size_t)W_ * B_; break;
+        case 3: src = dUd_.p; n = sizeof(double) * (size_t)W_ * std::max(V_, 1); break;
+        case 4: src = dPathAux_.p; n = sizeof(double) * (size_t)W_ * std::max((int)path_.size(), 1) * 2; break;
+        default: fail("copy_buffer: unknown buffer id");
+    }
+    if (bytes < n) fail("copy_buffer: destination smaller than the buffer (" + std::to_string(n) + " bytes)");
+    hip_check(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToDevice, s), "copy_buffer");
+}
+
+void Engine::setHillHeightsDevice(int w, int bi, const double* d, int n, hipStream_t s)
+{
+    HillHost* h = nullptr;
+    for (auto* x : hills_)
+        if (x->spec == bi) h = x;
+    if (!h) fail("set_hill_heights: not a hill bias");
+    if (w < 0 || w >= W_) fail("set_hill_heights: walker out of range");
+    if (n < 0 || n > h->d.cap) fail("set_hill_heights: count exceeds the capacity");
+    const int t = h->d.shared ? 0 : w;
+    if (n)
+        hip_check(hipMemcpyAsync(h->d.heights + (size_t)t * h->d.cap, d, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                                 s),
+                  "set_hill_heights");
+}
+
+void Engine::setPathLambda(int pi, double lambda)
+{
+    if (!initialized_) fail("set_path_lambda: initialize first");
+    if (pi < 0 || pi >= (int)path_.size()) fail("set_path_lambda: path index out of range");
+    if (!(lambda > 0.0)) fail("set_path_lambda: lambda must be positive");
+    path_[pi].lambda = lambda;
+    hip_check(hipMemcpy(dPath_.as<PathItem>() + pi, &path_[pi], sizeof(PathItem), hipMemcpyHostToDevice),
+              "set_path_lambda");
+}
+
+void Engine::setHillSigma(int bi, const double* sigma, int D)
+{
+    HillHost* h = nullptr;
+    int k = 0;
+    for (size_t i = 0; i < hills_.size(); i++)
+        if (hills_[i]->spec == bi) {
+            h = hills_[i];
+            k = (int)i;
+        }
+    if (!h) fail("set_hill_sigma: not a hill bias");
+    if (D != h->d.D) fail("set_hill_sigma: dimension mismatch");
+    for (int d = 0; d < D; d++) {
+        if (!(sigma[d] > 0.0)) fail("set_hill_sigma: sigma must be positive");
+        h->d.sigma[d] = sigma[d];
+        h->d.i2s2[d] = 1.0 / (2.0 * sigma[d] * sigma[d]);
+        h->d.is2[d] = 1.0 / (sigma[d] * sigma[d]);
+    }
+    // the batched stage reads the table from device memory (graph-safe)
+    hip_check(hipMemcpy(dHillTable_.as<HillBiasDev>() + k, &h->d, sizeof(HillBiasDev), hipMemcpyHostToDevice),
+              "set_hill_sigma");
+}
+
+void Engine::hillAdjoint(hipStream_t s, int bi, const double* gV, double* gH, double* gSigma)
+{
+    HillHost* h = nullptr;
+    for (auto* x : hills_)
+        if (x->spec == bi) h = x;
+    if (!h) fail("hill_adjoint: not a hill bias");
+    const size_t need = sizeof(double) * (size_t)hill_adjoint_blocks(h->d.cap) * (h->d.shared ? 1 : W_) * h->d.D;
+    if (dAdjPart_.bytes < need) dAdjPart_.alloc(need);
+    StepView v = view(lastPos_, lastForce_);
+    v.S = S_;
+    launch_hill_adjoint(v, h->d, gV, gH, dAdjPart_.as<double>(), gSigma, W_, s);
+    hip_check(hipGetLastError(), "hill_adjoint launch");
 }
 
 // ---------------------------------------------------------------------------

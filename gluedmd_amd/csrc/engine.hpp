@@ -100,6 +100,13 @@ class Engine {
     void updateState(hipStream_t s, int step);
     void executeStage(hipStream_t s, const float4* posq, long long* force, int stage);
 
+    // ---- autograd support ----
+    void copyBuffer(int which, void* dst, size_t bytes, hipStream_t s);
+    void setHillHeightsDevice(int w, int bias, const double* d, int n, hipStream_t s);
+    void setPathLambda(int path, double lambda);
+    void setHillSigma(int bias, const double* sigma, int D);
+    void hillAdjoint(hipStream_t s, int bias, const double* gV, double* gH, double* gSigma);
+
     // ---- read-backs ----
     std::vector<double> cvValues(int w);
     std::vector<double> biasEnergies(int w);
@@ -191,7 +198,7 @@ class Engine {
     DBuf dPath_, dPathSlots_, dPathFrames_, dPathW_;
     DBuf dSmallBias_, dGridTable_, dSmallConst_, dSmallVar_, dSmallScal_;
     DBuf dSegStart_, dSegRow_, dRowCv_, dCombStart_, dCombSlot_, dTestGrad_, dEnergyU_;
-    DBuf dHillTable_, dOpesTable_, dFusedPart_, dFusedCounter_;
+    DBuf dHillTable_, dOpesTable_, dFusedPart_, dFusedCounter_, dPathAux_, dAdjPart_;
     FusedBiasPlan fplan_{};
 
     // graph cache for the batched step

@@ -72,6 +72,8 @@ struct StepView {
     double* benergy;      // [W][B]
     double* dUdCV;        // [W][V]
     const double* energyU; // [W] value of the CV_ENERGY input
+    double* pathAux;       // [W][nPath][2] ds/dlambda, dz/dlambda
+    int nPath;
     int S, V, R, G, B;
     int pbc;
 };
@@ -117,10 +119,11 @@ struct CoordItem {
 struct PathItem {
     int begin, M;         // slots
     int N;                // frames
-    float lambda;
+    double lambda;        // FP64 (differentiable parameter)
     int frameOff;         // float offset of frame 0 (N*M*3 floats)
     int valueS;           // z at valueS + 1
     int row0;             // rows interleaved s/z per atom
+    int aux;              // index of this path in the [W][nPath][2] d(s,z)/d(lambda) buffer
 };
 
 }  // namespace glued

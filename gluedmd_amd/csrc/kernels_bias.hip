@@ -889,3 +889,107 @@ void launch_dep_small(const StepView& v, const SmallBiasState& st, const SmallBi
 }
 
 }  // namespace glued
+
+namespace glued {
+
+// ---------------------------------------------------------------------------
+// Adjoint of the hill-list bias with respect to the hill heights and widths,
+// for the autograd path (no counterpart in the reference):
+//   dL/dh_k     = sum_w g_w exp(-E_wk) [E_wk <= 8]
+//   dL/dsigma_d = sum_w g_w sum_k h_k exp(-E_wk) [E_wk <= 8] dc_wkd^2 / sigma_d^3
+// with the forward's exact arithmetic (FP64 exponent from the same host
+// reciprocals, FP32 exp).  Shared table: one thread per hill, walkers' CV
+// values and g_w staged through LDS in tiles of 256.  Private tables: one
+// y-row of blocks per walker.
+// ---------------------------------------------------------------------------
+constexpr int ADJ_BLOCK = 256;
+
+template <int D>
+__global__ __launch_bounds__(ADJ_BLOCK) void k_hill_adjoint(StepView v, HillBiasDev h, const double* __restrict__ gV,
+                                                            double* __restrict__ gH, double* __restrict__ sigPart,
+                                                            int W)
+{
+#pragma clang fp contract(off)
+    __shared__ double sS[ADJ_BLOCK * D];
+    __shared__ double sG[ADJ_BLOCK];
+    __shared__ double sh[4 * D];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tb = h.shared ? 0 : blockIdx.y;
+    const int n = h.count[tb];
+    const double* C = h.centers + (size_t)tb * D * h.cap;
+    double c[D], hk = 0.0;
+    const bool ok = k < n;
+#pragma unroll
+    for (int d = 0; d < D; d++) c[d] = ok ? C[(size_t)d * h.cap + k] : 0.0;
+    if (ok) hk = h.heights[(size_t)tb * h.cap + k];
+    double gh = 0.0, gs[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) gs[d] = 0.0;
+    const int wBegin = h.shared ? 0 : blockIdx.y;
+    const int wEnd = h.shared ? W : blockIdx.y + 1;
+    for (int w0 = wBegin; w0 < wEnd; w0 += ADJ_BLOCK) {
+        const int m = min(ADJ_BLOCK, wEnd - w0);
+        if (threadIdx.x < m) {
+            const double* cv = v.cvVal + (size_t)(w0 + threadIdx.x) * v.V;
+#pragma unroll
+            for (int d = 0; d < D; d++) sS[threadIdx.x * D + d] = cv[h.cv[d]];
+            sG[threadIdx.x] = gV[w0 + threadIdx.x];
+        }
+        __syncthreads();
+        if (ok)
+            for (int j = 0; j < m; j++) {
+                double e = 0.0, dc[D];
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    double x = sS[j * D + d] - c[d];
+                    if (h.period[d] > 0.0) x = x - rint(x * h.invPeriod[d]) * h.period[d];
+                    dc[d] = x;
+                    e = e + x * x * h.i2s2[d];
+                }
+                if (!(e <= 8.0)) continue;
+                const double t = sG[j] * (double)expf(-(float)e);
+                gh += t;
+#pragma unroll
+                for (int d = 0; d < D; d++) gs[d] += t * hk * dc[d] * dc[d];
+            }
+        __syncthreads();
+    }
+    if (ok) gH[(size_t)tb * h.cap + k] = gh;
+    // per-block partial of the sigma gradient (scaled by 1/sigma^3 on the host side of the reduction)
+#pragma unroll
+    for (int d = 0; d < D; d++) gs[d] = wsum(gs[d]);
+    if ((threadIdx.x & 63) == 0)
+        for (int d = 0; d < D; d++) sh[d * 4 + (threadIdx.x >> 6)] = gs[d];
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int d = 0; d < D; d++)
+            sigPart[((size_t)blockIdx.y * gridDim.x + blockIdx.x) * D + d] =
+                sh[d * 4] + sh[d * 4 + 1] + sh[d * 4 + 2] + sh[d * 4 + 3];
+}
+
+// Sum the per-block sigma partials in block order: gSigma[d] = sum / sigma_d^3.
+__global__ void k_hill_adjoint_sigma(HillBiasDev h, const double* sigPart, int nPart, double* gSigma)
+{
+    const int d = threadIdx.x;
+    if (d >= h.D) return;
+    double s = 0.0;
+    for (int i = 0; i < nPart; i++) s += sigPart[(size_t)i * h.D + d];
+    gSigma[d] = s / (h.sigma[d] * h.sigma[d] * h.sigma[d]);
+}
+
+int hill_adjoint_blocks(int cap) { return (cap + ADJ_BLOCK - 1) / ADJ_BLOCK; }
+
+void launch_hill_adjoint(const StepView& v, const HillBiasDev& h, const double* gV, double* gH, double* sigPart,
+                         double* gSigma, int W, hipStream_t s)
+{
+    dim3 g(hill_adjoint_blocks(h.cap), h.shared ? 1 : W), b(ADJ_BLOCK);
+    switch (h.D) {
+        case 1: hipLaunchKernelGGL(k_hill_adjoint<1>, g, b, 0, s, v, h, gV, gH, sigPart, W); break;
+        case 2: hipLaunchKernelGGL(k_hill_adjoint<2>, g, b, 0, s, v, h, gV, gH, sigPart, W); break;
+        case 3: hipLaunchKernelGGL(k_hill_adjoint<3>, g, b, 0, s, v, h, gV, gH, sigPart, W); break;
+        default: hipLaunchKernelGGL(k_hill_adjoint<4>, g, b, 0, s, v, h, gV, gH, sigPart, W); break;
+    }
+    hipLaunchKernelGGL(k_hill_adjoint_sigma, dim3(1), dim3(64), 0, s, h, sigPart, (int)(g.x * g.y), gSigma);
+}
+
+}  // namespace glued

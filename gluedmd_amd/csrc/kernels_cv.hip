@@ -883,6 +883,7 @@ __device__ void path_item(const StepView& v, const PathItem& it, const int* __re
     // frames processed in groups of up to PATH_BLOCK/P frames
     const int P = N >= PATH_BLOCK ? 1 : PATH_BLOCK / N;
     __shared__ double shW[1024];
+    __shared__ double shM[1024];
     for (int f0 = 0; f0 < N; f0 += PATH_BLOCK / P) {
         const int f = f0 + threadIdx.x / P, part = threadIdx.x % P;
         double acc = 0.0;
@@ -900,15 +901,16 @@ __device__ void path_item(const StepView& v, const PathItem& it, const int* __re
             double m = 0.0;
             for (int k = 0; k < P; k++) m += shPart[threadIdx.x + k];
             m /= M;
-            const double wv = exp(-(double)it.lambda * m);
-            if (f < 1024) shW[f] = wv;
+            const double wv = exp(-it.lambda * m);
+            shW[f] = wv;
+            shM[f] = m;
         }
         __syncthreads();
     }
     // S and sum (f+1) w in frame order by one thread (deterministic, N small)
     __shared__ double shS, shSn;
     if (threadIdx.x == 0) {
-        for (int f = 0; f < N && f < 1024; f++) {
+        for (int f = 0; f < N; f++) {
             S += shW[f];
             sNum += (f + 1) * shW[f];
         }
@@ -924,6 +926,15 @@ __device__ void path_item(const StepView& v, const PathItem& it, const int* __re
     if (threadIdx.x == 0) {
         v.cvVal[(size_t)w * v.V + it.valueS] = s;
         v.cvVal[(size_t)w * v.V + it.valueS + 1] = z;
+        // ds/dlambda = -(1/S) sum (f+1-s) msd_f w_f ; dz/dlambda = (<msd>_w - z)/lambda
+        double a = 0.0, b = 0.0;
+        for (int f = 0; f < N; f++) {
+            a += ((f + 1) - s) * shM[f] * shW[f];
+            b += shM[f] * shW[f];
+        }
+        double* aux = v.pathAux + ((size_t)w * v.nPath + it.aux) * 2;
+        aux[0] = S > 1e-300 ? -a / S : 0.0;
+        aux[1] = S > 1e-300 ? (b / S - z) / lam : 0.0;
     }
     const double scS = -2.0 * lam / (M * S), scZ = 2.0 / (M * S);
     for (int j = threadIdx.x; j < M; j += blockDim.x) {

@@ -159,6 +159,25 @@ int glued_set_hills(glued_handle h, int walker, int biasIdx, const double* cente
 int glued_get_hills(glued_handle h, int walker, int biasIdx, double* centers, double* heights, int cap, int* n);
 /* Whole MetaD grid (the grid section of the state blob). */
 int glued_set_grid(glued_handle h, int walker, int biasIdx, const double* grid, int n);
+/* ---- autograd support (no reference counterpart) ------------------------ */
+
+/* Device-to-device copy of per-walker state: 0 = CV values [W][V],
+ * 1 = per-bias gradients dV_b/ds_d [W][G], 2 = bias energies [W][B],
+ * 3 = dU/dCV [W][V], 4 = (ds/dlambda, dz/dlambda) of every path CV
+ * [W][nPath][2]. */
+int glued_copy_buffer(glued_handle h, int which, void* dst, size_t bytes, void* stream);
+/* Hill heights of a GLUED_BIAS_METAD_HILLS bias from device memory. */
+int glued_set_hill_heights(glued_handle h, int walker, int biasIdx, const double* dHeights, int n, void* stream);
+/* lambda of the pathIdx-th path CV (FP64). */
+int glued_set_path_lambda(glued_handle h, int pathIdx, double lambda);
+/* Widths of a GLUED_BIAS_METAD_HILLS bias. */
+int glued_set_hill_sigma(glued_handle h, int biasIdx, const double* sigma, int D);
+/* Adjoint of a hill bias: given g_w = dL/dV_w (device, [W]) writes
+ * dL/dh_k (device, [T][cap], T = 1 shared or W private tables) and
+ * dL/dsigma_d (device, [D]) for the CV values of the last evaluation. */
+int glued_hill_adjoint(glued_handle h, void* stream, int biasIdx, const double* dGradV, double* dGradHeights,
+                       double* dGradSigma);
+
 /* Plan sizes: Jacobian rows and distinct atoms of one walker. */
 int glued_plan_info(glued_handle h, int* rows, int* atoms);
 
