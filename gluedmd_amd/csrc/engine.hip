@@ -30,7 +30,7 @@ int hill_chunks_for(int);
 int opes_chunks_for(int);
 
 void launch_scatter(const StepView&, const ScatterPlan&, int, int, int, hipStream_t);
-void launch_scatter_dense(const StepView&, const ScatterPlan&, int, int, int, hipStream_t);
+void launch_scatter_dense(const StepView&, const float* const*, int, int, int, int, hipStream_t);
 void launch_step_bias(const StepView&, const FusedBiasPlan&, int, hipStream_t);
 void launch_step_bias_chunks(const StepView&, const FusedBiasPlan&, int, hipStream_t);
 void launch_cv_fused(const StepView&, const CVFusedPlan&, int, hipStream_t);
@@ -207,6 +207,11 @@ void Engine::initialize(int W, const int* natoms, const int* padded, int nMass, 
     for (int w = 0; w < W; w++)
         if (padded_[w] < natoms_[w]) fail("initialize: padded atom count smaller than atom count");
     validate(minAtoms);
+    bool hasEnergy = false;
+    for (const auto& cv : cvs_) hasEnergy |= cv.type == CV_ENERGY;
+    if (hasEnergy)  // the dense energy-CV scatter moves 4 field entries per 16-byte access
+        for (int w = 0; w < W; w++)
+            if (padded_[w] % 4) fail("initialize: with an energy CV every padded atom count must be a multiple of 4");
     posOff_.resize(W);
     forceOff_.resize(W);
     long long po = 0;
@@ -432,6 +437,7 @@ void Engine::buildPlan(int nMass, const double* masses)
     energyU_.assign(W_, 0.0);
     energyF_.assign(W_, nullptr);
     dEnergyU_.alloc(sizeof(double) * W_);
+    dEnergyFPtr_.alloc(sizeof(const float*) * W_);
 
     // ---- coordination buffers ----
     {
@@ -813,9 +819,13 @@ void Engine::setEnergyInput(int w, double U, const float* Fext, hipStream_t s)
     if (!initialized_) fail("set_energy_input: initialize first");
     if (w < 0 || w >= W_) fail("set_energy_input: walker out of range");
     if (energyValue_ < 0) fail("set_energy_input: the force has no CV_ENERGY");
+    if (Fext && (reinterpret_cast<uintptr_t>(Fext) & 15)) fail("set_energy_input: force array must be 16-byte aligned");
     energyU_[w] = U;
     energyF_[w] = Fext;
     hip_check(hipMemcpyAsync(dEnergyU_.as<double>() + w, &energyU_[w], sizeof(double), hipMemcpyHostToDevice, s),
+              "energy input");
+    hip_check(hipMemcpyAsync(dEnergyFPtr_.as<const float*>() + w, &energyF_[w], sizeof(const float*),
+                             hipMemcpyHostToDevice, s),
               "energy input");
     hip_check(hipStreamSynchronize(s), "energy input sync");
 }
@@ -950,16 +960,8 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         return;
     }
     if (!scattered) launch_scatter(v, sp, w0, nw, forces ? 1 : 0, s);
-    if (forces && energyValue_ >= 0) {
-        for (int w = w0; w < w0 + nw; w++) {
-            if (!energyF_[w]) continue;
-            ScatterPlan spw = sp;
-            // energyF_ holds one device pointer per walker; present it through the
-            // walker's force offset by shifting the base.
-            spw.energyF = energyF_[w] - forceOff_[w];
-            launch_scatter_dense(v, spw, padded_[w], w, 1, s);
-        }
-    }
+    if (forces && energyValue_ >= 0)
+        launch_scatter_dense(v, dEnergyFPtr_.as<const float*>(), energyValue_, maxPadded_, w0, nw, s);
     hip_check(hipGetLastError(), "kernel launch");
 }
 
@@ -999,9 +1001,17 @@ double Engine::execute(hipStream_t s, int w, const float4* posq, long long* forc
 void Engine::executeStage(hipStream_t s, const float4* posq, long long* force, int stage)
 {
     if (!initialized_) fail("execute_stage: initialize first");
-    if (stage < 0 || stage > 3) fail("execute_stage: stage must be 0, 1, 2 or 3");
+    if (stage < 0 || stage > 4) fail("execute_stage: stage must be 0, 1, 2, 3 or 4");
     lastPos_ = posq;
     lastForce_ = force;
+    if (stage == 4) {
+        // the dense energy-CV scatter alone (every walker)
+        if (energyValue_ >= 0)
+            launch_scatter_dense(view(lastPos_, lastForce_), dEnergyFPtr_.as<const float*>(), energyValue_,
+                                 maxPadded_, 0, W_, s);
+        hip_check(hipGetLastError(), "kernel launch");
+        return;
+    }
     if (stage == 3) {
         // the batched chunk kernel alone (every hill / OPES chunk of every walker)
         if (fplan_.totalChunks > 0 && testGrads_.empty()) {
@@ -1028,8 +1038,7 @@ void Engine::executeBatch(hipStream_t s, const float4* posq, long long* force, b
         cvReady_ = true;
         return;
     }
-    bool needsDense = energyValue_ >= 0;
-    if (needsDense || graphPos_ != posq || graphForce_ != force || graphForces_ != forces || !graphExec_) {
+    if (graphPos_ != posq || graphForce_ != force || graphForces_ != forces || !graphExec_) {
         if (graphExec_) {
             (void)hipGraphExecDestroy(graphExec_);
             (void)hipGraphDestroy(graph_);

@@ -32,19 +32,33 @@ __global__ __launch_bounds__(SCATTER_BLOCK) void k_scatter(StepView v, ScatterPl
     scatter_slot(v, sp, w, s, sU, MAX_LDS_VALUES);
 }
 
-// Dense scatter of an energy CV whose Jacobian is -F_ext over every atom:
-// F += dV/dU * F_ext, one thread per (atom, component), coalesced, HBM-bound.
-__global__ void k_scatter_dense(StepView v, ScatterPlan sp, int w0)
+// Dense scatter of an energy CV whose Jacobian is -F_ext over every atom
+// (the CV_ENERGY rows, CommonGluedKernels.cpp:4227-4230 / 2471-2503):
+// F += dV/dU * F_ext.  Streams 12 B of F_ext and read-modify-writes 24 B of
+// fixed-point force per atom; 4 consecutive field entries per thread with
+// 16-byte loads and stores.  One y-row of blocks per walker; the walker's
+// F_ext pointer comes from a device table (graph-safe).
+__global__ __launch_bounds__(256) void k_scatter_dense(StepView v, const float* const* __restrict__ fext, int energyValue,
+                                                     int w0)
 {
     const int w = w0 + blockIdx.y;
+    const float* Fe = fext[w];
+    if (!Fe) return;
     const WalkerInfo wi = v.walkers[w];
-    const double u = v.dUdCV[(size_t)w * v.V + sp.energyValue];
-    const float* Fe = sp.energyF + wi.forceOff;  // same field-major layout as the force block
+    const double u = v.dUdCV[(size_t)w * v.V + energyValue];
     long long* F = v.force + wi.forceOff;
-    const long long n = 3LL * wi.padded;
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
-        const double f = u * (double)Fe[i];
-        F[i] += (long long)(f * FIXED_SCALE);
+    const long long n4 = (3LL * wi.padded) >> 2;  // padded is a multiple of 4
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
+         i += (long long)gridDim.x * blockDim.x) {
+        const float4 e = reinterpret_cast<const float4*>(Fe)[i];
+        longlong2* p = reinterpret_cast<longlong2*>(F + 4 * i);
+        longlong2 a = p[0], b = p[1];
+        a.x += (long long)(u * (double)e.x * FIXED_SCALE);
+        a.y += (long long)(u * (double)e.y * FIXED_SCALE);
+        b.x += (long long)(u * (double)e.z * FIXED_SCALE);
+        b.y += (long long)(u * (double)e.w * FIXED_SCALE);
+        p[0] = a;
+        p[1] = b;
     }
 }
 
@@ -55,12 +69,13 @@ void launch_scatter(const StepView& v, const ScatterPlan& sp, int w0, int nw, in
     hipLaunchKernelGGL(k_scatter, dim3(nb, nw), dim3(SCATTER_BLOCK), 0, s, v, sp, w0, includeForces);
 }
 
-void launch_scatter_dense(const StepView& v, const ScatterPlan& sp, int maxPadded, int w0, int nw, hipStream_t s)
+void launch_scatter_dense(const StepView& v, const float* const* fext, int energyValue, int maxPadded, int w0, int nw,
+                          hipStream_t s)
 {
-    long long n = 3LL * maxPadded;
-    int nb = (int)((n + 255) / 256);
-    if (nb > 4096) nb = 4096;
-    hipLaunchKernelGGL(k_scatter_dense, dim3(nb, nw), dim3(256), 0, s, v, sp, w0);
+    const long long n4 = (3LL * maxPadded) / 4;
+    int nb = (int)((n4 + 255) / 256);
+    if (nb > 2048) nb = 2048;
+    hipLaunchKernelGGL(k_scatter_dense, dim3(nb, nw), dim3(256), 0, s, v, fext, energyValue, w0);
 }
 
 }  // namespace glued

@@ -104,6 +104,12 @@ def opes_prior_kernels(r, center, sigma0, gamma, n=5000, spread=20.0):
     return c, s, lw
 
 
+def opes_metad_cutoff2(gamma):
+    """Squared kernel cutoff of OPES-METAD with bias factor gamma:
+    2 gamma / invGF with invGF = (gamma - 1) / gamma (CommonBiasKernels.cpp:553-588)."""
+    return 2.0 * gamma * gamma / (gamma - 1.0)
+
+
 def opes_state_blob(centers, sigmas, logw, sum_uprob, n_metad=0, grids=()):
     """A 'GPUS' v1 bias-state blob holding one OPES bias
     (CommonSerialization.cpp:20-178 layout)."""
@@ -261,17 +267,23 @@ def config5(n_walkers=16, N=100000, n_cm=4096, n_drmsd=1500, path_frames=256, pa
     pos = [r.uniform(3 * N, 0.0, L).reshape(N, 3).astype(np.float32) for _ in range(n_walkers)]
     cm_pairs = r.integers(2 * n_cm, 0, N - 1).reshape(n_cm, 2)
     ref = pos[0].astype(np.float64)
+    # candidate pairs (a, b) are consecutive draws of one stream, kept while
+    # d_ref < 0.8 nm; drawn in blocks, with the stream rewound to just after
+    # the last candidate used
     pairs, d0 = [], []
     while len(pairs) < n_drmsd:
-        a, b = r.integers(2, 0, N - 1)
-        if a == b:
-            continue
-        d = ref[b] - ref[a]
+        start = r.state
+        cand = r.integers(2 * 65536, 0, N - 1).reshape(-1, 2)
+        d = ref[cand[:, 1]] - ref[cand[:, 0]]
         d -= L * np.round(d / L)
-        dist = float(np.linalg.norm(d))
-        if dist < 0.8:
-            pairs.append((int(a), int(b)))
-            d0.append(dist)
+        dist = np.sqrt(np.sum(d * d, axis=1))
+        keep = np.nonzero((cand[:, 0] != cand[:, 1]) & (dist < 0.8))[0][:n_drmsd - len(pairs)]
+        pairs += [(int(cand[i, 0]), int(cand[i, 1])) for i in keep]
+        d0 += [float(np.linalg.norm(d[i])) for i in keep]
+        if len(pairs) == n_drmsd:
+            used = 2 * (int(keep[-1]) + 1)
+            with np.errstate(over="ignore"):
+                r.state = start + np.uint64(used) * _GOLD
     path_idx = list(range(path_atoms))
     base = pos[0][:path_atoms].astype(np.float64)
     frames = [base.copy()]

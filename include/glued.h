@@ -125,8 +125,9 @@ int glued_execute(glued_handle h, void* stream, int walker, const void* posq, in
 int glued_execute_batch(glued_handle h, void* stream, const void* posq, int64_t* forces, int includeForces,
                         int useGraph);
 /* One stage of the batched step for every walker (0 = CVs, 1 = biases,
- * 2 = dU/dCV + force scatter, 3 = only the streamed hill / OPES-kernel sums),
- * for per-stage timing; no reference counterpart. */
+ * 2 = dU/dCV + force scatter, 3 = only the streamed hill / OPES-kernel sums,
+ * 4 = only the dense energy-CV scatter), for per-stage timing; no reference
+ * counterpart. */
 int glued_execute_stage(glued_handle h, void* stream, const void* posq, int64_t* forces, int stage);
 /* Replaces CalcGluedForceKernel::updateState (GluedKernels.h:52,
  * CommonGluedKernels.cpp:5050-5157) with GluedForceImpl's once-per-step guard
