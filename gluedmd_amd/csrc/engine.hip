@@ -13,7 +13,8 @@ namespace glued {
 void launch_cv_small(const StepView&, const SmallCVItem*, int, int, int, hipStream_t);
 void launch_cv_group(const StepView&, const GroupCVItem*, int, const int*, const float*, const float*, int, int,
                      hipStream_t);
-void launch_cv_path(const StepView&, const PathItem*, int, const int*, const float*, int, int, hipStream_t);
+void launch_cv_path(const StepView&, const PathItem*, int, int, int, const int*, const float*, int, int, hipStream_t);
+void launch_path_rows(const StepView&, const PathItem*, int, int, const int*, const float*, int, int, hipStream_t);
 
 void launch_coordination(const StepView&, const CoordBuffers&, const CoordItem*, const CoordItem&, int, int, int,
                          hipStream_t);
@@ -140,7 +141,8 @@ void Engine::validate(int minAtoms) const
                 needP(2);
                 int N = (int)c.params[1];
                 if (N < 2) err("path needs >= 2 frames");
-                if (N > 1024) err("path frame count " + std::to_string(N) + " exceeds 1024");
+                if (N > PATH_MAX_FRAMES)
+                    err("path frame count " + std::to_string(N) + " exceeds " + std::to_string(PATH_MAX_FRAMES));
                 needP(2 + (size_t)N * 3 * c.atoms.size());
                 break;
             }
@@ -390,6 +392,12 @@ void Engine::buildPlan(int nMass, const double* masses)
                     addRow(s, v + 1);
                 }
                 it.frameOff = (int)pathFrames.size();
+                it.msdOff = msdStride_;
+                it.msdBlk0 = pathMsdBlocks_;
+                it.rowBlk0 = pathRowBlocks_;
+                msdStride_ += N;
+                pathMsdBlocks_ += (N + PATH_FPB - 1) / PATH_FPB;
+                pathRowBlocks_ += (M + PATH_TILE - 1) / PATH_TILE;
                 for (int k = 0; k < N * M * 3; k++) pathFrames.push_back((float)c.params[2 + k]);
                 path_.push_back(it);
                 break;
@@ -433,6 +441,7 @@ void Engine::buildPlan(int nMass, const double* masses)
     dPath_.upload(path_);
     dPathSlots_.upload(pathSlots);
     dPathAux_.alloc(sizeof(double) * (size_t)W_ * std::max((int)path_.size(), 1) * 2);
+    dPathMsd_.alloc(sizeof(double) * (size_t)W_ * std::max(msdStride_, 1));
     dPathFrames_.upload(pathFrames);
     energyU_.assign(W_, 0.0);
     energyF_.assign(W_, nullptr);
@@ -848,6 +857,8 @@ StepView Engine::view(const float4* posq, long long* force) const
     v.energyU = dEnergyU_.as<double>();
     v.pathAux = dPathAux_.as<double>();
     v.nPath = std::max((int)path_.size(), 1);
+    v.pathMsd = dPathMsd_.as<double>();
+    v.msdStride = std::max(msdStride_, 1);
     v.S = std::max(S_, 1);
     v.V = std::max(V_, 1);
     v.R = std::max(R_, 1);
@@ -879,15 +890,18 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         cp.gref = dGroupRef_.as<float>();
         cp.path = dPath_.as<PathItem>();
         cp.nPath = (int)path_.size();
+        cp.nPathMsdBlocks = pathMsdBlocks_;
         cp.pslots = dPathSlots_.as<int>();
         cp.frames = dPathFrames_.as<float>();
         launch_cv_fused(v, cp, nw, s);
+        launch_path_rows(v, dPath_.as<PathItem>(), (int)path_.size(), pathRowBlocks_, dPathSlots_.as<int>(),
+                         dPathFrames_.as<float>(), w0, nw, s);
     } else {
         launch_cv_small(v, dSmall_.as<SmallCVItem>(), (int)small_.size(), w0, nw, s);
         launch_cv_group(v, dGroup_.as<GroupCVItem>(), (int)group_.size(), dGroupSlots_.as<int>(),
                         dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
-        launch_cv_path(v, dPath_.as<PathItem>(), (int)path_.size(), dPathSlots_.as<int>(), dPathFrames_.as<float>(),
-                       w0, nw, s);
+        launch_cv_path(v, dPath_.as<PathItem>(), (int)path_.size(), pathMsdBlocks_, pathRowBlocks_,
+                       dPathSlots_.as<int>(), dPathFrames_.as<float>(), w0, nw, s);
     }
     if (!coord_.empty()) {
         CoordBuffers cb;

@@ -198,7 +198,8 @@ class Engine {
     DBuf dPath_, dPathSlots_, dPathFrames_, dPathW_;
     DBuf dSmallBias_, dGridTable_, dSmallConst_, dSmallVar_, dSmallScal_;
     DBuf dSegStart_, dSegRow_, dRowCv_, dCombStart_, dCombSlot_, dTestGrad_, dEnergyU_, dEnergyFPtr_;
-    DBuf dHillTable_, dOpesTable_, dFusedPart_, dFusedCounter_, dPathAux_, dAdjPart_;
+    DBuf dHillTable_, dOpesTable_, dFusedPart_, dFusedCounter_, dPathAux_, dAdjPart_, dPathMsd_;
+    int msdStride_ = 0, pathMsdBlocks_ = 0, pathRowBlocks_ = 0;
     FusedBiasPlan fplan_{};
 
     // graph cache for the batched step

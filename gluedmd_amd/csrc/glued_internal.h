@@ -73,7 +73,9 @@ struct StepView {
     double* dUdCV;        // [W][V]
     const double* energyU; // [W] value of the CV_ENERGY input
     double* pathAux;       // [W][nPath][2] ds/dlambda, dz/dlambda
+    double* pathMsd;       // [W][msdStride] per-frame MSD of every path
     int nPath;
+    int msdStride;
     int S, V, R, G, B;
     int pbc;
 };
@@ -124,7 +126,17 @@ struct PathItem {
     int valueS;           // z at valueS + 1
     int row0;             // rows interleaved s/z per atom
     int aux;              // index of this path in the [W][nPath][2] d(s,z)/d(lambda) buffer
+    int msdOff;           // offset of frame 0 in a walker's msd row ([W][msdStride] FP64)
+    int msdBlk0;          // first MSD block of this path (PATH_FPB frames per block)
+    int rowBlk0;          // first row block of this path (PATH_TILE atoms per block)
 };
+
+// Path CV work decomposition (shared by the host planner and the kernels).
+constexpr int PATH_FPB = 16;           // frames per MSD block (4 per wave)
+constexpr int PATH_TILE = 64;          // atoms per row block (one per lane)
+constexpr int PATH_PHASES = 8;         // waves per row block, frames f = phase mod 8
+constexpr int PATH_MAX_FRAMES = 4096;  // frames per path (LDS weight table)
+constexpr int PATH_LDS_ATOMS = 1024;   // positions staged in LDS per chunk by an MSD block
 
 }  // namespace glued
 
@@ -262,6 +274,7 @@ struct CVFusedPlan {
     const float* gref;
     const PathItem* path;
     int nPath;
+    int nPathMsdBlocks;
     const int* pslots;
     const float* frames;
 };

@@ -865,85 +865,126 @@ namespace glued {
 
 // ---------------------------------------------------------------------------
 // Path CV (s, z), MSD metric, frames 1-indexed, no PBC:
-// CommonGluedKernels.cpp:667-794.  One workgroup per (path, walker).
-// Pass 1: msd_f for every frame, with (frame, part) thread tiles; x - r is
-// formed in FP64 (exact for FP32 inputs) and accumulated in FP64.
-// Pass 2: Jacobian rows per atom (s row, z row interleaved, :781-791).
-// No frame cap (the reference stops at 128 frames, :3901).
+// CommonGluedKernels.cpp:667-794, in two phases so that a path of many frames
+// and atoms spreads over the whole chip:
+//   MSD blocks  (walker, PATH_FPB frames): the walker's path atoms are staged
+//               in LDS, each wave accumulates 4 frames with lanes striding
+//               over atoms (frame reads coalesce across lanes); msd_f goes to
+//               a [W][msdStride] buffer.
+//   row blocks  (walker, PATH_TILE atoms): every block rebuilds w_f, S and s
+//               from the msd row with the same fixed-order reduction (so all
+//               blocks agree bit for bit); lane = atom, wave = frame phase;
+//               the per-phase partial rows are summed in LDS in phase order.
+// x - r is formed in FP64 (exact for FP32 inputs); sums are FP64.  The row
+// blocks of tile 0 also write s, z and d(s, z)/d(lambda):
+//   ds/dlambda = -(1/S) sum_f (f+1-s) msd_f w_f,  dz/dlambda = (<msd>_w - z)/lambda.
+// With S <= 1e-300 (every frame's weight underflows) s = z = 0 as in the
+// reference, and the rows and lambda derivatives are 0 (the reference divides
+// by S there).  No frame cap at 128 (:3901); up to PATH_MAX_FRAMES.
 // ---------------------------------------------------------------------------
-constexpr int PATH_BLOCK = 256;
-
-__device__ void path_item(const StepView& v, const PathItem& it, const int* __restrict__ slots,
-                          const float* __restrict__ frames, int w)
+__device__ void path_msd_block(const StepView& v, const PathItem& it, const int* __restrict__ slots,
+                               const float* __restrict__ frames, int fb, int w)
 {
-    __shared__ double shPart[PATH_BLOCK];
+    __shared__ float shX[3 * PATH_LDS_ATOMS];
     const int M = it.M, N = it.N;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int f0 = fb * PATH_FPB + wave * 4;
+    const int nf = f0 < N ? min(4, N - f0) : 0;
     const float* F = frames + it.frameOff;
-    double S = 0.0, sNum = 0.0;
-    // frames processed in groups of up to PATH_BLOCK/P frames
-    const int P = N >= PATH_BLOCK ? 1 : PATH_BLOCK / N;
-    __shared__ double shW[1024];
-    __shared__ double shM[1024];
-    for (int f0 = 0; f0 < N; f0 += PATH_BLOCK / P) {
-        const int f = f0 + threadIdx.x / P, part = threadIdx.x % P;
-        double acc = 0.0;
-        if (f < N && threadIdx.x / P < PATH_BLOCK / P) {
-            for (int j = part; j < M; j += P) {
-                float4 q = load_pos(v, w, slots[it.begin + j]);
-                const float* r = F + ((size_t)f * M + j) * 3;
-                double dx = (double)q.x - r[0], dy = (double)q.y - r[1], dz = (double)q.z - r[2];
-                acc += dx * dx + dy * dy + dz * dz;
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int c0 = 0; c0 < M; c0 += PATH_LDS_ATOMS) {
+        const int mc = min(PATH_LDS_ATOMS, M - c0);
+        __syncthreads();
+        for (int j = threadIdx.x; j < mc; j += blockDim.x) {
+            const float4 q = load_pos(v, w, slots[it.begin + c0 + j]);
+            shX[3 * j] = q.x;
+            shX[3 * j + 1] = q.y;
+            shX[3 * j + 2] = q.z;
+        }
+        __syncthreads();
+        if (nf == 0) continue;
+        for (int j = lane; j < mc; j += 64) {
+            const double x = shX[3 * j], y = shX[3 * j + 1], z = shX[3 * j + 2];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                if (k < nf) {
+                    const float* r = F + ((size_t)(f0 + k) * M + c0 + j) * 3;
+                    const double dx = x - (double)r[0], dy = y - (double)r[1], dz = z - (double)r[2];
+                    acc[k] += dx * dx + dy * dy + dz * dz;
+                }
             }
         }
-        shPart[threadIdx.x] = acc;
-        __syncthreads();
-        if (part == 0 && f < N && threadIdx.x / P < PATH_BLOCK / P) {
-            double m = 0.0;
-            for (int k = 0; k < P; k++) m += shPart[threadIdx.x + k];
-            m /= M;
-            const double wv = exp(-it.lambda * m);
-            shW[f] = wv;
-            shM[f] = m;
-        }
-        __syncthreads();
     }
-    // S and sum (f+1) w in frame order by one thread (deterministic, N small)
-    __shared__ double shS, shSn;
-    if (threadIdx.x == 0) {
-        for (int f = 0; f < N; f++) {
-            S += shW[f];
-            sNum += (f + 1) * shW[f];
-        }
-        shS = S;
-        shSn = sNum;
+    if (nf == 0) return;
+#pragma unroll
+    for (int k = 0; k < 4; k++) acc[k] = wave_sum(acc[k]);
+    if (lane < nf) {
+        const double a = lane == 0 ? acc[0] : lane == 1 ? acc[1] : lane == 2 ? acc[2] : acc[3];
+        v.pathMsd[(size_t)w * v.msdStride + it.msdOff + f0 + lane] = a / M;
     }
-    __syncthreads();
-    S = shS;
-    sNum = shSn;
+}
+
+__global__ __launch_bounds__(256) void k_path_msd(StepView v, const PathItem* __restrict__ items, int nItems,
+                                                  const int* __restrict__ slots, const float* __restrict__ frames,
+                                                  int w0)
+{
+    int p = 0;
+    while (p + 1 < nItems && (int)blockIdx.x >= items[p + 1].msdBlk0) p++;
+    path_msd_block(v, items[p], slots, frames, blockIdx.x - items[p].msdBlk0, w0 + blockIdx.y);
+}
+
+__global__ __launch_bounds__(64 * PATH_PHASES) void k_path_rows(StepView v, const PathItem* __restrict__ items,
+                                                                int nItems, const int* __restrict__ slots,
+                                                                const float* __restrict__ frames, int w0)
+{
+    __shared__ double shW[PATH_MAX_FRAMES];
+    __shared__ double shRed[6][64 * PATH_PHASES];
+    __shared__ double sh[4 * 16];
+    int p = 0;
+    while (p + 1 < nItems && (int)blockIdx.x >= items[p + 1].rowBlk0) p++;
+    const PathItem& it = items[p];
+    const int tile = blockIdx.x - it.rowBlk0;
+    const int w = w0 + blockIdx.y;
+    const int M = it.M, N = it.N;
     const double lam = it.lambda;
-    const double s = S > 1e-300 ? sNum / S : 0.0;
-    const double z = S > 1e-300 ? -log(S) / lam : 0.0;
-    if (threadIdx.x == 0) {
-        v.cvVal[(size_t)w * v.V + it.valueS] = s;
-        v.cvVal[(size_t)w * v.V + it.valueS + 1] = z;
-        // ds/dlambda = -(1/S) sum (f+1-s) msd_f w_f ; dz/dlambda = (<msd>_w - z)/lambda
-        double a = 0.0, b = 0.0;
-        for (int f = 0; f < N; f++) {
-            a += ((f + 1) - s) * shM[f] * shW[f];
-            b += shM[f] * shW[f];
-        }
-        double* aux = v.pathAux + ((size_t)w * v.nPath + it.aux) * 2;
-        aux[0] = S > 1e-300 ? -a / S : 0.0;
-        aux[1] = S > 1e-300 ? (b / S - z) / lam : 0.0;
+    const double* msd = v.pathMsd + (size_t)w * v.msdStride + it.msdOff;
+    double t[2] = {0.0, 0.0};
+    for (int f = threadIdx.x; f < N; f += blockDim.x) {
+        const double wv = exp(-lam * msd[f]);
+        shW[f] = wv;
+        t[0] += wv;
+        t[1] += (f + 1) * wv;
     }
-    const double scS = -2.0 * lam / (M * S), scZ = 2.0 / (M * S);
-    for (int j = threadIdx.x; j < M; j += blockDim.x) {
-        float4 q = load_pos(v, w, slots[it.begin + j]);
-        double js0 = 0, js1 = 0, js2 = 0, jz0 = 0, jz1 = 0, jz2 = 0;
-        for (int f = 0; f < N; f++) {
-            const float* r = F + ((size_t)f * M + j) * 3;
+    block_sum<2>(t, sh);  // ends with a barrier: shW is complete
+    const double S = t[0];
+    const bool ok = S > 1e-300;
+    const double s = ok ? t[1] / S : 0.0;
+    const double z = ok ? -log(S) / lam : 0.0;
+    if (tile == 0) {
+        double a[2] = {0.0, 0.0};
+        for (int f = threadIdx.x; f < N; f += blockDim.x) {
+            a[0] += ((f + 1) - s) * msd[f] * shW[f];
+            a[1] += msd[f] * shW[f];
+        }
+        block_sum<2>(a, sh);
+        if (threadIdx.x == 0) {
+            v.cvVal[(size_t)w * v.V + it.valueS] = s;
+            v.cvVal[(size_t)w * v.V + it.valueS + 1] = z;
+            double* aux = v.pathAux + ((size_t)w * v.nPath + it.aux) * 2;
+            aux[0] = ok ? -a[0] / S : 0.0;
+            aux[1] = ok ? (a[1] / S - z) / lam : 0.0;
+        }
+    }
+    const int lane = threadIdx.x & 63, ph = threadIdx.x >> 6;
+    const int j = tile * PATH_TILE + lane;
+    double js0 = 0, js1 = 0, js2 = 0, jz0 = 0, jz1 = 0, jz2 = 0;
+    if (j < M) {
+        const float4 q = load_pos(v, w, slots[it.begin + j]);
+        const float* F = frames + it.frameOff + (size_t)j * 3;
+        for (int f = ph; f < N; f += PATH_PHASES) {
+            const float* r = F + (size_t)f * M * 3;
             const double wv = shW[f];
-            double dx = (double)q.x - r[0], dy = (double)q.y - r[1], dz = (double)q.z - r[2];
+            const double dx = (double)q.x - r[0], dy = (double)q.y - r[1], dz = (double)q.z - r[2];
             const double a = ((f + 1) - s) * wv;
             js0 += a * dx;
             js1 += a * dy;
@@ -952,23 +993,42 @@ __device__ void path_item(const StepView& v, const PathItem& it, const int* __re
             jz1 += wv * dy;
             jz2 += wv * dz;
         }
-        store_row(v, w, it.row0 + 2 * j, scS * js0, scS * js1, scS * js2);
-        store_row(v, w, it.row0 + 2 * j + 1, scZ * jz0, scZ * jz1, scZ * jz2);
+    }
+    shRed[0][threadIdx.x] = js0;
+    shRed[1][threadIdx.x] = js1;
+    shRed[2][threadIdx.x] = js2;
+    shRed[3][threadIdx.x] = jz0;
+    shRed[4][threadIdx.x] = jz1;
+    shRed[5][threadIdx.x] = jz2;
+    __syncthreads();
+    if (ph == 0 && j < M) {
+        double r6[6];
+#pragma unroll
+        for (int k = 0; k < 6; k++) {
+            double acc = 0.0;
+            for (int q = 0; q < PATH_PHASES; q++) acc += shRed[k][q * 64 + lane];
+            r6[k] = acc;
+        }
+        const double scS = ok ? -2.0 * lam / (M * S) : 0.0, scZ = ok ? 2.0 / (M * S) : 0.0;
+        store_row(v, w, it.row0 + 2 * j, scS * r6[0], scS * r6[1], scS * r6[2]);
+        store_row(v, w, it.row0 + 2 * j + 1, scZ * r6[3], scZ * r6[4], scZ * r6[5]);
     }
 }
 
-__global__ __launch_bounds__(PATH_BLOCK) void k_cv_path(StepView v, const PathItem* __restrict__ items,
-                                                        const int* __restrict__ slots,
-                                                        const float* __restrict__ frames, int w0)
-{
-    path_item(v, items[blockIdx.x], slots, frames, w0 + blockIdx.y);
-}
-
-void launch_cv_path(const StepView& v, const PathItem* items, int n, const int* slots, const float* frames, int w0,
-                    int nw, hipStream_t s)
+void launch_path_rows(const StepView& v, const PathItem* items, int n, int nRowBlocks, const int* slots,
+                      const float* frames, int w0, int nw, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_cv_path, dim3(n, nw), dim3(PATH_BLOCK), 0, s, v, items, slots, frames, w0);
+    hipLaunchKernelGGL(k_path_rows, dim3(nRowBlocks, nw), dim3(64 * PATH_PHASES), 0, s, v, items, n, slots, frames,
+                       w0);
+}
+
+void launch_cv_path(const StepView& v, const PathItem* items, int n, int nMsdBlocks, int nRowBlocks,
+                    const int* slots, const float* frames, int w0, int nw, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_path_msd, dim3(nMsdBlocks, nw), dim3(256), 0, s, v, items, n, slots, frames, w0);
+    launch_path_rows(v, items, n, nRowBlocks, slots, frames, w0, nw, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -976,7 +1036,7 @@ void launch_cv_path(const StepView& v, const PathItem* items, int n, const int* 
 // in one launch (blocks [0, nGroup) group items, then path items, then
 // blocks of 256 few-atom items).  Same device code as the per-kind kernels.
 // ---------------------------------------------------------------------------
-static_assert(GROUP_BLOCK == PATH_BLOCK, "fused CV stage uses one block size");
+static_assert(GROUP_BLOCK == 256, "the MSD blocks of the fused CV stage assume 256 threads");
 
 __global__ __launch_bounds__(GROUP_BLOCK) void k_cv_fused(StepView v, CVFusedPlan p)
 {
@@ -984,17 +1044,20 @@ __global__ __launch_bounds__(GROUP_BLOCK) void k_cv_fused(StepView v, CVFusedPla
     const int b = blockIdx.x;
     if (b < p.nGroup) {
         group_item(v, p.group[b], p.gslots, p.gmass, p.gref, w);
-    } else if (b < p.nGroup + p.nPath) {
-        path_item(v, p.path[b - p.nGroup], p.pslots, p.frames, w);
+    } else if (b < p.nGroup + p.nPathMsdBlocks) {
+        const int lb = b - p.nGroup;
+        int q = 0;
+        while (q + 1 < p.nPath && lb >= p.path[q + 1].msdBlk0) q++;
+        path_msd_block(v, p.path[q], p.pslots, p.frames, lb - p.path[q].msdBlk0, w);
     } else {
-        const int i = (b - p.nGroup - p.nPath) * GROUP_BLOCK + threadIdx.x;
+        const int i = (b - p.nGroup - p.nPathMsdBlocks) * GROUP_BLOCK + threadIdx.x;
         if (i < p.nSmall) small_item(v, p.small[i], w);
     }
 }
 
 void launch_cv_fused(const StepView& v, const CVFusedPlan& p, int nw, hipStream_t s)
 {
-    const int nb = p.nGroup + p.nPath + (p.nSmall + GROUP_BLOCK - 1) / GROUP_BLOCK;
+    const int nb = p.nGroup + p.nPathMsdBlocks + (p.nSmall + GROUP_BLOCK - 1) / GROUP_BLOCK;
     if (nb <= 0) return;
     hipLaunchKernelGGL(k_cv_fused, dim3(nb, nw), dim3(GROUP_BLOCK), 0, s, v, p);
 }

@@ -454,7 +454,10 @@ EXPORT void orc_cv_path(const double* pos, const int* atoms, int M, const double
     double z = S > 1e-300 ? -log(S) / lambda : 0.0;
     sz[0] = s;
     sz[1] = z;
-    double scS = -2.0 * lambda / (M * S), scZ = 2.0 / (M * S);
+    /* every weight underflowed: s = z = 0 as in the reference; rows and the  */
+    /* lambda derivatives are 0 (the reference divides by S = 0 there)        */
+    const int ok = S > 1e-300;
+    double scS = ok ? -2.0 * lambda / (M * S) : 0.0, scZ = ok ? 2.0 / (M * S) : 0.0;
     for (int j = 0; j < M; j++) {
         double js[3] = {0, 0, 0}, jz[3] = {0, 0, 0};
         for (int f = 0; f < N; f++) {
@@ -476,8 +479,8 @@ EXPORT void orc_cv_path(const double* pos, const int* atoms, int M, const double
             a += ((f + 1) - s) * msd[f] * w[f];
             b += msd[f] * w[f];
         }
-        dlam[0] = -a / S;
-        dlam[1] = (b / S - z) / lambda;
+        dlam[0] = ok ? -a / S : 0.0;
+        dlam[1] = ok ? (b / S - z) / lambda : 0.0;
     }
     free(w);
     free(msd);

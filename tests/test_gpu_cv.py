@@ -7,7 +7,7 @@ import numpy as np
 import pytest
 
 from gluedmd_amd.glued import Force
-from tests.gpu_util import assert_close_rel, make_pair, run_both
+from tests.gpu_util import assert_close_rel, check, make_pair, run_both
 
 pytestmark = pytest.mark.gpu
 
@@ -119,6 +119,58 @@ def test_path_random():
     g, o = make_pair(f, M + 5)
     pos = np.vstack([base + rng.normal(0, 0.05, size=(M, 3)), rng.uniform(0, 2, size=(5, 3))])
     run_both(g, o, pos)
+
+
+@pytest.mark.parametrize("M,N,batched", [(1100, 37, False), (1100, 37, True), (70, 300, True)])
+def test_path_tiles(M, N, batched):
+    """Path shapes that exercise the work split: atom counts above the LDS
+    staging chunk (1,024) and not a multiple of the 64-atom row tile, frame
+    counts not a multiple of the 16-frame MSD block, two paths in one force,
+    plain and batched forms."""
+    from gluedmd_amd.context import Context
+    from oracle.oracle import OracleContext
+    rng = np.random.default_rng(11)
+    base = rng.uniform(0, 4, size=(M, 3))
+    frames = [base + 0.02 * k * rng.normal(size=(M, 3)) for k in range(N)]
+    f = Force()
+    f.add_path(list(range(M)), [fr.reshape(-1).tolist() for fr in frames], 40.0)
+    sub = list(range(0, M, 3))[:50]
+    f.add_path(sub, [fr[sub].reshape(-1).tolist() for fr in frames[:9]], 25.0)
+    f.setTestBiasGradients([1.3, -0.6, 0.7, 0.2])
+    W = 2 if batched else 1
+    g = Context(f, walkers=[M + 3] * W)
+    pos = [np.vstack([base + rng.normal(0, 0.04, size=(M, 3)), rng.uniform(0, 2, size=(3, 3))]).astype(np.float32)
+           for _ in range(W)]
+    for w in range(W):
+        g.set_positions(pos[w], w)
+    g.zero_forces()
+    if batched:
+        g.execute_batch(use_graph=True)
+    else:
+        g.execute(0)
+    for w in range(W):
+        o = OracleContext(f, M + 3)
+        r = o.execute(pos[w])
+        check(g, o, r, g.getLastBias(w), w)
+
+
+def test_path_all_weights_underflow():
+    """Every frame's weight underflows (lambda * msd > 745 for all frames):
+    s = z = 0 as in the reference, rows 0 instead of the reference's division
+    by S = 0; no NaN reaches the force buffer."""
+    rng = np.random.default_rng(12)
+    M, N = 30, 5
+    base = rng.uniform(0, 2, size=(M, 3))
+    frames = [base + 0.01 * rng.normal(size=(M, 3)) for _ in range(N)]
+    f = Force()
+    f.add_path(list(range(M)), [fr.reshape(-1).tolist() for fr in frames], 1e5)
+    f.setTestBiasGradients([1.0, 1.0])
+    g, o = make_pair(f, M)
+    pos = (base + 1.0).astype(np.float32)  # msd = 3 nm^2 to every frame
+    eg, r = run_both(g, o, pos, compare=False)
+    assert np.array_equal(g.getLastCVValues(), [0.0, 0.0]) and np.array_equal(r["cv"], [0.0, 0.0])
+    assert np.array_equal(g.get_forces_fixed(), np.zeros_like(g.get_forces_fixed()))
+    assert np.all(r["forces"] == 0.0)
 
 
 def test_atom_reorder():

@@ -335,6 +335,18 @@ def test_path_fd_and_lambda_derivatives(oracle):
     assert abs(dl[1] - (sp[1] - sm[1]) / (2 * h)) < 1e-6
 
 
+def test_path_weight_underflow_is_finite(oracle):
+    """Every weight underflows: s = z = 0 (the reference's guard) and the rows
+    and lambda derivatives are 0 rather than a division by S = 0."""
+    frames = [[(0.0, 0.0, 0.0)], [(0.1, 0.0, 0.0)]]
+    f = path_force(frames, 1e5)
+    c, r = ctx_for(oracle, f, [(3.0, 0, 0), (5, 5, 5)], grads=[1.0, 1.0])
+    assert list(r["cv"]) == [0.0, 0.0]
+    assert np.all(r["forces"] == 0.0)
+    sz, dl = c.path_lambda_derivs(np.array([(3.0, 0, 0), (5, 5, 5)]), 0)
+    assert list(dl) == [0.0, 0.0]
+
+
 # ---------------------------------------------------------------------------
 # contact map / DRMSD (tests/test_cv_contactmap.py:22-158, test_cv_drmsd.py:52-151)
 # ---------------------------------------------------------------------------
