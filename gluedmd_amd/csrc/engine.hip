@@ -688,16 +688,21 @@ void Engine::buildPlan(int nMass, const double* masses)
     }
 
     // ---- scatter segments: rows grouped by atom slot ----
-    std::vector<int> segStart(S_ + 1, 0), segRow(R_);
+    // (row r is stored at rowPos[r]; the rows of slot s occupy [segStart[s], segStart[s+1]))
+    std::vector<int> segStart(S_ + 1, 0), rowPos(R_), entryCv(R_);
     for (int r = 0; r < R_; r++) segStart[rowSlot_[r] + 1]++;
     for (int s = 0; s < S_; s++) segStart[s + 1] += segStart[s];
     {
         std::vector<int> cur(segStart.begin(), segStart.end() - 1);
-        for (int r = 0; r < R_; r++) segRow[cur[rowSlot_[r]]++] = r;
+        for (int r = 0; r < R_; r++) {
+            const int k = cur[rowSlot_[r]]++;
+            rowPos[r] = k;
+            entryCv[k] = rowCv_[r];
+        }
     }
     dSegStart_.upload(segStart);
-    dSegRow_.upload(segRow);
-    dRowCv_.upload(rowCv_);
+    dSegRow_.upload(rowPos);
+    dRowCv_.upload(entryCv);
 
     // ---- the batched bias + scatter stage ----
     {
@@ -729,6 +734,19 @@ void Engine::buildPlan(int nMass, const double* masses)
         fplan_.nSmall = (int)smallBias_.size();
         // one workgroup scatters a walker when its atom list is short
         fplan_.scatter = (S_ <= 4096 && V_ <= 1024) ? 1 : 0;
+        // all chunked biases of one type and dimension: a specialised chunk
+        // kernel (fewer registers, more resident workgroups)
+        fplan_.kind = 0;
+        if (!ht.empty() && ot.empty()) {
+            fplan_.kind = 1;
+            fplan_.kindD = ht[0].D;
+            for (auto& h : ht) fplan_.kind = h.D == fplan_.kindD ? fplan_.kind : 0;
+        } else if (ht.empty() && !ot.empty()) {
+            fplan_.kind = 2;
+            fplan_.kindD = ot[0].D;
+            for (auto& o : ot) fplan_.kind = o.D == fplan_.kindD ? fplan_.kind : 0;
+        }
+        if (fplan_.kind && (fplan_.kindD < 1 || fplan_.kindD > 3)) fplan_.kind = 0;
     }
 }
 
@@ -848,6 +866,7 @@ StepView Engine::view(const float4* posq, long long* force) const
     v.gidx = dGidx_.as<int>();
     v.box = dBox_.as<double>();
     v.cvVal = dCv_.as<double>();
+    v.rowPos = dSegRow_.as<int>();
     v.jx = dJx_.as<float>();
     v.jy = dJy_.as<float>();
     v.jz = dJz_.as<float>();
@@ -932,12 +951,10 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
     st.scalStride = 4 * (long long)std::max((int)smallBias_.size(), 1);
     ScatterPlan sp;
     sp.segStart = dSegStart_.as<int>();
-    sp.segRow = dSegRow_.as<int>();
-    sp.rowCv = dRowCv_.as<int>();
+    sp.entryCv = dRowCv_.as<int>();
     sp.combStart = dCombStart_.as<int>();
     sp.combSlot = dCombSlot_.as<int>();
     sp.testGrad = testGrads_.empty() ? nullptr : dTestGrad_.as<double>();
-    sp.energyF = nullptr;
     sp.energyValue = energyValue_;
     const bool biases = (stageMask & 10) && testGrads_.empty();
     if (biases || (stageMask & 4)) {

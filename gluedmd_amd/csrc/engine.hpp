@@ -197,7 +197,7 @@ class Engine {
     int cellOfStride_ = 0, cellStride_ = 0, sortStride_ = 0, partStride_ = 0;
     DBuf dPath_, dPathSlots_, dPathFrames_, dPathW_;
     DBuf dSmallBias_, dGridTable_, dSmallConst_, dSmallVar_, dSmallScal_;
-    DBuf dSegStart_, dSegRow_, dRowCv_, dCombStart_, dCombSlot_, dTestGrad_, dEnergyU_, dEnergyFPtr_;
+    DBuf dSegStart_, dSegRow_ /* rowPos */, dRowCv_ /* entryCv */, dCombStart_, dCombSlot_, dTestGrad_, dEnergyU_, dEnergyFPtr_;
     DBuf dHillTable_, dOpesTable_, dFusedPart_, dFusedCounter_, dPathAux_, dAdjPart_, dPathMsd_;
     int msdStride_ = 0, pathMsdBlocks_ = 0, pathRowBlocks_ = 0;
     FusedBiasPlan fplan_{};

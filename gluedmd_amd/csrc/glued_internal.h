@@ -65,6 +65,7 @@ struct StepView {
     const int* gidx;      // [W][S]
     const double* box;    // [W][12] = a, b, c, 1/ax, 1/by, 1/cz
     double* cvVal;        // [W][V]
+    const int* rowPos;    // [R] storage position of plan row r (rows are stored grouped by atom slot)
     float* jx;            // [W][R]
     float* jy;
     float* jz;
@@ -225,13 +226,11 @@ struct CoordBuffers {
 // Scatter plan: Jacobian rows grouped by atom slot (CSR), CV of every row,
 // and the gradient slots that sum into each dU/dCV.
 struct ScatterPlan {
-    const int* segStart;     // [S+1]
-    const int* segRow;       // row ids grouped by slot
-    const int* rowCv;        // [R]
+    const int* segStart;     // [S+1] Jacobian rows of slot s are stored at [segStart[s], segStart[s+1])
+    const int* entryCv;      // [R] CV value index of the row stored at each position
     const int* combStart;    // [V+1]
     const int* combSlot;
     const double* testGrad;  // [V] or null
-    const float* energyF;    // dense external force of the energy CV (force-block layout), or null
     int energyValue;         // CV value index of the energy CV (-1 if none)
 };
 
@@ -257,6 +256,8 @@ struct FusedBiasPlan {
     SmallBiasState st;
     ScatterPlan sp;
     int scatter;             // 1: the finalizing workgroup also scatters
+    int kind;                // chunk-kernel specialisation: 1 hills only, 2 OPES only (of one D), 0 mixed
+    int kindD;               // that D
 };
 
 }  // namespace glued

@@ -1,25 +1,21 @@
 // Bias evaluation and deposition kernels for gfx950.
 //
 // Chunked sums (hill-list metadynamics, OPES kernels) stream the bias state
-// once per evaluation: each workgroup reduces one chunk to FP64 partials,
-// the last workgroup of a walker to finish (atomic ticket) adds the partials
-// in chunk order -- deterministic -- and writes V and dV/ds.  Per-hill
+// once per evaluation: each workgroup reduces one chunk to FP64 partials;
+// the partials are then added in a fixed order (wave-parallel, see
+// reduce_partials) -- deterministic -- and V and dV/ds written.  Per-hill
 // exponents are formed in FP64 with unfused operations so the E <= 8
 // truncation predicate is bit-identical to the FP64 oracle; exp() runs in FP32
 // (its relative error bounds each term by 0.37 * 2^-24 of its height).
 #include "glued_internal.h"
 #include <hip/hip_runtime.h>
 #include "scatter_dev.h"
+#include "wave_reduce.h"
 #include <cmath>
 
 namespace glued {
 
-__device__ inline double wsum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+__device__ inline double wsum(double v) { return wave_sum1(v); }
 
 // ---------------------------------------------------------------------------
 // Grid MetaD evaluation: CommonGluedKernels.cpp:3574-3636.
@@ -168,16 +164,15 @@ void launch_bias_small(const StepView& v, const SmallBiasState& st, const SmallB
 //   hills: V = sum_k h_k exp(-E_k) [E_k <= 8], E_k = sum_d dc_d^2/(2 sigma_d^2)
 //   OPES : prob = sum_k h_k max(G_k - kfloor, 0), acc_d = sum_k h_k G_k (-dc_d/s_kd^2)
 // One 256-thread workgroup reduces one chunk of 1,024 entries (4 per thread,
-// coalesced SoA loads) to D+1 FP64 partials; the workgroup that finishes
-// last for a walker (atomic ticket) gathers the partials in parallel into LDS
-// and adds them in chunk order (deterministic), then finalizes.
+// coalesced SoA loads) to D+1 FP64 partials.  Plain form: the workgroup that
+// finishes last for a walker (atomic ticket) adds the partials and finalizes;
+// batched form: a second launch does it (no cross-workgroup fences).
 // ---------------------------------------------------------------------------
 constexpr int BIAS_BLOCK = 256;
 constexpr int PER_THREAD = 4;
 constexpr int CHUNK = BIAS_BLOCK * PER_THREAD;
 constexpr int HILL_BLOCK = BIAS_BLOCK;
 constexpr int OPES_BLOCK = BIAS_BLOCK;
-constexpr int PART_LDS = 1024;   // doubles of LDS for gathering partials
 constexpr int PW = MAX_BIAS_D + 1;  // partial width in the fused buffer
 
 int hill_chunks_for(int cap) { return (cap + CHUNK - 1) / CHUNK; }
@@ -185,10 +180,9 @@ int opes_chunks_for(int cap) { return (cap + CHUNK - 1) / CHUNK; }
 
 // Block-wide sum of NV values, written by thread 0 to dst.  sh >= NV*4.
 template <int NV>
-__device__ inline void block_partial(double* acc, double* sh, double* dst)
+__device__ inline void block_partial(double (&acc)[NV], double* sh, double* dst)
 {
-#pragma unroll
-    for (int d = 0; d < NV; d++) acc[d] = wsum(acc[d]);
+    wave_sum_n<NV>(acc);
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0)
 #pragma unroll
@@ -284,23 +278,30 @@ __device__ void opes_chunk(const StepView& v, const OpesBiasDev& o, int w, int c
     block_partial<D + 1>(acc, sh, dst);
 }
 
-// Sum n partial vectors of width wd (written by other workgroups), in index
-// order, into tot (thread 0).  Loads run in parallel through LDS.
-__device__ void sum_partials(const double* src, int n, int wd, double* tot, double* shbuf)
+// Sum of n partial vectors (width wd <= PW, stride ld) by wave 0 of the
+// workgroup: lane l adds entries l, l + 64, ... in index order, then a
+// butterfly over the wave (commutative pairwise adds: every lane ends with the
+// same bits).  The order depends only on n, so the plain and the batched forms
+// agree bit for bit.  ATOMIC: the partials were written by other workgroups of
+// the same launch (read with device-scope loads after the ticket).
+template <bool ATOMIC>
+__device__ inline void reduce_partials(const double* src, int n, int wd, int ld, double* tot)
 {
-    if (threadIdx.x == 0)
-        for (int d = 0; d < wd; d++) tot[d] = 0.0;
-    const int per = PART_LDS / wd;
-    for (int base = 0; base < n; base += per) {
-        const int m = min(n - base, per);
-        for (int i = threadIdx.x; i < m * wd; i += blockDim.x)
-            shbuf[i] = __hip_atomic_load(src + (size_t)base * wd + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
-        if (threadIdx.x == 0)
-            for (int j = 0; j < m; j++)
-                for (int d = 0; d < wd; d++) tot[d] += shbuf[j * wd + d];
-        __syncthreads();
+    const int lane = threadIdx.x & 63;
+    double a[PW];
+#pragma unroll
+    for (int d = 0; d < PW; d++) a[d] = 0.0;
+    for (int j = lane; j < n; j += 64) {
+#pragma unroll
+        for (int d = 0; d < PW; d++)
+            if (d < wd) {
+                const double* p = src + (size_t)j * ld + d;
+                a[d] += ATOMIC ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
+            }
     }
+    wave_sum_n<PW>(a);
+#pragma unroll
+    for (int d = 0; d < PW; d++) tot[d] = a[d];
 }
 
 __device__ inline void hill_finalize(const StepView& v, const HillBiasDev& h, int w, const double* tot)
@@ -352,12 +353,12 @@ template <int D>
 __global__ __launch_bounds__(BIAS_BLOCK) void k_bias_hills(StepView v, HillBiasDev h, int w0)
 {
     __shared__ double sh[(D + 1) * 4];
-    __shared__ double shbuf[PART_LDS];
-    __shared__ double tot[D + 1];
     const int w = w0 + blockIdx.y;
     hill_chunk<D>(v, h, w, blockIdx.x, sh, h.part + ((size_t)w * h.chunks + blockIdx.x) * (D + 1));
     if (!last_block(&h.counter[w], gridDim.x)) return;
-    sum_partials(h.part + (size_t)w * h.chunks * (D + 1), gridDim.x, D + 1, tot, shbuf);
+    if (threadIdx.x >= 64) return;
+    double tot[PW];
+    reduce_partials<true>(h.part + (size_t)w * h.chunks * (D + 1), gridDim.x, D + 1, D + 1, tot);
     if (threadIdx.x == 0) {
         hill_finalize(v, h, w, tot);
         h.counter[w] = 0u;
@@ -368,12 +369,12 @@ template <int D>
 __global__ __launch_bounds__(BIAS_BLOCK) void k_bias_opes(StepView v, OpesBiasDev o, int w0)
 {
     __shared__ double sh[(D + 1) * 4];
-    __shared__ double shbuf[PART_LDS];
-    __shared__ double tot[D + 1];
     const int w = w0 + blockIdx.y;
     opes_chunk<D>(v, o, w, blockIdx.x, sh, o.part + ((size_t)w * o.chunks + blockIdx.x) * (D + 1));
     if (!last_block(&o.counter[w], gridDim.x)) return;
-    sum_partials(o.part + (size_t)w * o.chunks * (D + 1), gridDim.x, D + 1, tot, shbuf);
+    if (threadIdx.x >= 64) return;
+    double tot[PW];
+    reduce_partials<true>(o.part + (size_t)w * o.chunks * (D + 1), gridDim.x, D + 1, D + 1, tot);
     if (threadIdx.x == 0) {
         opes_finalize(v, o, w, tot);
         o.counter[w] = 0u;
@@ -413,6 +414,10 @@ void launch_bias_opes(const StepView& v, const OpesBiasDev& o, int w0, int nw, h
 // ---------------------------------------------------------------------------
 constexpr int MAX_LDS_VALUES = 1024;
 
+// KIND 0: any mix of hill and OPES biases of any D; KIND 1 (2): only hill
+// (OPES) biases, all of dimension DD -- the specialisations keep register use
+// low enough for every workgroup of a typical step to be resident at once.
+template <int KIND, int DD>
 __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_chunks(StepView v, FusedBiasPlan fp)
 {
     __shared__ double sh[PW * 4];
@@ -422,7 +427,11 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_chunks(StepView v, Fus
     while (k + 1 < fp.nChunked && b >= fp.chunkStart[k + 1]) k++;
     const int chunk = b - fp.chunkStart[k];
     double* dst = fp.part + ((size_t)w * fp.totalChunks + b) * PW;
-    if (k < fp.nHill) {
+    if (KIND == 1) {
+        hill_chunk<DD>(v, fp.hills[k], w, chunk, sh, dst);
+    } else if (KIND == 2) {
+        opes_chunk<DD>(v, fp.opes[k], w, chunk, sh, dst);
+    } else if (k < fp.nHill) {
         const HillBiasDev& h = fp.hills[k];
         switch (h.D) {
             case 1: hill_chunk<1>(v, h, w, chunk, sh, dst); break;
@@ -441,38 +450,42 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_chunks(StepView v, Fus
     }
 }
 
-// One workgroup per walker: partial sums (written by the previous launch)
-// in chunk order, finalize, one-thread biases, dU/dCV, and the scatter.
+static void launch_chunks(const StepView& v, const FusedBiasPlan& fp, int nw, hipStream_t s)
+{
+    const dim3 g(fp.totalChunks, nw), b(BIAS_BLOCK);
+    if (fp.kind == 1 && fp.kindD == 1) hipLaunchKernelGGL((k_step_bias_chunks<1, 1>), g, b, 0, s, v, fp);
+    else if (fp.kind == 1 && fp.kindD == 2) hipLaunchKernelGGL((k_step_bias_chunks<1, 2>), g, b, 0, s, v, fp);
+    else if (fp.kind == 1 && fp.kindD == 3) hipLaunchKernelGGL((k_step_bias_chunks<1, 3>), g, b, 0, s, v, fp);
+    else if (fp.kind == 2 && fp.kindD == 1) hipLaunchKernelGGL((k_step_bias_chunks<2, 1>), g, b, 0, s, v, fp);
+    else if (fp.kind == 2 && fp.kindD == 2) hipLaunchKernelGGL((k_step_bias_chunks<2, 2>), g, b, 0, s, v, fp);
+    else if (fp.kind == 2 && fp.kindD == 3) hipLaunchKernelGGL((k_step_bias_chunks<2, 3>), g, b, 0, s, v, fp);
+    else hipLaunchKernelGGL((k_step_bias_chunks<0, 1>), g, b, 0, s, v, fp);
+}
+
+// One workgroup per walker: wave 0 reduces the chunk partials (written by the
+// previous launch) and finalizes the chunked biases while the other waves
+// evaluate the one-thread biases; then dU/dCV and the scatter.
 __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, FusedBiasPlan fp)
 {
-    __shared__ double shbuf[PART_LDS];
     __shared__ double sU[MAX_LDS_VALUES];
-    __shared__ double tot[PW];
     const int w = blockIdx.x;
-    for (int k = 0; k < fp.nChunked; k++) {
-        const int c0 = fp.chunkStart[k], nc = fp.chunkStart[k + 1] - c0;
-        const double* src = fp.part + ((size_t)w * fp.totalChunks + c0) * PW;
-        if (threadIdx.x == 0)
-            for (int d = 0; d < PW; d++) tot[d] = 0.0;
-        const int per = PART_LDS / PW;
-        for (int base = 0; base < nc; base += per) {
-            const int m = min(nc - base, per);
-            for (int i = threadIdx.x; i < m * PW; i += blockDim.x) shbuf[i] = src[(size_t)base * PW + i];
-            __syncthreads();
-            if (threadIdx.x == 0)
-                for (int j = 0; j < m; j++)
-                    for (int d = 0; d < PW; d++) tot[d] += shbuf[j * PW + d];
-            __syncthreads();
+    if (threadIdx.x < 64) {
+        for (int k = 0; k < fp.nChunked; k++) {
+            const int c0 = fp.chunkStart[k], nc = fp.chunkStart[k + 1] - c0;
+            const double* src = fp.part + ((size_t)w * fp.totalChunks + c0) * PW;
+            const int D = k < fp.nHill ? fp.hills[k].D : fp.opes[k - fp.nHill].D;
+            double tot[PW];
+            reduce_partials<false>(src, nc, D + 1, PW, tot);
+            if (threadIdx.x == 0) {
+                if (k < fp.nHill)
+                    hill_finalize(v, fp.hills[k], w, tot);
+                else
+                    opes_finalize(v, fp.opes[k - fp.nHill], w, tot);
+            }
         }
-        if (threadIdx.x == 0) {
-            if (k < fp.nHill)
-                hill_finalize(v, fp.hills[k], w, tot);
-            else
-                opes_finalize(v, fp.opes[k - fp.nHill], w, tot);
-        }
-        __syncthreads();
+    } else {
+        for (int i = threadIdx.x - 64; i < fp.nSmall; i += blockDim.x - 64) small_bias_eval(v, fp.st, fp.small[i], i, w);
     }
-    for (int i = threadIdx.x; i < fp.nSmall; i += blockDim.x) small_bias_eval(v, fp.st, fp.small[i], i, w);
     __syncthreads();
     if (fp.scatter)
         scatter_walker_block(v, fp.sp, w, sU, MAX_LDS_VALUES);
@@ -482,13 +495,12 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, F
 
 void launch_step_bias_chunks(const StepView& v, const FusedBiasPlan& fp, int nw, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_bias_chunks, dim3(fp.totalChunks, nw), dim3(BIAS_BLOCK), 0, s, v, fp);
+    launch_chunks(v, fp, nw, s);
 }
 
 void launch_step_bias(const StepView& v, const FusedBiasPlan& fp, int nw, hipStream_t s)
 {
-    if (fp.totalChunks > 0)
-        hipLaunchKernelGGL(k_step_bias_chunks, dim3(fp.totalChunks, nw), dim3(BIAS_BLOCK), 0, s, v, fp);
+    if (fp.totalChunks > 0) launch_chunks(v, fp, nw, s);
     hipLaunchKernelGGL(k_step_bias_finalize, dim3(nw), dim3(BIAS_BLOCK), 0, s, v, fp);
 }
 
@@ -956,8 +968,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_hill_adjoint(StepView v, HillBias
     }
     if (ok) gH[(size_t)tb * h.cap + k] = gh;
     // per-block partial of the sigma gradient (scaled by 1/sigma^3 on the host side of the reduction)
-#pragma unroll
-    for (int d = 0; d < D; d++) gs[d] = wsum(gs[d]);
+    wave_sum_n<D>(gs);
     if ((threadIdx.x & 63) == 0)
         for (int d = 0; d < D; d++) sh[d * 4 + (threadIdx.x >> 6)] = gs[d];
     __syncthreads();

@@ -11,6 +11,7 @@
 #include "glued_internal.h"
 #include <hip/hip_runtime.h>
 #include <cmath>
+#include "wave_reduce.h"
 
 namespace glued {
 
@@ -52,29 +53,33 @@ __device__ inline void pbc_f(float& x, float& y, float& z, const float* bx)
     x -= s1 * bx[0];
 }
 
-__device__ inline double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
+__device__ inline double wave_sum(double v) { return wave_sum1(v); }
 
-// Block-wide FP64 sum of NV values per thread; every thread receives the result.
-template <int NV>
-__device__ inline void block_sum(double (&v)[NV], double* sh /* >= NV * 16 */)
+// Block-wide FP64 sum of NV values per thread over NW waves (the block size,
+// fixed at compile time so the per-wave partials are read with independent
+// LDS loads rather than a loop of dependent ones); every thread receives the
+// result.  sh holds >= NV * 16 doubles.
+template <int NV, int NW = 4>
+__device__ inline void block_sum(double (&v)[NV], double* sh)
 {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = (blockDim.x + 63) >> 6;
-#pragma unroll
-    for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+    static_assert(NW >= 1 && NW <= 16, "1..16 waves");
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    wave_sum_n<NV>(v);
     __syncthreads();
     if (lane == 0)
 #pragma unroll
         for (int k = 0; k < NV; k++) sh[k * 16 + wid] = v[k];
     __syncthreads();
+    double p[NV][NW];
+#pragma unroll
+    for (int k = 0; k < NV; k++)
+#pragma unroll
+        for (int i = 0; i < NW; i++) p[k][i] = sh[k * 16 + i];
 #pragma unroll
     for (int k = 0; k < NV; k++) {
         double s = 0.0;
-        for (int i = 0; i < nw; i++) s += sh[k * 16 + i];
+#pragma unroll
+        for (int i = 0; i < NW; i++) s += p[k][i];
         v[k] = s;
     }
     __syncthreads();
@@ -89,7 +94,7 @@ __device__ inline float4 load_pos(const StepView& v, int w, int slot)
 
 __device__ inline void store_row(const StepView& v, int w, int row, double gx, double gy, double gz)
 {
-    size_t i = (size_t)w * v.R + row;
+    size_t i = (size_t)w * v.R + v.rowPos[row];
     v.jx[i] = (float)gx;
     v.jy[i] = (float)gy;
     v.jz[i] = (float)gz;
@@ -108,8 +113,10 @@ __device__ void small_item(const StepView& v, const SmallCVItem& it, int w)
     const double* bx = v.box + (size_t)w * 12;
     double p[4][3];
     const int na = it.type == CV_DISTANCE ? 2 : it.type == CV_ANGLE ? 3 : it.type == CV_DIHEDRAL ? 4 : 1;
-    for (int a = 0; a < na; a++) {
-        float4 q = load_pos(v, w, it.slot[a]);
+    // every gather issued before the first use (independent loads in flight together)
+#pragma unroll
+    for (int a = 0; a < 4; a++) {
+        const float4 q = load_pos(v, w, it.slot[a < na ? a : 0]);
         p[a][0] = q.x;
         p[a][1] = q.y;
         p[a][2] = q.z;
@@ -221,6 +228,25 @@ __device__ inline double det3(double a, double b, double c, double d, double e, 
     return a * (e * i - f * h) - b * (d * i - f * g) + c * (d * h - e * g);
 }
 
+// Minor of the 4x4 matrix A with row RR and column CC deleted (static indices).
+template <int RR, int CC>
+__device__ inline double minor4(const double (&A)[4][4])
+{
+    constexpr int r0 = RR == 0 ? 1 : 0, r1 = RR <= 1 ? 2 : 1, r2 = RR <= 2 ? 3 : 2;
+    constexpr int c0 = CC == 0 ? 1 : 0, c1 = CC <= 1 ? 2 : 1, c2 = CC <= 2 ? 3 : 2;
+    return det3(A[r0][c0], A[r0][c1], A[r0][c2], A[r1][c0], A[r1][c1], A[r1][c2], A[r2][c0], A[r2][c1], A[r2][c2]);
+}
+
+// Column CC of adj(A): adj(A)[i][CC] = (-1)^(i+CC) minor(CC, i).
+template <int CC>
+__device__ inline void adj_column(const double (&A)[4][4], double* q)
+{
+    q[0] = ((0 + CC) & 1 ? -1.0 : 1.0) * minor4<CC, 0>(A);
+    q[1] = ((1 + CC) & 1 ? -1.0 : 1.0) * minor4<CC, 1>(A);
+    q[2] = ((2 + CC) & 1 ? -1.0 : 1.0) * minor4<CC, 2>(A);
+    q[3] = ((3 + CC) & 1 ? -1.0 : 1.0) * minor4<CC, 3>(A);
+}
+
 __device__ void horn_rotation(const double* S, double Gx, double Gy, double* R, double* lmax)
 {
     const double Sxx = S[0], Sxy = S[1], Sxz = S[2], Syx = S[3], Syy = S[4], Syz = S[5], Szx = S[6], Szy = S[7],
@@ -236,25 +262,23 @@ __device__ void horn_rotation(const double* S, double Gx, double Gy, double* R, 
     K[2][2] = -Sxx + Syy - Szz;
     K[2][3] = K[3][2] = Syz + Szy;
     K[3][3] = -Sxx - Syy + Szz;
-    double K2[4][4];
+    // K symmetric: tr K^2 = sum_ij K_ij^2, tr K^3 = sum_ij (K^2)_ij K_ij
     double trK2 = 0.0, trK3 = 0.0;
+#pragma unroll
     for (int i = 0; i < 4; i++)
-        for (int j = 0; j < 4; j++) {
-            double s = 0.0;
-            for (int k = 0; k < 4; k++) s += K[i][k] * K[k][j];
-            K2[i][j] = s;
+#pragma unroll
+        for (int j = 0; j < 4; j++) trK2 += K[i][j] * K[i][j];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = i; j < 4; j++) {
+            double k2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < 4; k++) k2 += K[i][k] * K[k][j];
+            trK3 += (i == j ? 1.0 : 2.0) * k2 * K[i][j];
         }
-    for (int i = 0; i < 4; i++) {
-        trK2 += K2[i][i];
-        for (int k = 0; k < 4; k++) trK3 += K2[i][k] * K[k][i];
-    }
-    // det K by cofactor expansion along row 0
-    double detK = 0.0;
-    for (int c = 0; c < 4; c++) {
-        int c0 = c == 0 ? 1 : 0, c1 = c <= 1 ? 2 : 1, c2 = c <= 2 ? 3 : 2;
-        double m = det3(K[1][c0], K[1][c1], K[1][c2], K[2][c0], K[2][c1], K[2][c2], K[3][c0], K[3][c1], K[3][c2]);
-        detK += ((c & 1) ? -1.0 : 1.0) * K[0][c] * m;
-    }
+    const double detK = K[0][0] * minor4<0, 0>(K) - K[0][1] * minor4<0, 1>(K) + K[0][2] * minor4<0, 2>(K) -
+                        K[0][3] * minor4<0, 3>(K);
     const double e2 = -0.5 * trK2, e3 = trK3 / 3.0, e4 = detK;
     double l = 0.5 * (Gx + Gy);
     for (int it = 0; it < 60; it++) {
@@ -264,33 +288,34 @@ __device__ void horn_rotation(const double* S, double Gx, double Gy, double* R, 
         if (!(fabs(df) > 0.0)) break;
         double dl = f / df;
         l -= dl;
-        if (fabs(dl) <= 1e-15 * fabs(l) + 1e-300) break;
+        // quadratic convergence: after a step of relative size 1e-10 the
+        // remaining error is ~1e-20 relative for separated eigenvalues
+        if (fabs(dl) <= 1e-10 * fabs(l) + 1e-300) break;
     }
     *lmax = l;
     double A[4][4];
+#pragma unroll
     for (int i = 0; i < 4; i++)
+#pragma unroll
         for (int j = 0; j < 4; j++) A[i][j] = K[i][j] - (i == j ? l : 0.0);
-    // adjugate columns; adj(A)[i][j] = (-1)^(i+j) * minor(j, i)
-    double best = -1.0, q[4] = {1.0, 0.0, 0.0, 0.0};
-    for (int j = 0; j < 4; j++) {
-        double col[4], nn = 0.0;
-        for (int i = 0; i < 4; i++) {
-            // minor deleting row j and column i
-            int r[3], c[3];
-            for (int a = 0, k = 0; a < 4; a++)
-                if (a != j) r[k++] = a;
-            for (int a = 0, k = 0; a < 4; a++)
-                if (a != i) c[k++] = a;
-            double m = det3(A[r[0]][c[0]], A[r[0]][c[1]], A[r[0]][c[2]], A[r[1]][c[0]], A[r[1]][c[1]], A[r[1]][c[2]],
-                            A[r[2]][c[0]], A[r[2]][c[1]], A[r[2]][c[2]]);
-            col[i] = (((i + j) & 1) ? -1.0 : 1.0) * m;
-            nn += col[i] * col[i];
-        }
-        if (nn > best) {
-            best = nn;
-            for (int i = 0; i < 4; i++) q[i] = col[i];
-        }
+    // adj(A) of the rank-3 symmetric A is c q q^T: its diagonal is c q_i^2, so
+    // the column through the largest |diagonal cofactor| is the best-conditioned
+    // multiple of the eigenvector (first maximum on ties)
+    const double d0 = fabs(minor4<0, 0>(A)), d1 = fabs(minor4<1, 1>(A)), d2 = fabs(minor4<2, 2>(A)),
+                 d3 = fabs(minor4<3, 3>(A));
+    int ib = 0;
+    double db = d0;
+    if (d1 > db) { ib = 1; db = d1; }
+    if (d2 > db) { ib = 2; db = d2; }
+    if (d3 > db) { ib = 3; db = d3; }
+    double q[4];
+    switch (ib) {
+        case 0: adj_column<0>(A, q); break;
+        case 1: adj_column<1>(A, q); break;
+        case 2: adj_column<2>(A, q); break;
+        default: adj_column<3>(A, q); break;
     }
+    double best = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
     double qn = sqrt(best > 0.0 ? best : 1.0);
     if (!(best > 1e-300)) {
         q[0] = 1.0;
@@ -319,6 +344,7 @@ __device__ void horn_rotation(const double* S, double Gx, double Gy, double* R, 
 // groupRef: RMSD reference xyz / DRMSD d0 / contact-map [r0, n, m, w].
 // ---------------------------------------------------------------------------
 constexpr int GROUP_BLOCK = 256;
+constexpr int GROUP_REG = 4;  // atoms per thread held in registers by the group CVs
 
 __device__ inline double ipowd(double x, int n)
 {
@@ -327,26 +353,129 @@ __device__ inline double ipowd(double x, int n)
     return r;
 }
 
+// Kabsch RMSD of up to 64 * KABSCH_PER_LANE atoms by one wave: every lane
+// keeps its atoms and their centred reference in registers, one butterfly
+// gives every lane the raw moments
+//   m[3a+b] = sum y'_a x_b, m[9..11] = sum x, m[12] = sum |x|^2,
+//   m[13..15] = sum y', m[16] = sum |y'|^2   (y' = y - ybar),
+// from which the centred covariance sum y'(x - xbar)^T = m[3a+b] - (sum y'_a) xbar_b
+// and sum |x - xbar|^2 = sum |x|^2 - n |xbar|^2 (FP64).  Every lane then runs
+// the same Horn solve on the same bits (no broadcast) and writes its rows.
+constexpr int KABSCH_PER_LANE = 8;
+
+__device__ void kabsch_wave(const StepView& v, const GroupCVItem& it, const int* __restrict__ slots,
+                            const float* __restrict__ ref, int w)
+{
+    const int lane = threadIdx.x & 63;
+    const int n = it.end - it.begin;
+    const float* Y = ref + 3 * (size_t)it.refOff;
+    const double* yc = it.refCentroid;
+    float qx[KABSCH_PER_LANE][3], qy[KABSCH_PER_LANE][3];  // walker atom, raw FP32 reference
+    int rp[KABSCH_PER_LANE];
+    // lanes past the end load a valid entry (index clamped) and drop it: no
+    // divergent branch, so every gather of the wave is in flight at once
+#pragma unroll
+    for (int r = 0; r < KABSCH_PER_LANE; r++) {
+        const int j = min(lane + 64 * r, n - 1);
+        const float4 p = load_pos(v, w, slots[it.begin + j]);
+        qx[r][0] = p.x;
+        qx[r][1] = p.y;
+        qx[r][2] = p.z;
+#pragma unroll
+        for (int k = 0; k < 3; k++) qy[r][k] = Y[3 * j + k];
+        rp[r] = v.rowPos[it.row0 + j];
+    }
+    double m[17];
+#pragma unroll
+    for (int k = 0; k < 17; k++) m[k] = 0.0;
+#pragma unroll
+    for (int r = 0; r < KABSCH_PER_LANE; r++) {
+        if (lane + 64 * r >= n) break;
+        const double x[3] = {(double)qx[r][0], (double)qx[r][1], (double)qx[r][2]};
+        const double y[3] = {(double)qy[r][0] - yc[0], (double)qy[r][1] - yc[1], (double)qy[r][2] - yc[2]};
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) m[3 * a + b] += y[a] * x[b];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            m[9 + k] += x[k];
+            m[12] += x[k] * x[k];
+            m[13 + k] += y[k];
+            m[16] += y[k] * y[k];
+        }
+    }
+    wave_sum_n<17>(m);
+    const double xc[3] = {m[9] / n, m[10] / n, m[11] / n};
+    double S[9], R[9], l;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) S[3 * a + b] = m[3 * a + b] - m[13 + a] * xc[b];
+    const double Gx = m[12] - n * (xc[0] * xc[0] + xc[1] * xc[1] + xc[2] * xc[2]);
+    horn_rotation(S, Gx, m[16], R, &l);
+    double msd = (Gx + m[16] - 2.0 * l) / n;
+    if (msd < 0.0) msd = 0.0;
+    const double rv = sqrt(msd);
+    const double sc = (rv > 1e-10 ? 1.0 / rv : 0.0) / n;
+    if (lane == 0) v.cvVal[(size_t)w * v.V + it.value] = rv;
+    const size_t base = (size_t)w * v.R;
+#pragma unroll
+    for (int r = 0; r < KABSCH_PER_LANE; r++) {
+        const int j = lane + 64 * r;
+        if (j >= n) break;
+        const double x0 = qx[r][0] - xc[0], x1 = qx[r][1] - xc[1], x2 = qx[r][2] - xc[2];
+        const double y[3] = {(double)qy[r][0] - yc[0], (double)qy[r][1] - yc[1], (double)qy[r][2] - yc[2]};
+        const double r0 = R[0] * y[0] + R[1] * y[1] + R[2] * y[2];
+        const double r1 = R[3] * y[0] + R[4] * y[1] + R[5] * y[2];
+        const double r2 = R[6] * y[0] + R[7] * y[1] + R[8] * y[2];
+        v.jx[base + rp[r]] = (float)((x0 - r0) * sc);
+        v.jy[base + rp[r]] = (float)((x1 - r1) * sc);
+        v.jz[base + rp[r]] = (float)((x2 - r2) * sc);
+    }
+}
+
 __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* __restrict__ slots,
                            const float* __restrict__ mass, const float* __restrict__ ref, int w)
 {
-    __shared__ double sh[16 * 16];
+    __shared__ double sh[17 * 16];
     __shared__ double shR[9];
     const double* bx = v.box + (size_t)w * 12;
     double* cvv = v.cvVal + (size_t)w * v.V;
-    const int tid = threadIdx.x, nt = blockDim.x;
+    const int tid = threadIdx.x;
+    constexpr int nt = GROUP_BLOCK;  // every launch of the group CVs uses this block size
 
     if (it.type == CV_GYRATION || it.type == CV_COM_DISTANCE) {
         const int n = it.end - it.begin;
         const int n1 = it.type == CV_COM_DISTANCE ? it.split : n;
+        // atoms of this thread: j = tid + r * nt; the first GROUP_REG stay in
+        // registers (one gather round for every pass when n <= GROUP_REG * nt)
+        float4 q[GROUP_REG];
+        double mj[GROUP_REG];
+#pragma unroll
+        for (int r = 0; r < GROUP_REG; r++) {  // clamped index: no divergent loads
+            const int j = min(tid + r * nt, n - 1);
+            q[r] = load_pos(v, w, slots[it.begin + j]);
+            mj[r] = mass[it.begin + j];
+        }
+        auto atom = [&](int r, int j, float4& qq, double& mm) {
+            if (r < GROUP_REG) {
+                qq = q[r];
+                mm = mj[r];
+            } else {
+                qq = load_pos(v, w, slots[it.begin + j]);
+                mm = mass[it.begin + j];
+            }
+        };
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // m*x (g1), M1, m*x (g2), M2
-        for (int j = tid; j < n; j += nt) {
-            float4 q = load_pos(v, w, slots[it.begin + j]);
-            double m = mass[it.begin + j];
-            int o = j < n1 ? 0 : 4;
-            acc[o + 0] += m * q.x;
-            acc[o + 1] += m * q.y;
-            acc[o + 2] += m * q.z;
+        for (int j = tid, r = 0; j < n; j += nt, r++) {
+            float4 qq;
+            double m;
+            atom(r, j, qq, m);
+            const int o = j < n1 ? 0 : 4;
+            acc[o + 0] += m * qq.x;
+            acc[o + 1] += m * qq.y;
+            acc[o + 2] += m * qq.z;
             acc[o + 3] += m;
         }
         block_sum<8>(acc, sh);
@@ -354,21 +483,25 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
             const double M = acc[3];
             const double cx = acc[0] / M, cy = acc[1] / M, cz = acc[2] / M;
             double r2[1] = {0.0};
-            for (int j = tid; j < n; j += nt) {
-                float4 q = load_pos(v, w, slots[it.begin + j]);
-                double dx = q.x - cx, dy = q.y - cy, dz = q.z - cz;
+            for (int j = tid, r = 0; j < n; j += nt, r++) {
+                float4 qq;
+                double m;
+                atom(r, j, qq, m);
+                double dx = qq.x - cx, dy = qq.y - cy, dz = qq.z - cz;
                 if (v.pbc) pbc_exact(dx, dy, dz, bx);
-                r2[0] += (double)mass[it.begin + j] * (dx * dx + dy * dy + dz * dz);
+                r2[0] += m * (dx * dx + dy * dy + dz * dz);
             }
             block_sum<1>(r2, sh);
             const double rg = sqrt(r2[0] / M);
             const double inv = rg > 1e-10 ? 1.0 / rg : 0.0;
-            for (int j = tid; j < n; j += nt) {
-                float4 q = load_pos(v, w, slots[it.begin + j]);
-                double dx = q.x - cx, dy = q.y - cy, dz = q.z - cz;
+            for (int j = tid, r = 0; j < n; j += nt, r++) {
+                float4 qq;
+                double m;
+                atom(r, j, qq, m);
+                double dx = qq.x - cx, dy = qq.y - cy, dz = qq.z - cz;
                 if (v.pbc) pbc_exact(dx, dy, dz, bx);
-                double s = (double)mass[it.begin + j] / M * inv;
-                store_row(v, w, it.row0 + j, s * dx, s * dy, s * dz);
+                const double sc = m / M * inv;
+                store_row(v, w, it.row0 + j, sc * dx, sc * dy, sc * dz);
             }
             if (tid == 0) cvv[it.value] = rg;
         } else {
@@ -380,84 +513,120 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
             double inv = d > 1e-10 ? 1.0 / d : 0.0;
             for (int j = tid; j < n; j += nt) {
                 double m = mass[it.begin + j];
-                double s = (j < n1 ? -m / acc[3] : m / acc[7]) * inv;
-                store_row(v, w, it.row0 + j, s * dx, s * dy, s * dz);
+                double sc = (j < n1 ? -m / acc[3] : m / acc[7]) * inv;
+                store_row(v, w, it.row0 + j, sc * dx, sc * dy, sc * dz);
             }
             if (tid == 0) cvv[it.value] = d;
         }
         return;
     }
 
+    if (it.type == CV_RMSD && it.split && it.end - it.begin <= 64 * KABSCH_PER_LANE) {
+        if (tid < 64) kabsch_wave(v, it, slots, ref, w);
+        return;
+    }
     if (it.type == CV_RMSD) {
         const int n = it.end - it.begin;
-        const float* y = ref + 3 * (size_t)it.refOff;
-        if (!it.split) {
-            double s[1] = {0.0};
-            for (int j = tid; j < n; j += nt) {
-                float4 q = load_pos(v, w, slots[it.begin + j]);
-                double dx = (double)q.x - y[3 * j], dy = (double)q.y - y[3 * j + 1], dz = (double)q.z - y[3 * j + 2];
-                s[0] += dx * dx + dy * dy + dz * dz;
+        const float* Y = ref + 3 * (size_t)it.refOff;
+        const double* yc = it.refCentroid;
+        const bool fit = it.split != 0;
+        // one gather round: the first GROUP_REG atoms of this thread stay in
+        // registers with their reference (centred for the fit)
+        float4 q[GROUP_REG];
+        double yr[GROUP_REG][3];
+#pragma unroll
+        for (int r = 0; r < GROUP_REG; r++) {  // clamped index: no divergent loads
+            const int j = min(tid + r * nt, n - 1);
+            q[r] = load_pos(v, w, slots[it.begin + j]);
+#pragma unroll
+            for (int k = 0; k < 3; k++) yr[r][k] = (double)Y[3 * j + k] - (fit ? yc[k] : 0.0);
+        }
+        auto atom = [&](int r, int j, float4& qq, double* yy) {
+            if (r < GROUP_REG) {
+                qq = q[r];
+#pragma unroll
+                for (int k = 0; k < 3; k++) yy[k] = yr[r][k];
+            } else {
+                qq = load_pos(v, w, slots[it.begin + j]);
+#pragma unroll
+                for (int k = 0; k < 3; k++) yy[k] = (double)Y[3 * j + k] - (fit ? yc[k] : 0.0);
             }
-            block_sum<1>(s, sh);
-            const double r = sqrt(s[0] / n);
-            const double sc = (r > 1e-10 ? 1.0 / r : 0.0) / n;
-            for (int j = tid; j < n; j += nt) {
-                float4 q = load_pos(v, w, slots[it.begin + j]);
-                double dx = (double)q.x - y[3 * j], dy = (double)q.y - y[3 * j + 1], dz = (double)q.z - y[3 * j + 2];
+        };
+        if (!fit) {
+            double s1[1] = {0.0};
+            for (int j = tid, r = 0; j < n; j += nt, r++) {
+                float4 qq;
+                double yy[3];
+                atom(r, j, qq, yy);
+                const double dx = (double)qq.x - yy[0], dy = (double)qq.y - yy[1], dz = (double)qq.z - yy[2];
+                s1[0] += dx * dx + dy * dy + dz * dz;
+            }
+            block_sum<1>(s1, sh);
+            const double rv = sqrt(s1[0] / n);
+            const double sc = (rv > 1e-10 ? 1.0 / rv : 0.0) / n;
+            for (int j = tid, r = 0; j < n; j += nt, r++) {
+                float4 qq;
+                double yy[3];
+                atom(r, j, qq, yy);
+                const double dx = (double)qq.x - yy[0], dy = (double)qq.y - yy[1], dz = (double)qq.z - yy[2];
                 store_row(v, w, it.row0 + j, dx * sc, dy * sc, dz * sc);
             }
-            if (tid == 0) cvv[it.value] = r;
+            if (tid == 0) cvv[it.value] = rv;
             return;
         }
-        // Kabsch: centroid, covariance, Horn eigen-solve, gradient rows.
-        double c[3] = {0, 0, 0};
-        for (int j = tid; j < n; j += nt) {
-            float4 q = load_pos(v, w, slots[it.begin + j]);
-            c[0] += q.x;
-            c[1] += q.y;
-            c[2] += q.z;
+        // Kabsch: one block reduction of the raw moments
+        //   m[3a+b] = sum y'_a x_b, m[9..11] = sum x, m[12] = sum |x|^2,
+        //   m[13..15] = sum y', m[16] = sum |y'|^2   (y' = y - ybar),
+        // then the centred covariance sum y'(x - xbar)^T = m[3a+b] - (sum y'_a) xbar_b
+        // and sum |x - xbar|^2 = sum |x|^2 - n |xbar|^2, both in FP64.
+        double m[17];
+#pragma unroll
+        for (int k = 0; k < 17; k++) m[k] = 0.0;
+        for (int j = tid, r = 0; j < n; j += nt, r++) {
+            float4 qq;
+            double y[3];
+            atom(r, j, qq, y);
+            const double x[3] = {(double)qq.x, (double)qq.y, (double)qq.z};
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) m[3 * a + b] += y[a] * x[b];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                m[9 + k] += x[k];
+                m[12] += x[k] * x[k];
+                m[13 + k] += y[k];
+                m[16] += y[k] * y[k];
+            }
         }
-        block_sum<3>(c, sh);
-        const double xc = c[0] / n, yc0 = c[1] / n, zc = c[2] / n;
-        const double* yc = it.refCentroid;
-        double m[11] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
-        for (int j = tid; j < n; j += nt) {
-            float4 q = load_pos(v, w, slots[it.begin + j]);
-            double x0 = q.x - xc, x1 = q.y - yc0, x2 = q.z - zc;
-            double y0 = y[3 * j] - yc[0], y1 = y[3 * j + 1] - yc[1], y2 = y[3 * j + 2] - yc[2];
-            m[0] += y0 * x0;
-            m[1] += y0 * x1;
-            m[2] += y0 * x2;
-            m[3] += y1 * x0;
-            m[4] += y1 * x1;
-            m[5] += y1 * x2;
-            m[6] += y2 * x0;
-            m[7] += y2 * x1;
-            m[8] += y2 * x2;
-            m[9] += x0 * x0 + x1 * x1 + x2 * x2;
-            m[10] += y0 * y0 + y1 * y1 + y2 * y2;
-        }
-        block_sum<11>(m, sh);
+        block_sum<17>(m, sh);
+        const double xc[3] = {m[9] / n, m[10] / n, m[11] / n};
         __shared__ double shScale;
         if (tid == 0) {
-            double R[9], l;
-            horn_rotation(m, m[9], m[10], R, &l);
-            double msd = (m[9] + m[10] - 2.0 * l) / n;
+            double S[9], R[9], l;
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) S[3 * a + b] = m[3 * a + b] - m[13 + a] * xc[b];
+            const double Gx = m[12] - n * (xc[0] * xc[0] + xc[1] * xc[1] + xc[2] * xc[2]);
+            horn_rotation(S, Gx, m[16], R, &l);
+            double msd = (Gx + m[16] - 2.0 * l) / n;
             if (msd < 0.0) msd = 0.0;
-            double r = sqrt(msd);
+            const double rv = sqrt(msd);
             for (int k = 0; k < 9; k++) shR[k] = R[k];
-            shScale = (r > 1e-10 ? 1.0 / r : 0.0) / n;
-            cvv[it.value] = r;
+            shScale = (rv > 1e-10 ? 1.0 / rv : 0.0) / n;
+            cvv[it.value] = rv;
         }
         __syncthreads();
         const double sc = shScale;
-        for (int j = tid; j < n; j += nt) {
-            float4 q = load_pos(v, w, slots[it.begin + j]);
-            double x0 = q.x - xc, x1 = q.y - yc0, x2 = q.z - zc;
-            double y0 = y[3 * j] - yc[0], y1 = y[3 * j + 1] - yc[1], y2 = y[3 * j + 2] - yc[2];
-            double r0 = shR[0] * y0 + shR[1] * y1 + shR[2] * y2;
-            double r1 = shR[3] * y0 + shR[4] * y1 + shR[5] * y2;
-            double r2 = shR[6] * y0 + shR[7] * y1 + shR[8] * y2;
+        for (int j = tid, r = 0; j < n; j += nt, r++) {
+            float4 qq;
+            double y[3];
+            atom(r, j, qq, y);
+            const double x0 = qq.x - xc[0], x1 = qq.y - xc[1], x2 = qq.z - xc[2];
+            const double r0 = shR[0] * y[0] + shR[1] * y[1] + shR[2] * y[2];
+            const double r1 = shR[3] * y[0] + shR[4] * y[1] + shR[5] * y[2];
+            const double r2 = shR[6] * y[0] + shR[7] * y[1] + shR[8] * y[2];
             store_row(v, w, it.row0 + j, (x0 - r0) * sc, (x1 - r1) * sc, (x2 - r2) * sc);
         }
         return;
@@ -916,8 +1085,7 @@ __device__ void path_msd_block(const StepView& v, const PathItem& it, const int*
         }
     }
     if (nf == 0) return;
-#pragma unroll
-    for (int k = 0; k < 4; k++) acc[k] = wave_sum(acc[k]);
+    wave_sum_n<4>(acc);
     if (lane < nf) {
         const double a = lane == 0 ? acc[0] : lane == 1 ? acc[1] : lane == 2 ? acc[2] : acc[3];
         v.pathMsd[(size_t)w * v.msdStride + it.msdOff + f0 + lane] = a / M;
@@ -955,7 +1123,7 @@ __global__ __launch_bounds__(64 * PATH_PHASES) void k_path_rows(StepView v, cons
         t[0] += wv;
         t[1] += (f + 1) * wv;
     }
-    block_sum<2>(t, sh);  // ends with a barrier: shW is complete
+    block_sum<2, PATH_PHASES>(t, sh);  // ends with a barrier: shW is complete
     const double S = t[0];
     const bool ok = S > 1e-300;
     const double s = ok ? t[1] / S : 0.0;
@@ -966,7 +1134,7 @@ __global__ __launch_bounds__(64 * PATH_PHASES) void k_path_rows(StepView v, cons
             a[0] += ((f + 1) - s) * msd[f] * shW[f];
             a[1] += msd[f] * shW[f];
         }
-        block_sum<2>(a, sh);
+        block_sum<2, PATH_PHASES>(a, sh);
         if (threadIdx.x == 0) {
             v.cvVal[(size_t)w * v.V + it.valueS] = s;
             v.cvVal[(size_t)w * v.V + it.valueS + 1] = z;

@@ -1,7 +1,8 @@
 // Device side of the chain-rule force scatter, shared by the scatter kernel
 // and the fused bias + scatter kernel (both need it in their translation
 // unit).  CommonGluedKernels.cpp:2430-2456, re-organised: Jacobian rows are
-// grouped by atom slot at plan time, each thread owns one atom, sums
+// stored grouped by atom slot (the CV kernels write row r at rowPos[r]), so
+// the rows of one atom are contiguous; each thread owns one atom, sums
 // F = -sum_rows dU/dCV * J in FP64 and issues one 64-bit fixed-point atomic
 // per component.
 #pragma once
@@ -42,12 +43,11 @@ __device__ inline void scatter_slot(const StepView& v, const ScatterPlan& sp, in
     double fx = 0.0, fy = 0.0, fz = 0.0;
     const size_t base = (size_t)w * v.R;
     for (int k = b; k < e; k++) {
-        const int r = sp.segRow[k];
-        const int c = sp.rowCv[r];
+        const int c = sp.entryCv[k];
         const double u = c < nLds ? sU[c] : v.dUdCV[(size_t)w * v.V + c];
-        fx -= u * (double)v.jx[base + r];
-        fy -= u * (double)v.jy[base + r];
-        fz -= u * (double)v.jz[base + r];
+        fx -= u * (double)v.jx[base + k];
+        fy -= u * (double)v.jy[base + k];
+        fz -= u * (double)v.jz[base + k];
     }
     const WalkerInfo wi = v.walkers[w];
     const int g = v.gidx[(size_t)w * v.S + s];
