@@ -954,6 +954,7 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
     sp.entryCv = dRowCv_.as<int>();
     sp.combStart = dCombStart_.as<int>();
     sp.combSlot = dCombSlot_.as<int>();
+    sp.nComb = (int)combSlot_.size();
     sp.testGrad = testGrads_.empty() ? nullptr : dTestGrad_.as<double>();
     sp.energyValue = energyValue_;
     const bool biases = (stageMask & 10) && testGrads_.empty();

@@ -229,7 +229,8 @@ struct ScatterPlan {
     const int* segStart;     // [S+1] Jacobian rows of slot s are stored at [segStart[s], segStart[s+1])
     const int* entryCv;      // [R] CV value index of the row stored at each position
     const int* combStart;    // [V+1]
-    const int* combSlot;
+    const int* combSlot;     // [nComb]: bias-gradient slots summed into each CV's dU/dCV
+    int nComb;
     const double* testGrad;  // [V] or null
     int energyValue;         // CV value index of the energy CV (-1 if none)
 };

@@ -105,14 +105,20 @@ __device__ inline void grid_eval(const GridBiasDev& g, const double* grid, const
 // One-thread biases: grid MetaD eval, OPES expanded (:3050-3090),
 // multithermal (:3145-3172).  One thread per (item, walker).
 // ---------------------------------------------------------------------------
-__device__ void small_bias_eval(const StepView& v, const SmallBiasState& st, const SmallBiasItem& it, int i, int w)
+// bgL (optional, LDS): a copy of the walker's bias-gradient slots.
+__device__ void small_bias_eval(const StepView& v, const SmallBiasState& st, const SmallBiasItem& it, int i, int w,
+                                double* bgL = nullptr)
 {
     const double* cv = v.cvVal + (size_t)w * v.V;
     double* bg = v.bgrad + (size_t)w * v.G + it.gslot;
     double* be = v.benergy + (size_t)w * v.B + it.eslot;
+    double bgv[MAX_BIAS_D];
+    int nOut = 0;
     if (it.type == BIAS_METAD) {
         const GridBiasDev& g = st.grids[it.bias];
-        grid_eval(g, g.grid + (size_t)w * g.walkerStride, cv, be, bg);
+        grid_eval(g, g.grid + (size_t)w * g.walkerStride, cv, be, bgv);
+        nOut = g.D;
+        for (int d = 0; d < g.D; d++) bg[d] = bgv[d];
     } else if (it.type == BIAS_OPES_EXPANDED) {
         const double invKT = it.p[0];
         const double* lw = st.stateConst + it.stateOff;
@@ -123,7 +129,11 @@ __device__ void small_bias_eval(const StepView& v, const SmallBiasState& st, con
         for (int l = 0; l < it.nState; l++) se += exp(lw[l] - cv[it.cv[0] + l] * invKT - mx);
         const double logQ = mx + log(se);
         *be = -(1.0 / invKT) * (logQ - logZ);
-        for (int l = 0; l < it.nState; l++) bg[l] = exp(lw[l] - cv[it.cv[0] + l] * invKT - logQ);
+        for (int l = 0; l < it.nState; l++) {
+            const double pl = exp(lw[l] - cv[it.cv[0] + l] * invKT - logQ);
+            bg[l] = pl;
+            if (bgL) bgL[it.gslot + l] = pl;
+        }
     } else if (it.type == BIAS_OPES_MULTITHERMAL) {
         const double kT0 = it.p[0];
         const double* db = st.stateConst + it.stateOff;
@@ -138,8 +148,12 @@ __device__ void small_bias_eval(const StepView& v, const SmallBiasState& st, con
             ds += e * db[l];
         }
         *be = -kT0 * (mx + log(s / (double)it.nState));
-        bg[0] = kT0 * (ds / s);
+        bgv[0] = kT0 * (ds / s);
+        bg[0] = bgv[0];
+        nOut = 1;
     }
+    if (bgL)
+        for (int d = 0; d < nOut; d++) bgL[it.gslot + d] = bgv[d];
 }
 
 __global__ void k_bias_small(StepView v, SmallBiasState st, const SmallBiasItem* __restrict__ items, int n, int w0)
@@ -304,13 +318,18 @@ __device__ inline void reduce_partials(const double* src, int n, int wd, int ld,
     for (int d = 0; d < PW; d++) tot[d] = a[d];
 }
 
-__device__ inline void hill_finalize(const StepView& v, const HillBiasDev& h, int w, const double* tot)
+__device__ inline void hill_finalize(const StepView& v, const HillBiasDev& h, int w, const double* tot,
+                                     double* bgL = nullptr)
 {
     v.benergy[(size_t)w * v.B + h.eslot] = tot[0];
-    for (int d = 0; d < h.D; d++) v.bgrad[(size_t)w * v.G + h.gslot + d] = tot[1 + d];
+    for (int d = 0; d < h.D; d++) {
+        v.bgrad[(size_t)w * v.G + h.gslot + d] = tot[1 + d];
+        if (bgL) bgL[h.gslot + d] = tot[1 + d];
+    }
 }
 
-__device__ inline void opes_finalize(const StepView& v, const OpesBiasDev& o, int w, const double* tot)
+__device__ inline void opes_finalize(const StepView& v, const OpesBiasDev& o, int w, const double* tot,
+                                     double* bgL = nullptr)
 {
     const int tb = o.shared ? 0 : w;
     const int nk = o.ints[tb * 4 + 0];
@@ -318,7 +337,10 @@ __device__ inline void opes_finalize(const StepView& v, const OpesBiasDev& o, in
     double* bg = v.bgrad + (size_t)w * v.G + o.gslot;
     if (nk == 0) {
         *be = -o.barrier;
-        for (int d = 0; d < o.D; d++) bg[d] = 0.0;
+        for (int d = 0; d < o.D; d++) {
+            bg[d] = 0.0;
+            if (bgL) bgL[o.gslot + d] = 0.0;
+        }
         return;
     }
     const double prob = tot[0];
@@ -328,7 +350,10 @@ __device__ inline void opes_finalize(const StepView& v, const OpesBiasDev& o, in
     const double arg = pz + o.eps;
     *be = o.invGF * o.kT * log(arg > 1e-30 ? arg : 1e-30);
     const double f = prob > 0.0 ? o.invGF * o.kT * (pz / (pz + o.eps)) / prob : 0.0;
-    for (int d = 0; d < o.D; d++) bg[d] = f * tot[1 + d];
+    for (int d = 0; d < o.D; d++) {
+        bg[d] = f * tot[1 + d];
+        if (bgL) bgL[o.gslot + d] = f * tot[1 + d];
+    }
 }
 
 // Ticket: true in every thread of the workgroup that finished last.  Only
@@ -462,35 +487,110 @@ static void launch_chunks(const StepView& v, const FusedBiasPlan& fp, int nw, hi
     else hipLaunchKernelGGL((k_step_bias_chunks<0, 1>), g, b, 0, s, v, fp);
 }
 
-// One workgroup per walker: wave 0 reduces the chunk partials (written by the
-// previous launch) and finalizes the chunked biases while the other waves
-// evaluate the one-thread biases; then dU/dCV and the scatter.
+// One workgroup per walker.  Everything that does not depend on the biases
+// is loaded first and kept in registers (each thread's scatter slots: row
+// range, first row's CV and Jacobian, GPU atom index) or LDS (the dU/dCV
+// combine table).  Meanwhile wave 0 reduces the chunk partials (written by
+// the previous launch) and finalizes the chunked biases, and the other waves
+// evaluate the one-thread biases; both write their gradient slots to global
+// memory and to LDS.  After one barrier dU/dCV is formed from LDS and the
+// scatter needs only arithmetic and the fixed-point atomics.
+constexpr int FIN_SLOTS = 2;           // scatter slots per thread held in registers
+constexpr int FIN_LDS_G = 512;         // bias-gradient slots held in LDS
+constexpr int FIN_LDS_COMB = 1024;     // combine entries held in LDS
+
 __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, FusedBiasPlan fp)
 {
     __shared__ double sU[MAX_LDS_VALUES];
-    const int w = blockIdx.x;
-    if (threadIdx.x < 64) {
+    __shared__ double sBg[FIN_LDS_G];
+    __shared__ int sCombStart[MAX_LDS_VALUES + 1];
+    __shared__ int sCombSlot[FIN_LDS_COMB];
+    const int w = blockIdx.x, tid = threadIdx.x;
+    const ScatterPlan& sp = fp.sp;
+    const bool ldsComb = v.V <= MAX_LDS_VALUES && v.G <= FIN_LDS_G && sp.nComb <= FIN_LDS_COMB && !sp.testGrad;
+    double* bgL = ldsComb ? sBg : nullptr;
+    // bias-independent loads
+    const bool doScatter = fp.scatter != 0 && v.S > 0 && v.R > 0;
+    int sb[FIN_SLOTS], se[FIN_SLOTS], sc0[FIN_SLOTS], sg[FIN_SLOTS];
+    float sjx[FIN_SLOTS], sjy[FIN_SLOTS], sjz[FIN_SLOTS];
+    const size_t rbase = (size_t)w * v.R;
+    if (doScatter) {
+#pragma unroll
+        for (int r = 0; r < FIN_SLOTS; r++) {  // clamped slot: no divergent loads
+            const int s = min(tid + r * BIAS_BLOCK, v.S - 1);
+            sb[r] = sp.segStart[s];
+            se[r] = sp.segStart[s + 1];
+            sg[r] = v.gidx[(size_t)w * v.S + s];
+        }
+#pragma unroll
+        for (int r = 0; r < FIN_SLOTS; r++) {
+            const int k = min(sb[r], v.R - 1);  // a slot without rows reads a valid row and drops it
+            sc0[r] = sp.entryCv[k];
+            sjx[r] = v.jx[rbase + k];
+            sjy[r] = v.jy[rbase + k];
+            sjz[r] = v.jz[rbase + k];
+        }
+    }
+    if (ldsComb) {
+        for (int c = tid; c <= v.V; c += BIAS_BLOCK) sCombStart[c] = sp.combStart[c];
+        for (int k = tid; k < sp.nComb; k += BIAS_BLOCK) sCombSlot[k] = sp.combSlot[k];
+    }
+    if (tid < 64) {
         for (int k = 0; k < fp.nChunked; k++) {
             const int c0 = fp.chunkStart[k], nc = fp.chunkStart[k + 1] - c0;
             const double* src = fp.part + ((size_t)w * fp.totalChunks + c0) * PW;
-            const int D = k < fp.nHill ? fp.hills[k].D : fp.opes[k - fp.nHill].D;
+            const int D = fp.kind ? fp.kindD : (k < fp.nHill ? fp.hills[k].D : fp.opes[k - fp.nHill].D);
             double tot[PW];
             reduce_partials<false>(src, nc, D + 1, PW, tot);
-            if (threadIdx.x == 0) {
+            if (tid == 0) {
                 if (k < fp.nHill)
-                    hill_finalize(v, fp.hills[k], w, tot);
+                    hill_finalize(v, fp.hills[k], w, tot, bgL);
                 else
-                    opes_finalize(v, fp.opes[k - fp.nHill], w, tot);
+                    opes_finalize(v, fp.opes[k - fp.nHill], w, tot, bgL);
             }
         }
     } else {
-        for (int i = threadIdx.x - 64; i < fp.nSmall; i += blockDim.x - 64) small_bias_eval(v, fp.st, fp.small[i], i, w);
+        for (int i = tid - 64; i < fp.nSmall; i += BIAS_BLOCK - 64) small_bias_eval(v, fp.st, fp.small[i], i, w, bgL);
     }
     __syncthreads();
-    if (fp.scatter)
-        scatter_walker_block(v, fp.sp, w, sU, MAX_LDS_VALUES);
-    else
-        combine_dudcv(v, fp.sp, w, sU, MAX_LDS_VALUES, true);
+    if (!ldsComb) {
+        if (doScatter)
+            scatter_walker_block(v, sp, w, sU, MAX_LDS_VALUES);
+        else
+            combine_dudcv(v, sp, w, sU, MAX_LDS_VALUES, true);
+        return;
+    }
+    // dU/dCV from LDS (the combine order of combine_dudcv)
+    for (int c = tid; c < v.V; c += BIAS_BLOCK) {
+        double u = 0.0;
+        for (int k = sCombStart[c]; k < sCombStart[c + 1]; k++) u += sBg[sCombSlot[k]];
+        sU[c] = u;
+        v.dUdCV[(size_t)w * v.V + c] = u;
+    }
+    __syncthreads();
+    if (!doScatter) return;
+    const WalkerInfo wi = v.walkers[w];
+    long long* F = v.force + wi.forceOff;
+#pragma unroll
+    for (int r = 0; r < FIN_SLOTS; r++) {
+        const int s = tid + r * BIAS_BLOCK;
+        if (s >= v.S) break;
+        // first row from registers, further rows of this atom (rare) from memory,
+        // in the order of scatter_slot
+        if (sb[r] == se[r]) continue;
+        const double u0 = sU[sc0[r]];
+        double fx = -(u0 * (double)sjx[r]), fy = -(u0 * (double)sjy[r]), fz = -(u0 * (double)sjz[r]);
+        for (int k = sb[r] + 1; k < se[r]; k++) {
+            const double u = sU[sp.entryCv[k]];
+            fx -= u * (double)v.jx[rbase + k];
+            fy -= u * (double)v.jy[rbase + k];
+            fz -= u * (double)v.jz[rbase + k];
+        }
+        atomic_add_fixed(F + sg[r], fx);
+        atomic_add_fixed(F + sg[r] + wi.padded, fy);
+        atomic_add_fixed(F + sg[r] + 2 * (size_t)wi.padded, fz);
+    }
+    for (int s = tid + FIN_SLOTS * BIAS_BLOCK; s < v.S; s += BIAS_BLOCK) scatter_slot(v, sp, w, s, sU, MAX_LDS_VALUES);
 }
 
 void launch_step_bias_chunks(const StepView& v, const FusedBiasPlan& fp, int nw, hipStream_t s)

@@ -322,7 +322,8 @@ __device__ void horn_rotation(const double* S, double Gx, double Gy, double* R, 
         q[1] = q[2] = q[3] = 0.0;
         qn = 1.0;
     }
-    const double q0 = q[0] / qn, qx = q[1] / qn, qy = q[2] / qn, qz = q[3] / qn;
+    const double iq = 1.0 / qn;
+    const double q0 = q[0] * iq, qx = q[1] * iq, qy = q[2] * iq, qz = q[3] * iq;
     R[0] = q0 * q0 + qx * qx - qy * qy - qz * qz;
     R[1] = 2 * (qx * qy - q0 * qz);
     R[2] = 2 * (qx * qz + q0 * qy);
@@ -353,30 +354,31 @@ __device__ inline double ipowd(double x, int n)
     return r;
 }
 
-// Kabsch RMSD of up to 64 * KABSCH_PER_LANE atoms by one wave: every lane
-// keeps its atoms and their centred reference in registers, one butterfly
-// gives every lane the raw moments
+// Kabsch RMSD of up to 256 * KABSCH_PER_THREAD atoms by one 256-thread
+// workgroup: every thread keeps its atoms and their reference in registers,
+// a wave butterfly and one LDS exchange give every thread the raw moments
 //   m[3a+b] = sum y'_a x_b, m[9..11] = sum x, m[12] = sum |x|^2,
 //   m[13..15] = sum y', m[16] = sum |y'|^2   (y' = y - ybar),
 // from which the centred covariance sum y'(x - xbar)^T = m[3a+b] - (sum y'_a) xbar_b
-// and sum |x - xbar|^2 = sum |x|^2 - n |xbar|^2 (FP64).  Every lane then runs
-// the same Horn solve on the same bits (no broadcast) and writes its rows.
-constexpr int KABSCH_PER_LANE = 8;
+// and sum |x - xbar|^2 = sum |x|^2 - n |xbar|^2 (FP64).  Every thread then
+// runs the same Horn solve on the same bits (no broadcast) and writes its rows.
+constexpr int KABSCH_PER_THREAD = 4;
 
-__device__ void kabsch_wave(const StepView& v, const GroupCVItem& it, const int* __restrict__ slots,
-                            const float* __restrict__ ref, int w)
+__device__ void kabsch_block(const StepView& v, const GroupCVItem& it, const int* __restrict__ slots,
+                             const float* __restrict__ ref, int w)
 {
-    const int lane = threadIdx.x & 63;
+    __shared__ double shK[GROUP_BLOCK / 64][17];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int n = it.end - it.begin;
     const float* Y = ref + 3 * (size_t)it.refOff;
     const double* yc = it.refCentroid;
-    float qx[KABSCH_PER_LANE][3], qy[KABSCH_PER_LANE][3];  // walker atom, raw FP32 reference
-    int rp[KABSCH_PER_LANE];
-    // lanes past the end load a valid entry (index clamped) and drop it: no
+    float qx[KABSCH_PER_THREAD][3], qy[KABSCH_PER_THREAD][3];  // walker atom, raw FP32 reference
+    int rp[KABSCH_PER_THREAD];
+    // threads past the end load a valid entry (index clamped) and drop it: no
     // divergent branch, so every gather of the wave is in flight at once
 #pragma unroll
-    for (int r = 0; r < KABSCH_PER_LANE; r++) {
-        const int j = min(lane + 64 * r, n - 1);
+    for (int r = 0; r < KABSCH_PER_THREAD; r++) {
+        const int j = min(tid + GROUP_BLOCK * r, n - 1);
         const float4 p = load_pos(v, w, slots[it.begin + j]);
         qx[r][0] = p.x;
         qx[r][1] = p.y;
@@ -389,8 +391,8 @@ __device__ void kabsch_wave(const StepView& v, const GroupCVItem& it, const int*
 #pragma unroll
     for (int k = 0; k < 17; k++) m[k] = 0.0;
 #pragma unroll
-    for (int r = 0; r < KABSCH_PER_LANE; r++) {
-        if (lane + 64 * r >= n) break;
+    for (int r = 0; r < KABSCH_PER_THREAD; r++) {
+        if (tid + GROUP_BLOCK * r >= n) break;
         const double x[3] = {(double)qx[r][0], (double)qx[r][1], (double)qx[r][2]};
         const double y[3] = {(double)qy[r][0] - yc[0], (double)qy[r][1] - yc[1], (double)qy[r][2] - yc[2]};
 #pragma unroll
@@ -406,6 +408,17 @@ __device__ void kabsch_wave(const StepView& v, const GroupCVItem& it, const int*
         }
     }
     wave_sum_n<17>(m);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < 17; k++) shK[wid][k] = m[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 17; k++) {
+        double t = 0.0;
+#pragma unroll
+        for (int q = 0; q < GROUP_BLOCK / 64; q++) t += shK[q][k];
+        m[k] = t;
+    }
     const double xc[3] = {m[9] / n, m[10] / n, m[11] / n};
     double S[9], R[9], l;
 #pragma unroll
@@ -418,12 +431,11 @@ __device__ void kabsch_wave(const StepView& v, const GroupCVItem& it, const int*
     if (msd < 0.0) msd = 0.0;
     const double rv = sqrt(msd);
     const double sc = (rv > 1e-10 ? 1.0 / rv : 0.0) / n;
-    if (lane == 0) v.cvVal[(size_t)w * v.V + it.value] = rv;
+    if (tid == 0) v.cvVal[(size_t)w * v.V + it.value] = rv;
     const size_t base = (size_t)w * v.R;
 #pragma unroll
-    for (int r = 0; r < KABSCH_PER_LANE; r++) {
-        const int j = lane + 64 * r;
-        if (j >= n) break;
+    for (int r = 0; r < KABSCH_PER_THREAD; r++) {
+        if (tid + GROUP_BLOCK * r >= n) break;
         const double x0 = qx[r][0] - xc[0], x1 = qx[r][1] - xc[1], x2 = qx[r][2] - xc[2];
         const double y[3] = {(double)qy[r][0] - yc[0], (double)qy[r][1] - yc[1], (double)qy[r][2] - yc[2]};
         const double r0 = R[0] * y[0] + R[1] * y[1] + R[2] * y[2];
@@ -521,8 +533,8 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
         return;
     }
 
-    if (it.type == CV_RMSD && it.split && it.end - it.begin <= 64 * KABSCH_PER_LANE) {
-        if (tid < 64) kabsch_wave(v, it, slots, ref, w);
+    if (it.type == CV_RMSD && it.split && it.end - it.begin <= GROUP_BLOCK * KABSCH_PER_THREAD) {
+        kabsch_block(v, it, slots, ref, w);
         return;
     }
     if (it.type == CV_RMSD) {
