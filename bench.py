@@ -412,8 +412,9 @@ def main():
         cpu, err = cpu_baseline(wk, args.cpu_seconds)
         log(f"cpu baseline {cpu['value']} evals/s, force rel RMS {err}")
 
-    # ---- eval + deposit (pace 1: every walker deposits every step) ----
-    t_dep = time_loop(lambda k: (ctx.update_state(k + 1), ctx.execute_batch(use_graph=True)), 20, sync)
+    # ---- eval + deposit (pace 1: every walker deposits every step), one graph per step ----
+    ctx.step_batch(1)
+    t_dep = time_loop(lambda k: ctx.step_batch(k + 2), 20, sync)
 
     # ---- roofline of the dominant kernel ----
     t_launch = time_stage(ctx, wk.stage, 200)

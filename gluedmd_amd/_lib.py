@@ -36,6 +36,7 @@ SIGNATURES = {
     "glued_set_energy_input": [vp, i, d, vp, vp],
     "glued_execute": [vp, vp, i, vp, vp, i, i, c_double_p],
     "glued_execute_batch": [vp, vp, vp, vp, i, i],
+    "glued_step_batch": [vp, vp, i, vp, vp, i],
     "glued_update_state": [vp, vp, i],
     "glued_execute_stage": [vp, vp, vp, vp, i],
     "glued_get_cv_values": [vp, i, c_double_p, i],

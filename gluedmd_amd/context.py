@@ -179,6 +179,11 @@ class Context:
     def update_state(self, step):
         call("glued_update_state", self.h, self.stream, int(step))
 
+    def step_batch(self, step, include_forces=True):
+        """update_state(step) then execute_batch(), as one graph launch."""
+        call("glued_step_batch", self.h, self.stream, int(step), ctypes.c_void_p(self.posq.data_ptr()),
+             ctypes.c_void_p(self.forces.data_ptr()), int(include_forces))
+
     def md_step(self, walker=0):
         """One integrator step as OpenMM orders it: updateContextState(step)
         (deposition of the previous evaluation), then the force evaluation."""

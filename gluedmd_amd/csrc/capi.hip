@@ -157,6 +157,13 @@ int glued_update_state(glued_handle h, void* stream, int step)
     return guard([&] { E(h).updateState((hipStream_t)stream, step); });
 }
 
+int glued_step_batch(glued_handle h, void* stream, int step, const void* posq, int64_t* forces, int includeForces)
+{
+    return guard([&] {
+        E(h).stepBatch((hipStream_t)stream, step, (const float4*)posq, (long long*)forces, includeForces != 0);
+    });
+}
+
 int glued_get_cv_values(glued_handle h, int walker, double* out, int n)
 {
     return guard([&] { copy_out(E(h).cvValues(walker), out, n, "get_cv_values"); });

@@ -50,8 +50,7 @@ Engine::Engine() {}
 
 Engine::~Engine()
 {
-    if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
-    if (graph_) (void)hipGraphDestroy(graph_);
+    dropGraphs();
     if (captureStream_) (void)hipStreamDestroy(captureStream_);
     for (auto* g : grids_) delete g;
     for (auto* h : hills_) delete h;
@@ -830,12 +829,7 @@ void Engine::setBox(int w, const double* b9)
             dCellStart_.alloc(sizeof(int) * (size_t)W_ * (cellStride_ + coord_.size() + 1));
         }
         dCoord_.upload(coord_);
-        if (graphExec_) {
-            (void)hipGraphExecDestroy(graphExec_);
-            (void)hipGraphDestroy(graph_);
-            graphExec_ = nullptr;
-            graph_ = nullptr;
-        }
+        dropGraphs();  // the coordination launches are shaped by the cell grid
     }
 }
 
@@ -1111,59 +1105,168 @@ void Engine::executeBatch(hipStream_t s, const float4* posq, long long* force, b
 // skipped until one evaluation filled the CVs, once per step, never at step 0,
 // pace-gated; OPES, then OPES-expanded, multithermal, then MetaD.
 // ---------------------------------------------------------------------------
-void Engine::updateState(hipStream_t s, int step)
+int Engine::depositSites() const
 {
-    if (!initialized_) fail("update_state: initialize first");
-    if (!cvReady_) return;
-    if (step == lastUpdateStep_) return;
-    if (opes_.empty() && grids_.empty() && hills_.empty() && smallBias_.empty()) return;
-    lastUpdateStep_ = step;
+    return (int)(opes_.size() + smallBias_.size() + grids_.size() + hills_.size());
+}
+
+// The gating of updateState (CommonGluedKernels.cpp:5050-5157): nothing at
+// step 0; each site at its own pace; OPES only while its table has room.
+uint64_t Engine::depositMask(int step) const
+{
+    uint64_t m = 0;
+    int bit = 0;
+    if (step == 0) return 0;
+    for (auto* o : opes_) {
+        bool any = false;
+        for (int c : o->depositMirror) any |= c < o->d.cap;
+        if (step % o->pace == 0 && any) m |= 1ull << bit;
+        bit++;
+    }
+    for (size_t i = 0; i < smallBias_.size(); i++) {
+        const auto& hs = smallBiasHost_[i];
+        const bool dep = hs.type == BIAS_OPES_EXPANDED ? step % hs.pace == 0
+                         : hs.type == BIAS_OPES_MULTITHERMAL ? (hs.pace > 0 && step % hs.pace == 0)
+                                                             : false;
+        if (dep) m |= 1ull << bit;
+        bit++;
+    }
+    for (auto* g : grids_) {
+        if (step % g->pace == 0) m |= 1ull << bit;
+        bit++;
+    }
+    for (auto* h : hills_) {
+        if (step % h->pace == 0) m |= 1ull << bit;
+        bit++;
+    }
+    return m;
+}
+
+// Deposit launches in the reference's order: OPES, OPES-expanded,
+// multithermal, grid MetaD, hill lists.
+void Engine::launchDeposits(hipStream_t s, uint64_t mask)
+{
     StepView v = view(lastPos_, lastForce_);
     v.S = S_;
-    for (auto* o : opes_) {
-        if (step == 0 || step % o->pace != 0) continue;
-        const int T = (int)o->depositMirror.size();
-        bool any = false;
-        for (int t = 0; t < T; t++) {
-            if (o->depositMirror[t] >= o->d.cap) continue;
-            any = true;
-        }
-        if (!any) continue;
-        launch_dep_opes(v, o->d, 0, W_, s);
-        for (int t = 0; t < T; t++) {
-            bool will = o->d.adaptiveStride == 0 || o->nSamplesMirror[t] >= o->d.adaptiveStride;
-            if (will) o->depositMirror[t] += o->d.shared ? W_ : 1;
-        }
-    }
+    const int nO = (int)opes_.size(), nS = (int)smallBias_.size(), nG = (int)grids_.size();
+    for (int k = 0; k < nO; k++)
+        if (mask >> k & 1) launch_dep_opes(v, opes_[k]->d, 0, W_, s);
     SmallBiasState st;
     st.grids = dGridTable_.as<GridBiasDev>();
     st.stateConst = dSmallConst_.as<double>();
     st.stateVar = dSmallVar_.as<double>();
     st.scal = dSmallScal_.as<double>();
     st.stateVarStride = smallVarPerWalker_;
-    st.scalStride = 4 * (long long)std::max((int)smallBias_.size(), 1);
-    for (size_t i = 0; i < smallBias_.size(); i++) {
-        const auto& hs = smallBiasHost_[i];
-        if (hs.type != BIAS_OPES_EXPANDED) continue;
-        if (step == 0 || step % hs.pace != 0) continue;
-        launch_dep_small(v, st, dSmallBias_.as<SmallBiasItem>(), (int)smallBias_.size(), (int)i, 0, W_, s);
-    }
-    for (size_t i = 0; i < smallBias_.size(); i++) {
-        const auto& hs = smallBiasHost_[i];
-        if (hs.type != BIAS_OPES_MULTITHERMAL) continue;
-        if (step == 0 || hs.pace <= 0 || step % hs.pace != 0) continue;
-        launch_dep_small(v, st, dSmallBias_.as<SmallBiasItem>(), (int)smallBias_.size(), (int)i, 0, W_, s);
-    }
-    for (auto* g : grids_) {
-        if (step == 0 || step % g->pace != 0) continue;
-        launch_dep_grid(v, g->d, 0, W_, s);
-        for (auto& n : g->numDeposited) n += g->d.walkerStride ? 1 : W_;
-    }
-    for (auto* h : hills_) {
-        if (step == 0 || step % h->pace != 0) continue;
-        launch_dep_hills(v, h->d, 0, W_, s);
-    }
+    st.scalStride = 4 * (long long)std::max(nS, 1);
+    for (int pass = 0; pass < 2; pass++)  // expanded first, then multithermal
+        for (int i = 0; i < nS; i++) {
+            const int want = pass == 0 ? BIAS_OPES_EXPANDED : BIAS_OPES_MULTITHERMAL;
+            if (smallBiasHost_[i].type == want && (mask >> (nO + i) & 1))
+                launch_dep_small(v, st, dSmallBias_.as<SmallBiasItem>(), nS, i, 0, W_, s);
+        }
+    for (int k = 0; k < nG; k++)
+        if (mask >> (nO + nS + k) & 1) launch_dep_grid(v, grids_[k]->d, 0, W_, s);
+    for (int k = 0; k < (int)hills_.size(); k++)
+        if (mask >> (nO + nS + nG + k) & 1) launch_dep_hills(v, hills_[k]->d, 0, W_, s);
     hip_check(hipGetLastError(), "deposit launch");
+}
+
+void Engine::commitDeposits(uint64_t mask)
+{
+    const int nO = (int)opes_.size(), nS = (int)smallBias_.size();
+    for (int k = 0; k < nO; k++) {
+        if (!(mask >> k & 1)) continue;
+        OpesHost* o = opes_[k];
+        for (size_t t = 0; t < o->depositMirror.size(); t++) {
+            const bool will = o->d.adaptiveStride == 0 || o->nSamplesMirror[t] >= o->d.adaptiveStride;
+            if (will) o->depositMirror[t] += o->d.shared ? W_ : 1;
+        }
+    }
+    for (size_t k = 0; k < grids_.size(); k++)
+        if (mask >> (nO + nS + k) & 1)
+            for (auto& n : grids_[k]->numDeposited) n += grids_[k]->d.walkerStride ? 1 : W_;
+}
+
+void Engine::updateState(hipStream_t s, int step)
+{
+    if (!initialized_) fail("update_state: initialize first");
+    if (!cvReady_) return;
+    if (step == lastUpdateStep_) return;
+    if (depositSites() == 0) return;
+    lastUpdateStep_ = step;
+    const uint64_t m = depositMask(step);
+    if (depositSites() > 64) fail("update_state: more than 64 deposit sites");
+    launchDeposits(s, m);
+    commitDeposits(m);
+}
+
+void Engine::dropGraphs()
+{
+    if (graphExec_) (void)hipGraphExecDestroy(graphExec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graphExec_ = nullptr;
+    graph_ = nullptr;
+    for (auto& kv : stepGraphs_) {
+        (void)hipGraphExecDestroy(kv.second.second);
+        (void)hipGraphDestroy(kv.second.first);
+    }
+    stepGraphs_.clear();
+}
+
+// updateState(step) followed by the batched evaluation (the order OpenMM
+// uses: updateContextState, then calcForcesAndEnergy), as one graph launch.
+// Graphs are cached per deposit mask; the host mirrors advance exactly as
+// updateState + executeBatch would advance them.
+void Engine::stepBatch(hipStream_t s, int step, const float4* posq, long long* force, bool forces)
+{
+    if (!initialized_) fail("step_batch: initialize first");
+    if (V_ == 0) return;
+    if (depositSites() > 64) fail("step_batch: more than 64 deposit sites");
+    uint64_t m = 0;
+    if (cvReady_ && step != lastUpdateStep_ && depositSites() > 0) {
+        m = depositMask(step);
+        lastUpdateStep_ = step;
+    }
+    lastPos_ = posq;
+    lastForce_ = force;
+    if (stepGraphPos_ != posq || stepGraphForce_ != force || stepGraphForces_ != forces) {
+        for (auto& kv : stepGraphs_) {
+            (void)hipGraphExecDestroy(kv.second.second);
+            (void)hipGraphDestroy(kv.second.first);
+        }
+        stepGraphs_.clear();
+        stepGraphPos_ = posq;
+        stepGraphForce_ = force;
+        stepGraphForces_ = forces;
+    }
+    auto it = stepGraphs_.find(m);
+    if (it == stepGraphs_.end()) {
+        if (!captureStream_) hip_check(hipStreamCreateWithFlags(&captureStream_, hipStreamNonBlocking), "stream");
+        hip_check(hipStreamBeginCapture(captureStream_, hipStreamCaptureModeThreadLocal), "begin capture");
+        capturing_ = true;
+        try {
+            if (m) launchDeposits(captureStream_, m);
+            launchStep(captureStream_, 0, W_, forces, 7, true);
+            capturing_ = false;
+        } catch (...) {
+            capturing_ = false;
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(captureStream_, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        hip_check(hipStreamEndCapture(captureStream_, &g), "end capture");
+        hip_check(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "graph instantiate");
+        it = stepGraphs_.emplace(m, std::make_pair(g, ge)).first;
+    }
+    hip_check(hipGraphLaunch(it->second.second, s), "graph launch");
+    if (m) commitDeposits(m);
+    if (testGrads_.empty())
+        for (auto* o : opes_)
+            if (o->d.adaptiveStride > 0) bumpWelfordMirrors(o, 0, W_);
+    cvReady_ = true;
 }
 
 // ---------------------------------------------------------------------------

@@ -9,6 +9,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -98,6 +99,9 @@ class Engine {
     double execute(hipStream_t s, int w, const float4* posq, long long* force, bool forces, bool energy);
     void executeBatch(hipStream_t s, const float4* posq, long long* force, bool forces, bool useGraph);
     void updateState(hipStream_t s, int step);
+    // One MD step for every walker: updateState(step) then the batched
+    // evaluation, replayed as one hipGraph per deposit pattern.
+    void stepBatch(hipStream_t s, int step, const float4* posq, long long* force, bool forces);
     void executeStage(hipStream_t s, const float4* posq, long long* force, int stage);
 
     // ---- autograd support ----
@@ -182,6 +186,14 @@ class Engine {
     std::vector<HillHost*> hills_;
     std::vector<OpesHost*> opes_;
     void bumpWelfordMirrors(OpesHost* o, int w0, int nw);
+    // Deposit sites of updateState (bit per OPES table, OPES-expanded /
+    // multithermal item, grid, hill list, in that order): which deposit at
+    // `step`; the launches for a mask; the host mirrors for a mask.
+    int depositSites() const;
+    uint64_t depositMask(int step) const;
+    void launchDeposits(hipStream_t s, uint64_t mask);
+    void commitDeposits(uint64_t mask);
+    void dropGraphs();
     bool capturing_ = false;
     std::vector<SmallBiasItem> smallBias_;
     std::vector<SmallHost> smallBiasHost_;
@@ -209,6 +221,11 @@ class Engine {
     const void* graphForce_ = nullptr;
     hipStream_t captureStream_ = nullptr;
     bool graphForces_ = true;
+    // step graphs (deposits + evaluation), keyed by deposit mask
+    std::map<uint64_t, std::pair<hipGraph_t, hipGraphExec_t>> stepGraphs_;
+    const void* stepGraphPos_ = nullptr;
+    const void* stepGraphForce_ = nullptr;
+    bool stepGraphForces_ = true;
 
     bool cvReady_ = false;
     int lastUpdateStep_ = -1;

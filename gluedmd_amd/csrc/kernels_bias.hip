@@ -610,42 +610,68 @@ void launch_step_bias(const StepView& v, const FusedBiasPlan& fp, int nw, hipStr
 
 // Grid deposit, CommonGluedKernels.cpp:3645-3679.  One thread per grid node;
 // a shared grid takes the walkers' Gaussians in walker order (deterministic,
-// no atomics); a private grid is one y-block row per walker.
-__global__ void k_dep_grid(StepView v, GridBiasDev g, int w0, int nw)
+// no atomics); a private grid is one y-block row per walker.  The walkers'
+// heights h0 exp(-V hif) and centres are formed once per workgroup in LDS.
+// A node is skipped for a walker when some |dc_d| > 4 sigma_d (with a 1e-9
+// relative margin), which implies E > 8; otherwise E is formed with the
+// reference's expression, unfused, so the E <= 8 predicate is the oracle's.
+constexpr int DEP_BLOCK = 256;
+constexpr int DEP_WALKERS = 256;  // walkers staged in LDS per pass
+
+__global__ __launch_bounds__(DEP_BLOCK) void k_dep_grid(StepView v, GridBiasDev g, int w0, int nw)
 {
+#pragma clang fp contract(off)
+    __shared__ double sC[DEP_WALKERS][3];
+    __shared__ double sH[DEP_WALKERS];
     const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-    if (idx >= g.total) return;
-    const int wFirst = g.walkerStride ? w0 + blockIdx.y : w0;
-    const int wLast = g.walkerStride ? wFirst + 1 : w0 + nw;
-    double* grid = g.grid + (size_t)wFirst * g.walkerStride;
-    double pos[3];
+    const bool priv = g.walkerStride != 0;
+    const int wFirst = priv ? w0 + blockIdx.y : w0;
+    const int wCount = priv ? 1 : nw;
+    double pos[3] = {0.0, 0.0, 0.0}, per[3] = {0.0, 0.0, 0.0}, i2s2[3], far2[3];
     for (int d = 0; d < g.D; d++) {
-        int bi = (idx / g.strides[d]) % g.npts[d];
+        const int bi = (min(idx, g.total - 1) / g.strides[d]) % g.npts[d];
         pos[d] = g.origin[d] + (double)bi / g.invSpacing[d];
+        if (g.periodic[d]) per[d] = (double)g.npts[d] / g.invSpacing[d];
+        i2s2[d] = 2.0 * g.sigma[d] * g.sigma[d];
+        far2[d] = 16.0 * g.sigma[d] * g.sigma[d] * (1.0 + 1e-9);
     }
     double add = 0.0;
-    for (int w = wFirst; w < wLast; w++) {
-        const double* cv = v.cvVal + (size_t)w * v.V;
-        const double V = v.benergy[(size_t)w * v.B + g.eslot];
-        const double height = g.hif > 0.0 ? g.h0 * exp(-V * g.hif) : g.h0;
-        double e = 0.0;
-        for (int d = 0; d < g.D; d++) {
-            double dc = pos[d] - cv[g.cv[d]];
-            if (g.periodic[d]) {
-                double per = (double)g.npts[d] / g.invSpacing[d];
-                dc -= round(dc / per) * per;
-            }
-            e += dc * dc / (2.0 * g.sigma[d] * g.sigma[d]);
+    for (int base = 0; base < wCount; base += DEP_WALKERS) {
+        const int m = min(DEP_WALKERS, wCount - base);
+        __syncthreads();
+        for (int t = threadIdx.x; t < m; t += blockDim.x) {
+            const int w = wFirst + base + t;
+            const double* cv = v.cvVal + (size_t)w * v.V;
+            const double V = v.benergy[(size_t)w * v.B + g.eslot];
+            sH[t] = g.hif > 0.0 ? g.h0 * exp(-V * g.hif) : g.h0;
+            for (int d = 0; d < g.D; d++) sC[t][d] = cv[g.cv[d]];
         }
-        if (e <= 8.0) add += height * exp(-e);
+        __syncthreads();
+        for (int t = 0; t < m; t++) {
+            double e = 0.0;
+            bool near = true;
+            for (int d = 0; d < g.D && near; d++) {
+                double dc = pos[d] - sC[t][d];
+                if (g.periodic[d]) dc -= round(dc / per[d]) * per[d];
+                if (dc * dc > far2[d]) {
+                    near = false;
+                } else {
+                    e += dc * dc / i2s2[d];
+                }
+            }
+            if (near && e <= 8.0) add += sH[t] * exp(-e);
+        }
     }
-    grid[idx] += add;
+    if (idx < g.total) {
+        double* grid = g.grid + (size_t)wFirst * g.walkerStride;
+        grid[idx] += add;
+    }
 }
 
 void launch_dep_grid(const StepView& v, const GridBiasDev& g, int w0, int nw, hipStream_t s)
 {
-    dim3 grid((g.total + 255) / 256, g.walkerStride ? nw : 1);
-    hipLaunchKernelGGL(k_dep_grid, grid, dim3(256), 0, s, v, g, w0, nw);
+    dim3 grid((g.total + DEP_BLOCK - 1) / DEP_BLOCK, g.walkerStride ? nw : 1);
+    hipLaunchKernelGGL(k_dep_grid, grid, dim3(DEP_BLOCK), 0, s, v, g, w0, nw);
 }
 
 // Hill deposit: centre = current CVs, height = h0 exp(-V hif) (the WT rule of

@@ -133,6 +133,12 @@ int glued_execute_stage(glued_handle h, void* stream, const void* posq, int64_t*
  * CommonGluedKernels.cpp:5050-5157) with GluedForceImpl's once-per-step guard
  * (GluedForceImpl.cpp:100-109); all walkers. */
 int glued_update_state(glued_handle h, void* stream, int step);
+/* One MD step for every walker: glued_update_state(step) followed by
+ * glued_execute_batch (the order OpenMM uses: updateContextState, then the
+ * force evaluation; GluedForceImpl.cpp:100-109), replayed as one hipGraph per
+ * deposit pattern.  Results and host-side state are those of the two calls;
+ * no reference counterpart (the reference launches each kernel separately). */
+int glued_step_batch(glued_handle h, void* stream, int step, const void* posq, int64_t* forces, int includeForces);
 
 /* ---- read-backs ---------------------------------------------------------- */
 

@@ -193,3 +193,49 @@ def test_config4_deposits():
     for w in range(W):
         hc, hh = g.get_hills(c["hill_bias"], w)
         assert len(hh) == len(os_[w].states[c["hill_bias"]].heights)
+
+
+def _mixed_deposit_force():
+    """Distance + gyration CVs with a private OPES table (pace 1), a shared
+    grid MetaD (pace 2) and private hills (pace 3): every deposit pattern
+    occurs within a few steps."""
+    f = Force(pbc=True, temperature=300.0)
+    d = f.add_distance([0, 150])
+    rg = f.add_gyration(list(range(0, 120)))
+    f.add_opes([d, rg], sigma=[0.05, 0.02], gamma=12.0, pace=1, temperature=300.0, max_kernels=64)
+    gb = f.add_metad([d, rg], sigma=[0.05, 0.02], height=1.0, pace=2, grid_min=[0.0, 0.0], grid_max=[3.0, 2.0],
+                     bins=[60, 40], periodic=[False, False], gamma=8.0)
+    f.share_bias(gb)
+    f.add_metad_hills([d, rg], sigma=[0.05, 0.02], height=1.0, pace=3, gamma=8.0, max_hills=64)
+    return f
+
+
+def test_step_batch_matches_update_then_execute():
+    """glued_step_batch (deposits + evaluation as one graph per deposit
+    pattern) against update_state + execute_batch: forces, energies, hill
+    lists, grids and OPES tables bit-identical over steps with every pattern."""
+    from gluedmd_amd.context import Context
+    f = _mixed_deposit_force()
+    W, N = 4, 300
+    rng = np.random.default_rng(7)
+    L = 3.0
+    ga, gb_ = Context(f, walkers=[N] * W), Context(f, walkers=[N] * W)
+    for g in (ga, gb_):
+        g.set_box(L)
+    base = [rng.uniform(0, L, size=(N, 3)) for _ in range(W)]
+    for step in range(9):
+        pos = [(b + rng.normal(0, 0.02, size=b.shape)).astype(np.float32) for b in base]
+        for g in (ga, gb_):
+            g.set_all_positions(pos)
+            g.zero_forces()
+        ga.update_state(step)
+        ga.execute_batch(use_graph=True)
+        gb_.step_batch(step)
+        for w in range(W):
+            assert np.array_equal(ga.get_forces_fixed(w), gb_.get_forces_fixed(w)), (step, w)
+            assert np.array_equal(ga.getBiasEnergies(w), gb_.getBiasEnergies(w)), (step, w)
+    for w in range(W):
+        assert ga.getBiasState(w) == gb_.getBiasState(w)
+        ha, hb = ga.get_hills(2, w), gb_.get_hills(2, w)
+        assert np.array_equal(ha[0], hb[0]) and np.array_equal(ha[1], hb[1]) and len(ha[1]) == 2
+        assert np.array_equal(ga.getOPESMetrics(0, w), gb_.getOPESMetrics(0, w))
