@@ -608,14 +608,18 @@ void launch_step_bias(const StepView& v, const FusedBiasPlan& fp, int nw, hipStr
 // Deposition.
 // ---------------------------------------------------------------------------
 
-// Grid deposit, CommonGluedKernels.cpp:3645-3679.  One thread per grid node;
-// a shared grid takes the walkers' Gaussians in walker order (deterministic,
-// no atomics); a private grid is one y-block row per walker.  The walkers'
-// heights h0 exp(-V hif) and centres are formed once per workgroup in LDS.
-// A node is skipped for a walker when some |dc_d| > 4 sigma_d (with a 1e-9
-// relative margin), which implies E > 8; otherwise E is formed with the
-// reference's expression, unfused, so the E <= 8 predicate is the oracle's.
+// Grid deposit, CommonGluedKernels.cpp:3645-3679.  A workgroup covers 64
+// grid nodes; its four waves take every fourth walker of a shared grid (a
+// private grid is one y-block row per walker) and the four partial sums of a
+// node are added in wave order, then once to the grid (deterministic, no
+// atomics).  The walkers' heights h0 exp(-V hif) and centres are formed once
+// per workgroup in LDS.  A node is skipped for a walker when some
+// |dc_d| > 4 sigma_d (with a 1e-9 relative margin), which implies E > 8;
+// otherwise E is formed with the reference's expression, unfused, so the
+// E <= 8 predicate is the oracle's.
 constexpr int DEP_BLOCK = 256;
+constexpr int DEP_NODES = 64;
+constexpr int DEP_GROUPS = DEP_BLOCK / DEP_NODES;
 constexpr int DEP_WALKERS = 256;  // walkers staged in LDS per pass
 
 __global__ __launch_bounds__(DEP_BLOCK) void k_dep_grid(StepView v, GridBiasDev g, int w0, int nw)
@@ -623,7 +627,9 @@ __global__ __launch_bounds__(DEP_BLOCK) void k_dep_grid(StepView v, GridBiasDev 
 #pragma clang fp contract(off)
     __shared__ double sC[DEP_WALKERS][3];
     __shared__ double sH[DEP_WALKERS];
-    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double sAdd[DEP_GROUPS][DEP_NODES];
+    const int nl = threadIdx.x % DEP_NODES, grp = threadIdx.x / DEP_NODES;
+    const int idx = blockIdx.x * DEP_NODES + nl;
     const bool priv = g.walkerStride != 0;
     const int wFirst = priv ? w0 + blockIdx.y : w0;
     const int wCount = priv ? 1 : nw;
@@ -647,7 +653,7 @@ __global__ __launch_bounds__(DEP_BLOCK) void k_dep_grid(StepView v, GridBiasDev 
             for (int d = 0; d < g.D; d++) sC[t][d] = cv[g.cv[d]];
         }
         __syncthreads();
-        for (int t = 0; t < m; t++) {
+        for (int t = grp; t < m; t += DEP_GROUPS) {
             double e = 0.0;
             bool near = true;
             for (int d = 0; d < g.D && near; d++) {
@@ -662,15 +668,20 @@ __global__ __launch_bounds__(DEP_BLOCK) void k_dep_grid(StepView v, GridBiasDev 
             if (near && e <= 8.0) add += sH[t] * exp(-e);
         }
     }
-    if (idx < g.total) {
+    sAdd[grp][nl] = add;
+    __syncthreads();
+    if (grp == 0 && idx < g.total) {
+        double tot = sAdd[0][nl];
+#pragma unroll
+        for (int q = 1; q < DEP_GROUPS; q++) tot += sAdd[q][nl];
         double* grid = g.grid + (size_t)wFirst * g.walkerStride;
-        grid[idx] += add;
+        grid[idx] += tot;
     }
 }
 
 void launch_dep_grid(const StepView& v, const GridBiasDev& g, int w0, int nw, hipStream_t s)
 {
-    dim3 grid((g.total + DEP_BLOCK - 1) / DEP_BLOCK, g.walkerStride ? nw : 1);
+    dim3 grid((g.total + DEP_NODES - 1) / DEP_NODES, g.walkerStride ? nw : 1);
     hipLaunchKernelGGL(k_dep_grid, grid, dim3(DEP_BLOCK), 0, s, v, g, w0, nw);
 }
 
