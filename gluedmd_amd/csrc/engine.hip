@@ -14,11 +14,12 @@ void launch_cv_small(const StepView&, const SmallCVItem*, int, int, int, hipStre
 void launch_cv_group(const StepView&, const GroupCVItem*, int, const int*, const float*, const float*, int, int,
                      hipStream_t);
 void launch_cv_path(const StepView&, const PathItem*, int, int, int, const int*, const float*, int, int, hipStream_t);
+void launch_cv_group_big(const StepView&, const GroupCVItem*, int, const int*, const float*, const float*, int, int,
+                         hipStream_t);
 void launch_path_rows(const StepView&, const PathItem*, int, int, const int*, const float*, int, int, hipStream_t);
 
 void launch_coordination(const StepView&, const CoordBuffers&, const CoordItem*, const CoordItem&, int, int, int,
                          hipStream_t);
-int coord_blocks_A(int);
 void launch_bias_small(const StepView&, const SmallBiasState&, const SmallBiasItem*, int, int, int, hipStream_t);
 void launch_bias_hills(const StepView&, const HillBiasDev&, int, int, hipStream_t);
 void launch_bias_opes(const StepView&, const OpesBiasDev&, int, int, hipStream_t);
@@ -433,7 +434,21 @@ void Engine::buildPlan(int nMass, const double* masses)
     dJz_.alloc(sizeof(float) * (size_t)W_ * std::max(R_, 1));
     dUd_.alloc(sizeof(double) * (size_t)W_ * std::max(V_, 1));
     dSmall_.upload(small_);
+    {
+        // atom / pair lists longer than one 256-thread workgroup holds in
+        // registers go to the 1,024-thread group kernel
+        std::vector<GroupCVItem> small, big;
+        for (const auto& it : group_) {
+            const bool pairs = it.type == CV_DRMSD || it.type == CV_CONTACTMAP;
+            const int n = pairs ? (it.end - it.begin) / 2 : it.end - it.begin;
+            const bool fit = it.type == CV_RMSD && it.split;  // Kabsch stays on the 256-thread kernels
+            (n > 1024 && !fit ? big : small).push_back(it);
+        }
+        group_ = small;
+        groupBig_ = big;
+    }
     dGroup_.upload(group_);
+    dGroupBig_.upload(groupBig_);
     dGroupSlots_.upload(groupSlots);
     dGroupMass_.upload(groupMass);
     dGroupRef_.upload(groupRef);
@@ -449,28 +464,29 @@ void Engine::buildPlan(int nMass, const double* masses)
 
     // ---- coordination buffers ----
     {
-        int cellBase = 0, sortBase = 0, partBase = 0, cellOf = 0;
+        int cellBase = 0, sortBase = 0, cellOf = 0;
         for (auto& it : coord_) {
             it.useCells = 0;
             it.nc[0] = it.nc[1] = it.nc[2] = 1;
             it.cellBase = cellBase;
             it.sortBase = sortBase;
-            it.partBase = partBase;
             cellBase += 2;
             sortBase += it.nA + it.nB;
             cellOf += it.nA + it.nB;
-            partBase += coord_blocks_A(it.nA);
         }
         cellOfStride_ = std::max(cellOf, 1);
         sortStride_ = std::max(sortBase, 1);
-        partStride_ = std::max(partBase, 1);
         cellStride_ = std::max(cellBase, 1);
+        const size_t nci = (size_t)W_ * std::max((int)coord_.size(), 1);
         dCellOf_.alloc(sizeof(int) * (size_t)W_ * cellOfStride_);
         dSorted_.alloc(sizeof(float4) * (size_t)W_ * sortStride_);
-        dPartCN_.alloc(sizeof(double) * (size_t)W_ * partStride_);
-        dPartPairs_.alloc(sizeof(unsigned long long) * (size_t)W_ * partStride_);
-        dCounters_.alloc(sizeof(unsigned int) * (size_t)W_ * std::max((int)coord_.size(), 1));
-        dNpairs_.alloc(sizeof(long long) * (size_t)W_ * std::max((int)coord_.size(), 1));
+        dUnsorted_.alloc(sizeof(float4) * (size_t)W_ * sortStride_);
+        dRank_.alloc(sizeof(int) * (size_t)W_ * sortStride_);
+        layoutCellTables();
+        dCnAcc_.alloc(sizeof(long long) * nci);
+        dCnAccLo_.alloc(sizeof(unsigned long long) * nci);
+        dPairAcc_.alloc(sizeof(unsigned long long) * nci);
+        dNpairs_.alloc(sizeof(long long) * nci);
         dCellCount_.alloc(sizeof(int) * (size_t)W_ * cellStride_);
         dCellStart_.alloc(sizeof(int) * (size_t)W_ * (cellStride_ + coord_.size() + 1));
         dCoord_.upload(coord_);
@@ -828,8 +844,24 @@ void Engine::setBox(int w, const double* b9)
             dCellCount_.alloc(sizeof(int) * (size_t)W_ * cellStride_);
             dCellStart_.alloc(sizeof(int) * (size_t)W_ * (cellStride_ + coord_.size() + 1));
         }
+        layoutCellTables();
         dCoord_.upload(coord_);
         dropGraphs();  // the coordination launches are shaped by the cell grid
+    }
+}
+
+// The counting sort's (cell, workgroup) table: 2 * ncell * ceil(n / 1024)
+// entries per item and walker.
+void Engine::layoutCellTables()
+{
+    int blk = 0;
+    for (auto& it : coord_) {
+        it.blkBase = blk;
+        blk += 2 * it.nc[0] * it.nc[1] * it.nc[2] * ((it.nA + it.nB + 1023) / 1024);
+    }
+    if (std::max(blk, 1) > blkStride_) {
+        blkStride_ = std::max(blk, 1);
+        dBlockCnt_.alloc(sizeof(int) * (size_t)W_ * blkStride_);
     }
 }
 
@@ -907,12 +939,16 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         cp.pslots = dPathSlots_.as<int>();
         cp.frames = dPathFrames_.as<float>();
         launch_cv_fused(v, cp, nw, s);
+        launch_cv_group_big(v, dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupSlots_.as<int>(),
+                            dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
         launch_path_rows(v, dPath_.as<PathItem>(), (int)path_.size(), pathRowBlocks_, dPathSlots_.as<int>(),
                          dPathFrames_.as<float>(), w0, nw, s);
     } else {
         launch_cv_small(v, dSmall_.as<SmallCVItem>(), (int)small_.size(), w0, nw, s);
         launch_cv_group(v, dGroup_.as<GroupCVItem>(), (int)group_.size(), dGroupSlots_.as<int>(),
                         dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
+        launch_cv_group_big(v, dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupSlots_.as<int>(),
+                            dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
         launch_cv_path(v, dPath_.as<PathItem>(), (int)path_.size(), pathMsdBlocks_, pathRowBlocks_,
                        dPathSlots_.as<int>(), dPathFrames_.as<float>(), w0, nw, s);
     }
@@ -923,14 +959,17 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         cb.cellCount = dCellCount_.as<int>();
         cb.cellStart = dCellStart_.as<int>();
         cb.sorted = dSorted_.as<float4>();
-        cb.partCN = dPartCN_.as<double>();
-        cb.partPairs = dPartPairs_.as<unsigned long long>();
-        cb.counters = dCounters_.as<unsigned int>();
+        cb.unsorted = dUnsorted_.as<float4>();
+        cb.rank = dRank_.as<int>();
+        cb.blockCnt = dBlockCnt_.as<int>();
+        cb.blkStride = blkStride_;
+        cb.cnAcc = dCnAcc_.as<long long>();
+        cb.cnAccLo = dCnAccLo_.as<unsigned long long>();
+        cb.pairAcc = dPairAcc_.as<unsigned long long>();
         cb.npairs = dNpairs_.as<long long>();
         cb.cellOfStride = cellOfStride_;
         cb.cellStride = cellStride_;
         cb.sortStride = sortStride_;
-        cb.partStride = partStride_;
         cb.nItems = (int)coord_.size();
         for (size_t i = 0; i < coord_.size(); i++)
             launch_coordination(v, cb, dCoord_.as<CoordItem>(), coord_[i], (int)i, w0, nw, s);

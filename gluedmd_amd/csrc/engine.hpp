@@ -153,7 +153,7 @@ class Engine {
 
     // items
     std::vector<SmallCVItem> small_;
-    std::vector<GroupCVItem> group_;
+    std::vector<GroupCVItem> group_, groupBig_;  // groupBig_: more than 1,024 atoms / pairs
     std::vector<CoordItem> coord_;
     std::vector<PathItem> path_;
     int energyValue_ = -1;
@@ -194,6 +194,7 @@ class Engine {
     void launchDeposits(hipStream_t s, uint64_t mask);
     void commitDeposits(uint64_t mask);
     void dropGraphs();
+    void layoutCellTables();
     bool capturing_ = false;
     std::vector<SmallBiasItem> smallBias_;
     std::vector<SmallHost> smallBiasHost_;
@@ -203,10 +204,10 @@ class Engine {
 
     // device buffers
     DBuf dWalkers_, dGidx_, dBox_, dCv_, dJx_, dJy_, dJz_, dBgrad_, dBener_, dUd_;
-    DBuf dSmall_, dGroup_, dGroupSlots_, dGroupMass_, dGroupRef_;
-    DBuf dCoord_, dCoordSlots_, dCellOf_, dCellCount_, dCellStart_, dSorted_, dPartCN_, dPartPairs_, dCounters_,
-        dNpairs_;
-    int cellOfStride_ = 0, cellStride_ = 0, sortStride_ = 0, partStride_ = 0;
+    DBuf dSmall_, dGroup_, dGroupBig_, dGroupSlots_, dGroupMass_, dGroupRef_;
+    DBuf dCoord_, dCoordSlots_, dCellOf_, dCellCount_, dCellStart_, dSorted_, dUnsorted_, dCnAcc_, dCnAccLo_, dPairAcc_,
+        dNpairs_, dRank_, dBlockCnt_;
+    int cellOfStride_ = 0, cellStride_ = 0, sortStride_ = 0, blkStride_ = 0;
     DBuf dPath_, dPathSlots_, dPathFrames_, dPathW_;
     DBuf dSmallBias_, dGridTable_, dSmallConst_, dSmallVar_, dSmallScal_;
     DBuf dSegStart_, dSegRow_ /* rowPos */, dRowCv_ /* entryCv */, dCombStart_, dCombSlot_, dTestGrad_, dEnergyU_, dEnergyFPtr_;

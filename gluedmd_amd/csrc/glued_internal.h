@@ -115,7 +115,7 @@ struct CoordItem {
     int nc[3];            // cells per dimension
     int cellBase;         // offset into the cell arrays
     int sortBase;         // offset into the sorted-atom arrays (A then B)
-    int partBase;         // offset into the per-block CN partial sums
+    int blkBase;          // offset into the per-workgroup cell-count table of the sort
 };
 
 // Path CV (s, z), MSD metric.
@@ -215,12 +215,15 @@ struct CoordBuffers {
     int* cellOf;           // [W][cellOfStride]
     int* cellCount;        // [W][cellStride]
     int* cellStart;        // [W][cellStride + items]  (scan output, 1 extra per item)
-    float4* sorted;        // [W][sortStride]
-    double* partCN;        // [W][partStride]
-    unsigned long long* partPairs;  // [W][partStride]
-    unsigned int* counters;         // [W][items]
+    float4* sorted;        // [W][sortStride]  atoms in cell order (x, y, z, local index)
+    float4* unsorted;      // [W][sortStride]  atoms in list order (staging of the cell build)
+    int* rank;             // [W][sortStride]  rank of an atom among its sort workgroup's atoms of its cell
+    int* blockCnt;         // [W][blkStride]   per (cell, sort workgroup) counts, then offsets
+    long long* cnAcc;               // [W][items] CN: high part, units of 2^-24 (pass A adds, pass B converts and clears)
+    unsigned long long* cnAccLo;    // [W][items] CN: low part, units of 2^-64
+    unsigned long long* pairAcc;    // [W][items] pairs inside the cutoff (same life cycle)
     long long* npairs;              // [W][items]
-    int cellOfStride, cellStride, sortStride, partStride, nItems;
+    int cellOfStride, cellStride, sortStride, blkStride, nItems;
 };
 
 // Scatter plan: Jacobian rows grouped by atom slot (CSR), CV of every row,

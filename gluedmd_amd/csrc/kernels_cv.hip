@@ -55,14 +55,15 @@ __device__ inline void pbc_f(float& x, float& y, float& z, const float* bx)
 
 __device__ inline double wave_sum(double v) { return wave_sum1(v); }
 
-// Block-wide FP64 sum of NV values per thread over NW waves (the block size,
-// fixed at compile time so the per-wave partials are read with independent
-// LDS loads rather than a loop of dependent ones); every thread receives the
-// result.  sh holds >= NV * 16 doubles.
+// Block-wide FP64 sum of NV values per thread over NW waves (the block
+// size, fixed at compile time); every thread receives the result.  Thread k
+// (k < NV) adds the NW wave partials of value k in wave order, so each
+// thread reads NV values from LDS rather than NV * NW.  sh holds >= NV * 16
+// doubles.
 template <int NV, int NW = 4>
 __device__ inline void block_sum(double (&v)[NV], double* sh)
 {
-    static_assert(NW >= 1 && NW <= 16, "1..16 waves");
+    static_assert(NW >= 1 && NW <= 16 && NV <= 64, "1..16 waves, at most 64 values");
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     wave_sum_n<NV>(v);
     __syncthreads();
@@ -70,18 +71,15 @@ __device__ inline void block_sum(double (&v)[NV], double* sh)
 #pragma unroll
         for (int k = 0; k < NV; k++) sh[k * 16 + wid] = v[k];
     __syncthreads();
-    double p[NV][NW];
+    if (threadIdx.x < NV) {
+        double t = 0.0;
 #pragma unroll
-    for (int k = 0; k < NV; k++)
-#pragma unroll
-        for (int i = 0; i < NW; i++) p[k][i] = sh[k * 16 + i];
-#pragma unroll
-    for (int k = 0; k < NV; k++) {
-        double s = 0.0;
-#pragma unroll
-        for (int i = 0; i < NW; i++) s += p[k][i];
-        v[k] = s;
+        for (int i = 0; i < NW; i++) t += sh[threadIdx.x * 16 + i];
+        sh[threadIdx.x * 16] = t;
     }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = sh[k * 16];
     __syncthreads();
 }
 
@@ -364,10 +362,11 @@ __device__ inline double ipowd(double x, int n)
 // runs the same Horn solve on the same bits (no broadcast) and writes its rows.
 constexpr int KABSCH_PER_THREAD = 4;
 
+template <int NT>
 __device__ void kabsch_block(const StepView& v, const GroupCVItem& it, const int* __restrict__ slots,
                              const float* __restrict__ ref, int w)
 {
-    __shared__ double shK[GROUP_BLOCK / 64][17];
+    __shared__ double shK[NT / 64][17];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int n = it.end - it.begin;
     const float* Y = ref + 3 * (size_t)it.refOff;
@@ -378,7 +377,7 @@ __device__ void kabsch_block(const StepView& v, const GroupCVItem& it, const int
     // divergent branch, so every gather of the wave is in flight at once
 #pragma unroll
     for (int r = 0; r < KABSCH_PER_THREAD; r++) {
-        const int j = min(tid + GROUP_BLOCK * r, n - 1);
+        const int j = min(tid + NT * r, n - 1);
         const float4 p = load_pos(v, w, slots[it.begin + j]);
         qx[r][0] = p.x;
         qx[r][1] = p.y;
@@ -392,7 +391,7 @@ __device__ void kabsch_block(const StepView& v, const GroupCVItem& it, const int
     for (int k = 0; k < 17; k++) m[k] = 0.0;
 #pragma unroll
     for (int r = 0; r < KABSCH_PER_THREAD; r++) {
-        if (tid + GROUP_BLOCK * r >= n) break;
+        if (tid + NT * r >= n) break;
         const double x[3] = {(double)qx[r][0], (double)qx[r][1], (double)qx[r][2]};
         const double y[3] = {(double)qy[r][0] - yc[0], (double)qy[r][1] - yc[1], (double)qy[r][2] - yc[2]};
 #pragma unroll
@@ -416,7 +415,7 @@ __device__ void kabsch_block(const StepView& v, const GroupCVItem& it, const int
     for (int k = 0; k < 17; k++) {
         double t = 0.0;
 #pragma unroll
-        for (int q = 0; q < GROUP_BLOCK / 64; q++) t += shK[q][k];
+        for (int q = 0; q < NT / 64; q++) t += shK[q][k];
         m[k] = t;
     }
     const double xc[3] = {m[9] / n, m[10] / n, m[11] / n};
@@ -435,7 +434,7 @@ __device__ void kabsch_block(const StepView& v, const GroupCVItem& it, const int
     const size_t base = (size_t)w * v.R;
 #pragma unroll
     for (int r = 0; r < KABSCH_PER_THREAD; r++) {
-        if (tid + GROUP_BLOCK * r >= n) break;
+        if (tid + NT * r >= n) break;
         const double x0 = qx[r][0] - xc[0], x1 = qx[r][1] - xc[1], x2 = qx[r][2] - xc[2];
         const double y[3] = {(double)qy[r][0] - yc[0], (double)qy[r][1] - yc[1], (double)qy[r][2] - yc[2]};
         const double r0 = R[0] * y[0] + R[1] * y[1] + R[2] * y[2];
@@ -447,6 +446,7 @@ __device__ void kabsch_block(const StepView& v, const GroupCVItem& it, const int
     }
 }
 
+template <int NT, int REG>
 __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* __restrict__ slots,
                            const float* __restrict__ mass, const float* __restrict__ ref, int w)
 {
@@ -455,66 +455,64 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
     const double* bx = v.box + (size_t)w * 12;
     double* cvv = v.cvVal + (size_t)w * v.V;
     const int tid = threadIdx.x;
-    constexpr int nt = GROUP_BLOCK;  // every launch of the group CVs uses this block size
+    constexpr int nt = NT;
 
     if (it.type == CV_GYRATION || it.type == CV_COM_DISTANCE) {
         const int n = it.end - it.begin;
         const int n1 = it.type == CV_COM_DISTANCE ? it.split : n;
-        // atoms of this thread: j = tid + r * nt; the first GROUP_REG stay in
-        // registers (one gather round for every pass when n <= GROUP_REG * nt)
-        float4 q[GROUP_REG];
-        double mj[GROUP_REG];
+        // atoms of this thread: j = tid + r * nt; the first REG stay in
+        // registers (one gather round for every pass when n <= REG * nt)
+        float4 q[REG];
+        double mj[REG];
 #pragma unroll
-        for (int r = 0; r < GROUP_REG; r++) {  // clamped index: no divergent loads
+        for (int r = 0; r < REG; r++) {  // clamped index: no divergent loads
             const int j = min(tid + r * nt, n - 1);
             q[r] = load_pos(v, w, slots[it.begin + j]);
             mj[r] = mass[it.begin + j];
         }
-        auto atom = [&](int r, int j, float4& qq, double& mm) {
-            if (r < GROUP_REG) {
-                qq = q[r];
-                mm = mj[r];
-            } else {
-                qq = load_pos(v, w, slots[it.begin + j]);
-                mm = mass[it.begin + j];
+        // every atom of this thread: registers first (static indices), then
+        // the tail past REG * nt with fresh gathers
+        auto visit = [&](auto&& f) {
+#pragma unroll
+            for (int r = 0; r < REG; r++) {
+                const int j = tid + r * nt;
+                if (j < n) f(j, q[r], mj[r]);
             }
+            for (int j = tid + REG * nt; j < n; j += nt)
+                f(j, load_pos(v, w, slots[it.begin + j]), (double)mass[it.begin + j]);
         };
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // m*x (g1), M1, m*x (g2), M2
-        for (int j = tid, r = 0; j < n; j += nt, r++) {
-            float4 qq;
-            double m;
-            atom(r, j, qq, m);
-            const int o = j < n1 ? 0 : 4;
-            acc[o + 0] += m * qq.x;
-            acc[o + 1] += m * qq.y;
-            acc[o + 2] += m * qq.z;
-            acc[o + 3] += m;
-        }
-        block_sum<8>(acc, sh);
+        visit([&](int j, const float4& qq, double m) {
+            const bool g1 = j < n1;  // static indices only (a runtime index would spill acc)
+            const double m1 = g1 ? m : 0.0, m2 = g1 ? 0.0 : m;
+            acc[0] += m1 * qq.x;
+            acc[1] += m1 * qq.y;
+            acc[2] += m1 * qq.z;
+            acc[3] += m1;
+            acc[4] += m2 * qq.x;
+            acc[5] += m2 * qq.y;
+            acc[6] += m2 * qq.z;
+            acc[7] += m2;
+        });
+        block_sum<8, NT / 64>(acc, sh);
         if (it.type == CV_GYRATION) {
             const double M = acc[3];
             const double cx = acc[0] / M, cy = acc[1] / M, cz = acc[2] / M;
             double r2[1] = {0.0};
-            for (int j = tid, r = 0; j < n; j += nt, r++) {
-                float4 qq;
-                double m;
-                atom(r, j, qq, m);
+            visit([&](int j, const float4& qq, double m) {
                 double dx = qq.x - cx, dy = qq.y - cy, dz = qq.z - cz;
                 if (v.pbc) pbc_exact(dx, dy, dz, bx);
                 r2[0] += m * (dx * dx + dy * dy + dz * dz);
-            }
-            block_sum<1>(r2, sh);
+            });
+            block_sum<1, NT / 64>(r2, sh);
             const double rg = sqrt(r2[0] / M);
             const double inv = rg > 1e-10 ? 1.0 / rg : 0.0;
-            for (int j = tid, r = 0; j < n; j += nt, r++) {
-                float4 qq;
-                double m;
-                atom(r, j, qq, m);
+            visit([&](int j, const float4& qq, double m) {
                 double dx = qq.x - cx, dy = qq.y - cy, dz = qq.z - cz;
                 if (v.pbc) pbc_exact(dx, dy, dz, bx);
                 const double sc = m / M * inv;
                 store_row(v, w, it.row0 + j, sc * dx, sc * dy, sc * dz);
-            }
+            });
             if (tid == 0) cvv[it.value] = rg;
         } else {
             double dx = acc[4] / acc[7] - acc[0] / acc[3];
@@ -533,59 +531,67 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
         return;
     }
 
-    if (it.type == CV_RMSD && it.split && it.end - it.begin <= GROUP_BLOCK * KABSCH_PER_THREAD) {
-        kabsch_block(v, it, slots, ref, w);
-        return;
+    if constexpr (NT > GROUP_BLOCK) {
+        // the wide kernel takes no Kabsch item (the planner keeps every fitted
+        // RMSD on the 256-thread kernels, whose registers hold the Horn solve)
+        if (it.type == CV_RMSD && it.split) return;
+    } else {
+        if (it.type == CV_RMSD && it.split && it.end - it.begin <= NT * KABSCH_PER_THREAD) {
+            kabsch_block<NT>(v, it, slots, ref, w);
+            return;
+        }
     }
     if (it.type == CV_RMSD) {
         const int n = it.end - it.begin;
         const float* Y = ref + 3 * (size_t)it.refOff;
         const double* yc = it.refCentroid;
         const bool fit = it.split != 0;
-        // one gather round: the first GROUP_REG atoms of this thread stay in
+        // one gather round: the first REG atoms of this thread stay in
         // registers with their reference (centred for the fit)
-        float4 q[GROUP_REG];
-        double yr[GROUP_REG][3];
+        float4 q[REG];
+        float yr[REG][3];  // raw FP32 reference; centred on use
 #pragma unroll
-        for (int r = 0; r < GROUP_REG; r++) {  // clamped index: no divergent loads
+        for (int r = 0; r < REG; r++) {  // clamped index: no divergent loads
             const int j = min(tid + r * nt, n - 1);
             q[r] = load_pos(v, w, slots[it.begin + j]);
 #pragma unroll
-            for (int k = 0; k < 3; k++) yr[r][k] = (double)Y[3 * j + k] - (fit ? yc[k] : 0.0);
+            for (int k = 0; k < 3; k++) yr[r][k] = Y[3 * j + k];
         }
-        auto atom = [&](int r, int j, float4& qq, double* yy) {
-            if (r < GROUP_REG) {
-                qq = q[r];
+        // every atom of this thread with its (centred, for the fit) reference:
+        // registers first (static indices), then the tail with fresh gathers
+        auto visit = [&](auto&& f) {
 #pragma unroll
-                for (int k = 0; k < 3; k++) yy[k] = yr[r][k];
-            } else {
-                qq = load_pos(v, w, slots[it.begin + j]);
-#pragma unroll
-                for (int k = 0; k < 3; k++) yy[k] = (double)Y[3 * j + k] - (fit ? yc[k] : 0.0);
+            for (int r = 0; r < REG; r++) {
+                const int j = tid + r * nt;
+                if (j < n) {
+                    const double yy[3] = {(double)yr[r][0] - (fit ? yc[0] : 0.0), (double)yr[r][1] - (fit ? yc[1] : 0.0),
+                                          (double)yr[r][2] - (fit ? yc[2] : 0.0)};
+                    f(j, q[r], yy);
+                }
+            }
+            for (int j = tid + REG * nt; j < n; j += nt) {
+                const double yy[3] = {(double)Y[3 * j] - (fit ? yc[0] : 0.0), (double)Y[3 * j + 1] - (fit ? yc[1] : 0.0),
+                                      (double)Y[3 * j + 2] - (fit ? yc[2] : 0.0)};
+                f(j, load_pos(v, w, slots[it.begin + j]), yy);
             }
         };
         if (!fit) {
             double s1[1] = {0.0};
-            for (int j = tid, r = 0; j < n; j += nt, r++) {
-                float4 qq;
-                double yy[3];
-                atom(r, j, qq, yy);
+            visit([&](int j, const float4& qq, const double* yy) {
                 const double dx = (double)qq.x - yy[0], dy = (double)qq.y - yy[1], dz = (double)qq.z - yy[2];
                 s1[0] += dx * dx + dy * dy + dz * dz;
-            }
-            block_sum<1>(s1, sh);
+            });
+            block_sum<1, NT / 64>(s1, sh);
             const double rv = sqrt(s1[0] / n);
             const double sc = (rv > 1e-10 ? 1.0 / rv : 0.0) / n;
-            for (int j = tid, r = 0; j < n; j += nt, r++) {
-                float4 qq;
-                double yy[3];
-                atom(r, j, qq, yy);
+            visit([&](int j, const float4& qq, const double* yy) {
                 const double dx = (double)qq.x - yy[0], dy = (double)qq.y - yy[1], dz = (double)qq.z - yy[2];
                 store_row(v, w, it.row0 + j, dx * sc, dy * sc, dz * sc);
-            }
+            });
             if (tid == 0) cvv[it.value] = rv;
             return;
         }
+        if constexpr (NT > GROUP_BLOCK) return;
         // Kabsch: one block reduction of the raw moments
         //   m[3a+b] = sum y'_a x_b, m[9..11] = sum x, m[12] = sum |x|^2,
         //   m[13..15] = sum y', m[16] = sum |y'|^2   (y' = y - ybar),
@@ -594,10 +600,7 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
         double m[17];
 #pragma unroll
         for (int k = 0; k < 17; k++) m[k] = 0.0;
-        for (int j = tid, r = 0; j < n; j += nt, r++) {
-            float4 qq;
-            double y[3];
-            atom(r, j, qq, y);
+        visit([&](int j, const float4& qq, const double* y) {
             const double x[3] = {(double)qq.x, (double)qq.y, (double)qq.z};
 #pragma unroll
             for (int a = 0; a < 3; a++)
@@ -610,8 +613,8 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
                 m[13 + k] += y[k];
                 m[16] += y[k] * y[k];
             }
-        }
-        block_sum<17>(m, sh);
+        });
+        block_sum<17, NT / 64>(m, sh);
         const double xc[3] = {m[9] / n, m[10] / n, m[11] / n};
         __shared__ double shScale;
         if (tid == 0) {
@@ -631,16 +634,13 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
         }
         __syncthreads();
         const double sc = shScale;
-        for (int j = tid, r = 0; j < n; j += nt, r++) {
-            float4 qq;
-            double y[3];
-            atom(r, j, qq, y);
+        visit([&](int j, const float4& qq, const double* y) {
             const double x0 = qq.x - xc[0], x1 = qq.y - xc[1], x2 = qq.z - xc[2];
             const double r0 = shR[0] * y[0] + shR[1] * y[1] + shR[2] * y[2];
             const double r1 = shR[3] * y[0] + shR[4] * y[1] + shR[5] * y[2];
             const double r2 = shR[6] * y[0] + shR[7] * y[1] + shR[8] * y[2];
             store_row(v, w, it.row0 + j, (x0 - r0) * sc, (x1 - r1) * sc, (x2 - r2) * sc);
-        }
+        });
         return;
     }
 
@@ -656,7 +656,7 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
             double dev = sqrt(dx * dx + dy * dy + dz * dz) - (double)d0[p];
             s[0] += dev * dev;
         }
-        block_sum<1>(s, sh);
+        block_sum<1, NT / 64>(s, sh);
         const double val = sqrt(s[0] / np);
         const double inv = val > 1e-10 ? 1.0 / val : 0.0;
         for (int p = tid; p < np; p += nt) {
@@ -705,7 +705,7 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
             store_row(v, w, it.row0 + 2 * p, -coeff * dx, -coeff * dy, -coeff * dz);
             store_row(v, w, it.row0 + 2 * p + 1, coeff * dx, coeff * dy, coeff * dz);
         }
-        block_sum<1>(s, sh);
+        block_sum<1, NT / 64>(s, sh);
         if (tid == 0) cvv[it.value] = s[0];
         return;
     }
@@ -716,7 +716,7 @@ __global__ __launch_bounds__(GROUP_BLOCK) void k_cv_group(StepView v, const Grou
                                                           const float* __restrict__ mass,
                                                           const float* __restrict__ ref, int w0)
 {
-    group_item(v, items[blockIdx.x], slots, mass, ref, w0 + blockIdx.y);
+    group_item<GROUP_BLOCK, GROUP_REG>(v, items[blockIdx.x], slots, mass, ref, w0 + blockIdx.y);
 }
 
 void launch_cv_group(const StepView& v, const GroupCVItem* items, int n, const int* slots, const float* mass,
@@ -726,19 +726,46 @@ void launch_cv_group(const StepView& v, const GroupCVItem* items, int n, const i
     hipLaunchKernelGGL(k_cv_group, dim3(n, nw), dim3(GROUP_BLOCK), 0, s, v, items, slots, mass, ref, w0);
 }
 
+// Group CVs over many atoms or pairs (more than GROUP_BLOCK * GROUP_REG): one
+// 1,024-thread workgroup each, 8 atoms per thread in registers.
+constexpr int GROUP_BIG_BLOCK = 1024;
+constexpr int GROUP_BIG_REG = 8;
+
+__global__ __launch_bounds__(GROUP_BIG_BLOCK) void k_cv_group_big(StepView v, const GroupCVItem* __restrict__ items,
+                                                                  const int* __restrict__ slots,
+                                                                  const float* __restrict__ mass,
+                                                                  const float* __restrict__ ref, int w0)
+{
+    group_item<GROUP_BIG_BLOCK, GROUP_BIG_REG>(v, items[blockIdx.x], slots, mass, ref, w0 + blockIdx.y);
+}
+
+void launch_cv_group_big(const StepView& v, const GroupCVItem* items, int n, const int* slots, const float* mass,
+                         const float* ref, int w0, int nw, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cv_group_big, dim3(n, nw), dim3(GROUP_BIG_BLOCK), 0, s, v, items, slots, mass, ref, w0);
+}
+
 // ---------------------------------------------------------------------------
 // Coordination, CommonGluedKernels.cpp:899-1031, with an optional cutoff.
 //
-// Pair enumeration: a cell list per group (cells of edge >= cutoff, at least
-// 3 per dimension, orthorhombic box) or, without a usable cell list, the whole
-// partner group split into 32 chunks.  Each "home" atom owns 32 lanes of a
-// wave; lane j scans the partner atoms of neighbour cell j (27 used).
-// Two passes give deterministic per-atom Jacobian rows without atomics:
-// pass A (home = group A) also sums CN; pass B (home = group B) fills B rows.
-// The pair predicate r^2 < rc^2 is evaluated in FP64 exactly as the oracle;
-// an FP32 pre-check with a 1e-5 relative margin skips far pairs cheaply.
+// Cell list per group (cells of edge >= cutoff, at least 3 per dimension,
+// orthorhombic box), built by a counting sort of both groups into cell order
+// (k_coord_rank / k_coord_offsets / k_coord_place; the zero / bin / scan /
+// fill kernels are the fallback for cell grids too large for LDS).
+// Without a usable cell list the partner group is scanned in chunks.
+//
+// Pair passes: pass A (home = group A atom) forms CN and the A rows, pass B
+// (home = group B atom) the B rows; every in-cutoff pair is visited once per
+// pass, so each atom's row is a sum owned by one wave (no atomics on rows).
+// Within a wave, lanes scan their own neighbour cell with an FP32 pre-check
+// (1e-5 relative margin); the candidates are queued in LDS and evaluated
+// densely, 64 at a time, on the exact FP64 predicate r^2 < rc^2 (as the
+// oracle) and the FP64 switching function.  CN, pair count and gradients are
+// accumulated as 2^32 fixed-point integers, so the result does not depend on
+// the order atoms were placed in a cell, on the queue order or on the wave
+// layout: the sums are exact integer sums of rounded terms, bit-reproducible.
 // ---------------------------------------------------------------------------
-
 
 __device__ inline int cell_of(double x, double y, double z, const double* bx, const int* nc)
 {
@@ -822,6 +849,93 @@ __global__ void k_coord_fill(StepView v, CoordBuffers cb, const CoordItem* items
     cb.sorted[(size_t)w * cb.sortStride + it.sortBase + pos] = make_float4(q.x, q.y, q.z, __int_as_float(local));
 }
 
+constexpr int CELL_BLOCK = 1024;
+constexpr int CELL_LDS = 8192;  // cells (both groups) counted in LDS by the sort
+
+// Three-kernel counting sort (no global atomics, no single-CU scatter):
+//   rank    one 1,024-thread workgroup per 1,024 atoms: gather, cell, rank of
+//           the atom among the workgroup's atoms of that cell (LDS counter),
+//           and the workgroup's count per cell (cell-major table);
+//   offsets one workgroup per walker: exclusive scan of the table (cell-major,
+//           workgroup-minor) -> where each workgroup's atoms of a cell start;
+//   place   every atom to its slot.
+int coord_blocks(int n) { return (n + CELL_BLOCK - 1) / CELL_BLOCK; }
+int coord_sort_fits(int ncellBoth, int n) { return ncellBoth <= CELL_LDS && ncellBoth * coord_blocks(n) <= 65536; }
+
+__global__ __launch_bounds__(CELL_BLOCK) void k_coord_rank(StepView v, CoordBuffers cb, const CoordItem* items,
+                                                           int item, int w0)
+{
+    __shared__ int lcnt[CELL_LDS];
+    const int w = w0 + blockIdx.y, tid = threadIdx.x, b = blockIdx.x;
+    const CoordItem it = items[item];
+    const int n = it.nA + it.nB, nb = gridDim.x;
+    const int ncell = it.nc[0] * it.nc[1] * it.nc[2], n2 = 2 * ncell;
+    for (int c = tid; c < n2; c += CELL_BLOCK) lcnt[c] = 0;
+    __syncthreads();
+    const int i = b * CELL_BLOCK + tid;
+    if (i < n) {
+        const bool a = i < it.nA;
+        const float4 q = load_pos(v, w, cb.slots[a ? it.aBegin + i : it.bBegin + (i - it.nA)]);
+        const int c = cell_of(q.x, q.y, q.z, v.box + (size_t)w * 12, it.nc) + (a ? 0 : ncell);
+        const size_t o = (size_t)w * cb.sortStride + it.sortBase + i;
+        cb.cellOf[(size_t)w * cb.cellOfStride + it.sortBase + i] = c;
+        cb.rank[o] = atomicAdd(&lcnt[c], 1);
+        cb.unsorted[o] = make_float4(q.x, q.y, q.z, __int_as_float(a ? i : i - it.nA));
+    }
+    __syncthreads();
+    int* tab = cb.blockCnt + (size_t)w * cb.blkStride + it.blkBase;
+    for (int c = tid; c < n2; c += CELL_BLOCK) tab[(size_t)c * nb + b] = lcnt[c];
+}
+
+__global__ __launch_bounds__(CELL_BLOCK) void k_coord_offsets(CoordBuffers cb, const CoordItem* items, int item, int nb,
+                                                              int w0)
+{
+    __shared__ int waveTot[CELL_BLOCK / 64];
+    const int w = w0 + blockIdx.y, tid = threadIdx.x;
+    const CoordItem it = items[item];
+    const int ncell = it.nc[0] * it.nc[1] * it.nc[2], n2 = 2 * ncell, len = n2 * nb;
+    int* tab = cb.blockCnt + (size_t)w * cb.blkStride + it.blkBase;
+    int* st = cb.cellStart + (size_t)w * (cb.cellStride + cb.nItems) + it.cellBase + item;
+    const int per = (len + CELL_BLOCK - 1) / CELL_BLOCK;
+    const int b0 = tid * per;
+    int tsum = 0;
+    for (int k = 0; k < per; k++)
+        if (b0 + k < len) tsum += tab[b0 + k];
+    const int lane = tid & 63, wid = tid >> 6;
+    int incl = tsum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int t = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += t;
+    }
+    if (lane == 63) waveTot[wid] = incl;
+    __syncthreads();
+    int run = incl - tsum;
+    for (int q = 0; q < wid; q++) run += waveTot[q];
+    for (int k = 0; k < per; k++) {
+        const int e = b0 + k;
+        if (e >= len) break;
+        const int x = tab[e];
+        tab[e] = run;
+        if (e % nb == 0) st[e / nb] = run;  // start of cell e / nb
+        run += x;
+    }
+    if (tid == CELL_BLOCK - 1) st[n2] = run;
+}
+
+__global__ __launch_bounds__(CELL_BLOCK) void k_coord_place(CoordBuffers cb, const CoordItem* items, int item, int w0)
+{
+    const int w = w0 + blockIdx.y, b = blockIdx.x, nb = gridDim.x;
+    const CoordItem it = items[item];
+    const int n = it.nA + it.nB;
+    const int i = b * CELL_BLOCK + threadIdx.x;
+    if (i >= n) return;
+    const size_t o = (size_t)w * cb.sortStride + it.sortBase + i;
+    const int c = cb.cellOf[(size_t)w * cb.cellOfStride + it.sortBase + i];
+    const int pos = cb.blockCnt[(size_t)w * cb.blkStride + it.blkBase + (size_t)c * nb + b] + cb.rank[o];
+    cb.sorted[(size_t)w * cb.sortStride + it.sortBase + pos] = cb.unsorted[o];
+}
+
 struct SwitchD {
     double r0, rn, rm, ir0;
     double preRes, preDfunc, preSecDev;
@@ -874,153 +988,232 @@ __device__ inline void switch_eval(const SwitchD& s, double r, double& f, double
     }
 }
 
-constexpr int COORD_BLOCK = 256;  // 8 home atoms x 32 lanes
+constexpr int COORD_BLOCK = 256;  // 4 waves
+constexpr int COORD_PF = 8;       // partner loads in flight per lane
+constexpr double COORD_FIX = 1099511627776.0;  // 2^40: gradient terms (|term| < ~1e3, row sums < 2^23)
+constexpr double COORD_UNFIX = 1.0 / 1099511627776.0;
 
-template <bool SIDE_A>
+__device__ inline long long to_fix(double x) { return __double2ll_rn(x * COORD_FIX); }
+
+// CN terms f in [0, 1] as 128-bit fixed point in units of 2^-64 (hi * 2^64 +
+// lo): exact for f >= 2^-11, below that the error is < 2^-64.  Sums of these
+// are exact integer sums, independent of order.
+struct U128 {
+    unsigned long long lo;
+    long long hi;
+};
+
+__device__ inline void add128(U128& a, unsigned long long lo, long long hi)
+{
+    const unsigned long long t = a.lo + lo;
+    a.hi += hi + (t < lo ? 1 : 0);
+    a.lo = t;
+}
+
+__device__ inline void add_cn_term(U128& a, double f)
+{
+    const double ip = floor(f);
+    const double frac = f - ip;  // exact
+    add128(a, (unsigned long long)(frac * 18446744073709551616.0), (long long)ip);
+}
+
+// SIDE_A: one home atom per wave, lanes (cell, half) = (l / 2, l % 2) for l < 54.
+// SIDE_B: two home atoms per wave (lanes 0-31, 32-63), lane l % 32 = cell (< 27).
+template <bool SIDE_A, bool CELLS>
 __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBuffers cb,
                                                             const CoordItem* __restrict__ items, int item, int w0)
 {
-    __shared__ double shC[COORD_BLOCK / 64];
-    __shared__ unsigned long long shP[COORD_BLOCK / 64];
-    __shared__ int shLast;
+    constexpr int HPW = SIDE_A ? 1 : 2;   // homes per wave
+    constexpr int LPH = 64 / HPW;         // lanes per home
+    constexpr int LPC = SIDE_A ? 2 : 1;   // lanes per cell
+    __shared__ float4 qPos[COORD_BLOCK / 64][128];
+    __shared__ int qHome[COORD_BLOCK / 64][128];
     const int w = w0 + blockIdx.y;
     const CoordItem it = items[item];
     const int nHome = SIDE_A ? it.nA : it.nB;
     const int nPart = SIDE_A ? it.nB : it.nA;
-    const int home = blockIdx.x * (COORD_BLOCK / 32) + (threadIdx.x >> 5);
-    const int lane = threadIdx.x & 31;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int h = lane / LPH, l = lane % LPH;
+    if (!SIDE_A && blockIdx.x == 0 && threadIdx.x == 0) {
+        // pass A has finished (launch boundary): CN and the pair count to their
+        // outputs, accumulators cleared for the next step
+        const size_t k = (size_t)w * cb.nItems + item;
+        // CN = (H 2^40 + L) 2^-64 = H 2^-24 + L 2^-64
+        v.cvVal[(size_t)w * v.V + it.value] =
+            (double)cb.cnAcc[k] * 5.9604644775390625e-08 + (double)cb.cnAccLo[k] * 5.421010862427522e-20;
+        cb.npairs[k] = (long long)cb.pairAcc[k];
+        cb.cnAcc[k] = 0;
+        cb.cnAccLo[k] = 0;
+        cb.pairAcc[k] = 0;
+    }
+    const int home0 = (blockIdx.x * (COORD_BLOCK / 64) + wv) * HPW;
+    if (home0 >= nHome) return;  // whole wave idle (no workgroup barrier below)
+    const int home = home0 + h;
+    const bool homeOk = home < nHome;
     const double* bx = v.box + (size_t)w * 12;
     float bxf[12];
     for (int k = 0; k < 12; k++) bxf[k] = (float)bx[k];
     const SwitchD sw = make_switch(it);
     const float rc2f = (float)(it.rc2 * (1.0 + 1e-5));
 
-    double cn = 0.0;
-    unsigned long long np = 0;
-    double gx = 0.0, gy = 0.0, gz = 0.0;
-    if (home < nHome) {
-        const int hslot = cb.slots[SIDE_A ? it.aBegin + home : it.bBegin + home];
-        const float4 ph = load_pos(v, w, hslot);
-        int pBegin = 0, pEnd = 0;
-        const float4* part = cb.sorted + (size_t)w * cb.sortStride + it.sortBase + (SIDE_A ? it.nA : 0);
-        if (it.useCells) {
-            if (lane < 27) {
-                const int ncell = it.nc[0] * it.nc[1] * it.nc[2];
-                const int hc = cb.cellOf[(size_t)w * cb.cellOfStride + it.sortBase + (SIDE_A ? home : it.nA + home)] -
-                               (SIDE_A ? 0 : ncell);
-                int cx = hc % it.nc[0], cy = (hc / it.nc[0]) % it.nc[1], cz = hc / (it.nc[0] * it.nc[1]);
-                cx = (cx + (lane % 3) - 1 + it.nc[0]) % it.nc[0];
-                cy = (cy + ((lane / 3) % 3) - 1 + it.nc[1]) % it.nc[1];
-                cz = (cz + (lane / 9) - 1 + it.nc[2]) % it.nc[2];
-                const int pc = (cz * it.nc[1] + cy) * it.nc[0] + cx + (SIDE_A ? ncell : 0);
-                const int* st = cb.cellStart + (size_t)w * (cb.cellStride + cb.nItems) + it.cellBase + item;
-                const int base = SIDE_A ? st[ncell] : 0;  // partner atoms start after the other group
-                pBegin = st[pc] - base;
-                pEnd = st[pc + 1] - base;
-            }
-        } else {
-            const int chunk = (nPart + 31) / 32;
-            pBegin = lane * chunk;
-            pEnd = min(nPart, pBegin + chunk);
+    // home positions of this wave (both halves), this lane's partner range
+    const int hslot = cb.slots[SIDE_A ? it.aBegin + min(home, nHome - 1) : it.bBegin + min(home, nHome - 1)];
+    const float4 ph = load_pos(v, w, hslot);
+    float4 hp[HPW];
+#pragma unroll
+    for (int k = 0; k < HPW; k++) {
+        const int src = k * LPH;
+        hp[k] = make_float4(__shfl(ph.x, src, 64), __shfl(ph.y, src, 64), __shfl(ph.z, src, 64), 0.f);
+    }
+    const float4* part = cb.sorted + (size_t)w * cb.sortStride + it.sortBase + (SIDE_A ? it.nA : 0);
+    int pBegin = 0, pCount = 0, pStride = 1;
+    if (CELLS) {
+        const int cell = l / LPC, sub = l % LPC;
+        if (homeOk && cell < 27) {
+            const int ncell = it.nc[0] * it.nc[1] * it.nc[2];
+            const int hc = cb.cellOf[(size_t)w * cb.cellOfStride + it.sortBase + (SIDE_A ? home : it.nA + home)] -
+                           (SIDE_A ? 0 : ncell);
+            int cx = hc % it.nc[0], cy = (hc / it.nc[0]) % it.nc[1], cz = hc / (it.nc[0] * it.nc[1]);
+            cx = (cx + (cell % 3) - 1 + it.nc[0]) % it.nc[0];
+            cy = (cy + ((cell / 3) % 3) - 1 + it.nc[1]) % it.nc[1];
+            cz = (cz + (cell / 9) - 1 + it.nc[2]) % it.nc[2];
+            const int pc = (cz * it.nc[1] + cy) * it.nc[0] + cx + (SIDE_A ? ncell : 0);
+            const int* st = cb.cellStart + (size_t)w * (cb.cellStride + cb.nItems) + it.cellBase + item;
+            const int base = SIDE_A ? st[ncell] : 0;  // partner atoms start after the other group
+            const int b = st[pc] - base, e = st[pc + 1] - base;
+            pBegin = b + sub;
+            pCount = e > pBegin ? (e - pBegin + LPC - 1) / LPC : 0;
+            pStride = LPC;
         }
-        for (int j = pBegin; j < pEnd; j++) {
-            const float4 pp = it.useCells ? part[j] : load_pos(v, w, cb.slots[SIDE_A ? it.bBegin + j : it.aBegin + j]);
-            // dr = r_A - r_B (reference order) in both passes
-            const float4 pa = SIDE_A ? ph : pp;
-            const float4 pb = SIDE_A ? pp : ph;
-            if (it.useCells) {  // cells guarantee box >= 3 cutoffs: the FP32 image is never ambiguous
+    } else if (homeOk) {
+        const int chunk = (nPart + LPH - 1) / LPH;
+        pBegin = l * chunk;
+        pCount = max(0, min(nPart, pBegin + chunk) - pBegin);
+    }
+    int maxCount = pCount;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) maxCount = max(maxCount, __shfl_xor(maxCount, o, 64));
+
+    long long gx[HPW], gy[HPW], gz[HPW];
+#pragma unroll
+    for (int k = 0; k < HPW; k++) gx[k] = gy[k] = gz[k] = 0;
+    U128 cn{0ull, 0ll};
+    unsigned long long np = 0;
+    float4* Q = qPos[wv];
+    int* QH = qHome[wv];
+    unsigned head = 0, tail = 0;
+
+    // FP64 evaluation of queue entry e (exact predicate, reference switching)
+    auto evaluate = [&](int e) {
+        const float4 pp = Q[e & 127];
+        const int hh = QH[e & 127];
+        const float4 pH = hh == 0 ? hp[0] : hp[HPW - 1];
+        const float4 pa = SIDE_A ? pH : pp;  // dr = r_A - r_B in both passes
+        const float4 pb = SIDE_A ? pp : pH;
+        double dx = (double)pa.x - (double)pb.x, dy = (double)pa.y - (double)pb.y, dz = (double)pa.z - (double)pb.z;
+        if (v.pbc) pbc_exact(dx, dy, dz, bx);
+        const double r2 = dot3_exact(dx, dy, dz);
+        if (r2 < 1e-20) return;
+        if (it.rc2 > 0.0 && !(r2 < it.rc2)) return;
+        const double r = sqrt(r2);
+        if (r * sw.ir0 > 1000.0) return;
+        double f, dfdr;
+        switch_eval(sw, r, f, dfdr);
+        const double s = (SIDE_A ? 1.0 : -1.0) * dfdr / r;
+        const long long qx = to_fix(s * dx), qy = to_fix(s * dy), qz = to_fix(s * dz);
+        if (hh == 0) {
+            gx[0] += qx;
+            gy[0] += qy;
+            gz[0] += qz;
+        } else {
+            gx[HPW - 1] += qx;
+            gy[HPW - 1] += qy;
+            gz[HPW - 1] += qz;
+        }
+        if (SIDE_A) {
+            add_cn_term(cn, f);
+            np++;
+        }
+    };
+
+    // partners in batches of COORD_PF: every load of a batch in flight before the tests
+    for (int k0 = 0; k0 < maxCount; k0 += COORD_PF) {
+        float4 pf[COORD_PF];
+#pragma unroll
+        for (int u = 0; u < COORD_PF; u++) {  // clamped to a valid partner: no divergent loads
+            const int j = pCount > 0 ? pBegin + min(k0 + u, pCount - 1) * pStride : 0;
+            pf[u] = CELLS ? part[j] : load_pos(v, w, cb.slots[SIDE_A ? it.bBegin + j : it.aBegin + j]);
+        }
+#pragma unroll
+        for (int u = 0; u < COORD_PF; u++) {
+            const int k = k0 + u;
+            if (k >= maxCount) break;
+            const float4 pp = pf[u];
+            const bool have = k < pCount;
+            bool cand = have;
+            if (CELLS && have) {  // cells guarantee box >= 3 cutoffs: the FP32 image is never ambiguous
+                const float4 pa = SIDE_A ? hp[h] : pp;
+                const float4 pb = SIDE_A ? pp : hp[h];
                 float fx = pa.x - pb.x, fy = pa.y - pb.y, fz = pa.z - pb.z;
                 if (v.pbc) pbc_f(fx, fy, fz, bxf);
-                if (!(fx * fx + fy * fy + fz * fz < rc2f)) continue;
+                cand = fx * fx + fy * fy + fz * fz < rc2f;
             }
-            double dx = (double)pa.x - (double)pb.x, dy = (double)pa.y - (double)pb.y,
-                   dz = (double)pa.z - (double)pb.z;
-            if (v.pbc) pbc_exact(dx, dy, dz, bx);
-            const double r2 = dot3_exact(dx, dy, dz);
-            if (r2 < 1e-20) continue;
-            if (it.rc2 > 0.0 && !(r2 < it.rc2)) continue;
-            const double r = sqrt(r2);
-            if (r * sw.ir0 > 1000.0) continue;
-            double f, dfdr;
-            switch_eval(sw, r, f, dfdr);
-            const double s = dfdr / r;
-            if (SIDE_A) {
-                cn += f;
-                np++;
-                gx += s * dx;
-                gy += s * dy;
-                gz += s * dz;
-            } else {
-                gx -= s * dx;
-                gy -= s * dy;
-                gz -= s * dz;
+            const unsigned long long m = __ballot(cand);
+            if (cand) {
+                const unsigned slot = tail + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
+                Q[slot & 127] = pp;
+                QH[slot & 127] = h;
+            }
+            tail += (unsigned)__popcll(m);
+            if (tail - head >= 64) {
+                __builtin_amdgcn_wave_barrier();
+                evaluate((int)(head + lane));
+                head += 64;
+                __builtin_amdgcn_wave_barrier();
             }
         }
     }
-    // reduce the 32 lanes of each home atom
-#pragma unroll
-    for (int o = 16; o > 0; o >>= 1) {
-        gx += __shfl_xor(gx, o, 32);
-        gy += __shfl_xor(gy, o, 32);
-        gz += __shfl_xor(gz, o, 32);
-    }
-    if (home < nHome && lane == 0) store_row(v, w, (SIDE_A ? it.rowA : it.rowB) + home, gx, gy, gz);
-    if (!SIDE_A) return;
+    __builtin_amdgcn_wave_barrier();
+    if (lane < (int)(tail - head)) evaluate((int)(head + lane));
 
-    // CN and pair count: block partials, last block sums them in block order.
-    cn = wave_sum(cn);
+    // integer (exact) wave sums; lane 0 of each home's half writes its row
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) np += __shfl_xor(np, o, 64);
-    if ((threadIdx.x & 63) == 0) {
-        shC[threadIdx.x >> 6] = cn;
-        shP[threadIdx.x >> 6] = np;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double c = 0.0;
-        unsigned long long p = 0;
-        for (int i = 0; i < COORD_BLOCK / 64; i++) {
-            c += shC[i];
-            p += shP[i];
+    for (int k = 0; k < HPW; k++) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            gx[k] += __shfl_xor(gx[k], o, 64);
+            gy[k] += __shfl_xor(gy[k], o, 64);
+            gz[k] += __shfl_xor(gz[k], o, 64);
         }
-        cb.partCN[(size_t)w * cb.partStride + it.partBase + blockIdx.x] = c;
-        cb.partPairs[(size_t)w * cb.partStride + it.partBase + blockIdx.x] = p;
-        __threadfence();
-        unsigned int done = atomicAdd(&cb.counters[(size_t)w * cb.nItems + item], 1u);
-        shLast = (done == gridDim.x - 1);
     }
-    __syncthreads();
-    if (!shLast) return;
-    __threadfence();
-    // gather the block partials in parallel, add them in block order
-    __shared__ double shc[COORD_BLOCK];
-    __shared__ unsigned long long shp[COORD_BLOCK];
-    double c = 0.0;
-    unsigned long long p = 0;
-    const double* pc = cb.partCN + (size_t)w * cb.partStride + it.partBase;
-    const unsigned long long* pp = cb.partPairs + (size_t)w * cb.partStride + it.partBase;
-    for (unsigned base = 0; base < gridDim.x; base += COORD_BLOCK) {
-        const unsigned m = min((unsigned)COORD_BLOCK, gridDim.x - base);
-        if (threadIdx.x < m) {
-            shc[threadIdx.x] = __hip_atomic_load(pc + base + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            shp[threadIdx.x] = __hip_atomic_load(pp + base + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (l == 0 && homeOk) {
+        const int k = h;
+        const long long sx = k == 0 ? gx[0] : gx[HPW - 1], sy = k == 0 ? gy[0] : gy[HPW - 1],
+                        sz = k == 0 ? gz[0] : gz[HPW - 1];
+        store_row(v, w, (SIDE_A ? it.rowA : it.rowB) + home, (double)sx * COORD_UNFIX, (double)sy * COORD_UNFIX,
+                  (double)sz * COORD_UNFIX);
+    }
+    if (SIDE_A) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const unsigned long long lo = __shfl_xor(cn.lo, o, 64);
+            const long long hi = __shfl_xor(cn.hi, o, 64);
+            add128(cn, lo, hi);
+            np += __shfl_xor(np, o, 64);
         }
-        __syncthreads();
-        if (threadIdx.x == 0)
-            for (unsigned j = 0; j < m; j++) {
-                c += shc[j];
-                p += shp[j];
-            }
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        v.cvVal[(size_t)w * v.V + it.value] = c;
-        cb.npairs[(size_t)w * cb.nItems + item] = (long long)p;
-        cb.counters[(size_t)w * cb.nItems + item] = 0u;  // self-reset for the next step
+        if (lane == 0) {
+            // the wave's 128-bit sum V as (V >> 40) and (V mod 2^40): the low
+            // accumulator absorbs 2^24 waves without overflow, so two 64-bit
+            // atomics add V exactly
+            const size_t k = (size_t)w * cb.nItems + item;
+            const unsigned long long L = cn.lo & ((1ull << 40) - 1ull);
+            const long long H = (long long)((unsigned long long)cn.hi << 24) + (long long)(cn.lo >> 40);
+            atomicAdd(&cb.cnAccLo[k], L);
+            atomicAdd((unsigned long long*)&cb.cnAcc[k], (unsigned long long)H);
+            atomicAdd(&cb.pairAcc[k], np);
+        }
     }
 }
-
-int coord_blocks_A(int nA) { return (nA + COORD_BLOCK / 32 - 1) / (COORD_BLOCK / 32); }
 
 void launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordItem* dItems, const CoordItem& hIt,
                          int item, int w0, int nw, hipStream_t s)
@@ -1028,16 +1221,28 @@ void launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordI
     const int n = hIt.nA + hIt.nB;
     if (hIt.useCells) {
         const int ncell2 = 2 * hIt.nc[0] * hIt.nc[1] * hIt.nc[2];
-        hipLaunchKernelGGL(k_coord_zero, dim3((ncell2 + 255) / 256, nw), dim3(256), 0, s, cb, hIt.cellBase, ncell2,
-                           w0);
-        hipLaunchKernelGGL(k_coord_bin, dim3((n + 255) / 256, nw), dim3(256), 0, s, v, cb, dItems, item, w0);
-        hipLaunchKernelGGL(k_coord_scan, dim3(1, nw), dim3(1024), 0, s, cb, dItems, item, w0);
-        hipLaunchKernelGGL(k_coord_fill, dim3((n + 255) / 256, nw), dim3(256), 0, s, v, cb, dItems, item, w0);
+        if (coord_sort_fits(ncell2, n)) {
+            const int nb = coord_blocks(n);
+            hipLaunchKernelGGL(k_coord_rank, dim3(nb, nw), dim3(CELL_BLOCK), 0, s, v, cb, dItems, item, w0);
+            hipLaunchKernelGGL(k_coord_offsets, dim3(1, nw), dim3(CELL_BLOCK), 0, s, cb, dItems, item, nb, w0);
+            hipLaunchKernelGGL(k_coord_place, dim3(nb, nw), dim3(CELL_BLOCK), 0, s, cb, dItems, item, w0);
+        } else {
+            hipLaunchKernelGGL(k_coord_zero, dim3((ncell2 + 255) / 256, nw), dim3(256), 0, s, cb, hIt.cellBase,
+                               ncell2, w0);
+            hipLaunchKernelGGL(k_coord_bin, dim3((n + 255) / 256, nw), dim3(256), 0, s, v, cb, dItems, item, w0);
+            hipLaunchKernelGGL(k_coord_scan, dim3(1, nw), dim3(1024), 0, s, cb, dItems, item, w0);
+            hipLaunchKernelGGL(k_coord_fill, dim3((n + 255) / 256, nw), dim3(256), 0, s, v, cb, dItems, item, w0);
+        }
     }
-    hipLaunchKernelGGL(k_coord_pairs<true>, dim3(coord_blocks_A(hIt.nA), nw), dim3(COORD_BLOCK), 0, s, v, cb, dItems,
-                       item, w0);
-    hipLaunchKernelGGL(k_coord_pairs<false>, dim3(coord_blocks_A(hIt.nB), nw), dim3(COORD_BLOCK), 0, s, v, cb,
-                       dItems, item, w0);
+    const int wavesA = hIt.nA, wavesB = (hIt.nB + 1) / 2;
+    const dim3 gA((wavesA + 3) / 4, nw), gB((wavesB + 3) / 4, nw);
+    if (hIt.useCells) {
+        hipLaunchKernelGGL((k_coord_pairs<true, true>), gA, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
+        hipLaunchKernelGGL((k_coord_pairs<false, true>), gB, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
+    } else {
+        hipLaunchKernelGGL((k_coord_pairs<true, false>), gA, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
+        hipLaunchKernelGGL((k_coord_pairs<false, false>), gB, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
+    }
 }
 
 }  // namespace glued
@@ -1223,7 +1428,7 @@ __global__ __launch_bounds__(GROUP_BLOCK) void k_cv_fused(StepView v, CVFusedPla
     const int w = blockIdx.y;
     const int b = blockIdx.x;
     if (b < p.nGroup) {
-        group_item(v, p.group[b], p.gslots, p.gmass, p.gref, w);
+        group_item<GROUP_BLOCK, GROUP_REG>(v, p.group[b], p.gslots, p.gmass, p.gref, w);
     } else if (b < p.nGroup + p.nPathMsdBlocks) {
         const int lb = b - p.nGroup;
         int q = 0;
