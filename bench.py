@@ -8,13 +8,16 @@ atoms/nm^3), Kabsch RMSD over 500 atoms + 2 torsions, well-tempered MetaD on
 One eval = every CV, the bias energy and dV/ds, and the fixed-point force
 scatter, for one walker.
 
-  value        batched evals/s: every walker per step as one captured
-               hipGraph
+  value        batched evals/s: every walker per step, the stage kernels
+               launched directly on the stream (on this ROCm a captured
+               hipGraph replay of the same kernels costs ~4 us more per step;
+               its rate is reported as graph_evals_per_s)
   plain        the reference's structure: one walker per launch sequence
                (one launch per CV kind, per bias, and the scatter), host loop
                over walkers
   roofline     the dominant kernel of the workload, timed live with HIP events
-               on the launch stream over back-to-back launches of that stage
+               on the launch stream over back-to-back launches of that stage;
+               traffic = HBM bytes per launch from rocprofv3 PMC counters
   cpu_baseline the FP64 oracle on a bounded sample of the same walkers, one
                host core; its forces also give the force error vs FP64
 
@@ -47,6 +50,30 @@ FP64_VALU_TFLOPS = 78.6
 FP32_VALU_TFLOPS = 157.3
 FIXED_SCALE = 4294967296.0
 
+# HBM traffic per launch of each workload's roofline kernel(s), from two
+# rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch,
+# averaged over 20 batched steps of `bench.py --workload W --batched-only
+# --steps 20 --warmup 2`).  On gfx950 FETCH_SIZE reads half the bytes that a
+# copy kernel of known size moves and WRITE_SIZE reads them in full, so
+# traffic = 2 * FETCH + 1 * WRITE.  Summaries under profiles/ (*_pmc.md).
+PMC_FETCH_SCALE, PMC_WRITE_SCALE = 2.0, 1.0
+PMC_KIB = {
+    "multiwalker": {"k_step_bias_chunks<1, 2>": (15052.2, 60.0)},
+    "mixed-large": {"k_scatter_dense": (28157.5, 37512.1)},
+    "coordination-opes": {"k_coord_rank": (278.5, 792.4), "k_coord_offsets": (21.1, 38.8),
+                          "k_coord_place": (359.4, 418.8), "k_coord_pairs<true, true>": (1656.8, 234.4),
+                          "k_coord_pairs<false, true>": (1151.6, 269.7), "k_cv_group_big": (95.5, 117.2)},
+    "path-batch": {"k_cv_fused (path MSD blocks)": (3377.4, 128.0), "k_path_rows": (2133.0, 1816.0)},
+    "small": {"k_step_bias_chunks<1, 2>": (21.7, 0.1)},
+}
+
+
+def pmc_traffic(workload):
+    t = PMC_KIB.get(workload)
+    if not t:
+        return None
+    return round(sum(PMC_FETCH_SCALE * f + PMC_WRITE_SCALE * w for f, w in t.values()) * 1024)
+
 
 def log(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
@@ -58,6 +85,10 @@ def log(msg):
 class Workload:
     name = ""
     walkers = 1
+
+    def time_dominant(self, reps):
+        """Average duration of one launch of the roofline kernel(s) (s)."""
+        return time_stage(self.ctx, self.stage, reps)
 
     def roofline(self, t_launch):
         raise NotImplementedError
@@ -94,7 +125,7 @@ class MultiWalker(Workload):
                         f"WT MetaD shared 200x200 grid + {self.H // 1000}k private hills/walker",
             "walkers": self.W,
             "hills_per_walker": self.H,
-            "parallelism": "batched walkers, one hipGraph per step (3 kernels)",
+            "parallelism": "batched walkers, 3 kernel launches per step (CVs, hill-sum chunks, finalize + scatter)",
         }
 
     def roofline(self, t_launch):
@@ -150,7 +181,7 @@ class MixedLarge(Workload):
             "workload": f"mixed large: {W} walkers x 100k atoms, contact map 4096 pairs, DRMSD 1500 pairs, "
                         "path 256 frames x 1000 atoms, multithermal 32 T on a dense energy CV",
             "walkers": W,
-            "parallelism": "batched walkers, one hipGraph per step",
+            "parallelism": "batched walkers, direct kernel launches",
         }
 
     def roofline(self, t_launch):
@@ -193,7 +224,7 @@ class CoordinationOpes(Workload):
             "workload": "coordination + OPES: 50k atoms, CN 2000 x 20000 (cutoff 1.2 nm, cell list), Rg(5000), "
                         "OPES-METAD 5000 kernels",
             "walkers": 1,
-            "parallelism": "one walker, one hipGraph per step",
+            "parallelism": "one walker, direct kernel launches",
         }
 
     def roofline(self, t_launch):
@@ -249,7 +280,7 @@ class PathBatch(Workload):
             "workload": "path batch: 256 configurations x 300 atoms, path CV 64 frames (lambda 50), "
                         "MetaD 20k hills shared",
             "walkers": B,
-            "parallelism": "configurations as walkers, one hipGraph per step",
+            "parallelism": "configurations as walkers, direct kernel launches",
         }
 
     def roofline(self, t_launch):
@@ -269,7 +300,7 @@ class PathBatch(Workload):
 
 class Small(Workload):
     name = "small"
-    stage = 1
+    stage = 3
 
     def __init__(self, args):
         from gluedmd_amd import workloads as wl
@@ -284,13 +315,13 @@ class Small(Workload):
             "workload": "small: 22 atoms, distance + 2 torsions, WT MetaD hill list (1000 prior hills, "
                         "1 deposit per frame)",
             "walkers": 1,
-            "parallelism": "one walker, one hipGraph per step",
+            "parallelism": "one walker, direct kernel launches",
         }
 
     def roofline(self, t_launch):
         n = len(self.ctx.get_hills(0)[1])
         nbytes = n * 24
-        return dict(bound="hbm", kernel="bias stage (hill list sum)", achieved=nbytes / t_launch / 1e9,
+        return dict(bound="hbm", kernel="k_step_bias_chunks (hill list sum)", achieved=nbytes / t_launch / 1e9,
                     peak=HBM_PEAK_GBS, unit="GB/s", per_unit="24 B per hill", units_per_launch=n,
                     bytes_per_launch=nbytes)
 
@@ -317,20 +348,26 @@ def time_loop(fn, steps, sync):
     return (time.perf_counter() - t0) / steps
 
 
-def time_stage(ctx, stage, reps):
-    """Average duration of one launch of a stage: HIP events on the stream the
-    engine launches on, around `reps` back-to-back launches."""
+def time_fn(fn, reps):
+    """Average GPU duration of one call: HIP events on the stream the engine
+    launches on (torch's current stream, which the Context hands to every
+    C-ABI call), around `reps` back-to-back calls."""
     import torch
-    stream = torch.cuda.current_stream(ctx.device)
+    stream = torch.cuda.current_stream()
     for _ in range(10):
-        ctx.execute_stage(stage)
+        fn()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
     for _ in range(reps):
-        ctx.execute_stage(stage)
+        fn()
     e1.record(stream)
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps * 1e-3
+
+
+def time_stage(ctx, stage, reps):
+    """Average duration of one launch of a stage."""
+    return time_fn(lambda: ctx.execute_stage(stage), reps)
 
 
 def cpu_baseline(wk, seconds):
@@ -340,7 +377,7 @@ def cpu_baseline(wk, seconds):
     import torch
     ctx = wk.ctx
     ctx.zero_forces()
-    ctx.execute_batch(use_graph=True)
+    ctx.execute_batch(use_graph=False)
     torch.cuda.synchronize()
     evals, t_used, num, den, amax, w = 0, 0.0, 0.0, 0.0, 0.0, 0
     while t_used < seconds and evals < 1000 * wk.W:
@@ -390,10 +427,10 @@ def main():
     sync = torch.cuda.synchronize
     log(f"{wk.name} ready ({W} walkers)")
 
-    # ---- batched: one graph per step ----
+    # ---- batched: every walker per step, direct launches ----
     for _ in range(args.warmup):
-        ctx.execute_batch(use_graph=True)
-    t_batched = time_loop(lambda k: ctx.execute_batch(use_graph=True), args.steps, sync)
+        ctx.execute_batch(use_graph=False)
+    t_batched = time_loop(lambda k: ctx.execute_batch(use_graph=False), args.steps, sync)
     log(f"batched {t_batched * 1e6:.1f} us/step")
     if args.batched_only:
         print(json.dumps({"metric": "batched step only", "workload": wk.name, "ms_per_step": round(t_batched * 1e3, 4),
@@ -412,16 +449,28 @@ def main():
         cpu, err = cpu_baseline(wk, args.cpu_seconds)
         log(f"cpu baseline {cpu['value']} evals/s, force rel RMS {err}")
 
-    # ---- eval + deposit (pace 1: every walker deposits every step), one graph per step ----
-    ctx.step_batch(1)
-    t_dep = time_loop(lambda k: ctx.step_batch(k + 2), 20, sync)
+    # ---- the same batched step replayed as one captured hipGraph ----
+    for _ in range(10):
+        ctx.execute_batch(use_graph=True)
+    t_graph = time_loop(lambda k: ctx.execute_batch(use_graph=True), args.steps, sync)
+
+    # ---- eval + deposit (pace 1: every walker deposits every step) ----
+    ctx.update_state(1)
+    ctx.execute_batch(use_graph=False)
+    # (10 steps per form: the smallest hill capacity, path-batch's, holds 32 steps of 256 deposits)
+    t_dep = time_loop(lambda k: (ctx.update_state(k + 2), ctx.execute_batch(use_graph=False)), 10, sync)
+    # as one graph per deposit pattern (glued_step_batch)
+    ctx.step_batch(100)
+    t_dep_graph = time_loop(lambda k: ctx.step_batch(k + 101), 10, sync)
 
     # ---- roofline of the dominant kernel ----
-    t_launch = time_stage(ctx, wk.stage, 200)
+    t_launch = wk.time_dominant(200)
     rf = wk.roofline(t_launch)
     rf["frac"] = round(rf["achieved"] / rf["peak"], 4)
     rf["achieved"] = round(rf["achieved"], 3)
-    rf["traffic"] = None
+    rf["traffic"] = pmc_traffic(wk.name)
+    if rf["traffic"] is not None:
+        rf["traffic_kernels"] = sorted(PMC_KIB[wk.name])
     rf["avg_launch_us"] = round(t_launch * 1e6, 3)
     stage_t = {}
     for st, name in ((0, "cv"), (1, "bias"), (2, "scatter")):
@@ -443,7 +492,9 @@ def main():
         "config": wk.config,
         "plain_evals_per_s": round(W / t_plain, 1),
         "batched_over_plain": round(t_plain / t_batched, 2),
+        "graph_evals_per_s": round(W / t_graph, 1),
         "eval_plus_deposit_evals_per_s": round(W / t_dep, 1),
+        "eval_plus_deposit_graph_evals_per_s": round(W / t_dep_graph, 1),
         "stage_us": stage_t,
         "roofline": rf,
     }
