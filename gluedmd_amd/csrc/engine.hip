@@ -316,6 +316,16 @@ void Engine::buildPlan(int nMass, const double* masses)
                         cs[k % 3] += (double)f;
                     }
                     for (int k = 0; k < 3; k++) it.refCentroid[k] = cs[k] / n;
+                    // the reference moments of the Kabsch covariance do not change
+                    // between evaluations: formed once here, in FP64
+                    for (int k = 0; k < 3; k++) it.refSum[k] = 0.0;
+                    it.refSq = 0.0;
+                    for (int j = 0; j < n; j++)
+                        for (int k = 0; k < 3; k++) {
+                            const double y = (double)(float)c.params[3 * j + k] - it.refCentroid[k];
+                            it.refSum[k] += y;
+                            it.refSq += y * y;
+                        }
                 }
                 group_.push_back(it);
                 break;
@@ -417,7 +427,12 @@ void Engine::buildPlan(int nMass, const double* masses)
     }
     dWalkers_.upload(wi);
     userToGpu_.assign(W_, std::vector<int>());
+    if (posOff_[W_ - 1] + padded_[W_ - 1] > (long long)INT32_MAX) fail("initialize: more than 2^31 atoms in total");
     dGidx_.alloc(sizeof(int) * (size_t)W_ * std::max(S_, 1));
+    groupSlots_ = groupSlots;
+    pathSlots_ = pathSlots;
+    dGroupPos_.alloc(sizeof(int) * (size_t)W_ * std::max((int)groupSlots_.size(), 1));
+    dPathPos_.alloc(sizeof(int) * (size_t)W_ * std::max((int)pathSlots_.size(), 1));
     for (int w = 0; w < W_; w++) uploadGidx(w);
     boxHost_.assign((size_t)W_ * 12, 0.0);
     for (int w = 0; w < W_; w++) {
@@ -449,11 +464,9 @@ void Engine::buildPlan(int nMass, const double* masses)
     }
     dGroup_.upload(group_);
     dGroupBig_.upload(groupBig_);
-    dGroupSlots_.upload(groupSlots);
     dGroupMass_.upload(groupMass);
     dGroupRef_.upload(groupRef);
     dPath_.upload(path_);
-    dPathSlots_.upload(pathSlots);
     dPathAux_.alloc(sizeof(double) * (size_t)W_ * std::max((int)path_.size(), 1) * 2);
     dPathMsd_.alloc(sizeof(double) * (size_t)W_ * std::max(msdStride_, 1));
     dPathFrames_.upload(pathFrames);
@@ -775,6 +788,17 @@ void Engine::uploadGidx(int w)
     hip_check(hipMemcpy(dGidx_.as<int>() + (size_t)w * std::max(S_, 1), g.data(), sizeof(int) * g.size(),
                         hipMemcpyHostToDevice),
               "upload atom indices");
+    // resolved gather tables: posq index of every group / path entry of this walker
+    auto resolve = [&](const std::vector<int>& slots, DBuf& dst) {
+        if (slots.empty()) return;
+        std::vector<int> p(slots.size());
+        for (size_t e = 0; e < slots.size(); e++) p[e] = (int)posOff_[w] + g[slots[e]];
+        hip_check(hipMemcpy(dst.as<int>() + (size_t)w * slots.size(), p.data(), sizeof(int) * p.size(),
+                            hipMemcpyHostToDevice),
+                  "upload gather table");
+    };
+    resolve(groupSlots_, dGroupPos_);
+    resolve(pathSlots_, dPathPos_);
 }
 
 // rebuildGpuAtomIndices, CommonCVKernels.cpp:113-317: user -> GPU order.
@@ -904,6 +928,8 @@ StepView Engine::view(const float4* posq, long long* force) const
     v.nPath = std::max((int)path_.size(), 1);
     v.pathMsd = dPathMsd_.as<double>();
     v.msdStride = std::max(msdStride_, 1);
+    v.gStride = (int)groupSlots_.size();
+    v.pStride = (int)pathSlots_.size();
     v.S = std::max(S_, 1);
     v.V = std::max(V_, 1);
     v.R = std::max(R_, 1);
@@ -930,27 +956,27 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         cp.nSmall = (int)small_.size();
         cp.group = dGroup_.as<GroupCVItem>();
         cp.nGroup = (int)group_.size();
-        cp.gslots = dGroupSlots_.as<int>();
+        cp.gslots = dGroupPos_.as<int>();
         cp.gmass = dGroupMass_.as<float>();
         cp.gref = dGroupRef_.as<float>();
         cp.path = dPath_.as<PathItem>();
         cp.nPath = (int)path_.size();
         cp.nPathMsdBlocks = pathMsdBlocks_;
-        cp.pslots = dPathSlots_.as<int>();
+        cp.pslots = dPathPos_.as<int>();
         cp.frames = dPathFrames_.as<float>();
         launch_cv_fused(v, cp, nw, s);
-        launch_cv_group_big(v, dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupSlots_.as<int>(),
+        launch_cv_group_big(v, dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupPos_.as<int>(),
                             dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
-        launch_path_rows(v, dPath_.as<PathItem>(), (int)path_.size(), pathRowBlocks_, dPathSlots_.as<int>(),
+        launch_path_rows(v, dPath_.as<PathItem>(), (int)path_.size(), pathRowBlocks_, dPathPos_.as<int>(),
                          dPathFrames_.as<float>(), w0, nw, s);
     } else {
         launch_cv_small(v, dSmall_.as<SmallCVItem>(), (int)small_.size(), w0, nw, s);
-        launch_cv_group(v, dGroup_.as<GroupCVItem>(), (int)group_.size(), dGroupSlots_.as<int>(),
+        launch_cv_group(v, dGroup_.as<GroupCVItem>(), (int)group_.size(), dGroupPos_.as<int>(),
                         dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
-        launch_cv_group_big(v, dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupSlots_.as<int>(),
+        launch_cv_group_big(v, dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupPos_.as<int>(),
                             dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
         launch_cv_path(v, dPath_.as<PathItem>(), (int)path_.size(), pathMsdBlocks_, pathRowBlocks_,
-                       dPathSlots_.as<int>(), dPathFrames_.as<float>(), w0, nw, s);
+                       dPathPos_.as<int>(), dPathFrames_.as<float>(), w0, nw, s);
     }
     if (!coord_.empty()) {
         CoordBuffers cb;

@@ -147,6 +147,7 @@ class Engine {
     std::vector<int> natoms_, padded_;
     std::vector<long long> posOff_, forceOff_;
     std::vector<int> slotAtom_;          // user atom of every slot
+    std::vector<int> groupSlots_, pathSlots_;  // slot of every group / path gather entry
     std::vector<std::vector<int>> userToGpu_;
     std::vector<double> boxHost_;        // [W][12]
     std::vector<int> rowSlot_, rowCv_;
@@ -204,11 +205,11 @@ class Engine {
 
     // device buffers
     DBuf dWalkers_, dGidx_, dBox_, dCv_, dJx_, dJy_, dJz_, dBgrad_, dBener_, dUd_;
-    DBuf dSmall_, dGroup_, dGroupBig_, dGroupSlots_, dGroupMass_, dGroupRef_;
+    DBuf dSmall_, dGroup_, dGroupBig_, dGroupPos_, dGroupMass_, dGroupRef_;
     DBuf dCoord_, dCoordSlots_, dCellOf_, dCellCount_, dCellStart_, dSorted_, dUnsorted_, dCnAcc_, dCnAccLo_, dPairAcc_,
         dNpairs_, dRank_, dBlockCnt_;
     int cellOfStride_ = 0, cellStride_ = 0, sortStride_ = 0, blkStride_ = 0;
-    DBuf dPath_, dPathSlots_, dPathFrames_, dPathW_;
+    DBuf dPath_, dPathPos_, dPathFrames_, dPathW_;
     DBuf dSmallBias_, dGridTable_, dSmallConst_, dSmallVar_, dSmallScal_;
     DBuf dSegStart_, dSegRow_ /* rowPos */, dRowCv_ /* entryCv */, dCombStart_, dCombSlot_, dTestGrad_, dEnergyU_, dEnergyFPtr_;
     DBuf dHillTable_, dOpesTable_, dFusedPart_, dFusedCounter_, dPathAux_, dAdjPart_, dPathMsd_;

@@ -77,6 +77,8 @@ struct StepView {
     double* pathMsd;       // [W][msdStride] per-frame MSD of every path
     int nPath;
     int msdStride;
+    int gStride;          // entries per walker of the resolved group / path gather tables
+    int pStride;
     int S, V, R, G, B;
     int pbc;
 };
@@ -91,7 +93,10 @@ struct SmallCVItem {
 };
 
 // Generic "group" CVs (gyration, COM distance, RMSD, DRMSD, contact map):
-// atoms / pairs are listed in slot space; one workgroup per (walker, item).
+// atoms / pairs are listed in entry space (begin..end); one workgroup per
+// (walker, item).  The gather table holds, per walker, the posq index of
+// every entry (walker offset + GPU index), rebuilt on every atom reorder, so
+// a gather is one dependent load (table) before the position load.
 struct GroupCVItem {
     int type;
     int begin, end;       // range into the slot list / pair list
@@ -100,6 +105,8 @@ struct GroupCVItem {
     int row0;
     int refOff;           // RMSD reference (float3 units) / DRMSD ref / CM params
     double refCentroid[3];
+    double refSum[3];     // Kabsch: sum of y - refCentroid over the reference (FP64, host)
+    double refSq;         // Kabsch: sum of |y - refCentroid|^2
 };
 
 // Coordination CV with optional cutoff (cell list when the box allows it).

@@ -298,24 +298,40 @@ __device__ void opes_chunk(const StepView& v, const OpesBiasDev& o, int w, int c
 // same bits).  The order depends only on n, so the plain and the batched forms
 // agree bit for bit.  ATOMIC: the partials were written by other workgroups of
 // the same launch (read with device-scope loads after the ticket).
+template <bool ATOMIC, int WD>
+__device__ inline void reduce_partials_w(const double* src, int n, int ld, double* tot)
+{
+    const int lane = threadIdx.x & 63;
+    double a[WD];
+#pragma unroll
+    for (int d = 0; d < WD; d++) a[d] = 0.0;
+    for (int j = lane; j < n; j += 64) {
+#pragma unroll
+        for (int d = 0; d < WD; d++) {
+            const double* p = src + (size_t)j * ld + d;
+            a[d] += ATOMIC ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
+        }
+    }
+    wave_sum_n<WD>(a);
+#pragma unroll
+    for (int d = 0; d < PW; d++) tot[d] = d < WD ? a[d < WD ? d : 0] : 0.0;
+}
+
+// The butterfly covers exactly the wd = D + 1 values of the bias.
 template <bool ATOMIC>
 __device__ inline void reduce_partials(const double* src, int n, int wd, int ld, double* tot)
 {
-    const int lane = threadIdx.x & 63;
-    double a[PW];
-#pragma unroll
-    for (int d = 0; d < PW; d++) a[d] = 0.0;
-    for (int j = lane; j < n; j += 64) {
-#pragma unroll
-        for (int d = 0; d < PW; d++)
-            if (d < wd) {
-                const double* p = src + (size_t)j * ld + d;
-                a[d] += ATOMIC ? __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *p;
-            }
+    switch (wd) {
+        case 1: reduce_partials_w<ATOMIC, 1>(src, n, ld, tot); break;
+        case 2: reduce_partials_w<ATOMIC, 2>(src, n, ld, tot); break;
+        case 3: reduce_partials_w<ATOMIC, 3>(src, n, ld, tot); break;
+        case 4: reduce_partials_w<ATOMIC, 4>(src, n, ld, tot); break;
+        case 5: reduce_partials_w<ATOMIC, 5>(src, n, ld, tot); break;
+        case 6: reduce_partials_w<ATOMIC, 6>(src, n, ld, tot); break;
+        case 7: reduce_partials_w<ATOMIC, 7>(src, n, ld, tot); break;
+        case 8: reduce_partials_w<ATOMIC, 8>(src, n, ld, tot); break;
+        default: reduce_partials_w<ATOMIC, PW>(src, n, ld, tot); break;
     }
-    wave_sum_n<PW>(a);
-#pragma unroll
-    for (int d = 0; d < PW; d++) tot[d] = a[d];
 }
 
 __device__ inline void hill_finalize(const StepView& v, const HillBiasDev& h, int w, const double* tot,

@@ -63,13 +63,20 @@ __device__ inline double wave_sum(double v) { return wave_sum1(v); }
 template <int NV, int NW = 4>
 __device__ inline void block_sum(double (&v)[NV], double* sh)
 {
-    static_assert(NW >= 1 && NW <= 16 && NV <= 64, "1..16 waves, at most 64 values");
+    static_assert(NW >= 1 && NW <= 16 && NV <= 16, "1..16 waves, at most 16 values");
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    wave_sum_n<NV>(v);
-    __syncthreads();
-    if (lane == 0)
+    if constexpr (NV <= 2) {
+        wave_sum_n<NV>(v);
+        __syncthreads();
+        if (lane == 0)
 #pragma unroll
-        for (int k = 0; k < NV; k++) sh[k * 16 + wid] = v[k];
+            for (int k = 0; k < NV; k++) sh[k * 16 + wid] = v[k];
+    } else {
+        // lane k of every wave holds the wave total of value k
+        const double t = wave_reduce_scatter<NV>(v);
+        __syncthreads();
+        if (lane < NV) sh[lane * 16 + wid] = t;
+    }
     __syncthreads();
     if (threadIdx.x < NV) {
         double t = 0.0;
@@ -88,6 +95,12 @@ __device__ inline float4 load_pos(const StepView& v, int w, int slot)
     const WalkerInfo wi = v.walkers[w];
     int g = v.gidx[(size_t)w * v.S + slot];
     return v.posq[wi.posOff + g];
+}
+
+// Position of entry e of walker w's gather table ([W][stride] posq indices).
+__device__ inline float4 gather_pos(const StepView& v, const int* __restrict__ table, int stride, int w, int e)
+{
+    return v.posq[table[(size_t)w * stride + e]];
 }
 
 __device__ inline void store_row(const StepView& v, int w, int row, double gx, double gy, double gz)
@@ -354,19 +367,20 @@ __device__ inline double ipowd(double x, int n)
 
 // Kabsch RMSD of up to 256 * KABSCH_PER_THREAD atoms by one 256-thread
 // workgroup: every thread keeps its atoms and their reference in registers,
-// a wave butterfly and one LDS exchange give every thread the raw moments
-//   m[3a+b] = sum y'_a x_b, m[9..11] = sum x, m[12] = sum |x|^2,
-//   m[13..15] = sum y', m[16] = sum |y'|^2   (y' = y - ybar),
+// a wave reduce-scatter and one LDS exchange give every thread the raw moments
+//   m[3a+b] = sum y'_a x_b, m[9..11] = sum x, m[12] = sum |x|^2   (y' = y - ybar),
 // from which the centred covariance sum y'(x - xbar)^T = m[3a+b] - (sum y'_a) xbar_b
-// and sum |x - xbar|^2 = sum |x|^2 - n |xbar|^2 (FP64).  Every thread then
+// and sum |x - xbar|^2 = sum |x|^2 - n |xbar|^2 (FP64); sum y' and sum |y'|^2
+// are constants of the reference, formed on the host.  Every thread then
 // runs the same Horn solve on the same bits (no broadcast) and writes its rows.
 constexpr int KABSCH_PER_THREAD = 4;
+constexpr int KABSCH_MOMENTS = 13;  // sum y'x^T (9), sum x (3), sum |x|^2; the reference sums come from the host
 
 template <int NT>
 __device__ void kabsch_block(const StepView& v, const GroupCVItem& it, const int* __restrict__ slots,
                              const float* __restrict__ ref, int w)
 {
-    __shared__ double shK[NT / 64][17];
+    __shared__ double shK[NT / 64][16];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const int n = it.end - it.begin;
     const float* Y = ref + 3 * (size_t)it.refOff;
@@ -378,7 +392,7 @@ __device__ void kabsch_block(const StepView& v, const GroupCVItem& it, const int
 #pragma unroll
     for (int r = 0; r < KABSCH_PER_THREAD; r++) {
         const int j = min(tid + NT * r, n - 1);
-        const float4 p = load_pos(v, w, slots[it.begin + j]);
+        const float4 p = gather_pos(v, slots, v.gStride, w, it.begin + j);
         qx[r][0] = p.x;
         qx[r][1] = p.y;
         qx[r][2] = p.z;
@@ -386,9 +400,9 @@ __device__ void kabsch_block(const StepView& v, const GroupCVItem& it, const int
         for (int k = 0; k < 3; k++) qy[r][k] = Y[3 * j + k];
         rp[r] = v.rowPos[it.row0 + j];
     }
-    double m[17];
+    double m[KABSCH_MOMENTS];
 #pragma unroll
-    for (int k = 0; k < 17; k++) m[k] = 0.0;
+    for (int k = 0; k < KABSCH_MOMENTS; k++) m[k] = 0.0;
 #pragma unroll
     for (int r = 0; r < KABSCH_PER_THREAD; r++) {
         if (tid + NT * r >= n) break;
@@ -402,17 +416,15 @@ __device__ void kabsch_block(const StepView& v, const GroupCVItem& it, const int
         for (int k = 0; k < 3; k++) {
             m[9 + k] += x[k];
             m[12] += x[k] * x[k];
-            m[13 + k] += y[k];
-            m[16] += y[k] * y[k];
         }
     }
-    wave_sum_n<17>(m);
-    if (lane == 0)
-#pragma unroll
-        for (int k = 0; k < 17; k++) shK[wid][k] = m[k];
+    // lane k of each wave holds the wave total of moment k; the waves' totals
+    // meet in LDS and every thread adds them in wave order
+    const double tw = wave_reduce_scatter<KABSCH_MOMENTS>(m);
+    if (lane < KABSCH_MOMENTS) shK[wid][lane] = tw;
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 17; k++) {
+    for (int k = 0; k < KABSCH_MOMENTS; k++) {
         double t = 0.0;
 #pragma unroll
         for (int q = 0; q < NT / 64; q++) t += shK[q][k];
@@ -423,10 +435,10 @@ __device__ void kabsch_block(const StepView& v, const GroupCVItem& it, const int
 #pragma unroll
     for (int a = 0; a < 3; a++)
 #pragma unroll
-        for (int b = 0; b < 3; b++) S[3 * a + b] = m[3 * a + b] - m[13 + a] * xc[b];
+        for (int b = 0; b < 3; b++) S[3 * a + b] = m[3 * a + b] - it.refSum[a] * xc[b];
     const double Gx = m[12] - n * (xc[0] * xc[0] + xc[1] * xc[1] + xc[2] * xc[2]);
-    horn_rotation(S, Gx, m[16], R, &l);
-    double msd = (Gx + m[16] - 2.0 * l) / n;
+    horn_rotation(S, Gx, it.refSq, R, &l);
+    double msd = (Gx + it.refSq - 2.0 * l) / n;
     if (msd < 0.0) msd = 0.0;
     const double rv = sqrt(msd);
     const double sc = (rv > 1e-10 ? 1.0 / rv : 0.0) / n;
@@ -467,7 +479,7 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
 #pragma unroll
         for (int r = 0; r < REG; r++) {  // clamped index: no divergent loads
             const int j = min(tid + r * nt, n - 1);
-            q[r] = load_pos(v, w, slots[it.begin + j]);
+            q[r] = gather_pos(v, slots, v.gStride, w, it.begin + j);
             mj[r] = mass[it.begin + j];
         }
         // every atom of this thread: registers first (static indices), then
@@ -479,7 +491,7 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
                 if (j < n) f(j, q[r], mj[r]);
             }
             for (int j = tid + REG * nt; j < n; j += nt)
-                f(j, load_pos(v, w, slots[it.begin + j]), (double)mass[it.begin + j]);
+                f(j, gather_pos(v, slots, v.gStride, w, it.begin + j), (double)mass[it.begin + j]);
         };
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // m*x (g1), M1, m*x (g2), M2
         visit([&](int j, const float4& qq, double m) {
@@ -553,7 +565,7 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
 #pragma unroll
         for (int r = 0; r < REG; r++) {  // clamped index: no divergent loads
             const int j = min(tid + r * nt, n - 1);
-            q[r] = load_pos(v, w, slots[it.begin + j]);
+            q[r] = gather_pos(v, slots, v.gStride, w, it.begin + j);
 #pragma unroll
             for (int k = 0; k < 3; k++) yr[r][k] = Y[3 * j + k];
         }
@@ -572,7 +584,7 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
             for (int j = tid + REG * nt; j < n; j += nt) {
                 const double yy[3] = {(double)Y[3 * j] - (fit ? yc[0] : 0.0), (double)Y[3 * j + 1] - (fit ? yc[1] : 0.0),
                                       (double)Y[3 * j + 2] - (fit ? yc[2] : 0.0)};
-                f(j, load_pos(v, w, slots[it.begin + j]), yy);
+                f(j, gather_pos(v, slots, v.gStride, w, it.begin + j), yy);
             }
         };
         if (!fit) {
@@ -593,13 +605,13 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
         }
         if constexpr (NT > GROUP_BLOCK) return;
         // Kabsch: one block reduction of the raw moments
-        //   m[3a+b] = sum y'_a x_b, m[9..11] = sum x, m[12] = sum |x|^2,
-        //   m[13..15] = sum y', m[16] = sum |y'|^2   (y' = y - ybar),
+        //   m[3a+b] = sum y'_a x_b, m[9..11] = sum x, m[12] = sum |x|^2   (y' = y - ybar),
         // then the centred covariance sum y'(x - xbar)^T = m[3a+b] - (sum y'_a) xbar_b
-        // and sum |x - xbar|^2 = sum |x|^2 - n |xbar|^2, both in FP64.
-        double m[17];
+        // and sum |x - xbar|^2 = sum |x|^2 - n |xbar|^2, both in FP64; sum y' and
+        // sum |y'|^2 are the host's (the reference does not move).
+        double m[KABSCH_MOMENTS];
 #pragma unroll
-        for (int k = 0; k < 17; k++) m[k] = 0.0;
+        for (int k = 0; k < KABSCH_MOMENTS; k++) m[k] = 0.0;
         visit([&](int j, const float4& qq, const double* y) {
             const double x[3] = {(double)qq.x, (double)qq.y, (double)qq.z};
 #pragma unroll
@@ -610,11 +622,9 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
             for (int k = 0; k < 3; k++) {
                 m[9 + k] += x[k];
                 m[12] += x[k] * x[k];
-                m[13 + k] += y[k];
-                m[16] += y[k] * y[k];
             }
         });
-        block_sum<17, NT / 64>(m, sh);
+        block_sum<KABSCH_MOMENTS, NT / 64>(m, sh);
         const double xc[3] = {m[9] / n, m[10] / n, m[11] / n};
         __shared__ double shScale;
         if (tid == 0) {
@@ -622,10 +632,10 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
 #pragma unroll
             for (int a = 0; a < 3; a++)
 #pragma unroll
-                for (int b = 0; b < 3; b++) S[3 * a + b] = m[3 * a + b] - m[13 + a] * xc[b];
+                for (int b = 0; b < 3; b++) S[3 * a + b] = m[3 * a + b] - it.refSum[a] * xc[b];
             const double Gx = m[12] - n * (xc[0] * xc[0] + xc[1] * xc[1] + xc[2] * xc[2]);
-            horn_rotation(S, Gx, m[16], R, &l);
-            double msd = (Gx + m[16] - 2.0 * l) / n;
+            horn_rotation(S, Gx, it.refSq, R, &l);
+            double msd = (Gx + it.refSq - 2.0 * l) / n;
             if (msd < 0.0) msd = 0.0;
             const double rv = sqrt(msd);
             for (int k = 0; k < 9; k++) shR[k] = R[k];
@@ -649,8 +659,8 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
         const float* d0 = ref + it.refOff;
         double s[1] = {0.0};
         for (int p = tid; p < np; p += nt) {
-            float4 a = load_pos(v, w, slots[it.begin + 2 * p]);
-            float4 b = load_pos(v, w, slots[it.begin + 2 * p + 1]);
+            float4 a = gather_pos(v, slots, v.gStride, w, it.begin + 2 * p);
+            float4 b = gather_pos(v, slots, v.gStride, w, it.begin + 2 * p + 1);
             double dx = (double)b.x - a.x, dy = (double)b.y - a.y, dz = (double)b.z - a.z;
             if (v.pbc) pbc_exact(dx, dy, dz, bx);
             double dev = sqrt(dx * dx + dy * dy + dz * dz) - (double)d0[p];
@@ -660,8 +670,8 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
         const double val = sqrt(s[0] / np);
         const double inv = val > 1e-10 ? 1.0 / val : 0.0;
         for (int p = tid; p < np; p += nt) {
-            float4 a = load_pos(v, w, slots[it.begin + 2 * p]);
-            float4 b = load_pos(v, w, slots[it.begin + 2 * p + 1]);
+            float4 a = gather_pos(v, slots, v.gStride, w, it.begin + 2 * p);
+            float4 b = gather_pos(v, slots, v.gStride, w, it.begin + 2 * p + 1);
             double dx = (double)b.x - a.x, dy = (double)b.y - a.y, dz = (double)b.z - a.z;
             if (v.pbc) pbc_exact(dx, dy, dz, bx);
             double d = sqrt(dx * dx + dy * dy + dz * dz);
@@ -679,8 +689,8 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
         const float* prm = ref + it.refOff;
         double s[1] = {0.0};
         for (int p = tid; p < np; p += nt) {
-            float4 a = load_pos(v, w, slots[it.begin + 2 * p]);
-            float4 b = load_pos(v, w, slots[it.begin + 2 * p + 1]);
+            float4 a = gather_pos(v, slots, v.gStride, w, it.begin + 2 * p);
+            float4 b = gather_pos(v, slots, v.gStride, w, it.begin + 2 * p + 1);
             double dx = (double)b.x - a.x, dy = (double)b.y - a.y, dz = (double)b.z - a.z;
             if (v.pbc) pbc_exact(dx, dy, dz, bx);
             const double r0 = prm[4 * p], wgt = prm[4 * p + 3];
@@ -1282,7 +1292,7 @@ __device__ void path_msd_block(const StepView& v, const PathItem& it, const int*
         const int mc = min(PATH_LDS_ATOMS, M - c0);
         __syncthreads();
         for (int j = threadIdx.x; j < mc; j += blockDim.x) {
-            const float4 q = load_pos(v, w, slots[it.begin + c0 + j]);
+            const float4 q = gather_pos(v, slots, v.pStride, w, it.begin + c0 + j);
             shX[3 * j] = q.x;
             shX[3 * j + 1] = q.y;
             shX[3 * j + 2] = q.z;
@@ -1364,7 +1374,7 @@ __global__ __launch_bounds__(64 * PATH_PHASES) void k_path_rows(StepView v, cons
     const int j = tile * PATH_TILE + lane;
     double js0 = 0, js1 = 0, js2 = 0, jz0 = 0, jz1 = 0, jz2 = 0;
     if (j < M) {
-        const float4 q = load_pos(v, w, slots[it.begin + j]);
+        const float4 q = gather_pos(v, slots, v.pStride, w, it.begin + j);
         const float* F = frames + it.frameOff + (size_t)j * 3;
         for (int f = ph; f < N; f += PATH_PHASES) {
             const float* r = F + (size_t)f * M * 3;

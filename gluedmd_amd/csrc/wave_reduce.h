@@ -51,6 +51,42 @@ __device__ inline void wave_sum_n(double (&v)[NV])
     }
 }
 
+// Reduce-scatter of up to 16 FP64 sums over a wave.  Four DPP steps inside
+// each 16-lane row halve the values a lane carries (row mirror, half
+// mirror, quad xor 2, quad xor 1: the partners of every step agree on the
+// lane bits earlier steps split on, so they carry the same value subset),
+// then two xor shuffles add across rows.  Afterwards every lane l holds the
+// wave total of value (l & 15); values NV..15 are zero padding.  Each total
+// is formed by the same additions whatever the data.
+template <int CTRL, int H>
+__device__ inline void rs_step(double (&v)[16], bool upper)
+{
+#pragma unroll
+    for (int j = 0; j < H; j++) {
+        const double send = upper ? v[j] : v[j + H];
+        const double keep = upper ? v[j + H] : v[j];
+        v[j] = keep + dpp_f64<CTRL>(send);
+    }
+}
+
+template <int NV>
+__device__ inline double wave_reduce_scatter(const double (&x)[NV])
+{
+    static_assert(NV >= 1 && NV <= 16, "at most 16 values");
+    const int lane = threadIdx.x & 63;
+    double v[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = k < NV ? x[k < NV ? k : 0] : 0.0;
+    rs_step<DPP_ROW_MIRROR, 8>(v, lane & 8);
+    rs_step<DPP_ROW_HALF_MIRROR, 4>(v, lane & 4);
+    rs_step<DPP_QUAD_XOR2, 2>(v, lane & 2);
+    rs_step<DPP_QUAD_XOR1, 1>(v, lane & 1);
+    double t = v[0];
+    t += __shfl_xor(t, 16, 64);
+    t += __shfl_xor(t, 32, 64);
+    return t;
+}
+
 __device__ inline double wave_sum1(double x)
 {
     double v[1] = {x};
