@@ -552,6 +552,7 @@ void Engine::buildPlan(int nMass, const double* masses)
             it.gslot = gslot;
             it.eslot = (int)b;
             it.bias = (int)gridTable.size();
+            it.grid = d;
             gridTable.push_back(d);
             grids_.push_back(g);
             smallBias_.push_back(it);

@@ -202,6 +202,7 @@ struct SmallBiasItem {
     int nState;               // OPES_EXPANDED / MULTITHERMAL states
     int stateOff;             // offset into the state arrays
     double p[8];              // type-specific scalars (k, at, kT0, invKT, ...)
+    GridBiasDev grid;         // METAD: a copy of the grid descriptor (one load less before the CV reads)
 };
 
 struct SmallBiasState {

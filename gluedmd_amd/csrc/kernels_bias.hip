@@ -115,7 +115,7 @@ __device__ void small_bias_eval(const StepView& v, const SmallBiasState& st, con
     double bgv[MAX_BIAS_D];
     int nOut = 0;
     if (it.type == BIAS_METAD) {
-        const GridBiasDev& g = st.grids[it.bias];
+        const GridBiasDev& g = it.grid;
         grid_eval(g, g.grid + (size_t)w * g.walkerStride, cv, be, bgv);
         nOut = g.D;
         for (int d = 0; d < g.D; d++) bg[d] = bgv[d];
@@ -503,15 +503,19 @@ static void launch_chunks(const StepView& v, const FusedBiasPlan& fp, int nw, hi
     else hipLaunchKernelGGL((k_step_bias_chunks<0, 1>), g, b, 0, s, v, fp);
 }
 
-// One workgroup per walker.  Everything that does not depend on the biases
-// is loaded first and kept in registers (each thread's scatter slots: row
-// range, first row's CV and Jacobian, GPU atom index) or LDS (the dU/dCV
-// combine table).  Meanwhile wave 0 reduces the chunk partials (written by
-// the previous launch) and finalizes the chunked biases, and the other waves
-// evaluate the one-thread biases; both write their gradient slots to global
+// One workgroup per walker; each wave has one chain of dependent loads, so
+// the slowest chain, not their sum, sets the time to the barrier:
+//   wave 0     reduces the chunk partials (written by the previous launch)
+//              and finalizes the chunked biases;
+//   wave 1     evaluates the one-thread biases (item, CV values, grid
+//              corners: three dependent loads);
+//   waves 2-3  copy the dU/dCV combine table into LDS.
+// Waves 0, 2 and 3 also load their scatter slots into registers (row range,
+// first row's CV and Jacobian, GPU atom index).  Gradient slots go to global
 // memory and to LDS.  After one barrier dU/dCV is formed from LDS and the
 // scatter needs only arithmetic and the fixed-point atomics.
-constexpr int FIN_SLOTS = 2;           // scatter slots per thread held in registers
+constexpr int FIN_SLOTS = 3;           // scatter slots per slot-owning thread held in registers
+constexpr int FIN_OWNERS = 3 * 64;     // slot-owning threads (waves 0, 2, 3)
 constexpr int FIN_LDS_G = 512;         // bias-gradient slots held in LDS
 constexpr int FIN_LDS_COMB = 1024;     // combine entries held in LDS
 
@@ -526,14 +530,17 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, F
     const bool ldsComb = v.V <= MAX_LDS_VALUES && v.G <= FIN_LDS_G && sp.nComb <= FIN_LDS_COMB && !sp.testGrad;
     double* bgL = ldsComb ? sBg : nullptr;
     // bias-independent loads
+    const int wave = tid >> 6;
+    const int owner = wave == 0 ? tid : tid - 64;  // slot owner index (waves 0, 2, 3)
     const bool doScatter = fp.scatter != 0 && v.S > 0 && v.R > 0;
+    const bool ownsSlots = doScatter && wave != 1;
     int sb[FIN_SLOTS], se[FIN_SLOTS], sc0[FIN_SLOTS], sg[FIN_SLOTS];
     float sjx[FIN_SLOTS], sjy[FIN_SLOTS], sjz[FIN_SLOTS];
     const size_t rbase = (size_t)w * v.R;
-    if (doScatter) {
+    if (ownsSlots) {
 #pragma unroll
         for (int r = 0; r < FIN_SLOTS; r++) {  // clamped slot: no divergent loads
-            const int s = min(tid + r * BIAS_BLOCK, v.S - 1);
+            const int s = min(owner + r * FIN_OWNERS, v.S - 1);
             sb[r] = sp.segStart[s];
             se[r] = sp.segStart[s + 1];
             sg[r] = v.gidx[(size_t)w * v.S + s];
@@ -547,11 +554,11 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, F
             sjz[r] = v.jz[rbase + k];
         }
     }
-    if (ldsComb) {
-        for (int c = tid; c <= v.V; c += BIAS_BLOCK) sCombStart[c] = sp.combStart[c];
-        for (int k = tid; k < sp.nComb; k += BIAS_BLOCK) sCombSlot[k] = sp.combSlot[k];
+    if (ldsComb && wave >= 2) {
+        for (int c = tid - 128; c <= v.V; c += 128) sCombStart[c] = sp.combStart[c];
+        for (int k = tid - 128; k < sp.nComb; k += 128) sCombSlot[k] = sp.combSlot[k];
     }
-    if (tid < 64) {
+    if (wave == 0) {
         for (int k = 0; k < fp.nChunked; k++) {
             const int c0 = fp.chunkStart[k], nc = fp.chunkStart[k + 1] - c0;
             const double* src = fp.part + ((size_t)w * fp.totalChunks + c0) * PW;
@@ -565,8 +572,8 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, F
                     opes_finalize(v, fp.opes[k - fp.nHill], w, tot, bgL);
             }
         }
-    } else {
-        for (int i = tid - 64; i < fp.nSmall; i += BIAS_BLOCK - 64) small_bias_eval(v, fp.st, fp.small[i], i, w, bgL);
+    } else if (wave == 1) {
+        for (int i = tid - 64; i < fp.nSmall; i += 64) small_bias_eval(v, fp.st, fp.small[i], i, w, bgL);
     }
     __syncthreads();
     if (!ldsComb) {
@@ -587,26 +594,28 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, F
     if (!doScatter) return;
     const WalkerInfo wi = v.walkers[w];
     long long* F = v.force + wi.forceOff;
+    if (ownsSlots) {
 #pragma unroll
-    for (int r = 0; r < FIN_SLOTS; r++) {
-        const int s = tid + r * BIAS_BLOCK;
-        if (s >= v.S) break;
-        // first row from registers, further rows of this atom (rare) from memory,
-        // in the order of scatter_slot
-        if (sb[r] == se[r]) continue;
-        const double u0 = sU[sc0[r]];
-        double fx = -(u0 * (double)sjx[r]), fy = -(u0 * (double)sjy[r]), fz = -(u0 * (double)sjz[r]);
-        for (int k = sb[r] + 1; k < se[r]; k++) {
-            const double u = sU[sp.entryCv[k]];
-            fx -= u * (double)v.jx[rbase + k];
-            fy -= u * (double)v.jy[rbase + k];
-            fz -= u * (double)v.jz[rbase + k];
+        for (int r = 0; r < FIN_SLOTS; r++) {
+            const int s = owner + r * FIN_OWNERS;
+            if (s >= v.S) break;
+            // first row from registers, further rows of this atom (rare) from memory,
+            // in the order of scatter_slot
+            if (sb[r] == se[r]) continue;
+            const double u0 = sU[sc0[r]];
+            double fx = -(u0 * (double)sjx[r]), fy = -(u0 * (double)sjy[r]), fz = -(u0 * (double)sjz[r]);
+            for (int k = sb[r] + 1; k < se[r]; k++) {
+                const double u = sU[sp.entryCv[k]];
+                fx -= u * (double)v.jx[rbase + k];
+                fy -= u * (double)v.jy[rbase + k];
+                fz -= u * (double)v.jz[rbase + k];
+            }
+            atomic_add_fixed(F + sg[r], fx);
+            atomic_add_fixed(F + sg[r] + wi.padded, fy);
+            atomic_add_fixed(F + sg[r] + 2 * (size_t)wi.padded, fz);
         }
-        atomic_add_fixed(F + sg[r], fx);
-        atomic_add_fixed(F + sg[r] + wi.padded, fy);
-        atomic_add_fixed(F + sg[r] + 2 * (size_t)wi.padded, fz);
     }
-    for (int s = tid + FIN_SLOTS * BIAS_BLOCK; s < v.S; s += BIAS_BLOCK) scatter_slot(v, sp, w, s, sU, MAX_LDS_VALUES);
+    for (int s = tid + FIN_SLOTS * FIN_OWNERS; s < v.S; s += BIAS_BLOCK) scatter_slot(v, sp, w, s, sU, MAX_LDS_VALUES);
 }
 
 void launch_step_bias_chunks(const StepView& v, const FusedBiasPlan& fp, int nw, hipStream_t s)

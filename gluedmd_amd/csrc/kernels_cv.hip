@@ -297,7 +297,10 @@ __device__ void horn_rotation(const double* S, double Gx, double Gy, double* R, 
         double f = (l2 + e2) * l2 - e3 * l + e4;
         double df = 4.0 * l2 * l + 2.0 * e2 * l - e3;
         if (!(fabs(df) > 0.0)) break;
-        double dl = f / df;
+        // approximate reciprocal (v_rcp_f64): Newton converges to the same
+        // root with any slope estimate this close; only the rate of the last
+        // step changes, from quadratic to a factor ~1e-7 per step
+        double dl = f * __builtin_amdgcn_rcp(df);
         l -= dl;
         // quadratic convergence: after a step of relative size 1e-10 the
         // remaining error is ~1e-20 relative for separated eigenvalues
