@@ -382,6 +382,14 @@ void Engine::buildPlan(int nMass, const double* masses)
                              it.rn <= 64 && it.rm <= 64);
                 const double rc = c.params.size() > 3 ? c.params[3] : 0.0;
                 it.rc2 = rc > 0.0 ? rc * rc : 0.0;
+                {
+                    // CommonGluedKernels.cpp:979-1007 (Taylor branch constants)
+                    const double r0 = it.r0, rn = it.rn, rm = it.rm;
+                    it.ir0 = 1.0 / r0;
+                    it.preRes = rn / rm;
+                    it.preDfunc = 0.5 * rn * (rn - rm) / rm;
+                    it.preSecDev = rn * (rm * rm - 3.0 * rm * (rn - 1.0) + rn * (2.0 * rn - 3.0)) / (6.0 * rm);
+                }
                 coord_.push_back(it);
                 break;
             }

@@ -116,6 +116,9 @@ struct CoordItem {
     float r0, rn, rm;
     int intPow;           // 1 when n, m are small integers (fast integer powers)
     double rc2;           // cutoff^2 (0 = no cutoff)
+    // switching constants formed once on the host (FP64, unfused):
+    // 1/r0 and the Taylor coefficients used for |r/r0 - 1| < 1e-4
+    double ir0, preRes, preDfunc, preSecDev;
     int value;
     int rowA, rowB;
     int useCells;         // cell list or brute force

@@ -961,10 +961,10 @@ __device__ inline SwitchD make_switch(const CoordItem& it)
     s.r0 = it.r0;
     s.rn = it.rn;
     s.rm = it.rm;
-    s.ir0 = 1.0 / s.r0;
-    s.preRes = s.rn / s.rm;
-    s.preDfunc = 0.5 * s.rn * (s.rn - s.rm) / s.rm;
-    s.preSecDev = s.rn * (s.rm * s.rm - 3.0 * s.rm * (s.rn - 1.0) + s.rn * (2.0 * s.rn - 3.0)) / (6.0 * s.rm);
+    s.ir0 = it.ir0;
+    s.preRes = it.preRes;
+    s.preDfunc = it.preDfunc;
+    s.preSecDev = it.preSecDev;
     s.intPow = it.intPow;
     s.in = (int)s.rn;
     s.im = (int)s.rm;
@@ -1069,9 +1069,20 @@ __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBu
     const SwitchD sw = make_switch(it);
     const float rc2f = (float)(it.rc2 * (1.0 + 1e-5));
 
-    // home positions of this wave (both halves), this lane's partner range
-    const int hslot = cb.slots[SIDE_A ? it.aBegin + min(home, nHome - 1) : it.bBegin + min(home, nHome - 1)];
-    const float4 ph = load_pos(v, w, hslot);
+    // home positions of this wave (both halves), this lane's partner range.
+    // With cells the homes are taken in cell order: the sorted entry holds the
+    // position and the atom's index in its group, and its cell is recomputed
+    // as the sort computed it.
+    float4 ph;
+    int homeIdx = home;
+    if (CELLS) {
+        const float4 hq =
+            cb.sorted[(size_t)w * cb.sortStride + it.sortBase + (SIDE_A ? 0 : it.nA) + min(home, nHome - 1)];
+        ph = hq;
+        homeIdx = __float_as_int(hq.w);
+    } else {
+        ph = load_pos(v, w, cb.slots[SIDE_A ? it.aBegin + min(home, nHome - 1) : it.bBegin + min(home, nHome - 1)]);
+    }
     float4 hp[HPW];
 #pragma unroll
     for (int k = 0; k < HPW; k++) {
@@ -1084,8 +1095,8 @@ __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBu
         const int cell = l / LPC, sub = l % LPC;
         if (homeOk && cell < 27) {
             const int ncell = it.nc[0] * it.nc[1] * it.nc[2];
-            const int hc = cb.cellOf[(size_t)w * cb.cellOfStride + it.sortBase + (SIDE_A ? home : it.nA + home)] -
-                           (SIDE_A ? 0 : ncell);
+            const float4 hme = h == 0 ? hp[0] : hp[HPW - 1];
+            const int hc = cell_of(hme.x, hme.y, hme.z, bx, it.nc);
             int cx = hc % it.nc[0], cy = (hc / it.nc[0]) % it.nc[1], cz = hc / (it.nc[0] * it.nc[1]);
             cx = (cx + (cell % 3) - 1 + it.nc[0]) % it.nc[0];
             cy = (cy + ((cell / 3) % 3) - 1 + it.nc[1]) % it.nc[1];
@@ -1203,7 +1214,7 @@ __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBu
         const int k = h;
         const long long sx = k == 0 ? gx[0] : gx[HPW - 1], sy = k == 0 ? gy[0] : gy[HPW - 1],
                         sz = k == 0 ? gz[0] : gz[HPW - 1];
-        store_row(v, w, (SIDE_A ? it.rowA : it.rowB) + home, (double)sx * COORD_UNFIX, (double)sy * COORD_UNFIX,
+        store_row(v, w, (SIDE_A ? it.rowA : it.rowB) + homeIdx, (double)sx * COORD_UNFIX, (double)sy * COORD_UNFIX,
                   (double)sz * COORD_UNFIX);
     }
     if (SIDE_A) {
@@ -1225,6 +1236,159 @@ __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBu
             atomicAdd((unsigned long long*)&cb.cnAcc[k], (unsigned long long)H);
             atomicAdd(&cb.pairAcc[k], np);
         }
+    }
+}
+
+// Pass A over the cell list with one 192-thread workgroup per A atom: wave q
+// takes the 3x3 neighbour cells of the slab dz = q - 1, seven lanes per cell,
+// so each wave walks a third of the candidates of k_coord_pairs<true, true>
+// (shorter dependent chains, three times the waves in flight).  The waves'
+// gradient, CN and pair-count sums are integers, added in LDS: the results
+// equal those of the one-wave-per-atom form bit for bit.
+constexpr int COORD_A_BLOCK = 192;
+constexpr int COORD_A_LPC = 7;  // lanes per cell: 9 cells x 7 = 63 lanes
+constexpr int COORD_A_PF = 4;   // partner loads in flight per lane (register budget: 4 waves per SIMD)
+
+__global__ __launch_bounds__(COORD_A_BLOCK) void k_coord_pairs_a(StepView v, CoordBuffers cb,
+                                                                 const CoordItem* __restrict__ items, int item, int w0)
+{
+    __shared__ float4 qPos[COORD_A_BLOCK / 64][128];
+    __shared__ long long sG[COORD_A_BLOCK / 64][3];
+    __shared__ unsigned long long sLo[COORD_A_BLOCK / 64], sNp[COORD_A_BLOCK / 64];
+    __shared__ long long sHi[COORD_A_BLOCK / 64];
+    const int w = w0 + blockIdx.y;
+    const CoordItem it = items[item];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const double* bx = v.box + (size_t)w * 12;
+    float bxf[12];
+    for (int k = 0; k < 12; k++) bxf[k] = (float)bx[k];
+    const SwitchD sw = make_switch(it);
+    const float rc2f = (float)(it.rc2 * (1.0 + 1e-5));
+    // homes in cell order: the sorted entry holds the position and the atom's
+    // index in group A; its cell is recomputed as the sort computed it
+    const float4 hq = cb.sorted[(size_t)w * cb.sortStride + it.sortBase + blockIdx.x];  // grid.x = nA
+    const float4 hp = make_float4(hq.x, hq.y, hq.z, 0.f);
+    const int home = __float_as_int(hq.w);
+    const float4* part = cb.sorted + (size_t)w * cb.sortStride + it.sortBase + it.nA;
+    int pBegin = 0, pCount = 0;
+    {
+        const int ci = lane / COORD_A_LPC, sub = lane % COORD_A_LPC;
+        if (ci < 9) {
+            const int ncell = it.nc[0] * it.nc[1] * it.nc[2];
+            const int hc = cell_of(hq.x, hq.y, hq.z, bx, it.nc);
+            int cx = hc % it.nc[0], cy = (hc / it.nc[0]) % it.nc[1], cz = hc / (it.nc[0] * it.nc[1]);
+            cx = (cx + (ci % 3) - 1 + it.nc[0]) % it.nc[0];
+            cy = (cy + (ci / 3) - 1 + it.nc[1]) % it.nc[1];
+            cz = (cz + wv - 1 + it.nc[2]) % it.nc[2];
+            const int pc = (cz * it.nc[1] + cy) * it.nc[0] + cx + ncell;
+            const int* st = cb.cellStart + (size_t)w * (cb.cellStride + cb.nItems) + it.cellBase + item;
+            const int base = st[ncell];  // B atoms start after group A
+            const int b = st[pc] - base, e = st[pc + 1] - base;
+            pBegin = b + sub;
+            pCount = e > pBegin ? (e - pBegin + COORD_A_LPC - 1) / COORD_A_LPC : 0;
+        }
+    }
+    int maxCount = pCount;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) maxCount = max(maxCount, __shfl_xor(maxCount, o, 64));
+
+    long long gx = 0, gy = 0, gz = 0;
+    U128 cn{0ull, 0ll};
+    unsigned long long np = 0;
+    float4* Q = qPos[wv];
+    unsigned head = 0, tail = 0;
+    // FP64 evaluation of queue entry e (exact predicate, reference switching), dr = r_A - r_B
+    auto evaluate = [&](int e) {
+        const float4 pp = Q[e & 127];
+        double dx = (double)hp.x - (double)pp.x, dy = (double)hp.y - (double)pp.y, dz = (double)hp.z - (double)pp.z;
+        if (v.pbc) pbc_exact(dx, dy, dz, bx);
+        const double r2 = dot3_exact(dx, dy, dz);
+        if (r2 < 1e-20) return;
+        if (it.rc2 > 0.0 && !(r2 < it.rc2)) return;
+        const double r = sqrt(r2);
+        if (r * sw.ir0 > 1000.0) return;
+        double f, dfdr;
+        switch_eval(sw, r, f, dfdr);
+        const double sc = dfdr / r;
+        gx += to_fix(sc * dx);
+        gy += to_fix(sc * dy);
+        gz += to_fix(sc * dz);
+        add_cn_term(cn, f);
+        np++;
+    };
+    for (int k0 = 0; k0 < maxCount; k0 += COORD_A_PF) {
+        float4 pf[COORD_A_PF];
+#pragma unroll
+        for (int u = 0; u < COORD_A_PF; u++) {  // clamped to a valid partner: no divergent loads
+            const int j = pCount > 0 ? pBegin + min(k0 + u, pCount - 1) * COORD_A_LPC : 0;
+            pf[u] = part[j];
+        }
+#pragma unroll
+        for (int u = 0; u < COORD_A_PF; u++) {
+            const int k = k0 + u;
+            if (k >= maxCount) break;
+            const float4 pp = pf[u];
+            bool cand = k < pCount;
+            if (cand) {  // cells guarantee box >= 3 cutoffs: the FP32 image is never ambiguous
+                float fx = hp.x - pp.x, fy = hp.y - pp.y, fz = hp.z - pp.z;
+                if (v.pbc) pbc_f(fx, fy, fz, bxf);
+                cand = fx * fx + fy * fy + fz * fz < rc2f;
+            }
+            const unsigned long long m = __ballot(cand);
+            if (cand) Q[(tail + (unsigned)__popcll(m & ((1ull << lane) - 1ull))) & 127] = pp;
+            tail += (unsigned)__popcll(m);
+            if (tail - head >= 64) {
+                __builtin_amdgcn_wave_barrier();
+                evaluate((int)(head + lane));
+                head += 64;
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (lane < (int)(tail - head)) evaluate((int)(head + lane));
+
+    // integer (exact) sums: wave butterfly, then the three waves in LDS
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        gx += __shfl_xor(gx, o, 64);
+        gy += __shfl_xor(gy, o, 64);
+        gz += __shfl_xor(gz, o, 64);
+        const unsigned long long lo = __shfl_xor(cn.lo, o, 64);
+        const long long hi = __shfl_xor(cn.hi, o, 64);
+        add128(cn, lo, hi);
+        np += __shfl_xor(np, o, 64);
+    }
+    if (lane == 0) {
+        sG[wv][0] = gx;
+        sG[wv][1] = gy;
+        sG[wv][2] = gz;
+        sLo[wv] = cn.lo;
+        sHi[wv] = cn.hi;
+        sNp[wv] = np;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long tx = 0, ty = 0, tz = 0;
+        U128 t{0ull, 0ll};
+        unsigned long long tn = 0;
+        for (int q = 0; q < COORD_A_BLOCK / 64; q++) {
+            tx += sG[q][0];
+            ty += sG[q][1];
+            tz += sG[q][2];
+            add128(t, sLo[q], sHi[q]);
+            tn += sNp[q];
+        }
+        store_row(v, w, it.rowA + home, (double)tx * COORD_UNFIX, (double)ty * COORD_UNFIX, (double)tz * COORD_UNFIX);
+        // the workgroup's 128-bit CN sum V as (V >> 40) and (V mod 2^40): the low
+        // accumulator absorbs 2^24 workgroups without overflow, so two 64-bit
+        // atomics add V exactly
+        const size_t k = (size_t)w * cb.nItems + item;
+        const unsigned long long L = t.lo & ((1ull << 40) - 1ull);
+        const long long H = (long long)((unsigned long long)t.hi << 24) + (long long)(t.lo >> 40);
+        atomicAdd(&cb.cnAccLo[k], L);
+        atomicAdd((unsigned long long*)&cb.cnAcc[k], (unsigned long long)H);
+        atomicAdd(&cb.pairAcc[k], tn);
     }
 }
 
@@ -1250,7 +1414,8 @@ void launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordI
     const int wavesA = hIt.nA, wavesB = (hIt.nB + 1) / 2;
     const dim3 gA((wavesA + 3) / 4, nw), gB((wavesB + 3) / 4, nw);
     if (hIt.useCells) {
-        hipLaunchKernelGGL((k_coord_pairs<true, true>), gA, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
+        if (hIt.nA > 0)
+            hipLaunchKernelGGL(k_coord_pairs_a, dim3(hIt.nA, nw), dim3(COORD_A_BLOCK), 0, s, v, cb, dItems, item, w0);
         hipLaunchKernelGGL((k_coord_pairs<false, true>), gB, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
     } else {
         hipLaunchKernelGGL((k_coord_pairs<true, false>), gA, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
