@@ -14,7 +14,10 @@ the batch.  ``PathMetadFunction`` is its ``torch.autograd.Function``:
             dL/dsigma_d = sum_b g_b sum_k h_k exp(-E_bk) dc_bkd^2 / sigma_d^3;
             dL/dlambda  = sum_b g_b (dV/ds ds/dlambda + dV/dz dz/dlambda), with
                           ds/dlambda = -(1/S) sum_f (f+1-s) msd_f w_f and
-                          dz/dlambda = (<msd>_w - z)/lambda from the path kernel.
+                          dz/dlambda = (<msd>_w - z)/lambda from the path kernel;
+            dL/dr_fj    = sum_b c_bf (x_bj - r_fj) with
+                          c_bf = g_b (2/M)(w_bf/S_b)[lambda Vs_b (f+1-s_b) - Vz_b]
+                          (k_path_frames_coef / k_path_frames_grad).
 If the engine was used for another evaluation between forward and backward,
 backward recomputes the forward first (nothing per pair is stored).
 """
@@ -52,16 +55,21 @@ class PathMetadPipeline:
         self.device = self.ctx.device
         self._lam = float(lam)
         self._sigma = [float(x) for x in sigma]
+        self._frames = frames.astype(np.float32)  # as the engine stores them
         self.generation = 0
         self.grad_sigma = None
 
-    def __call__(self, x, heights, lam, sigma=None):
+    def __call__(self, x, heights, lam, sigma=None, frames=None):
+        """V_b for configurations x [B, M, 3]; frames [N, M, 3] (optional,
+        differentiable) replaces the path's reference frames."""
         if sigma is None:
             sigma = torch.tensor(self._sigma, dtype=torch.float64, device=self.device)
-        return PathMetadFunction.apply(x, heights, lam, sigma, self)
+        if frames is None:
+            frames = torch.as_tensor(self._frames.astype(np.float64), device=self.device)
+        return PathMetadFunction.apply(x, heights, lam, sigma, frames, self)
 
     # ---- engine plumbing ----
-    def _evaluate(self, x, heights, lam, sigma):
+    def _evaluate(self, x, heights, lam, sigma, frames):
         e = self.ctx
         P = e.padded[0]
         posq = e.posq.view(self.B, P, 4)
@@ -77,6 +85,11 @@ class PathMetadPipeline:
             arr = (ctypes.c_double * len(sig))(*sig)
             call("glued_set_hill_sigma", e.h, self.bias, arr, len(sig))
             self._sigma = sig
+        fr = frames.detach().to("cpu", torch.float64).numpy().reshape(self.N, self.M, 3)
+        if not np.array_equal(fr.astype(np.float32), self._frames):
+            flat = np.ascontiguousarray(fr, dtype=np.float64).reshape(-1)
+            call("glued_set_path_frames", e.h, 0, flat.ctypes.data_as(ctypes.c_void_p), flat.size)
+            self._frames = fr.astype(np.float32)
         e.zero_forces()
         e.execute_batch(use_graph=True)
         self.generation += 1
@@ -95,27 +108,27 @@ class PathMetadPipeline:
 
 class PathMetadFunction(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, x, heights, lam, sigma, pipe):
-        gen = pipe._evaluate(x, heights, lam, sigma)
+    def forward(ctx, x, heights, lam, sigma, frames, pipe):
+        gen = pipe._evaluate(x, heights, lam, sigma, frames)
         V = pipe.energies().clone()
         ctx.pipe = pipe
         ctx.gen = gen
         ctx.lam_shape = lam.shape if torch.is_tensor(lam) else None
         ctx.lam_device = lam.device if torch.is_tensor(lam) else None
         ctx.save_for_backward(x.detach(), heights.detach(), torch.as_tensor(float(lam), dtype=torch.float64),
-                              sigma.detach())
+                              sigma.detach(), frames.detach())
         return V
 
     @staticmethod
     def backward(ctx, gV):
         pipe = ctx.pipe
-        x, heights, lam, sigma = ctx.saved_tensors
+        x, heights, lam, sigma, frames = ctx.saved_tensors
         if pipe.generation != ctx.gen:  # the engine moved on: recompute this forward
-            pipe._evaluate(x, heights, lam, sigma)
+            pipe._evaluate(x, heights, lam, sigma, frames)
         e = pipe.ctx
         B, M, P = pipe.B, pipe.M, e.padded[0]
         g = gV.detach().to(torch.float64).contiguous()
-        grad_x = grad_h = grad_lam = grad_sig = None
+        grad_x = grad_h = grad_lam = grad_sig = grad_fr = None
         if ctx.needs_input_grad[0]:
             F = e.forces.view(B, 3, P)[:, :, :M].to(torch.float64) / FIXED_SCALE  # = -dV/dx
             grad_x = (-F.transpose(1, 2) * g[:, None, None]).to(x.dtype)
@@ -135,4 +148,8 @@ class PathMetadFunction(torch.autograd.Function):
             grad_lam = (g * dl).sum()
             if ctx.lam_shape is not None:
                 grad_lam = grad_lam.reshape(ctx.lam_shape).to(ctx.lam_device)
-        return grad_x, grad_h, grad_lam, grad_sig, None
+        if ctx.needs_input_grad[4]:
+            gf = torch.empty(pipe.N * pipe.M * 3, dtype=torch.float64, device=pipe.device)
+            call("glued_path_frames_adjoint", e.h, e.stream, 0, _p(g), _p(gf))
+            grad_fr = gf.view(pipe.N, pipe.M, 3).to(frames.dtype).to(frames.device)
+        return grad_x, grad_h, grad_lam, grad_sig, grad_fr, None

@@ -262,6 +262,16 @@ int glued_hill_adjoint(glued_handle h, void* stream, int biasIdx, const double* 
     return guard([&] { E(h).hillAdjoint((hipStream_t)stream, biasIdx, dGradV, dGradHeights, dGradSigma); });
 }
 
+int glued_set_path_frames(glued_handle h, int pathIdx, const double* frames, int n)
+{
+    return guard([&] { E(h).setPathFrames(pathIdx, frames, n); });
+}
+
+int glued_path_frames_adjoint(glued_handle h, void* stream, int pathIdx, const double* dGradV, double* dGradFrames)
+{
+    return guard([&] { E(h).pathFramesAdjoint((hipStream_t)stream, pathIdx, dGradV, dGradFrames); });
+}
+
 int glued_plan_info(glued_handle h, int* rows, int* atoms)
 {
     return guard([&] {

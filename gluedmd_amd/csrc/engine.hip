@@ -39,6 +39,8 @@ void launch_cv_fused(const StepView&, const CVFusedPlan&, int, hipStream_t);
 void launch_hill_adjoint(const StepView&, const HillBiasDev&, const double*, double*, double*, double*, int,
                          hipStream_t);
 int hill_adjoint_blocks(int);
+void launch_path_frames_adjoint(const StepView&, const PathItem*, int, int, int, const int*, const float*,
+                                const double*, double*, double*, int, hipStream_t);
 
 void hip_check(hipError_t e, const char* what)
 {
@@ -1423,6 +1425,41 @@ void Engine::hillAdjoint(hipStream_t s, int bi, const double* gV, double* gH, do
     v.S = S_;
     launch_hill_adjoint(v, h->d, gV, gH, dAdjPart_.as<double>(), gSigma, W_, s);
     hip_check(hipGetLastError(), "hill_adjoint launch");
+}
+
+// Frames as the CV stores them (FP32, [N][M][3]); the spec keeps the new
+// values so that a re-plan uses them too.
+void Engine::setPathFrames(int pi, const double* frames, int n)
+{
+    if (!initialized_) fail("set_path_frames: initialize first");
+    if (pi < 0 || pi >= (int)path_.size()) fail("set_path_frames: path index out of range");
+    const PathItem& it = path_[pi];
+    if (n != it.N * it.M * 3) fail("set_path_frames: expected N*M*3 = " + std::to_string(it.N * it.M * 3) + " values");
+    std::vector<float> f((size_t)n);
+    for (int k = 0; k < n; k++) {
+        if (!std::isfinite(frames[k])) fail("set_path_frames: non-finite coordinate");
+        f[k] = (float)frames[k];
+    }
+    int seen = 0;
+    for (auto& c : cvs_)
+        if (c.type == CV_PATH && seen++ == pi)
+            for (int k = 0; k < n; k++) c.params[2 + k] = (double)f[k];
+    hip_check(hipMemcpy(dPathFrames_.as<float>() + it.frameOff, f.data(), sizeof(float) * n, hipMemcpyHostToDevice),
+              "set_path_frames");
+}
+
+void Engine::pathFramesAdjoint(hipStream_t s, int pi, const double* gV, double* gF)
+{
+    if (!initialized_ || !cvReady_) fail("path_frames_adjoint: evaluate first");
+    if (pi < 0 || pi >= (int)path_.size()) fail("path_frames_adjoint: path index out of range");
+    const PathItem& it = path_[pi];
+    const size_t need = sizeof(double) * (size_t)W_ * it.N;
+    if (dPathCoef_.bytes < need) dPathCoef_.alloc(need);
+    StepView v = view(lastPos_, lastForce_);
+    v.S = S_;
+    launch_path_frames_adjoint(v, dPath_.as<PathItem>(), pi, it.N, it.M, dPathPos_.as<int>(),
+                               dPathFrames_.as<float>(), gV, dPathCoef_.as<double>(), gF, W_, s);
+    hip_check(hipGetLastError(), "path_frames_adjoint launch");
 }
 
 // ---------------------------------------------------------------------------

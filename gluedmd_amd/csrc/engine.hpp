@@ -110,6 +110,8 @@ class Engine {
     void setPathLambda(int path, double lambda);
     void setHillSigma(int bias, const double* sigma, int D);
     void hillAdjoint(hipStream_t s, int bias, const double* gV, double* gH, double* gSigma);
+    void setPathFrames(int path, const double* frames, int n);
+    void pathFramesAdjoint(hipStream_t s, int path, const double* gV, double* gF);
 
     // ---- read-backs ----
     std::vector<double> cvValues(int w);
@@ -212,7 +214,7 @@ class Engine {
     DBuf dPath_, dPathPos_, dPathFrames_, dPathW_;
     DBuf dSmallBias_, dGridTable_, dSmallConst_, dSmallVar_, dSmallScal_;
     DBuf dSegStart_, dSegRow_ /* rowPos */, dRowCv_ /* entryCv */, dCombStart_, dCombSlot_, dTestGrad_, dEnergyU_, dEnergyFPtr_;
-    DBuf dHillTable_, dOpesTable_, dFusedPart_, dFusedCounter_, dPathAux_, dAdjPart_, dPathMsd_;
+    DBuf dHillTable_, dOpesTable_, dFusedPart_, dFusedCounter_, dPathAux_, dAdjPart_, dPathMsd_, dPathCoef_;
     int msdStride_ = 0, pathMsdBlocks_ = 0, pathRowBlocks_ = 0;
     FusedBiasPlan fplan_{};
 

@@ -1578,6 +1578,81 @@ __global__ __launch_bounds__(64 * PATH_PHASES) void k_path_rows(StepView v, cons
     }
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of the path CV with respect to its reference frames (an extension:
+// the reference has no frame derivatives).  With V(s, z) and the MSD metric,
+//   dV/dr_fj = (2/M) (w_f/S) (x_j - r_fj) [lambda Vs (f+1-s) - Vz],
+// Vs, Vz = dU/ds, dU/dz of the last evaluation.  Over the walkers, with
+// upstream weights g_b,
+//   dL/dr_fj = sum_b c_bf (x_bj - r_fj),
+//   c_bf = g_b (2/M) (w_bf/S_b) [lambda Vs_b (f+1-s_b) - Vz_b].
+// Stage 1 forms c (one workgroup per walker, w_f, S and s exactly as the
+// forward's row blocks form them); stage 2 runs one thread per (frame, atom)
+// and adds the walkers in index order (deterministic), FP64.  Where every
+// weight underflowed (S = 0) the walker contributes nothing, as its rows.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64 * PATH_PHASES) void k_path_frames_coef(StepView v, const PathItem* __restrict__ items,
+                                                                       int pi, const double* __restrict__ gV,
+                                                                       double* __restrict__ coef)
+{
+    __shared__ double sh[4 * 16];
+    const PathItem& it = items[pi];
+    const int w = blockIdx.x;
+    const int N = it.N;
+    const double lam = it.lambda;
+    const double* msd = v.pathMsd + (size_t)w * v.msdStride + it.msdOff;
+    double t[2] = {0.0, 0.0};
+    for (int f = threadIdx.x; f < N; f += blockDim.x) {
+        const double wv = exp(-lam * msd[f]);
+        t[0] += wv;
+        t[1] += (f + 1) * wv;
+    }
+    block_sum<2, PATH_PHASES>(t, sh);
+    const double S = t[0];
+    const bool ok = S > 1e-300;
+    const double s = ok ? t[1] / S : 0.0;
+    const double Vs = v.dUdCV[(size_t)w * v.V + it.valueS], Vz = v.dUdCV[(size_t)w * v.V + it.valueS + 1];
+    const double pre = ok ? gV[w] * 2.0 / ((double)it.M * S) : 0.0;
+    for (int f = threadIdx.x; f < N; f += blockDim.x) {
+        const double wv = exp(-lam * msd[f]);
+        coef[(size_t)w * N + f] = pre * wv * (lam * Vs * ((f + 1) - s) - Vz);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_path_frames_grad(StepView v, const PathItem* __restrict__ items, int pi,
+                                                          const int* __restrict__ slots,
+                                                          const float* __restrict__ frames,
+                                                          const double* __restrict__ coef, double* __restrict__ gF,
+                                                          int W)
+{
+    const PathItem& it = items[pi];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= it.N * it.M) return;
+    const int f = t / it.M, j = t - f * it.M;
+    const float* r = frames + it.frameOff + (size_t)t * 3;  // frame f, atom j
+    const double r0 = r[0], r1 = r[1], r2 = r[2];
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+    for (int b = 0; b < W; b++) {
+        const float4 q = gather_pos(v, slots, v.pStride, b, it.begin + j);
+        const double c = coef[(size_t)b * it.N + f];
+        a0 += c * ((double)q.x - r0);
+        a1 += c * ((double)q.y - r1);
+        a2 += c * ((double)q.z - r2);
+    }
+    gF[(size_t)t * 3] = a0;
+    gF[(size_t)t * 3 + 1] = a1;
+    gF[(size_t)t * 3 + 2] = a2;
+}
+
+void launch_path_frames_adjoint(const StepView& v, const PathItem* items, int pi, int N, int M, const int* slots,
+                                const float* frames, const double* gV, double* coef, double* gF, int W, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_path_frames_coef, dim3(W), dim3(64 * PATH_PHASES), 0, s, v, items, pi, gV, coef);
+    const int n = N * M;
+    hipLaunchKernelGGL(k_path_frames_grad, dim3((n + 255) / 256), dim3(256), 0, s, v, items, pi, slots, frames, coef,
+                       gF, W);
+}
+
 void launch_path_rows(const StepView& v, const PathItem* items, int n, int nRowBlocks, const int* slots,
                       const float* frames, int w0, int nw, hipStream_t s)
 {

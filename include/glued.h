@@ -184,6 +184,15 @@ int glued_set_hill_sigma(glued_handle h, int biasIdx, const double* sigma, int D
  * dL/dsigma_d (device, [D]) for the CV values of the last evaluation. */
 int glued_hill_adjoint(glued_handle h, void* stream, int biasIdx, const double* dGradV, double* dGradHeights,
                        double* dGradSigma);
+/* Reference frames of the pathIdx-th path CV (host, [N][M][3], stored as
+ * FP32 like the CV's parameters); n = N*M*3.  No reference counterpart: the
+ * reference fixes the frames at creation (CommonGluedKernels.cpp:4011-4024). */
+int glued_set_path_frames(glued_handle h, int pathIdx, const double* frames, int n);
+/* Adjoint of the pathIdx-th path CV with respect to its frames: given
+ * g_w = dL/dV_w (device, [W]) writes dL/dr (device, [N][M][3]) for the
+ * positions, CV values and dU/dCV of the last evaluation.  No reference
+ * counterpart (the reference has no frame derivatives). */
+int glued_path_frames_adjoint(glued_handle h, void* stream, int pathIdx, const double* dGradV, double* dGradFrames);
 
 /* Plan sizes: Jacobian rows and distinct atoms of one walker. */
 int glued_plan_info(glued_handle h, int* rows, int* atoms);

@@ -130,3 +130,53 @@ def test_backward_recomputes_after_other_forward(setup):
     _ = pipe(x * 1.001, h, torch.tensor(c["lam"], dtype=torch.float64))  # moves the engine on
     V2.sum().backward()
     assert torch.equal(x.grad, g1)
+
+
+def test_backward_frames(setup):
+    """dL/d(reference frames) from the frame adjoint against FP64 central
+    finite differences of the oracle (path CV + hill sum), and the
+    translation identity: V depends on x - r_f only, so per component the
+    sum over frames and atoms of dL/dr equals minus the sum of dL/dx."""
+    from oracle import oracle as o
+    c, pipe, hc, hh, sig = setup
+    B = len(c["confs"])
+    rng = np.random.default_rng(5)
+    gvec = rng.normal(size=B)
+    frames0 = np.asarray(c["frames"], dtype=np.float64)
+    x = torch.as_tensor(c["confs"], dtype=torch.float64, device=pipe.device).requires_grad_(True)
+    h = torch.as_tensor(hh, device=pipe.device)
+    fr = torch.as_tensor(frames0, device=pipe.device).requires_grad_(True)
+    V = pipe(x, h, torch.tensor(c["lam"], dtype=torch.float64), frames=fr)
+    (V * torch.as_tensor(gvec, device=pipe.device)).sum().backward()
+    gF = fr.grad.cpu().numpy()
+    gX = x.grad.cpu().numpy()
+    assert gF.shape == frames0.shape
+    lhs = gF.sum(axis=(0, 1))
+    rhs = -gX.sum(axis=(0, 1))
+    scale = np.abs(gX).sum(axis=(0, 1))
+    assert np.all(np.abs(lhs - rhs) <= 1e-6 * scale + 1e-8), (lhs, rhs)
+    # FD in FP64 on the oracle, around the frames as the engine stores them (FP32 values)
+    fr32 = frames0.astype(np.float32).astype(np.float64)
+    xs = [c["confs"][b].astype(np.float32).astype(np.float64) for b in range(B)]
+    step = 1e-5
+    checked = 0
+    for (f, j, k) in [(0, 0, 0), (3, 11, 1), (9, 40, 2), (15, 59, 0), (7, 30, 1)]:
+        fp, fm = fr32.copy(), fr32.copy()
+        fp[f, j, k] += step
+        fm[f, j, k] -= step
+        Lp = Lm = 0.0
+        ok = True
+        for b in range(B):
+            Vp, szp = oracle_V(fp, c["lam"], xs[b], hc, hh, sig)
+            Vm, szm = oracle_V(fm, c["lam"], xs[b], hc, hh, sig)
+            ok &= min(o.hill_margin(szp, hc, sig, np.zeros(2)), o.hill_margin(szm, hc, sig, np.zeros(2))) >= 1e-6
+            Lp += gvec[b] * Vp
+            Lm += gvec[b] * Vm
+        if not ok:  # a hill crosses the E = 8 edge within the step
+            continue
+        fd = (Lp - Lm) / (2 * step)
+        assert abs(gF[f, j, k] - fd) <= 1e-5 * max(abs(fd), 1e-3), (f, j, k, gF[f, j, k], fd)
+        checked += 1
+    assert checked >= 3
+    # the frames are restored for the other tests
+    pipe(x.detach(), h, torch.tensor(c["lam"], dtype=torch.float64), frames=torch.as_tensor(frames0, device=pipe.device))
