@@ -59,7 +59,7 @@ FIXED_SCALE = 4294967296.0
 PMC_FETCH_SCALE, PMC_WRITE_SCALE = 2.0, 1.0
 PMC_KIB = {
     "multiwalker": {"k_step_bias_chunks<1, 2>": (15052.2, 60.0)},
-    "mixed-large": {"k_scatter_dense": (28157.5, 37512.1)},
+    "mixed-large": {"k_scatter_dense_rows": (47457.5, 37512.1)},
     "coordination-opes": {"k_coord_rank": (278.5, 792.4), "k_coord_offsets": (21.1, 38.8),
                           "k_coord_place": (359.4, 418.8), "k_coord_pairs<true, true>": (1656.8, 234.4),
                           "k_coord_pairs<false, true>": (1151.6, 269.7), "k_cv_group_big": (95.5, 117.2)},
@@ -186,11 +186,14 @@ class MixedLarge(Workload):
 
     def roofline(self, t_launch):
         n = self.W * self.c["natoms"]
-        nbytes = n * (12 + 48)  # F_ext FP32 xyz read + int64 xyz read-modify-write
-        return dict(bound="hbm", kernel="k_scatter_dense (energy-CV force scatter)",
+        rows, _ = self.ctx.plan_info()
+        nr = self.W * rows
+        nbytes = n * (12 + 48) + nr * 12  # F_ext + fixed-point force RMW per atom; FP32 xyz per Jacobian row
+        return dict(bound="hbm", kernel="k_scatter_dense_rows (energy-CV + atom-slot force scatter, one pass)",
                     achieved=nbytes / t_launch / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                    per_unit="60 B per atom (12 B FP32 F_ext + 48 B fixed-point force read+write)",
-                    units_per_launch=n, bytes_per_launch=nbytes)
+                    per_unit="60 B per atom (12 B FP32 F_ext + 48 B fixed-point force read+write) "
+                             "+ 12 B per Jacobian row",
+                    units_per_launch={"atoms": n, "rows": nr}, bytes_per_launch=nbytes)
 
     def oracle_walker(self, w):
         import numpy as np

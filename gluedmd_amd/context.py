@@ -183,7 +183,8 @@ class Context:
 
     def execute_stage(self, stage):
         """0 = CVs, 1 = biases, 2 = dU/dCV + scatter, 3 = streamed hill /
-        OPES-kernel sums only, 4 = dense energy-CV scatter only; every walker."""
+        OPES-kernel sums only, 4 = the force scatter only (the combined dense
+        energy-CV + atom-slot pass where the batched step uses it); every walker."""
         call("glued_execute_stage", self.h, self.stream, ctypes.c_void_p(self.posq.data_ptr()),
              ctypes.c_void_p(self.forces.data_ptr()), int(stage))
 

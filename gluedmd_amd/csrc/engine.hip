@@ -33,6 +33,8 @@ int opes_chunks_for(int);
 
 void launch_scatter(const StepView&, const ScatterPlan&, int, int, int, hipStream_t);
 void launch_scatter_dense(const StepView&, const float* const*, int, int, int, int, hipStream_t);
+void launch_scatter_dense_rows(const StepView&, const ScatterPlan&, const float* const*, int, const int*,
+                               const unsigned*, int, int, int, int, int, hipStream_t);
 void launch_step_bias(const StepView&, const FusedBiasPlan&, int, hipStream_t);
 void launch_step_bias_chunks(const StepView&, const FusedBiasPlan&, int, hipStream_t);
 void launch_cv_fused(const StepView&, const CVFusedPlan&, int, hipStream_t);
@@ -443,6 +445,16 @@ void Engine::buildPlan(int nMass, const double* masses)
     pathSlots_ = pathSlots;
     dGroupPos_.alloc(sizeof(int) * (size_t)W_ * std::max((int)groupSlots_.size(), 1));
     dPathPos_.alloc(sizeof(int) * (size_t)W_ * std::max((int)pathSlots_.size(), 1));
+    if (energyValue_ >= 0) {
+        // GPU atom -> slot tables for the combined dense + slot scatter
+        slotStride_ = maxPadded_;
+        maskStride_ = (maxPadded_ + 31) / 32;
+        dSlotOf_.alloc(sizeof(int) * (size_t)W_ * slotStride_);
+        dSlotMask_.alloc(sizeof(unsigned) * (size_t)W_ * maskStride_);
+    } else {
+        dSlotOf_.release();
+        dSlotMask_.release();
+    }
     for (int w = 0; w < W_; w++) uploadGidx(w);
     boxHost_.assign((size_t)W_ * 12, 0.0);
     for (int w = 0; w < W_; w++) {
@@ -810,6 +822,20 @@ void Engine::uploadGidx(int w)
     };
     resolve(groupSlots_, dGroupPos_);
     resolve(pathSlots_, dPathPos_);
+    if (dSlotMask_.bytes) {
+        std::vector<int> so((size_t)slotStride_, -1);
+        std::vector<unsigned> mk((size_t)maskStride_, 0u);
+        for (int s = 0; s < S_; s++) {
+            so[g[s]] = s;
+            mk[g[s] >> 5] |= 1u << (g[s] & 31);
+        }
+        hip_check(hipMemcpy(dSlotOf_.as<int>() + (size_t)w * slotStride_, so.data(), sizeof(int) * so.size(),
+                            hipMemcpyHostToDevice),
+                  "upload slot table");
+        hip_check(hipMemcpy(dSlotMask_.as<unsigned>() + (size_t)w * maskStride_, mk.data(),
+                            sizeof(unsigned) * mk.size(), hipMemcpyHostToDevice),
+                  "upload slot mask");
+    }
 }
 
 // rebuildGpuAtomIndices, CommonCVKernels.cpp:113-317: user -> GPU order.
@@ -1019,14 +1045,7 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
     st.scal = dSmallScal_.as<double>();
     st.stateVarStride = smallVarPerWalker_;
     st.scalStride = 4 * (long long)std::max((int)smallBias_.size(), 1);
-    ScatterPlan sp;
-    sp.segStart = dSegStart_.as<int>();
-    sp.entryCv = dRowCv_.as<int>();
-    sp.combStart = dCombStart_.as<int>();
-    sp.combSlot = dCombSlot_.as<int>();
-    sp.nComb = (int)combSlot_.size();
-    sp.testGrad = testGrads_.empty() ? nullptr : dTestGrad_.as<double>();
-    sp.energyValue = energyValue_;
+    const ScatterPlan sp = scatterPlan();
     const bool biases = (stageMask & 10) && testGrads_.empty();
     if (biases || (stageMask & 4)) {
         for (auto* o : opes_)
@@ -1035,7 +1054,7 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
                 if (!capturing_) bumpWelfordMirrors(o, w0, nw);
             }
     }
-    bool scattered = false;
+    bool scattered = false, denseDone = false;
     if (fused && (stageMask & 6) == 6 && w0 == 0 && nw == W_) {
         // batched: chunked sums + finalize + one-thread biases + dU/dCV (+ scatter) in one launch
         FusedBiasPlan fp = fplan_;
@@ -1049,7 +1068,17 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         fp.scatter = fplan_.scatter && forces;
         launch_step_bias(v, fp, nw, s);
         scattered = fp.scatter != 0;
-        if (forces && !scattered) launch_scatter(v, sp, w0, nw, 1, s);
+        if (forces && !scattered) {
+            if (energyValue_ >= 0 && dSlotMask_.bytes) {
+                // slots and the dense energy CV in one pass (dU/dCV from the finalize)
+                launch_scatter_dense_rows(v, sp, dEnergyFPtr_.as<const float*>(), energyValue_, dSlotOf_.as<int>(),
+                                          dSlotMask_.as<unsigned>(), slotStride_, maskStride_, maxPadded_, w0, nw,
+                                          s);
+                denseDone = true;
+            } else {
+                launch_scatter(v, sp, w0, nw, 1, s);
+            }
+        }
         scattered = true;
     } else if (biases) {
         for (auto* o : opes_) launch_bias_opes(v, o->d, w0, nw, s);
@@ -1062,9 +1091,22 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         return;
     }
     if (!scattered) launch_scatter(v, sp, w0, nw, forces ? 1 : 0, s);
-    if (forces && energyValue_ >= 0)
+    if (forces && energyValue_ >= 0 && !denseDone)
         launch_scatter_dense(v, dEnergyFPtr_.as<const float*>(), energyValue_, maxPadded_, w0, nw, s);
     hip_check(hipGetLastError(), "kernel launch");
+}
+
+ScatterPlan Engine::scatterPlan() const
+{
+    ScatterPlan sp;
+    sp.segStart = dSegStart_.as<int>();
+    sp.entryCv = dRowCv_.as<int>();
+    sp.combStart = dCombStart_.as<int>();
+    sp.combSlot = dCombSlot_.as<int>();
+    sp.nComb = (int)combSlot_.size();
+    sp.testGrad = testGrads_.empty() ? nullptr : dTestGrad_.as<double>();
+    sp.energyValue = energyValue_;
+    return sp;
 }
 
 // The host mirror of the adaptive-sigma sample count (the reference keeps
@@ -1107,10 +1149,20 @@ void Engine::executeStage(hipStream_t s, const float4* posq, long long* force, i
     lastPos_ = posq;
     lastForce_ = force;
     if (stage == 4) {
-        // the dense energy-CV scatter alone (every walker)
-        if (energyValue_ >= 0)
-            launch_scatter_dense(view(lastPos_, lastForce_), dEnergyFPtr_.as<const float*>(), energyValue_,
-                                 maxPadded_, 0, W_, s);
+        // the force scatter of the batched step alone when it is the combined
+        // dense + slot pass (energy CV, slots not scattered by the finalize),
+        // else the dense energy-CV scatter alone; every walker, dU/dCV of the
+        // last evaluation
+        if (energyValue_ >= 0) {
+            StepView v = view(lastPos_, lastForce_);
+            v.S = S_;
+            if (dSlotMask_.bytes && !fplan_.scatter)
+                launch_scatter_dense_rows(v, scatterPlan(), dEnergyFPtr_.as<const float*>(), energyValue_,
+                                          dSlotOf_.as<int>(), dSlotMask_.as<unsigned>(), slotStride_, maskStride_,
+                                          maxPadded_, 0, W_, s);
+            else
+                launch_scatter_dense(v, dEnergyFPtr_.as<const float*>(), energyValue_, maxPadded_, 0, W_, s);
+        }
         hip_check(hipGetLastError(), "kernel launch");
         return;
     }

@@ -135,6 +135,7 @@ class Engine {
     void buildPlan(int nMass, const double* masses);
     void launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMask = 7, bool fused = false);
     StepView view(const float4* posq, long long* force) const;
+    ScatterPlan scatterPlan() const;
     void uploadGidx(int w);
     void sync(hipStream_t s);
 
@@ -234,6 +235,8 @@ class Engine {
     bool cvReady_ = false;
     int lastUpdateStep_ = -1;
     int maxPadded_ = 0;
+    DBuf dSlotOf_, dSlotMask_;     // [W][maxPadded] slot of each GPU atom (-1: none), [W][maxPadded/32] bit per atom
+    int slotStride_ = 0, maskStride_ = 0;
     const float4* lastPos_ = nullptr;
     long long* lastForce_ = nullptr;
 };

@@ -103,33 +103,50 @@ __device__ inline void grid_eval(const GridBiasDev& g, const double* grid, const
 
 // ---------------------------------------------------------------------------
 // One-thread biases: grid MetaD eval, OPES expanded (:3050-3090),
-// multithermal (:3145-3172).  One thread per (item, walker).
+// multithermal (:3145-3172).  One wave per (item, walker): every lane calls
+// with the same item.  Grid MetaD runs on lane 0; OPES expanded and
+// multithermal give lane l the states l, l + 64, ...: the maximum is exact
+// and the sums are wave butterflies in a fixed order, so the plain and the
+// batched forms (both call this) agree bit for bit.
 // ---------------------------------------------------------------------------
+__device__ inline double wave_max1(double x)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x = fmax(x, __shfl_xor(x, o, 64));
+    return x;
+}
+
 // bgL (optional, LDS): a copy of the walker's bias-gradient slots.
 __device__ void small_bias_eval(const StepView& v, const SmallBiasState& st, const SmallBiasItem& it, int i, int w,
                                 double* bgL = nullptr)
 {
+    const int lane = threadIdx.x & 63;
     const double* cv = v.cvVal + (size_t)w * v.V;
     double* bg = v.bgrad + (size_t)w * v.G + it.gslot;
     double* be = v.benergy + (size_t)w * v.B + it.eslot;
-    double bgv[MAX_BIAS_D];
-    int nOut = 0;
     if (it.type == BIAS_METAD) {
-        const GridBiasDev& g = it.grid;
-        grid_eval(g, g.grid + (size_t)w * g.walkerStride, cv, be, bgv);
-        nOut = g.D;
-        for (int d = 0; d < g.D; d++) bg[d] = bgv[d];
+        if (lane == 0) {
+            const GridBiasDev& g = it.grid;
+            double bgv[MAX_BIAS_D];
+            grid_eval(g, g.grid + (size_t)w * g.walkerStride, cv, be, bgv);
+            for (int d = 0; d < g.D; d++) {
+                bg[d] = bgv[d];
+                if (bgL) bgL[it.gslot + d] = bgv[d];
+            }
+        }
     } else if (it.type == BIAS_OPES_EXPANDED) {
         const double invKT = it.p[0];
         const double* lw = st.stateConst + it.stateOff;
         const double logZ = st.scal[(size_t)w * st.scalStride + 4 * i + 0];
         double mx = -1e300;
-        for (int l = 0; l < it.nState; l++) mx = fmax(mx, lw[l] - cv[it.cv[0] + l] * invKT);
+        for (int l = lane; l < it.nState; l += 64) mx = fmax(mx, lw[l] - cv[it.cv[0] + l] * invKT);
+        mx = wave_max1(mx);
         double se = 0.0;
-        for (int l = 0; l < it.nState; l++) se += exp(lw[l] - cv[it.cv[0] + l] * invKT - mx);
+        for (int l = lane; l < it.nState; l += 64) se += exp(lw[l] - cv[it.cv[0] + l] * invKT - mx);
+        se = wave_sum1(se);
         const double logQ = mx + log(se);
-        *be = -(1.0 / invKT) * (logQ - logZ);
-        for (int l = 0; l < it.nState; l++) {
+        if (lane == 0) *be = -(1.0 / invKT) * (logQ - logZ);
+        for (int l = lane; l < it.nState; l += 64) {
             const double pl = exp(lw[l] - cv[it.cv[0] + l] * invKT - logQ);
             bg[l] = pl;
             if (bgL) bgL[it.gslot + l] = pl;
@@ -140,35 +157,36 @@ __device__ void small_bias_eval(const StepView& v, const SmallBiasState& st, con
         const double* dF = st.stateVar + (size_t)w * st.stateVarStride + it.stateOff;
         const double U = cv[it.cv[0]];
         double mx = -1e300;
-        for (int l = 0; l < it.nState; l++) mx = fmax(mx, -db[l] * U + dF[l] / kT0);
-        double s = 0.0, ds = 0.0;
-        for (int l = 0; l < it.nState; l++) {
-            double e = exp(-db[l] * U + dF[l] / kT0 - mx);
-            s += e;
-            ds += e * db[l];
+        for (int l = lane; l < it.nState; l += 64) mx = fmax(mx, -db[l] * U + dF[l] / kT0);
+        mx = wave_max1(mx);
+        double sd[2] = {0.0, 0.0};
+        for (int l = lane; l < it.nState; l += 64) {
+            const double e = exp(-db[l] * U + dF[l] / kT0 - mx);
+            sd[0] += e;
+            sd[1] += e * db[l];
         }
-        *be = -kT0 * (mx + log(s / (double)it.nState));
-        bgv[0] = kT0 * (ds / s);
-        bg[0] = bgv[0];
-        nOut = 1;
+        wave_sum_n<2>(sd);
+        if (lane == 0) {
+            *be = -kT0 * (mx + log(sd[0] / (double)it.nState));
+            const double g0 = kT0 * (sd[1] / sd[0]);
+            bg[0] = g0;
+            if (bgL) bgL[it.gslot] = g0;
+        }
     }
-    if (bgL)
-        for (int d = 0; d < nOut; d++) bgL[it.gslot + d] = bgv[d];
 }
 
-__global__ void k_bias_small(StepView v, SmallBiasState st, const SmallBiasItem* __restrict__ items, int n, int w0)
+// one 64-thread workgroup (one wave) per (item, walker)
+__global__ __launch_bounds__(64) void k_bias_small(StepView v, SmallBiasState st,
+                                                   const SmallBiasItem* __restrict__ items, int n, int w0)
 {
-    const int w = w0 + blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    small_bias_eval(v, st, items[i], i, w);
+    small_bias_eval(v, st, items[blockIdx.x], blockIdx.x, w0 + blockIdx.y);
 }
 
 void launch_bias_small(const StepView& v, const SmallBiasState& st, const SmallBiasItem* items, int n, int w0, int nw,
                        hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_bias_small, dim3((n + 63) / 64, nw), dim3(64), 0, s, v, st, items, n, w0);
+    hipLaunchKernelGGL(k_bias_small, dim3(n, nw), dim3(64), 0, s, v, st, items, n, w0);
 }
 
 // ---------------------------------------------------------------------------
@@ -573,7 +591,7 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, F
             }
         }
     } else if (wave == 1) {
-        for (int i = tid - 64; i < fp.nSmall; i += 64) small_bias_eval(v, fp.st, fp.small[i], i, w, bgL);
+        for (int i = 0; i < fp.nSmall; i++) small_bias_eval(v, fp.st, fp.small[i], i, w, bgL);
     }
     __syncthreads();
     if (!ldsComb) {

@@ -126,7 +126,9 @@ int glued_execute_batch(glued_handle h, void* stream, const void* posq, int64_t*
                         int useGraph);
 /* One stage of the batched step for every walker (0 = CVs, 1 = biases,
  * 2 = dU/dCV + force scatter, 3 = only the streamed hill / OPES-kernel sums,
- * 4 = only the dense energy-CV scatter), for per-stage timing; no reference
+ * 4 = only the force scatter: the combined dense energy-CV + atom-slot pass
+ * where the batched step uses it, else the dense energy-CV scatter), for
+ * per-stage timing; no reference
  * counterpart. */
 int glued_execute_stage(glued_handle h, void* stream, const void* posq, int64_t* forces, int stage);
 /* Replaces CalcGluedForceKernel::updateState (GluedKernels.h:52,

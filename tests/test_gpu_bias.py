@@ -239,3 +239,61 @@ def test_step_batch_matches_update_then_execute():
         ha, hb = ga.get_hills(2, w), gb_.get_hills(2, w)
         assert np.array_equal(ha[0], hb[0]) and np.array_equal(ha[1], hb[1]) and len(ha[1]) == 2
         assert np.array_equal(ga.getOPESMetrics(0, w), gb_.getOPESMetrics(0, w))
+
+
+def test_dense_and_slot_scatter_combined():
+    """The batched step's combined dense energy-CV + atom-slot scatter
+    (k_scatter_dense_rows) against the plain form's separate slot and dense
+    scatter kernels, bit for bit, and against the oracle.  Multithermal on the
+    energy CV with the ladder's weights spread (nonzero dV/dU), hill MetaD on
+    a distance and a contact map (nonzero dU/dCV on the slot rows); walker 2
+    has no energy input (its slots go through the atomic branch)."""
+    import torch
+    from gluedmd_amd.context import Context
+    from oracle.oracle import OracleContext
+    rng = np.random.default_rng(21)
+    N, W, L = 700, 3, 2.0
+    f = Force(pbc=True, temperature=300.0)
+    d = f.add_distance([3, 500])
+    pairs = [int(x) for a, b in rng.integers(0, N, size=(48, 2)) if a != b for x in (a, b)]
+    cm = f.add_contact_map(pairs, r0=0.5)
+    hb = f.add_metad_hills([d, cm], sigma=[0.2, 1.0], height=1.0, pace=1000, gamma=None, max_hills=64)
+    f.add_multithermal(300.0, 320.0, 8)
+    pos = [rng.uniform(0, L, size=(N, 3)).astype(np.float32) for _ in range(W)]
+    g = Context(f, walkers=[N] * W)
+    g.set_box(L)
+    os_ = []
+    for w in range(W):
+        g.set_positions(pos[w], w)
+        o = OracleContext(f, N)
+        o.set_box(L)
+        if w < 2:
+            U = -50.0 + 10.0 * w
+            F = rng.normal(0.0, 100.0, size=(N, 3)).astype(np.float32)
+            p = g.padded[w]
+            Fext = torch.zeros(3 * p, dtype=torch.float32, device=g.device)
+            Fext.view(3, p)[:, :N] = torch.as_tensor(F.T.copy(), device=g.device)
+            g.set_energy_input(U, Fext, w)
+            o.set_energy_input(U, F.astype(np.float64))
+        else:  # no energy input on the GPU: U = 0, no energy force
+            o.set_energy_input(0.0, np.zeros((N, 3)))
+        os_.append(o)
+    g.execute_batch(use_graph=False)  # CV values, to place hills around them
+    for w in range(W):
+        cv = g.getLastCVValues(w)
+        hc = np.stack([cv[0] + rng.normal(0, 0.15, 16), cv[1] + rng.normal(0, 0.8, 16)], axis=1)
+        hh = rng.uniform(0.5, 1.5, 16)
+        g.set_hills(hb, hc, hh, w)
+        os_[w].states[hb].centers = hc.copy()
+        os_[w].states[hb].heights = hh.copy()
+    g.zero_forces()
+    g.execute_batch(use_graph=False)
+    Fb = [g.get_forces_fixed(w) for w in range(W)]
+    for w in range(W):
+        assert np.abs(g.getDUDCV(w)).max() > 0.0, w
+        r = os_[w].execute(pos[w])
+        check(g, os_[w], r, g.getLastBias(w), w)
+    g.zero_forces()
+    g.execute_all_plain()
+    for w in range(W):
+        assert np.array_equal(g.get_forces_fixed(w), Fb[w]), w
