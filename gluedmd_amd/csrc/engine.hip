@@ -35,6 +35,7 @@ void launch_scatter(const StepView&, const ScatterPlan&, int, int, int, hipStrea
 void launch_scatter_dense(const StepView&, const float* const*, int, int, int, int, hipStream_t);
 void launch_scatter_dense_rows(const StepView&, const ScatterPlan&, const float* const*, int, const int*,
                                const unsigned*, int, int, int, int, int, hipStream_t);
+void launch_step_tiny(const StepView&, const SmallCVItem*, int, const FusedBiasPlan&, int, hipStream_t);
 void launch_step_bias(const StepView&, const FusedBiasPlan&, int, hipStream_t);
 void launch_step_bias_chunks(const StepView&, const FusedBiasPlan&, int, hipStream_t);
 void launch_cv_fused(const StepView&, const CVFusedPlan&, int, hipStream_t);
@@ -982,10 +983,41 @@ StepView Engine::view(const float4* posq, long long* force) const
 // One step for walkers [w0, w0+nw): CVs by kind, then biases, then the
 // scatter (the ordering contract of CommonGluedKernels.cpp:4646-5048).
 // ---------------------------------------------------------------------------
+// Plans small enough for the whole batched step to run in one workgroup per
+// walker: few-atom CVs only, no dense energy CV, the finalize's LDS limits,
+// and at most 16 chunks of streamed bias state per walker.
+bool Engine::tinyPlan() const
+{
+    return group_.empty() && groupBig_.empty() && path_.empty() && coord_.empty() && testGrads_.empty() &&
+           energyValue_ < 0 && fplan_.scatter && V_ <= 1024 && G_ <= 512 && combSlot_.size() <= 1024 &&
+           fplan_.totalChunks <= 16;
+}
+
 void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMask, bool fused)
 {
     StepView v = view(lastPos_, lastForce_);
     v.S = S_;
+    if (fused && stageMask == 7 && w0 == 0 && nw == W_ && tinyPlan()) {
+        // the whole step in one launch (CVs, chunks, finalize and scatter per workgroup)
+        FusedBiasPlan fp = fplan_;
+        fp.st.grids = dGridTable_.as<GridBiasDev>();
+        fp.st.stateConst = dSmallConst_.as<double>();
+        fp.st.stateVar = dSmallVar_.as<double>();
+        fp.st.scal = dSmallScal_.as<double>();
+        fp.st.stateVarStride = smallVarPerWalker_;
+        fp.st.scalStride = 4 * (long long)std::max((int)smallBias_.size(), 1);
+        fp.sp = scatterPlan();
+        fp.scatter = forces ? 1 : 0;
+        launch_step_tiny(v, dSmall_.as<SmallCVItem>(), (int)small_.size(), fp, W_, s);
+        // the every-step Welford update reads only the CV values
+        for (auto* o : opes_)
+            if (o->d.adaptiveStride > 0) {
+                launch_opes_welford(v, o->d, 0, W_, s);
+                if (!capturing_) bumpWelfordMirrors(o, 0, W_);
+            }
+        hip_check(hipGetLastError(), "kernel launch");
+        return;
+    }
     if (stageMask & 1) {
     if (fused && w0 == 0 && nw == W_) {
         CVFusedPlan cp;

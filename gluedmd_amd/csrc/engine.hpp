@@ -136,6 +136,7 @@ class Engine {
     void launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMask = 7, bool fused = false);
     StepView view(const float4* posq, long long* force) const;
     ScatterPlan scatterPlan() const;
+    bool tinyPlan() const;
     void uploadGidx(int w);
     void sync(hipStream_t s);
 

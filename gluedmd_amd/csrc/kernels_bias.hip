@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 #include "scatter_dev.h"
 #include "wave_reduce.h"
+#include "cv_small_dev.h"
 #include <cmath>
 
 namespace glued {
@@ -226,14 +227,37 @@ __device__ inline void block_partial(double (&acc)[NV], double* sh, double* dst)
     __syncthreads();
 }
 
+// One chunk of a hill list in two phases: the loads (every load of the chunk
+// issued before any arithmetic) and the sum at the walker's CV values.
 template <int D>
-__device__ void hill_chunk(const StepView& v, const HillBiasDev& h, int w, int chunk, double* sh, double* dst)
+struct HillChunkRegs {
+    double cx[PER_THREAD][D];
+    double hk[PER_THREAD];  // 0 past the end of the list: such entries add exactly zero
+};
+
+template <int D>
+__device__ inline void hill_chunk_load(const HillBiasDev& h, int w, int chunk, HillChunkRegs<D>& R)
 {
-#pragma clang fp contract(off)
     const int tb = h.shared ? 0 : w;
     const double* __restrict__ C = h.centers + (size_t)tb * D * h.cap;
     const double* __restrict__ H = h.heights + (size_t)tb * h.cap;
     const int n = h.count[tb];
+    const int k0 = chunk * CHUNK + threadIdx.x;
+#pragma unroll
+    for (int r = 0; r < PER_THREAD; r++) {
+        const int k = k0 + r * BIAS_BLOCK;
+        const bool ok = k < n;
+#pragma unroll
+        for (int d = 0; d < D; d++) R.cx[r][d] = ok ? C[(size_t)d * h.cap + k] : 0.0;
+        R.hk[r] = ok ? H[k] : 0.0;
+    }
+}
+
+template <int D>
+__device__ inline void hill_chunk_sum(const StepView& v, const HillBiasDev& h, int w, const HillChunkRegs<D>& R,
+                                      double* sh, double* dst)
+{
+#pragma clang fp contract(off)
     const double* cv = v.cvVal + (size_t)w * v.V;
     double s[D];
 #pragma unroll
@@ -241,34 +265,30 @@ __device__ void hill_chunk(const StepView& v, const HillBiasDev& h, int w, int c
     double acc[D + 1];
 #pragma unroll
     for (int d = 0; d <= D; d++) acc[d] = 0.0;
-    const int k0 = chunk * CHUNK + threadIdx.x;
-    // issue every load of the chunk before the arithmetic
-    double cx[PER_THREAD][D], hk[PER_THREAD];
-#pragma unroll
-    for (int r = 0; r < PER_THREAD; r++) {
-        const int k = k0 + r * BIAS_BLOCK;
-        const bool ok = k < n;
-#pragma unroll
-        for (int d = 0; d < D; d++) cx[r][d] = ok ? C[(size_t)d * h.cap + k] : 0.0;
-        hk[r] = ok ? H[k] : 0.0;
-    }
 #pragma unroll
     for (int r = 0; r < PER_THREAD; r++) {
         double e = 0.0, dc[D];
 #pragma unroll
         for (int d = 0; d < D; d++) {
-            double x = s[d] - cx[r][d];
+            double x = s[d] - R.cx[r][d];
             if (h.period[d] > 0.0) x = x - rint(x * h.invPeriod[d]) * h.period[d];
             dc[d] = x;
             e = e + x * x * h.i2s2[d];
         }
-        // hk = 0 for entries past the end: they add exactly zero
-        const double t = (e <= 8.0) ? hk[r] * (double)expf(-(float)e) : 0.0;
+        const double t = (e <= 8.0) ? R.hk[r] * (double)expf(-(float)e) : 0.0;
         acc[0] += t;
 #pragma unroll
         for (int d = 0; d < D; d++) acc[1 + d] -= t * dc[d] * h.is2[d];
     }
     block_partial<D + 1>(acc, sh, dst);
+}
+
+template <int D>
+__device__ void hill_chunk(const StepView& v, const HillBiasDev& h, int w, int chunk, double* sh, double* dst)
+{
+    HillChunkRegs<D> R;
+    hill_chunk_load<D>(h, w, chunk, R);
+    hill_chunk_sum<D>(v, h, w, R, sh, dst);
 }
 
 template <int D>
@@ -477,11 +497,8 @@ constexpr int MAX_LDS_VALUES = 1024;
 // (OPES) biases, all of dimension DD -- the specialisations keep register use
 // low enough for every workgroup of a typical step to be resident at once.
 template <int KIND, int DD>
-__global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_chunks(StepView v, FusedBiasPlan fp)
+__device__ inline void chunk_block(const StepView& v, const FusedBiasPlan& fp, int b, int w, double* sh)
 {
-    __shared__ double sh[PW * 4];
-    const int w = blockIdx.y;
-    const int b = blockIdx.x;
     int k = 0;
     while (k + 1 < fp.nChunked && b >= fp.chunkStart[k + 1]) k++;
     const int chunk = b - fp.chunkStart[k];
@@ -507,6 +524,13 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_chunks(StepView v, Fus
             default: opes_chunk<4>(v, o, w, chunk, sh, dst); break;
         }
     }
+}
+
+template <int KIND, int DD>
+__global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_chunks(StepView v, FusedBiasPlan fp)
+{
+    __shared__ double sh[PW * 4];
+    chunk_block<KIND, DD>(v, fp, blockIdx.x, blockIdx.y, sh);
 }
 
 static void launch_chunks(const StepView& v, const FusedBiasPlan& fp, int nw, hipStream_t s)
@@ -537,13 +561,13 @@ constexpr int FIN_OWNERS = 3 * 64;     // slot-owning threads (waves 0, 2, 3)
 constexpr int FIN_LDS_G = 512;         // bias-gradient slots held in LDS
 constexpr int FIN_LDS_COMB = 1024;     // combine entries held in LDS
 
-__global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, FusedBiasPlan fp)
+__device__ void finalize_walker(const StepView& v, const FusedBiasPlan& fp, int w)
 {
     __shared__ double sU[MAX_LDS_VALUES];
     __shared__ double sBg[FIN_LDS_G];
     __shared__ int sCombStart[MAX_LDS_VALUES + 1];
     __shared__ int sCombSlot[FIN_LDS_COMB];
-    const int w = blockIdx.x, tid = threadIdx.x;
+    const int tid = threadIdx.x;
     const ScatterPlan& sp = fp.sp;
     const bool ldsComb = v.V <= MAX_LDS_VALUES && v.G <= FIN_LDS_G && sp.nComb <= FIN_LDS_COMB && !sp.testGrad;
     double* bgL = ldsComb ? sBg : nullptr;
@@ -634,6 +658,65 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, F
         }
     }
     for (int s = tid + FIN_SLOTS * FIN_OWNERS; s < v.S; s += BIAS_BLOCK) scatter_slot(v, sp, w, s, sU, MAX_LDS_VALUES);
+}
+
+__global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, FusedBiasPlan fp)
+{
+    finalize_walker(v, fp, blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// The whole batched step of a small plan in one workgroup per walker:
+// few-atom CVs, then every chunk of the chunked biases in index order (the
+// chunk workgroups' code, partials to the same buffer), then the finalize.
+// One launch instead of three for plans whose launches are mostly latency
+// (a few atoms, a hill list of a few thousand entries); the same device
+// code in the same order, so the results equal the other forms bit for bit.
+// Every hand-off between waves stays inside the workgroup (barriers).
+// ---------------------------------------------------------------------------
+constexpr int TINY_PREFETCH_CHUNKS = 4;  // one hill list of up to 4,096 entries: loads issued before the CVs
+
+template <int KIND, int DD>
+__global__ __launch_bounds__(BIAS_BLOCK) void k_step_tiny(StepView v, const SmallCVItem* __restrict__ cvItems, int nCv,
+                                                          FusedBiasPlan fp)
+{
+    __shared__ double sh[PW * 4];
+    const int w = blockIdx.x;
+    if constexpr (KIND == 1) {
+        if (fp.nChunked == 1 && fp.totalChunks <= TINY_PREFETCH_CHUNKS) {
+            // the hill loads do not depend on the CVs: in flight during the CV evaluation
+            const HillBiasDev& h = fp.hills[0];
+            HillChunkRegs<DD> R[TINY_PREFETCH_CHUNKS];
+#pragma unroll
+            for (int c = 0; c < TINY_PREFETCH_CHUNKS; c++)
+                if (c < fp.totalChunks) hill_chunk_load<DD>(h, w, c, R[c]);
+            for (int i = threadIdx.x; i < nCv; i += BIAS_BLOCK) small_item(v, cvItems[i], w);
+            __syncthreads();
+#pragma unroll
+            for (int c = 0; c < TINY_PREFETCH_CHUNKS; c++)
+                if (c < fp.totalChunks)
+                    hill_chunk_sum<DD>(v, h, w, R[c], sh, fp.part + ((size_t)w * fp.totalChunks + c) * PW);
+            finalize_walker(v, fp, w);
+            return;
+        }
+    }
+    for (int i = threadIdx.x; i < nCv; i += BIAS_BLOCK) small_item(v, cvItems[i], w);
+    __syncthreads();
+    for (int b = 0; b < fp.totalChunks; b++) chunk_block<KIND, DD>(v, fp, b, w, sh);  // each ends with a barrier
+    finalize_walker(v, fp, w);
+}
+
+void launch_step_tiny(const StepView& v, const SmallCVItem* cvItems, int nCv, const FusedBiasPlan& fp, int nw,
+                      hipStream_t s)
+{
+    const dim3 g(nw), b(BIAS_BLOCK);
+    if (fp.kind == 1 && fp.kindD == 1) hipLaunchKernelGGL((k_step_tiny<1, 1>), g, b, 0, s, v, cvItems, nCv, fp);
+    else if (fp.kind == 1 && fp.kindD == 2) hipLaunchKernelGGL((k_step_tiny<1, 2>), g, b, 0, s, v, cvItems, nCv, fp);
+    else if (fp.kind == 1 && fp.kindD == 3) hipLaunchKernelGGL((k_step_tiny<1, 3>), g, b, 0, s, v, cvItems, nCv, fp);
+    else if (fp.kind == 2 && fp.kindD == 1) hipLaunchKernelGGL((k_step_tiny<2, 1>), g, b, 0, s, v, cvItems, nCv, fp);
+    else if (fp.kind == 2 && fp.kindD == 2) hipLaunchKernelGGL((k_step_tiny<2, 2>), g, b, 0, s, v, cvItems, nCv, fp);
+    else if (fp.kind == 2 && fp.kindD == 3) hipLaunchKernelGGL((k_step_tiny<2, 3>), g, b, 0, s, v, cvItems, nCv, fp);
+    else hipLaunchKernelGGL((k_step_tiny<0, 1>), g, b, 0, s, v, cvItems, nCv, fp);
 }
 
 void launch_step_bias_chunks(const StepView& v, const FusedBiasPlan& fp, int nw, hipStream_t s)

@@ -297,3 +297,59 @@ def test_dense_and_slot_scatter_combined():
     g.execute_all_plain()
     for w in range(W):
         assert np.array_equal(g.get_forces_fixed(w), Fb[w]), w
+
+
+def test_tiny_step_matches_plain():
+    """Plans of few-atom CVs run the whole batched step in one workgroup per
+    walker (k_step_tiny).  Config 1 over 60 frames with a WT deposit per frame
+    (batched vs plain: forces, energies and hill lists bit for bit), then 6
+    walkers with OPES, shared grid MetaD and private hills over deposits
+    (batched vs plain per walker, forces, energies and bias state; the plain
+    form is checked against the oracle by the other tests)."""
+    from gluedmd_amd.context import Context
+    c = wl.config1(n_frames=60)
+    f = c["force"]
+    ga = Context(f, c["natoms"])
+    gp = Context(f, c["natoms"])
+    for g in (ga, gp):
+        g.set_box(c["box"])
+        g.set_hills(0, c["hill_centers"], c["hill_heights"])
+    for step, pos in enumerate(c["frames"]):
+        for g in (ga, gp):
+            g.update_state(step)
+            g.set_positions(pos)
+            g.zero_forces()
+        ga.execute_batch(use_graph=(step % 2 == 0))
+        gp.execute_all_plain()
+        assert np.array_equal(ga.get_forces_fixed(0), gp.get_forces_fixed(0)), step
+        assert np.array_equal(ga.getBiasEnergies(0), gp.getBiasEnergies(0)), step
+    assert np.array_equal(ga.get_hills(0)[1], gp.get_hills(0)[1])
+
+    f2 = Force(pbc=True, temperature=300.0)
+    d = f2.add_distance([0, 7])
+    phi = f2.add_dihedral([1, 2, 3, 4])
+    f2.add_opes([d, phi], sigma=[0.05, 0.2], gamma=10.0, pace=2, max_kernels=64)
+    gb = f2.add_metad([d, phi], sigma=[0.05, 0.3], height=1.0, pace=3, grid_min=[0.0, -math.pi],
+                      grid_max=[3.0, math.pi], bins=[60, 40], periodic=[False, True], gamma=8.0)
+    f2.share_bias(gb)
+    f2.add_metad_hills([d, phi], sigma=[0.05, 0.3], height=1.0, pace=1, gamma=8.0, max_hills=32,
+                       period=[0.0, 2 * math.pi])
+    W, N, L = 6, 12, 2.0
+    rng = np.random.default_rng(31)
+    gA, gP = Context(f2, walkers=[N] * W), Context(f2, walkers=[N] * W)
+    for g in (gA, gP):
+        g.set_box(L)
+    base = [rng.uniform(0, L, size=(N, 3)) for _ in range(W)]
+    for step in range(8):
+        pos = [(b + rng.normal(0, 0.03, size=b.shape)).astype(np.float32) for b in base]
+        for g in (gA, gP):
+            g.update_state(step)
+            g.set_all_positions(pos)
+            g.zero_forces()
+        gA.execute_batch(use_graph=False)
+        gP.execute_all_plain()
+        for w in range(W):
+            assert np.array_equal(gA.get_forces_fixed(w), gP.get_forces_fixed(w)), (step, w)
+            assert np.array_equal(gA.getBiasEnergies(w), gP.getBiasEnergies(w)), (step, w)
+    for w in range(W):
+        assert gA.getBiasState(w) == gP.getBiasState(w)
