@@ -29,6 +29,28 @@ __device__ inline void pbc_exact(double& x, double& y, double& z, const double* 
     x = x - s1 * bx[0];
 }
 
+// The walker's box (the 3x3 matrix and the reciprocal diagonal) read once
+// into scalar registers.  Every caller's walker index is uniform over the
+// workgroup.  Minimum-image code that reads the box from memory inside a loop
+// would issue those loads after earlier row stores, and on this target such a
+// load waits for the stores to complete.
+struct WalkerBox {
+    double b[12];
+};
+
+__device__ inline WalkerBox load_box(const StepView& v, int w)
+{
+    WalkerBox r;
+    const double* bx = v.box + (size_t)w * 12;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        const double x = bx[k];
+        r.b[k] = __hiloint2double(__builtin_amdgcn_readfirstlane(__double2hiint(x)),
+                                  __builtin_amdgcn_readfirstlane(__double2loint(x)));
+    }
+    return r;
+}
+
 __device__ inline double dot3_exact(double x, double y, double z)
 {
 #pragma clang fp contract(off)
@@ -42,12 +64,25 @@ __device__ inline float4 load_pos(const StepView& v, int w, int slot)
     return v.posq[wi.posOff + g];
 }
 
-__device__ inline void store_row(const StepView& v, int w, int row, double gx, double gy, double gz)
+// Index of Jacobian row `row` of walker w in the row arrays.  CV code reads
+// these with its operands, before any row is stored: the vector memory
+// counter also counts stores, so a table load issued after a row store first
+// waits for that store to complete.
+__device__ inline size_t row_index(const StepView& v, int w, int row)
 {
-    size_t i = (size_t)w * v.R + v.rowPos[row];
+    return (size_t)w * v.R + v.rowPos[row];
+}
+
+__device__ inline void store_row_at(const StepView& v, size_t i, double gx, double gy, double gz)
+{
     v.jx[i] = (float)gx;
     v.jy[i] = (float)gy;
     v.jz[i] = (float)gz;
+}
+
+__device__ inline void store_row(const StepView& v, int w, int row, double gx, double gy, double gz)
+{
+    store_row_at(v, row_index(v, w, row), gx, gy, gz);
 }
 
 // ---------------------------------------------------------------------------
@@ -60,7 +95,6 @@ __device__ inline void small_item(const StepView& v, const SmallCVItem& it, int 
         v.cvVal[(size_t)w * v.V + it.value] = v.energyU[w];
         return;
     }
-    const double* bx = v.box + (size_t)w * 12;
     double p[4][3];
     const int na = it.type == CV_DISTANCE ? 2 : it.type == CV_ANGLE ? 3 : it.type == CV_DIHEDRAL ? 4 : 1;
     // every gather issued before the first use (independent loads in flight together)
@@ -71,6 +105,11 @@ __device__ inline void small_item(const StepView& v, const SmallCVItem& it, int 
         p[a][1] = q.y;
         p[a][2] = q.z;
     }
+    size_t ri[4];
+#pragma unroll
+    for (int a = 0; a < 4; a++) ri[a] = row_index(v, w, it.row0 + (a < na ? a : 0));
+    const WalkerBox wb = load_box(v, w);
+    const double* bx = wb.b;
     double val = 0.0;
     double* cvv = v.cvVal + (size_t)w * v.V;
     if (it.type == CV_DISTANCE) {
@@ -79,8 +118,8 @@ __device__ inline void small_item(const StepView& v, const SmallCVItem& it, int 
         double d = sqrt(dx * dx + dy * dy + dz * dz);
         double inv = d > 1e-10 ? 1.0 / d : 0.0;
         val = d;
-        store_row(v, w, it.row0, -dx * inv, -dy * inv, -dz * inv);
-        store_row(v, w, it.row0 + 1, dx * inv, dy * inv, dz * inv);
+        store_row_at(v, ri[0], -dx * inv, -dy * inv, -dz * inv);
+        store_row_at(v, ri[1], dx * inv, dy * inv, dz * inv);
     } else if (it.type == CV_ANGLE) {
         double r1[3], r2[3];
         for (int k = 0; k < 3; k++) {
@@ -109,9 +148,9 @@ __device__ inline void small_item(const StepView& v, const SmallCVItem& it, int 
             ga[k] = -(n2[k] - c * n1[k]) * i1 * is;
             gc[k] = -(n1[k] - c * n2[k]) * i2 * is;
         }
-        store_row(v, w, it.row0, ga[0], ga[1], ga[2]);
-        store_row(v, w, it.row0 + 1, -(ga[0] + gc[0]), -(ga[1] + gc[1]), -(ga[2] + gc[2]));
-        store_row(v, w, it.row0 + 2, gc[0], gc[1], gc[2]);
+        store_row_at(v, ri[0], ga[0], ga[1], ga[2]);
+        store_row_at(v, ri[1], -(ga[0] + gc[0]), -(ga[1] + gc[1]), -(ga[2] + gc[2]));
+        store_row_at(v, ri[2], gc[0], gc[1], gc[2]);
     } else if (it.type == CV_DIHEDRAL) {
         double b1[3], b2[3], b3[3];
         for (int k = 0; k < 3; k++) {
@@ -140,13 +179,13 @@ __device__ inline void small_item(const StepView& v, const SmallCVItem& it, int 
             jd[k] = aU * u[k];
             jb[k] = bT * t[k] + bU * u[k];
         }
-        store_row(v, w, it.row0, ja[0], ja[1], ja[2]);
-        store_row(v, w, it.row0 + 1, jb[0], jb[1], jb[2]);
-        store_row(v, w, it.row0 + 2, -(ja[0] + jb[0] + jd[0]), -(ja[1] + jb[1] + jd[1]), -(ja[2] + jb[2] + jd[2]));
-        store_row(v, w, it.row0 + 3, jd[0], jd[1], jd[2]);
+        store_row_at(v, ri[0], ja[0], ja[1], ja[2]);
+        store_row_at(v, ri[1], jb[0], jb[1], jb[2]);
+        store_row_at(v, ri[2], -(ja[0] + jb[0] + jd[0]), -(ja[1] + jb[1] + jd[1]), -(ja[2] + jb[2] + jd[2]));
+        store_row_at(v, ri[3], jd[0], jd[1], jd[2]);
     } else if (it.type == CV_POSITION) {
         val = p[0][it.comp];
-        store_row(v, w, it.row0, it.comp == 0, it.comp == 1, it.comp == 2);
+        store_row_at(v, ri[0], it.comp == 0, it.comp == 1, it.comp == 2);
     }
     cvv[it.value] = val;
 }

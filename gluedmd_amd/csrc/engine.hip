@@ -39,6 +39,7 @@ void launch_step_tiny(const StepView&, const SmallCVItem*, int, const FusedBiasP
 void launch_step_bias(const StepView&, const FusedBiasPlan&, int, hipStream_t);
 void launch_step_bias_chunks(const StepView&, const FusedBiasPlan&, int, hipStream_t);
 void launch_cv_fused(const StepView&, const CVFusedPlan&, int, hipStream_t);
+void launch_cv_cm_chunks(const StepView&, const CVFusedPlan&, int, int, hipStream_t);
 void launch_hill_adjoint(const StepView&, const HillBiasDev&, const double*, double*, double*, double*, int,
                          hipStream_t);
 int hill_adjoint_blocks(int);
@@ -474,11 +475,21 @@ void Engine::buildPlan(int nMass, const double* masses)
     dSmall_.upload(small_);
     {
         // atom / pair lists longer than one 256-thread workgroup holds in
-        // registers go to the 1,024-thread group kernel
+        // registers go to the 1,024-thread group kernel; contact maps longer
+        // than one chunk are evaluated in chunks (partial sums + last chunk)
         std::vector<GroupCVItem> small, big;
-        for (const auto& it : group_) {
+        cm_.clear();
+        cmBlocks_ = 0;
+        for (GroupCVItem it : group_) {
             const bool pairs = it.type == CV_DRMSD || it.type == CV_CONTACTMAP;
             const int n = pairs ? (it.end - it.begin) / 2 : it.end - it.begin;
+            if (it.type == CV_CONTACTMAP && n > CM_CHUNK) {
+                it.chunk0 = cmBlocks_;
+                it.chunkItem = (int)cm_.size();
+                cmBlocks_ += (n + CM_CHUNK - 1) / CM_CHUNK;
+                cm_.push_back(it);
+                continue;
+            }
             const bool fit = it.type == CV_RMSD && it.split;  // Kabsch stays on the 256-thread kernels
             (n > 1024 && !fit ? big : small).push_back(it);
         }
@@ -487,6 +498,9 @@ void Engine::buildPlan(int nMass, const double* masses)
     }
     dGroup_.upload(group_);
     dGroupBig_.upload(groupBig_);
+    dCm_.upload(cm_);
+    dCmPart_.alloc(sizeof(double) * (size_t)W_ * std::max(cmBlocks_, 1));
+    dCmCount_.alloc(sizeof(unsigned int) * (size_t)W_ * std::max((int)cm_.size(), 1));
     dGroupMass_.upload(groupMass);
     dGroupRef_.upload(groupRef);
     dPath_.upload(path_);
@@ -988,7 +1002,7 @@ StepView Engine::view(const float4* posq, long long* force) const
 // and at most 16 chunks of streamed bias state per walker.
 bool Engine::tinyPlan() const
 {
-    return group_.empty() && groupBig_.empty() && path_.empty() && coord_.empty() && testGrads_.empty() &&
+    return group_.empty() && groupBig_.empty() && cm_.empty() && path_.empty() && coord_.empty() && testGrads_.empty() &&
            energyValue_ < 0 && fplan_.scatter && V_ <= 1024 && G_ <= 512 && combSlot_.size() <= 1024 &&
            fplan_.totalChunks <= 16;
 }
@@ -1019,20 +1033,25 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         return;
     }
     if (stageMask & 1) {
+    CVFusedPlan cp;
+    cp.small = dSmall_.as<SmallCVItem>();
+    cp.nSmall = (int)small_.size();
+    cp.group = dGroup_.as<GroupCVItem>();
+    cp.nGroup = (int)group_.size();
+    cp.gslots = dGroupPos_.as<int>();
+    cp.gmass = dGroupMass_.as<float>();
+    cp.gref = dGroupRef_.as<float>();
+    cp.path = dPath_.as<PathItem>();
+    cp.nPath = (int)path_.size();
+    cp.nPathMsdBlocks = pathMsdBlocks_;
+    cp.pslots = dPathPos_.as<int>();
+    cp.frames = dPathFrames_.as<float>();
+    cp.cm = dCm_.as<GroupCVItem>();
+    cp.nCm = (int)cm_.size();
+    cp.nCmBlocks = cmBlocks_;
+    cp.cmPart = dCmPart_.as<double>();
+    cp.cmCount = dCmCount_.as<unsigned int>();
     if (fused && w0 == 0 && nw == W_) {
-        CVFusedPlan cp;
-        cp.small = dSmall_.as<SmallCVItem>();
-        cp.nSmall = (int)small_.size();
-        cp.group = dGroup_.as<GroupCVItem>();
-        cp.nGroup = (int)group_.size();
-        cp.gslots = dGroupPos_.as<int>();
-        cp.gmass = dGroupMass_.as<float>();
-        cp.gref = dGroupRef_.as<float>();
-        cp.path = dPath_.as<PathItem>();
-        cp.nPath = (int)path_.size();
-        cp.nPathMsdBlocks = pathMsdBlocks_;
-        cp.pslots = dPathPos_.as<int>();
-        cp.frames = dPathFrames_.as<float>();
         launch_cv_fused(v, cp, nw, s);
         launch_cv_group_big(v, dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupPos_.as<int>(),
                             dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
@@ -1042,6 +1061,7 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         launch_cv_small(v, dSmall_.as<SmallCVItem>(), (int)small_.size(), w0, nw, s);
         launch_cv_group(v, dGroup_.as<GroupCVItem>(), (int)group_.size(), dGroupPos_.as<int>(),
                         dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
+        launch_cv_cm_chunks(v, cp, w0, nw, s);
         launch_cv_group_big(v, dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupPos_.as<int>(),
                             dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
         launch_cv_path(v, dPath_.as<PathItem>(), (int)path_.size(), pathMsdBlocks_, pathRowBlocks_,

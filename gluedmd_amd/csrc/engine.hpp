@@ -159,6 +159,8 @@ class Engine {
     // items
     std::vector<SmallCVItem> small_;
     std::vector<GroupCVItem> group_, groupBig_;  // groupBig_: more than 1,024 atoms / pairs
+    std::vector<GroupCVItem> cm_;                // contact maps of more than CM_CHUNK pairs, in chunks
+    int cmBlocks_ = 0;
     std::vector<CoordItem> coord_;
     std::vector<PathItem> path_;
     int energyValue_ = -1;
@@ -210,6 +212,7 @@ class Engine {
     // device buffers
     DBuf dWalkers_, dGidx_, dBox_, dCv_, dJx_, dJy_, dJz_, dBgrad_, dBener_, dUd_;
     DBuf dSmall_, dGroup_, dGroupBig_, dGroupPos_, dGroupMass_, dGroupRef_;
+    DBuf dCm_, dCmPart_, dCmCount_;
     DBuf dCoord_, dCoordSlots_, dCellOf_, dCellCount_, dCellStart_, dSorted_, dUnsorted_, dCnAcc_, dCnAccLo_, dPairAcc_,
         dNpairs_, dRank_, dBlockCnt_;
     int cellOfStride_ = 0, cellStride_ = 0, sortStride_ = 0, blkStride_ = 0;

@@ -107,7 +107,13 @@ struct GroupCVItem {
     double refCentroid[3];
     double refSum[3];     // Kabsch: sum of y - refCentroid over the reference (FP64, host)
     double refSq;         // Kabsch: sum of |y - refCentroid|^2
+    int chunk0;           // contact map in chunks: first chunk (over all chunked maps)
+    int chunkItem;        // contact map in chunks: index of its partial counter
 };
+
+// Contact maps longer than this many pairs are evaluated in chunks of this
+// many pairs, one 256-thread workgroup per chunk (2 pairs per thread).
+constexpr int CM_CHUNK = 512;
 
 // Coordination CV with optional cutoff (cell list when the box allows it).
 struct CoordItem {
@@ -293,6 +299,11 @@ struct CVFusedPlan {
     int nPathMsdBlocks;
     const int* pslots;
     const float* frames;
+    const GroupCVItem* cm;  // contact maps in chunks
+    int nCm;
+    int nCmBlocks;          // chunks over all of them
+    double* cmPart;         // [W][nCmBlocks] chunk partial sums
+    unsigned int* cmCount;  // [W][nCm] chunks finished (the last resets it)
 };
 
 }  // namespace glued

@@ -410,23 +410,6 @@ __device__ inline void opes_finalize(const StepView& v, const OpesBiasDev& o, in
     }
 }
 
-// Ticket: true in every thread of the workgroup that finished last.  Only
-// thread 0 wrote this workgroup's partial, so only it fences (an agent-scope
-// fence writes back the XCD's L2 and is expensive on this part: one per
-// workgroup, never one per wave).
-__device__ inline bool last_block(unsigned int* counter, unsigned int total)
-{
-    __shared__ int shLast;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        shLast = atomicAdd(counter, 1u) == total - 1;
-        if (shLast) __threadfence();
-    }
-    __syncthreads();
-    return shLast != 0;
-}
-
 // ---- plain form: one launch per chunked bias ----
 template <int D>
 __global__ __launch_bounds__(BIAS_BLOCK) void k_bias_hills(StepView v, HillBiasDev h, int w0)

@@ -232,6 +232,47 @@ __device__ inline double ipowd(double x, int n)
     return r;
 }
 
+// x^n for a small non-negative integer n by repeated squaring.
+__device__ inline double ipow_sq(double x, int n)
+{
+    double r = 1.0;
+    while (n > 0) {
+        if (n & 1) r *= x;
+        x *= x;
+        n >>= 1;
+    }
+    return r;
+}
+
+// One contact-map pair, CommonGluedKernels.cpp:1236-1323 (in FP64: the
+// reference's FP32 loses digits in 1 - x^m near x = 1), for the minimum-image
+// difference dr = b - a and the pair's parameters {r0, n, m, w}.  Returns
+// w s(|dr|) and sets c, the factor of the pair's rows: row(b) = c dr,
+// row(a) = -c dr.  |1 - x^m| < 1e-8 gives s = n/m and no derivative.
+__device__ inline double contact_pair(double dx, double dy, double dz, const float4& par, double& c)
+{
+    const double r0 = par.x, wgt = par.w;
+    const int nn = (int)par.y, mm = (int)par.z;
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    const double invd = d > 1e-10 ? 1.0 / d : 0.0;
+    const double ir0 = 1.0 / r0, x = d * ir0;
+    // x^(n-1), x^(m-1) by squaring; x^n, x^m one product further
+    const double xn1 = nn > 0 ? ipow_sq(x, nn - 1) : 0.0, xm1 = mm > 0 ? ipow_sq(x, mm - 1) : 0.0;
+    const double xn = nn > 0 ? xn1 * x : 1.0, xm = mm > 0 ? xm1 * x : 1.0;
+    const double num = 1.0 - xn, den = 1.0 - xm;
+    double sw, dsdd;
+    if (fabs(den) < 1e-8) {
+        sw = (double)nn / (double)mm;
+        dsdd = 0.0;
+    } else {
+        const double iden = 1.0 / den;
+        sw = num * iden;
+        dsdd = (-(double)nn * xn1 * den + (double)mm * xm1 * num) * iden * iden * ir0;
+    }
+    c = wgt * dsdd * invd;
+    return wgt * sw;
+}
+
 // Kabsch RMSD of up to 256 * KABSCH_PER_THREAD atoms by one 256-thread
 // workgroup: every thread keeps its atoms and their reference in registers,
 // a wave reduce-scatter and one LDS exchange give every thread the raw moments
@@ -331,7 +372,8 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
 {
     __shared__ double sh[17 * 16];
     __shared__ double shR[9];
-    const double* bx = v.box + (size_t)w * 12;
+    const WalkerBox wb = load_box(v, w);
+    const double* bx = wb.b;
     double* cvv = v.cvVal + (size_t)w * v.V;
     const int tid = threadIdx.x;
     constexpr int nt = NT;
@@ -343,11 +385,13 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
         // registers (one gather round for every pass when n <= REG * nt)
         float4 q[REG];
         double mj[REG];
+        size_t rr[REG];
 #pragma unroll
         for (int r = 0; r < REG; r++) {  // clamped index: no divergent loads
             const int j = min(tid + r * nt, n - 1);
             q[r] = gather_pos(v, slots, v.gStride, w, it.begin + j);
             mj[r] = mass[it.begin + j];
+            rr[r] = row_index(v, w, it.row0 + j);
         }
         // every atom of this thread: registers first (static indices), then
         // the tail past REG * nt with fresh gathers
@@ -355,13 +399,14 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
 #pragma unroll
             for (int r = 0; r < REG; r++) {
                 const int j = tid + r * nt;
-                if (j < n) f(j, q[r], mj[r]);
+                if (j < n) f(j, q[r], mj[r], rr[r]);
             }
             for (int j = tid + REG * nt; j < n; j += nt)
-                f(j, gather_pos(v, slots, v.gStride, w, it.begin + j), (double)mass[it.begin + j]);
+                f(j, gather_pos(v, slots, v.gStride, w, it.begin + j), (double)mass[it.begin + j],
+                  row_index(v, w, it.row0 + j));
         };
         double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // m*x (g1), M1, m*x (g2), M2
-        visit([&](int j, const float4& qq, double m) {
+        visit([&](int j, const float4& qq, double m, size_t) {
             const bool g1 = j < n1;  // static indices only (a runtime index would spill acc)
             const double m1 = g1 ? m : 0.0, m2 = g1 ? 0.0 : m;
             acc[0] += m1 * qq.x;
@@ -378,7 +423,7 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
             const double M = acc[3];
             const double cx = acc[0] / M, cy = acc[1] / M, cz = acc[2] / M;
             double r2[1] = {0.0};
-            visit([&](int j, const float4& qq, double m) {
+            visit([&](int j, const float4& qq, double m, size_t) {
                 double dx = qq.x - cx, dy = qq.y - cy, dz = qq.z - cz;
                 if (v.pbc) pbc_exact(dx, dy, dz, bx);
                 r2[0] += m * (dx * dx + dy * dy + dz * dz);
@@ -386,11 +431,11 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
             block_sum<1, NT / 64>(r2, sh);
             const double rg = sqrt(r2[0] / M);
             const double inv = rg > 1e-10 ? 1.0 / rg : 0.0;
-            visit([&](int j, const float4& qq, double m) {
+            visit([&](int j, const float4& qq, double m, size_t ri) {
                 double dx = qq.x - cx, dy = qq.y - cy, dz = qq.z - cz;
                 if (v.pbc) pbc_exact(dx, dy, dz, bx);
                 const double sc = m / M * inv;
-                store_row(v, w, it.row0 + j, sc * dx, sc * dy, sc * dz);
+                store_row_at(v, ri, sc * dx, sc * dy, sc * dz);
             });
             if (tid == 0) cvv[it.value] = rg;
         } else {
@@ -400,11 +445,10 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
             if (v.pbc) pbc_exact(dx, dy, dz, bx);
             double d = sqrt(dx * dx + dy * dy + dz * dz);
             double inv = d > 1e-10 ? 1.0 / d : 0.0;
-            for (int j = tid; j < n; j += nt) {
-                double m = mass[it.begin + j];
+            visit([&](int j, const float4&, double m, size_t ri) {
                 double sc = (j < n1 ? -m / acc[3] : m / acc[7]) * inv;
-                store_row(v, w, it.row0 + j, sc * dx, sc * dy, sc * dz);
-            }
+                store_row_at(v, ri, sc * dx, sc * dy, sc * dz);
+            });
             if (tid == 0) cvv[it.value] = d;
         }
         return;
@@ -429,12 +473,14 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
         // registers with their reference (centred for the fit)
         float4 q[REG];
         float yr[REG][3];  // raw FP32 reference; centred on use
+        size_t rr[REG];
 #pragma unroll
         for (int r = 0; r < REG; r++) {  // clamped index: no divergent loads
             const int j = min(tid + r * nt, n - 1);
             q[r] = gather_pos(v, slots, v.gStride, w, it.begin + j);
 #pragma unroll
             for (int k = 0; k < 3; k++) yr[r][k] = Y[3 * j + k];
+            rr[r] = row_index(v, w, it.row0 + j);
         }
         // every atom of this thread with its (centred, for the fit) reference:
         // registers first (static indices), then the tail with fresh gathers
@@ -445,27 +491,27 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
                 if (j < n) {
                     const double yy[3] = {(double)yr[r][0] - (fit ? yc[0] : 0.0), (double)yr[r][1] - (fit ? yc[1] : 0.0),
                                           (double)yr[r][2] - (fit ? yc[2] : 0.0)};
-                    f(j, q[r], yy);
+                    f(j, q[r], yy, rr[r]);
                 }
             }
             for (int j = tid + REG * nt; j < n; j += nt) {
                 const double yy[3] = {(double)Y[3 * j] - (fit ? yc[0] : 0.0), (double)Y[3 * j + 1] - (fit ? yc[1] : 0.0),
                                       (double)Y[3 * j + 2] - (fit ? yc[2] : 0.0)};
-                f(j, gather_pos(v, slots, v.gStride, w, it.begin + j), yy);
+                f(j, gather_pos(v, slots, v.gStride, w, it.begin + j), yy, row_index(v, w, it.row0 + j));
             }
         };
         if (!fit) {
             double s1[1] = {0.0};
-            visit([&](int j, const float4& qq, const double* yy) {
+            visit([&](int j, const float4& qq, const double* yy, size_t) {
                 const double dx = (double)qq.x - yy[0], dy = (double)qq.y - yy[1], dz = (double)qq.z - yy[2];
                 s1[0] += dx * dx + dy * dy + dz * dz;
             });
             block_sum<1, NT / 64>(s1, sh);
             const double rv = sqrt(s1[0] / n);
             const double sc = (rv > 1e-10 ? 1.0 / rv : 0.0) / n;
-            visit([&](int j, const float4& qq, const double* yy) {
+            visit([&](int j, const float4& qq, const double* yy, size_t ri) {
                 const double dx = (double)qq.x - yy[0], dy = (double)qq.y - yy[1], dz = (double)qq.z - yy[2];
-                store_row(v, w, it.row0 + j, dx * sc, dy * sc, dz * sc);
+                store_row_at(v, ri, dx * sc, dy * sc, dz * sc);
             });
             if (tid == 0) cvv[it.value] = rv;
             return;
@@ -479,7 +525,7 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
         double m[KABSCH_MOMENTS];
 #pragma unroll
         for (int k = 0; k < KABSCH_MOMENTS; k++) m[k] = 0.0;
-        visit([&](int j, const float4& qq, const double* y) {
+        visit([&](int j, const float4& qq, const double* y, size_t) {
             const double x[3] = {(double)qq.x, (double)qq.y, (double)qq.z};
 #pragma unroll
             for (int a = 0; a < 3; a++)
@@ -511,77 +557,87 @@ __device__ void group_item(const StepView& v, const GroupCVItem& it, const int* 
         }
         __syncthreads();
         const double sc = shScale;
-        visit([&](int j, const float4& qq, const double* y) {
+        visit([&](int j, const float4& qq, const double* y, size_t ri) {
             const double x0 = qq.x - xc[0], x1 = qq.y - xc[1], x2 = qq.z - xc[2];
             const double r0 = shR[0] * y[0] + shR[1] * y[1] + shR[2] * y[2];
             const double r1 = shR[3] * y[0] + shR[4] * y[1] + shR[5] * y[2];
             const double r2 = shR[6] * y[0] + shR[7] * y[1] + shR[8] * y[2];
-            store_row(v, w, it.row0 + j, (x0 - r0) * sc, (x1 - r1) * sc, (x2 - r2) * sc);
+            store_row_at(v, ri, (x0 - r0) * sc, (x1 - r1) * sc, (x2 - r2) * sc);
         });
         return;
     }
 
+    // Pair CVs (DRMSD, contact map): the first PR pairs of this thread, with
+    // their per-pair parameters, are gathered in one round before any
+    // arithmetic, so a thread waits for memory once rather than once per
+    // pair; pairs past PR * nt are gathered on use.  Pairs are visited in the
+    // same order either way (p = tid, tid + nt, ...), so the sums do not
+    // depend on PR.
+    if (it.type != CV_DRMSD && it.type != CV_CONTACTMAP) return;
+    constexpr int PR = REG / 2 > 0 ? REG / 2 : 1;
+    const int np = (it.end - it.begin) / 2;  // >= 1 (validated)
+    const float* prm = ref + it.refOff;       // DRMSD: d0 per pair; contact map: r0, n, m, w
+    const bool cm = it.type == CV_CONTACTMAP;
+    float4 pa[PR], pb[PR], pp[PR];
+    size_t ra[PR], rb[PR];
+#pragma unroll
+    for (int r = 0; r < PR; r++) {  // clamped index: no divergent loads
+        const int p = min(tid + r * nt, np - 1);
+        pa[r] = gather_pos(v, slots, v.gStride, w, it.begin + 2 * p);
+        pb[r] = gather_pos(v, slots, v.gStride, w, it.begin + 2 * p + 1);
+        ra[r] = row_index(v, w, it.row0 + 2 * p);
+        rb[r] = row_index(v, w, it.row0 + 2 * p + 1);
+        const float* pq = prm + (cm ? 4 * p : p);  // DRMSD reads its one value four times
+        pp[r] = make_float4(pq[0], pq[cm ? 1 : 0], pq[cm ? 2 : 0], pq[cm ? 3 : 0]);
+    }
+    auto visit_pairs = [&](auto&& f) {
+#pragma unroll
+        for (int r = 0; r < PR; r++) {
+            const int p = tid + r * nt;
+            if (p < np) f(p, pa[r], pb[r], pp[r], ra[r], rb[r]);
+        }
+        for (int p = tid + PR * nt; p < np; p += nt) {
+            const float4 par = cm ? make_float4(prm[4 * p], prm[4 * p + 1], prm[4 * p + 2], prm[4 * p + 3])
+                                  : make_float4(prm[p], 0.f, 0.f, 0.f);
+            f(p, gather_pos(v, slots, v.gStride, w, it.begin + 2 * p),
+              gather_pos(v, slots, v.gStride, w, it.begin + 2 * p + 1), par, row_index(v, w, it.row0 + 2 * p),
+              row_index(v, w, it.row0 + 2 * p + 1));
+        }
+    };
+
     if (it.type == CV_DRMSD) {
-        const int np = (it.end - it.begin) / 2;
-        const float* d0 = ref + it.refOff;
         double s[1] = {0.0};
-        for (int p = tid; p < np; p += nt) {
-            float4 a = gather_pos(v, slots, v.gStride, w, it.begin + 2 * p);
-            float4 b = gather_pos(v, slots, v.gStride, w, it.begin + 2 * p + 1);
+        visit_pairs([&](int p, const float4& a, const float4& b, const float4& par, size_t, size_t) {
             double dx = (double)b.x - a.x, dy = (double)b.y - a.y, dz = (double)b.z - a.z;
             if (v.pbc) pbc_exact(dx, dy, dz, bx);
-            double dev = sqrt(dx * dx + dy * dy + dz * dz) - (double)d0[p];
+            double dev = sqrt(dx * dx + dy * dy + dz * dz) - (double)par.x;
             s[0] += dev * dev;
-        }
+        });
         block_sum<1, NT / 64>(s, sh);
         const double val = sqrt(s[0] / np);
         const double inv = val > 1e-10 ? 1.0 / val : 0.0;
-        for (int p = tid; p < np; p += nt) {
-            float4 a = gather_pos(v, slots, v.gStride, w, it.begin + 2 * p);
-            float4 b = gather_pos(v, slots, v.gStride, w, it.begin + 2 * p + 1);
+        visit_pairs([&](int p, const float4& a, const float4& b, const float4& par, size_t ia, size_t ib) {
             double dx = (double)b.x - a.x, dy = (double)b.y - a.y, dz = (double)b.z - a.z;
             if (v.pbc) pbc_exact(dx, dy, dz, bx);
             double d = sqrt(dx * dx + dy * dy + dz * dz);
-            double coeff = (d - (double)d0[p]) / np * inv * (d > 1e-10 ? 1.0 / d : 0.0);
-            store_row(v, w, it.row0 + 2 * p, -coeff * dx, -coeff * dy, -coeff * dz);
-            store_row(v, w, it.row0 + 2 * p + 1, coeff * dx, coeff * dy, coeff * dz);
-        }
+            double coeff = (d - (double)par.x) / np * inv * (d > 1e-10 ? 1.0 / d : 0.0);
+            store_row_at(v, ia, -coeff * dx, -coeff * dy, -coeff * dz);
+            store_row_at(v, ib, coeff * dx, coeff * dy, coeff * dz);
+        });
         if (tid == 0) cvv[it.value] = val;
         return;
     }
 
     if (it.type == CV_CONTACTMAP) {
-        // FP64: a few thousand listed pairs; FP32 loses digits in 1 - x^m near x = 1
-        const int np = (it.end - it.begin) / 2;
-        const float* prm = ref + it.refOff;
         double s[1] = {0.0};
-        for (int p = tid; p < np; p += nt) {
-            float4 a = gather_pos(v, slots, v.gStride, w, it.begin + 2 * p);
-            float4 b = gather_pos(v, slots, v.gStride, w, it.begin + 2 * p + 1);
+        visit_pairs([&](int p, const float4& a, const float4& b, const float4& par, size_t ia, size_t ib) {
             double dx = (double)b.x - a.x, dy = (double)b.y - a.y, dz = (double)b.z - a.z;
             if (v.pbc) pbc_exact(dx, dy, dz, bx);
-            const double r0 = prm[4 * p], wgt = prm[4 * p + 3];
-            const int nn = (int)prm[4 * p + 1], mm = (int)prm[4 * p + 2];
-            const double d = sqrt(dx * dx + dy * dy + dz * dz);
-            const double invd = d > 1e-10 ? 1.0 / d : 0.0;
-            const double ir0 = 1.0 / r0, x = d * ir0;
-            const double xn = ipowd(x, nn), xm = ipowd(x, mm);
-            const double num = 1.0 - xn, den = 1.0 - xm;
-            double sw, dsdd;
-            if (fabs(den) < 1e-8) {
-                sw = (double)nn / (double)mm;
-                dsdd = 0.0;
-            } else {
-                sw = num / den;
-                const double dxn = nn > 0 ? nn * ipowd(x, nn - 1) : 0.0;
-                const double dxm = mm > 0 ? mm * ipowd(x, mm - 1) : 0.0;
-                dsdd = ((-dxn * den + dxm * num) / (den * den)) * ir0;
-            }
-            s[0] += wgt * sw;
-            const double coeff = wgt * dsdd * invd;
-            store_row(v, w, it.row0 + 2 * p, -coeff * dx, -coeff * dy, -coeff * dz);
-            store_row(v, w, it.row0 + 2 * p + 1, coeff * dx, coeff * dy, coeff * dz);
-        }
+            double coeff;
+            s[0] += contact_pair(dx, dy, dz, par, coeff);
+            store_row_at(v, ia, -coeff * dx, -coeff * dy, -coeff * dz);
+            store_row_at(v, ib, coeff * dx, coeff * dy, coeff * dz);
+        });
         block_sum<1, NT / 64>(s, sh);
         if (tid == 0) cvv[it.value] = s[0];
         return;
@@ -927,7 +983,8 @@ __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBu
     if (home0 >= nHome) return;  // whole wave idle (no workgroup barrier below)
     const int home = home0 + h;
     const bool homeOk = home < nHome;
-    const double* bx = v.box + (size_t)w * 12;
+    const WalkerBox wb = load_box(v, w);
+    const double* bx = wb.b;
     float bxf[12];
     for (int k = 0; k < 12; k++) bxf[k] = (float)bx[k];
     const SwitchD sw = make_switch(it);
@@ -1123,7 +1180,8 @@ __global__ __launch_bounds__(COORD_A_BLOCK) void k_coord_pairs_a(StepView v, Coo
     const int w = w0 + blockIdx.y;
     const CoordItem it = items[item];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const double* bx = v.box + (size_t)w * 12;
+    const WalkerBox wb = load_box(v, w);
+    const double* bx = wb.b;
     float bxf[12];
     for (int k = 0; k < 12; k++) bxf[k] = (float)bx[k];
     const SwitchD sw = make_switch(it);
@@ -1310,35 +1368,78 @@ namespace glued {
 // reference, and the rows and lambda derivatives are 0 (the reference divides
 // by S there).  No frame cap at 128 (:3901); up to PATH_MAX_FRAMES.
 // ---------------------------------------------------------------------------
+constexpr int PATH_MSD_THREADS = 256;  // threads of an MSD block (k_path_msd, k_cv_fused)
+constexpr int PATH_MSD_UNROLL = 4;     // atoms per lane whose frame loads are issued together
+constexpr int PATH_ROWS_UNROLL = 8;    // frames per lane whose loads are issued together (row blocks)
+
 __device__ void path_msd_block(const StepView& v, const PathItem& it, const int* __restrict__ slots,
                                const float* __restrict__ frames, int fb, int w)
 {
     __shared__ float shX[3 * PATH_LDS_ATOMS];
+    constexpr int PER = PATH_LDS_ATOMS / PATH_MSD_THREADS;
     const int M = it.M, N = it.N;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int f0 = fb * PATH_FPB + wave * 4;
     const int nf = f0 < N ? min(4, N - f0) : 0;
     const float* F = frames + it.frameOff;
+    const int* tab = slots + (size_t)w * v.pStride + it.begin;
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
     for (int c0 = 0; c0 < M; c0 += PATH_LDS_ATOMS) {
         const int mc = min(PATH_LDS_ATOMS, M - c0);
         __syncthreads();
-        for (int j = threadIdx.x; j < mc; j += blockDim.x) {
-            const float4 q = gather_pos(v, slots, v.pStride, w, it.begin + c0 + j);
-            shX[3 * j] = q.x;
-            shX[3 * j + 1] = q.y;
-            shX[3 * j + 2] = q.z;
+        // staging: the table entries of all of this thread's atoms, then
+        // their positions (two memory round trips per chunk)
+        // (indices clamped into the chunk: every load is issued, none waits
+        // on a branch)
+        int gi[PER];
+#pragma unroll
+        for (int u = 0; u < PER; u++) gi[u] = tab[c0 + min((int)threadIdx.x + u * PATH_MSD_THREADS, mc - 1)];
+        float4 q[PER];
+#pragma unroll
+        for (int u = 0; u < PER; u++) q[u] = v.posq[gi[u]];
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const int j = threadIdx.x + u * PATH_MSD_THREADS;
+            if (j < mc) {
+                shX[3 * j] = q[u].x;
+                shX[3 * j + 1] = q[u].y;
+                shX[3 * j + 2] = q[u].z;
+            }
         }
         __syncthreads();
         if (nf == 0) continue;
-        for (int j = lane; j < mc; j += 64) {
-            const double x = shX[3 * j], y = shX[3 * j + 1], z = shX[3 * j + 2];
+        // lanes stride over atoms (frame reads coalesce across lanes); the
+        // frame values of PATH_MSD_UNROLL atoms are loaded before any is used,
+        // and each lane still adds its atoms in increasing order.  Frame and
+        // atom indices are clamped into range so that every load is issued
+        // unconditionally (a load under a lane condition is waited for at
+        // once); the clamped values are never added.
+        for (int jb = lane; jb < mc; jb += 64 * PATH_MSD_UNROLL) {
+            float r[PATH_MSD_UNROLL][4][3];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                if (k < nf) {
-                    const float* r = F + ((size_t)(f0 + k) * M + c0 + j) * 3;
-                    const double dx = x - (double)r[0], dy = y - (double)r[1], dz = z - (double)r[2];
-                    acc[k] += dx * dx + dy * dy + dz * dz;
+            for (int u = 0; u < PATH_MSD_UNROLL; u++) {
+                const int j = min(jb + 64 * u, mc - 1);
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const float* rp = F + ((size_t)(f0 + min(k, nf - 1)) * M + c0 + j) * 3;
+                    r[u][k][0] = rp[0];
+                    r[u][k][1] = rp[1];
+                    r[u][k][2] = rp[2];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < PATH_MSD_UNROLL; u++) {
+                const int j = jb + 64 * u;
+                if (j < mc) {
+                    const double x = shX[3 * j], y = shX[3 * j + 1], z = shX[3 * j + 2];
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        if (k < nf) {
+                            const double dx = x - (double)r[u][k][0], dy = y - (double)r[u][k][1],
+                                         dz = z - (double)r[u][k][2];
+                            acc[k] += dx * dx + dy * dy + dz * dz;
+                        }
+                    }
                 }
             }
         }
@@ -1351,7 +1452,7 @@ __device__ void path_msd_block(const StepView& v, const PathItem& it, const int*
     }
 }
 
-__global__ __launch_bounds__(256) void k_path_msd(StepView v, const PathItem* __restrict__ items, int nItems,
+__global__ __launch_bounds__(PATH_MSD_THREADS) void k_path_msd(StepView v, const PathItem* __restrict__ items, int nItems,
                                                   const int* __restrict__ slots, const float* __restrict__ frames,
                                                   int w0)
 {
@@ -1375,6 +1476,13 @@ __global__ __launch_bounds__(64 * PATH_PHASES) void k_path_rows(StepView v, cons
     const int M = it.M, N = it.N;
     const double lam = it.lambda;
     const double* msd = v.pathMsd + (size_t)w * v.msdStride + it.msdOff;
+    const int lane = threadIdx.x & 63, ph = threadIdx.x >> 6;
+    const int j = tile * PATH_TILE + lane;
+    // this lane's position and row indices, loaded before the weight table
+    // so that their round trips overlap the MSD loads (clamped: no branch)
+    const int jc = min(j, M - 1);
+    const float4 q = gather_pos(v, slots, v.pStride, w, it.begin + jc);
+    const size_t rowS = row_index(v, w, it.row0 + 2 * jc), rowZ = row_index(v, w, it.row0 + 2 * jc + 1);
     double t[2] = {0.0, 0.0};
     for (int f = threadIdx.x; f < N; f += blockDim.x) {
         const double wv = exp(-lam * msd[f]);
@@ -1402,23 +1510,35 @@ __global__ __launch_bounds__(64 * PATH_PHASES) void k_path_rows(StepView v, cons
             aux[1] = ok ? (a[1] / S - z) / lam : 0.0;
         }
     }
-    const int lane = threadIdx.x & 63, ph = threadIdx.x >> 6;
-    const int j = tile * PATH_TILE + lane;
     double js0 = 0, js1 = 0, js2 = 0, jz0 = 0, jz1 = 0, jz2 = 0;
     if (j < M) {
-        const float4 q = gather_pos(v, slots, v.pStride, w, it.begin + j);
+        // frames f = ph, ph + PATH_PHASES, ... in increasing order; the loads
+        // of PATH_ROWS_UNROLL of them are issued before any is used
         const float* F = frames + it.frameOff + (size_t)j * 3;
-        for (int f = ph; f < N; f += PATH_PHASES) {
-            const float* r = F + (size_t)f * M * 3;
-            const double wv = shW[f];
-            const double dx = (double)q.x - r[0], dy = (double)q.y - r[1], dz = (double)q.z - r[2];
-            const double a = ((f + 1) - s) * wv;
-            js0 += a * dx;
-            js1 += a * dy;
-            js2 += a * dz;
-            jz0 += wv * dx;
-            jz1 += wv * dy;
-            jz2 += wv * dz;
+        for (int fb = ph; fb < N; fb += PATH_PHASES * PATH_ROWS_UNROLL) {
+            float r[PATH_ROWS_UNROLL][3];
+#pragma unroll
+            for (int u = 0; u < PATH_ROWS_UNROLL; u++) {  // clamped frame: unconditional loads
+                const float* rp = F + (size_t)min(fb + u * PATH_PHASES, N - 1) * M * 3;
+                r[u][0] = rp[0];
+                r[u][1] = rp[1];
+                r[u][2] = rp[2];
+            }
+#pragma unroll
+            for (int u = 0; u < PATH_ROWS_UNROLL; u++) {
+                const int f = fb + u * PATH_PHASES;
+                if (f < N) {
+                    const double wv = shW[f];
+                    const double dx = (double)q.x - r[u][0], dy = (double)q.y - r[u][1], dz = (double)q.z - r[u][2];
+                    const double a = ((f + 1) - s) * wv;
+                    js0 += a * dx;
+                    js1 += a * dy;
+                    js2 += a * dz;
+                    jz0 += wv * dx;
+                    jz1 += wv * dy;
+                    jz2 += wv * dz;
+                }
+            }
         }
     }
     shRed[0][threadIdx.x] = js0;
@@ -1437,8 +1557,8 @@ __global__ __launch_bounds__(64 * PATH_PHASES) void k_path_rows(StepView v, cons
             r6[k] = acc;
         }
         const double scS = ok ? -2.0 * lam / (M * S) : 0.0, scZ = ok ? 2.0 / (M * S) : 0.0;
-        store_row(v, w, it.row0 + 2 * j, scS * r6[0], scS * r6[1], scS * r6[2]);
-        store_row(v, w, it.row0 + 2 * j + 1, scZ * r6[3], scZ * r6[4], scZ * r6[5]);
+        store_row_at(v, rowS, scS * r6[0], scS * r6[1], scS * r6[2]);
+        store_row_at(v, rowZ, scZ * r6[3], scZ * r6[4], scZ * r6[5]);
     }
 }
 
@@ -1529,16 +1649,90 @@ void launch_cv_path(const StepView& v, const PathItem* items, int n, int nMsdBlo
                     const int* slots, const float* frames, int w0, int nw, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_path_msd, dim3(nMsdBlocks, nw), dim3(256), 0, s, v, items, n, slots, frames, w0);
+    hipLaunchKernelGGL(k_path_msd, dim3(nMsdBlocks, nw), dim3(PATH_MSD_THREADS), 0, s, v, items, n, slots, frames, w0);
     launch_path_rows(v, items, n, nRowBlocks, slots, frames, w0, nw, s);
 }
 
 // ---------------------------------------------------------------------------
-// Batched CV stage: every group CV, path CV and few-atom CV of every walker
-// in one launch (blocks [0, nGroup) group items, then path items, then
-// blocks of 256 few-atom items).  Same device code as the per-kind kernels.
+// Contact maps of more than CM_CHUNK pairs, in chunks of CM_CHUNK pairs (one
+// 256-thread workgroup each, every gather of a thread issued before its
+// arithmetic), so a long map spreads over many CUs.  Each chunk writes its
+// rows and its partial sum; the last chunk of the map to finish adds the
+// partials in chunk order and writes the value, so the value does not depend
+// on which chunk finished last.
 // ---------------------------------------------------------------------------
-static_assert(GROUP_BLOCK == 256, "the MSD blocks of the fused CV stage assume 256 threads");
+constexpr int CM_THREADS = 256;
+constexpr int CM_PER_THREAD = CM_CHUNK / CM_THREADS;
+static_assert(CM_CHUNK % CM_THREADS == 0, "whole pairs per thread");
+
+__device__ void cm_chunk_block(const StepView& v, const CVFusedPlan& p, int lb, int w)
+{
+    __shared__ double sh[16];
+    int q = 0;
+    while (q + 1 < p.nCm && lb >= p.cm[q + 1].chunk0) q++;
+    const GroupCVItem& it = p.cm[q];
+    const int c = lb - it.chunk0;
+    const int np = (it.end - it.begin) / 2;
+    const int nChunks = (np + CM_CHUNK - 1) / CM_CHUNK;
+    const int p0 = c * CM_CHUNK, n = min(CM_CHUNK, np - p0);
+    const int tid = threadIdx.x;
+    const WalkerBox wb = load_box(v, w);
+    const float* prm = p.gref + it.refOff;
+    float4 pa[CM_PER_THREAD], pb[CM_PER_THREAD], pp[CM_PER_THREAD];
+    size_t ra[CM_PER_THREAD], rb[CM_PER_THREAD];
+#pragma unroll
+    for (int r = 0; r < CM_PER_THREAD; r++) {  // clamped index: no divergent loads
+        const int k = p0 + min(tid + r * CM_THREADS, n - 1);
+        pa[r] = gather_pos(v, p.gslots, v.gStride, w, it.begin + 2 * k);
+        pb[r] = gather_pos(v, p.gslots, v.gStride, w, it.begin + 2 * k + 1);
+        pp[r] = make_float4(prm[4 * k], prm[4 * k + 1], prm[4 * k + 2], prm[4 * k + 3]);
+        ra[r] = row_index(v, w, it.row0 + 2 * k);
+        rb[r] = row_index(v, w, it.row0 + 2 * k + 1);
+    }
+    double s[1] = {0.0};
+#pragma unroll
+    for (int r = 0; r < CM_PER_THREAD; r++) {
+        if (tid + r * CM_THREADS < n) {
+            double dx = (double)pb[r].x - pa[r].x, dy = (double)pb[r].y - pa[r].y, dz = (double)pb[r].z - pa[r].z;
+            if (v.pbc) pbc_exact(dx, dy, dz, wb.b);
+            double coeff;
+            s[0] += contact_pair(dx, dy, dz, pp[r], coeff);
+            store_row_at(v, ra[r], -coeff * dx, -coeff * dy, -coeff * dz);
+            store_row_at(v, rb[r], coeff * dx, coeff * dy, coeff * dz);
+        }
+    }
+    block_sum<1, CM_THREADS / 64>(s, sh);
+    double* part = p.cmPart + (size_t)w * p.nCmBlocks + it.chunk0;
+    unsigned int* cnt = p.cmCount + (size_t)w * p.nCm + it.chunkItem;
+    if (tid == 0) part[c] = s[0];
+    if (!last_block(cnt, (unsigned int)nChunks)) return;
+    if (tid >= 64) return;
+    const double tot = wave_sum_partials(part, nChunks);
+    if (tid == 0) {
+        v.cvVal[(size_t)w * v.V + it.value] = tot;
+        *cnt = 0u;
+    }
+}
+
+__global__ __launch_bounds__(CM_THREADS) void k_cv_cm_chunks(StepView v, CVFusedPlan p, int w0)
+{
+    cm_chunk_block(v, p, blockIdx.x, w0 + blockIdx.y);
+}
+
+void launch_cv_cm_chunks(const StepView& v, const CVFusedPlan& p, int w0, int nw, hipStream_t s)
+{
+    if (p.nCmBlocks <= 0) return;
+    hipLaunchKernelGGL(k_cv_cm_chunks, dim3(p.nCmBlocks, nw), dim3(CM_THREADS), 0, s, v, p, w0);
+}
+
+// ---------------------------------------------------------------------------
+// Batched CV stage: every group CV, path CV, contact-map chunk and few-atom
+// CV of every walker in one launch (blocks [0, nGroup) group items, then path
+// MSD blocks, then contact-map chunks, then blocks of 256 few-atom items).
+// Same device code as the per-kind kernels.
+// ---------------------------------------------------------------------------
+static_assert(GROUP_BLOCK == PATH_MSD_THREADS, "the MSD blocks of the fused CV stage run on GROUP_BLOCK threads");
+static_assert(GROUP_BLOCK == CM_THREADS, "the contact-map chunks of the fused CV stage run on GROUP_BLOCK threads");
 
 __global__ __launch_bounds__(GROUP_BLOCK) void k_cv_fused(StepView v, CVFusedPlan p)
 {
@@ -1551,15 +1745,17 @@ __global__ __launch_bounds__(GROUP_BLOCK) void k_cv_fused(StepView v, CVFusedPla
         int q = 0;
         while (q + 1 < p.nPath && lb >= p.path[q + 1].msdBlk0) q++;
         path_msd_block(v, p.path[q], p.pslots, p.frames, lb - p.path[q].msdBlk0, w);
+    } else if (b < p.nGroup + p.nPathMsdBlocks + p.nCmBlocks) {
+        cm_chunk_block(v, p, b - p.nGroup - p.nPathMsdBlocks, w);
     } else {
-        const int i = (b - p.nGroup - p.nPathMsdBlocks) * GROUP_BLOCK + threadIdx.x;
+        const int i = (b - p.nGroup - p.nPathMsdBlocks - p.nCmBlocks) * GROUP_BLOCK + threadIdx.x;
         if (i < p.nSmall) small_item(v, p.small[i], w);
     }
 }
 
 void launch_cv_fused(const StepView& v, const CVFusedPlan& p, int nw, hipStream_t s)
 {
-    const int nb = p.nGroup + p.nPathMsdBlocks + (p.nSmall + GROUP_BLOCK - 1) / GROUP_BLOCK;
+    const int nb = p.nGroup + p.nPathMsdBlocks + p.nCmBlocks + (p.nSmall + GROUP_BLOCK - 1) / GROUP_BLOCK;
     if (nb <= 0) return;
     hipLaunchKernelGGL(k_cv_fused, dim3(nb, nw), dim3(GROUP_BLOCK), 0, s, v, p);
 }

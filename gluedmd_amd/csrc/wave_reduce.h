@@ -94,4 +94,32 @@ __device__ inline double wave_sum1(double x)
     return v[0];
 }
 
+// Ticket of a cross-workgroup reduction: true in every thread of the
+// workgroup that finished last.  Only thread 0 wrote this workgroup's
+// partial, so only it fences (an agent-scope fence writes back the XCD's L2
+// and is expensive on this part: one per workgroup, never one per wave).
+__device__ inline bool last_block(unsigned int* counter, unsigned int total)
+{
+    __shared__ int shLast;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        shLast = atomicAdd(counter, 1u) == total - 1;
+        if (shLast) __threadfence();
+    }
+    __syncthreads();
+    return shLast != 0;
+}
+
+// Sum of n partials written by other workgroups of the same launch (read
+// with device-scope loads after the ticket), by one wave: lane l adds
+// entries l, l + 64, ... in index order, then the butterfly; the order
+// depends only on n.
+__device__ inline double wave_sum_partials(const double* src, int n)
+{
+    double a = 0.0;
+    for (int j = threadIdx.x & 63; j < n; j += 64) a += __hip_atomic_load(src + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return wave_sum1(a);
+}
+
 }  // namespace glued

@@ -36,10 +36,12 @@ def test_kabsch_size_boundaries(n):
     _batched_equals_plain(g)
 
 
-@pytest.mark.parametrize("n", [2, 1024, 1025, 5000])
+@pytest.mark.parametrize("n", [2, 512, 513, 1024, 1025, 5000])
 def test_group_size_boundaries(n):
     """Gyration over n atoms and a contact map over n pairs, on both sides of
-    the 1,024-entry split between the 256- and the 1,024-thread group kernels."""
+    the 1,024-entry split between the 256- and the 1,024-thread group kernels
+    and of the 512-pair chunk of a contact map (longer maps are summed from
+    chunk partials by the last chunk to finish)."""
     rng = np.random.default_rng(200 + n)
     N = max(2 * n + 10, 60)
     L = (N / 100) ** (1 / 3)
