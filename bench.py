@@ -59,7 +59,7 @@ FIXED_SCALE = 4294967296.0
 PMC_FETCH_SCALE, PMC_WRITE_SCALE = 2.0, 1.0
 PMC_KIB = {
     "multiwalker": {"k_step_bias_chunks<1, 2>": (15052.2, 60.0)},
-    "mixed-large": {"k_scatter_dense_rows": (47457.5, 37512.1)},
+    "mixed-large": {"k_scatter_dense_rows": (41545.6, 49685.4)},
     "coordination-opes": {"k_coord_rank": (278.5, 792.4), "k_coord_offsets": (21.1, 38.8),
                           "k_coord_place": (359.4, 418.8), "k_coord_pairs<true, true>": (1656.8, 234.4),
                           "k_coord_pairs<false, true>": (1151.6, 269.7), "k_cv_group_big": (95.5, 117.2)},

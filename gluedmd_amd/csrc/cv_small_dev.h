@@ -75,9 +75,7 @@ __device__ inline size_t row_index(const StepView& v, int w, int row)
 
 __device__ inline void store_row_at(const StepView& v, size_t i, double gx, double gy, double gz)
 {
-    v.jx[i] = (float)gx;
-    v.jy[i] = (float)gy;
-    v.jz[i] = (float)gz;
+    v.jrow[i] = make_float4((float)gx, (float)gy, (float)gz, 0.f);
 }
 
 __device__ inline void store_row(const StepView& v, int w, int row, double gx, double gy, double gz)

@@ -33,8 +33,8 @@ int opes_chunks_for(int);
 
 void launch_scatter(const StepView&, const ScatterPlan&, int, int, int, hipStream_t);
 void launch_scatter_dense(const StepView&, const float* const*, int, int, int, int, hipStream_t);
-void launch_scatter_dense_rows(const StepView&, const ScatterPlan&, const float* const*, int, const int*,
-                               const unsigned*, int, int, int, int, int, hipStream_t);
+void launch_scatter_dense_rows(const StepView&, const ScatterPlan&, const float* const*, int, const int2*,
+                               const uint2*, int, int, int, int, int, hipStream_t);
 void launch_step_tiny(const StepView&, const SmallCVItem*, int, const FusedBiasPlan&, int, hipStream_t);
 void launch_step_bias(const StepView&, const FusedBiasPlan&, int, hipStream_t);
 void launch_step_bias_chunks(const StepView&, const FusedBiasPlan&, int, hipStream_t);
@@ -430,6 +430,10 @@ void Engine::buildPlan(int nMass, const double* masses)
     }
     S_ = (int)slotAtom_.size();
     R_ = (int)rowSlot_.size();
+    // rows grouped by slot: the rows of slot s occupy [segStart_[s], segStart_[s + 1])
+    segStart_.assign(S_ + 1, 0);
+    for (int r = 0; r < R_; r++) segStart_[rowSlot_[r] + 1]++;
+    for (int s = 0; s < S_; s++) segStart_[s + 1] += segStart_[s];
 
     // ---- walkers, atom order, box ----
     std::vector<WalkerInfo> wi(W_);
@@ -449,13 +453,12 @@ void Engine::buildPlan(int nMass, const double* masses)
     dPathPos_.alloc(sizeof(int) * (size_t)W_ * std::max((int)pathSlots_.size(), 1));
     if (energyValue_ >= 0) {
         // GPU atom -> slot tables for the combined dense + slot scatter
-        slotStride_ = maxPadded_;
         maskStride_ = (maxPadded_ + 31) / 32;
-        dSlotOf_.alloc(sizeof(int) * (size_t)W_ * slotStride_);
-        dSlotMask_.alloc(sizeof(unsigned) * (size_t)W_ * maskStride_);
+        dSlotMask_.alloc(sizeof(uint2) * (size_t)W_ * maskStride_);
+        dSlotList_.alloc(sizeof(int2) * (size_t)W_ * std::max(S_, 1));
     } else {
-        dSlotOf_.release();
         dSlotMask_.release();
+        dSlotList_.release();
     }
     for (int w = 0; w < W_; w++) uploadGidx(w);
     boxHost_.assign((size_t)W_ * 12, 0.0);
@@ -468,9 +471,7 @@ void Engine::buildPlan(int nMass, const double* masses)
 
     // ---- per-walker step buffers ----
     dCv_.alloc(sizeof(double) * (size_t)W_ * std::max(V_, 1));
-    dJx_.alloc(sizeof(float) * (size_t)W_ * std::max(R_, 1));
-    dJy_.alloc(sizeof(float) * (size_t)W_ * std::max(R_, 1));
-    dJz_.alloc(sizeof(float) * (size_t)W_ * std::max(R_, 1));
+    dJ_.alloc(sizeof(float4) * (size_t)W_ * std::max(R_, 1));
     dUd_.alloc(sizeof(double) * (size_t)W_ * std::max(V_, 1));
     dSmall_.upload(small_);
     {
@@ -755,9 +756,8 @@ void Engine::buildPlan(int nMass, const double* masses)
 
     // ---- scatter segments: rows grouped by atom slot ----
     // (row r is stored at rowPos[r]; the rows of slot s occupy [segStart[s], segStart[s+1]))
-    std::vector<int> segStart(S_ + 1, 0), rowPos(R_), entryCv(R_);
-    for (int r = 0; r < R_; r++) segStart[rowSlot_[r] + 1]++;
-    for (int s = 0; s < S_; s++) segStart[s + 1] += segStart[s];
+    const std::vector<int>& segStart = segStart_;
+    std::vector<int> rowPos(R_), entryCv(R_);
     {
         std::vector<int> cur(segStart.begin(), segStart.end() - 1);
         for (int r = 0; r < R_; r++) {
@@ -838,18 +838,33 @@ void Engine::uploadGidx(int w)
     resolve(groupSlots_, dGroupPos_);
     resolve(pathSlots_, dPathPos_);
     if (dSlotMask_.bytes) {
-        std::vector<int> so((size_t)slotStride_, -1);
-        std::vector<unsigned> mk((size_t)maskStride_, 0u);
-        for (int s = 0; s < S_; s++) {
-            so[g[s]] = s;
-            mk[g[s] >> 5] |= 1u << (g[s] & 31);
+        // per 32-atom word: the bits of the atoms with Jacobian rows and the
+        // count of such atoms in the earlier words; the row ranges of those
+        // atoms in GPU order
+        std::vector<int> so((size_t)maskStride_ * 32, -1);
+        for (int s = 0; s < S_; s++) so[g[s]] = s;
+        std::vector<uint2> mk((size_t)maskStride_);
+        std::vector<int2> sl;
+        sl.reserve(std::max(S_, 1));
+        for (int wd = 0; wd < maskStride_; wd++) {
+            unsigned bits = 0;
+            const unsigned before = (unsigned)sl.size();
+            for (int b = 0; b < 32; b++) {
+                const int s = so[(size_t)wd * 32 + b];
+                if (s >= 0) {
+                    bits |= 1u << b;
+                    sl.push_back(make_int2(segStart_[s], segStart_[s + 1]));
+                }
+            }
+            mk[wd] = make_uint2(bits, before);
         }
-        hip_check(hipMemcpy(dSlotOf_.as<int>() + (size_t)w * slotStride_, so.data(), sizeof(int) * so.size(),
+        sl.resize(std::max(S_, 1), make_int2(0, 0));
+        hip_check(hipMemcpy(dSlotMask_.as<uint2>() + (size_t)w * maskStride_, mk.data(), sizeof(uint2) * mk.size(),
                             hipMemcpyHostToDevice),
-                  "upload slot table");
-        hip_check(hipMemcpy(dSlotMask_.as<unsigned>() + (size_t)w * maskStride_, mk.data(),
-                            sizeof(unsigned) * mk.size(), hipMemcpyHostToDevice),
                   "upload slot mask");
+        hip_check(hipMemcpy(dSlotList_.as<int2>() + (size_t)w * std::max(S_, 1), sl.data(), sizeof(int2) * sl.size(),
+                            hipMemcpyHostToDevice),
+                  "upload slot list");
     }
 }
 
@@ -969,9 +984,7 @@ StepView Engine::view(const float4* posq, long long* force) const
     v.box = dBox_.as<double>();
     v.cvVal = dCv_.as<double>();
     v.rowPos = dSegRow_.as<int>();
-    v.jx = dJx_.as<float>();
-    v.jy = dJy_.as<float>();
-    v.jz = dJz_.as<float>();
+    v.jrow = dJ_.as<float4>();
     v.bgrad = dBgrad_.as<double>();
     v.benergy = dBener_.as<double>();
     v.dUdCV = dUd_.as<double>();
@@ -1123,8 +1136,8 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         if (forces && !scattered) {
             if (energyValue_ >= 0 && dSlotMask_.bytes) {
                 // slots and the dense energy CV in one pass (dU/dCV from the finalize)
-                launch_scatter_dense_rows(v, sp, dEnergyFPtr_.as<const float*>(), energyValue_, dSlotOf_.as<int>(),
-                                          dSlotMask_.as<unsigned>(), slotStride_, maskStride_, maxPadded_, w0, nw,
+                launch_scatter_dense_rows(v, sp, dEnergyFPtr_.as<const float*>(), energyValue_, dSlotList_.as<int2>(),
+                                          dSlotMask_.as<uint2>(), std::max(S_, 1), maskStride_, maxPadded_, w0, nw,
                                           s);
                 denseDone = true;
             } else {
@@ -1210,7 +1223,7 @@ void Engine::executeStage(hipStream_t s, const float4* posq, long long* force, i
             v.S = S_;
             if (dSlotMask_.bytes && !fplan_.scatter)
                 launch_scatter_dense_rows(v, scatterPlan(), dEnergyFPtr_.as<const float*>(), energyValue_,
-                                          dSlotOf_.as<int>(), dSlotMask_.as<unsigned>(), slotStride_, maskStride_,
+                                          dSlotList_.as<int2>(), dSlotMask_.as<uint2>(), std::max(S_, 1), maskStride_,
                                           maxPadded_, 0, W_, s);
             else
                 launch_scatter_dense(v, dEnergyFPtr_.as<const float*>(), energyValue_, maxPadded_, 0, W_, s);

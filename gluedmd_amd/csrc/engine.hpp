@@ -210,7 +210,7 @@ class Engine {
     std::vector<int> combStart_, combSlot_;
 
     // device buffers
-    DBuf dWalkers_, dGidx_, dBox_, dCv_, dJx_, dJy_, dJz_, dBgrad_, dBener_, dUd_;
+    DBuf dWalkers_, dGidx_, dBox_, dCv_, dJ_, dBgrad_, dBener_, dUd_;
     DBuf dSmall_, dGroup_, dGroupBig_, dGroupPos_, dGroupMass_, dGroupRef_;
     DBuf dCm_, dCmPart_, dCmCount_;
     DBuf dCoord_, dCoordSlots_, dCellOf_, dCellCount_, dCellStart_, dSorted_, dUnsorted_, dCnAcc_, dCnAccLo_, dPairAcc_,
@@ -239,8 +239,12 @@ class Engine {
     bool cvReady_ = false;
     int lastUpdateStep_ = -1;
     int maxPadded_ = 0;
-    DBuf dSlotOf_, dSlotMask_;     // [W][maxPadded] slot of each GPU atom (-1: none), [W][maxPadded/32] bit per atom
-    int slotStride_ = 0, maskStride_ = 0;
+    // combined dense + slot scatter: per walker, {bit per GPU atom with rows,
+    // rows-atoms in earlier words} per 32-atom word, and the slots of those
+    // atoms in GPU order
+    DBuf dSlotMask_, dSlotList_;   // [W][maxPadded/32] uint2, [W][S] int2 row range
+    std::vector<int> segStart_;    // [S + 1] row range of every slot (host copy)
+    int maskStride_ = 0;
     const float4* lastPos_ = nullptr;
     long long* lastForce_ = nullptr;
 };

@@ -66,9 +66,7 @@ struct StepView {
     const double* box;    // [W][12] = a, b, c, 1/ax, 1/by, 1/cz
     double* cvVal;        // [W][V]
     const int* rowPos;    // [R] storage position of plan row r (rows are stored grouped by atom slot)
-    float* jx;            // [W][R]
-    float* jy;
-    float* jz;
+    float4* jrow;         // [W][R] Jacobian rows: FP32 x, y, z (w unused), one 16-byte load or store per row
     double* bgrad;        // [W][G]
     double* benergy;      // [W][B]
     double* dUdCV;        // [W][V]

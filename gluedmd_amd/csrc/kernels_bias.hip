@@ -560,7 +560,7 @@ __device__ void finalize_walker(const StepView& v, const FusedBiasPlan& fp, int 
     const bool doScatter = fp.scatter != 0 && v.S > 0 && v.R > 0;
     const bool ownsSlots = doScatter && wave != 1;
     int sb[FIN_SLOTS], se[FIN_SLOTS], sc0[FIN_SLOTS], sg[FIN_SLOTS];
-    float sjx[FIN_SLOTS], sjy[FIN_SLOTS], sjz[FIN_SLOTS];
+    float4 sj[FIN_SLOTS];
     const size_t rbase = (size_t)w * v.R;
     if (ownsSlots) {
 #pragma unroll
@@ -574,9 +574,7 @@ __device__ void finalize_walker(const StepView& v, const FusedBiasPlan& fp, int 
         for (int r = 0; r < FIN_SLOTS; r++) {
             const int k = min(sb[r], v.R - 1);  // a slot without rows reads a valid row and drops it
             sc0[r] = sp.entryCv[k];
-            sjx[r] = v.jx[rbase + k];
-            sjy[r] = v.jy[rbase + k];
-            sjz[r] = v.jz[rbase + k];
+            sj[r] = v.jrow[rbase + k];
         }
     }
     if (ldsComb && wave >= 2) {
@@ -628,12 +626,13 @@ __device__ void finalize_walker(const StepView& v, const FusedBiasPlan& fp, int 
             // in the order of scatter_slot
             if (sb[r] == se[r]) continue;
             const double u0 = sU[sc0[r]];
-            double fx = -(u0 * (double)sjx[r]), fy = -(u0 * (double)sjy[r]), fz = -(u0 * (double)sjz[r]);
+            double fx = -(u0 * (double)sj[r].x), fy = -(u0 * (double)sj[r].y), fz = -(u0 * (double)sj[r].z);
             for (int k = sb[r] + 1; k < se[r]; k++) {
                 const double u = sU[sp.entryCv[k]];
-                fx -= u * (double)v.jx[rbase + k];
-                fy -= u * (double)v.jy[rbase + k];
-                fz -= u * (double)v.jz[rbase + k];
+                const float4 j = v.jrow[rbase + k];
+                fx -= u * (double)j.x;
+                fy -= u * (double)j.y;
+                fz -= u * (double)j.z;
             }
             atomic_add_fixed(F + sg[r], fx);
             atomic_add_fixed(F + sg[r] + wi.padded, fy);

@@ -360,9 +360,7 @@ __device__ void kabsch_block(const StepView& v, const GroupCVItem& it, const int
         const double r0 = R[0] * y[0] + R[1] * y[1] + R[2] * y[2];
         const double r1 = R[3] * y[0] + R[4] * y[1] + R[5] * y[2];
         const double r2 = R[6] * y[0] + R[7] * y[1] + R[8] * y[2];
-        v.jx[base + rp[r]] = (float)((x0 - r0) * sc);
-        v.jy[base + rp[r]] = (float)((x1 - r1) * sc);
-        v.jz[base + rp[r]] = (float)((x2 - r2) * sc);
+        store_row_at(v, base + rp[r], (x0 - r0) * sc, (x1 - r1) * sc, (x2 - r2) * sc);
     }
 }
 

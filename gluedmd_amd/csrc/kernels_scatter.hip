@@ -62,35 +62,49 @@ __global__ __launch_bounds__(256) void k_scatter_dense(StepView v, const float* 
     }
 }
 
-// The dense energy-CV scatter and the atom-slot scatter in one pass: each
-// thread owns 4 consecutive entries of one force component, adds the dense
-// term and, for the atoms that have Jacobian rows (one bit per GPU atom),
-// the rows' sum in storage order, each rounded to fixed point on its own as
-// the separate kernels round them: the forces equal those of k_scatter +
-// k_scatter_dense bit for bit, with one read-modify-write per entry instead
-// of an atomic per component and slot.  dU/dCV comes from the finalize.
-__global__ __launch_bounds__(256) void k_scatter_dense_rows(StepView v, ScatterPlan sp,
-                                                            const float* const* __restrict__ fext, int energyValue,
-                                                            const int* __restrict__ slotOf,
-                                                            const unsigned* __restrict__ mask, int slotStride,
-                                                            int maskStride, int w0)
+// The dense energy-CV scatter and the atom-slot scatter in one pass,
+// atom-major: each thread owns 4 consecutive GPU atoms of one walker and all
+// three components (16-byte loads and stores per component array), adds the
+// dense term and, for the atoms that have Jacobian rows (one bit per GPU
+// atom), the rows' sum in storage order, each rounded to fixed point on its
+// own as the separate kernels round them: the forces equal those of
+// k_scatter + k_scatter_dense bit for bit, with one read-modify-write per
+// entry instead of an atomic per component and slot, and every row of a
+// masked atom read once (one 16-byte row).  dU/dCV comes from the finalize.
+//
+// Dependent loads of a thread with rows: {mask, running count} of its
+// 32-atom word (issued with the field loads), the row ranges of its atoms
+// (a list in GPU order: a wave reads it nearly in sequence), then the rows
+// and their CV indices; dU/dCV is staged in LDS behind the one barrier,
+// which comes after every first-round load is in flight.
+constexpr int DENSE_ROWS_BLOCK = 256;
+constexpr int DENSE_ROWS_LDS_U = 256;  // dU/dCV values staged in LDS (more are read from memory)
+
+__global__ __launch_bounds__(DENSE_ROWS_BLOCK) void k_scatter_dense_rows(StepView v, ScatterPlan sp,
+                                                                         const float* const* __restrict__ fext,
+                                                                         int energyValue,
+                                                                         const int2* __restrict__ rowList,
+                                                                         const uint2* __restrict__ mask,
+                                                                         int listStride, int maskStride, int w0)
 {
+    __shared__ double sU[DENSE_ROWS_LDS_U];
     const int w = w0 + blockIdx.y;
     const float* Fe = fext[w];
     const WalkerInfo wi = v.walkers[w];
     long long* F = v.force + wi.forceOff;
     const double* U = v.dUdCV + (size_t)w * v.V;
     const size_t rbase = (size_t)w * v.R;
-    if (!Fe) {  // no energy input for this walker: the slots alone, with atomics
+    if (!Fe) {  // no energy input for this walker (uniform over the workgroup): the slots alone, with atomics
         for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < v.S; s += gridDim.x * blockDim.x) {
             const int b = sp.segStart[s], e = sp.segStart[s + 1];
             if (b == e) continue;
             double fx = 0.0, fy = 0.0, fz = 0.0;
             for (int k = b; k < e; k++) {
                 const double u = U[sp.entryCv[k]];
-                fx -= u * (double)v.jx[rbase + k];
-                fy -= u * (double)v.jy[rbase + k];
-                fz -= u * (double)v.jz[rbase + k];
+                const float4 j = v.jrow[rbase + k];
+                fx -= u * (double)j.x;
+                fy -= u * (double)j.y;
+                fz -= u * (double)j.z;
             }
             const int g = v.gidx[(size_t)w * v.S + s];
             atomic_add_fixed(F + g, fx);
@@ -99,51 +113,87 @@ __global__ __launch_bounds__(256) void k_scatter_dense_rows(StepView v, ScatterP
         }
         return;
     }
-    const double u = U[energyValue];
-    const int* sOf = slotOf + (size_t)w * slotStride;
-    const unsigned* msk = mask + (size_t)w * maskStride;
     const long long P = wi.padded;
-    const long long n4 = (3LL * P) >> 2;  // padded is a multiple of 4
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n4;
-         i += (long long)gridDim.x * blockDim.x) {
-        const float4 ef = reinterpret_cast<const float4*>(Fe)[i];
-        const long long e0 = 4 * i;
-        const int c = (int)(e0 / P);                   // component of all four entries
-        const int g0 = (int)(e0 - (long long)c * P);   // first GPU atom (a multiple of 4)
-        const unsigned bits = (msk[g0 >> 5] >> (g0 & 31)) & 0xFu;
-        longlong2* p = reinterpret_cast<longlong2*>(F + e0);
-        longlong2 a = p[0], b = p[1];
-        long long q[4] = {(long long)(u * (double)ef.x * FIXED_SCALE), (long long)(u * (double)ef.y * FIXED_SCALE),
-                          (long long)(u * (double)ef.z * FIXED_SCALE), (long long)(u * (double)ef.w * FIXED_SCALE)};
-        if (bits) {
-            const float* J = c == 0 ? v.jx : (c == 1 ? v.jy : v.jz);
+    const long long n4 = P >> 2;  // groups of 4 atoms (padded is a multiple of 4)
+    const long long i = (long long)blockIdx.x * DENSE_ROWS_BLOCK + threadIdx.x;
+    const bool own = i < n4;
+    const long long g0 = 4 * (own ? i : n4 - 1);  // clamped: every thread loads, none waits on a branch
+    float4 ef[3];
+    longlong2 fa[3], fb[3];
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                if (!((bits >> t) & 1u)) continue;
-                const int s = sOf[g0 + t];
-                double f = 0.0;
-                for (int k = sp.segStart[s]; k < sp.segStart[s + 1]; k++) f -= U[sp.entryCv[k]] * (double)J[rbase + k];
-                q[t] += (long long)(f * FIXED_SCALE);
+    for (int c = 0; c < 3; c++) {
+        ef[c] = reinterpret_cast<const float4*>(Fe + c * P + g0)[0];
+        const longlong2* p = reinterpret_cast<const longlong2*>(F + c * P + g0);
+        fa[c] = p[0];
+        fb[c] = p[1];
+    }
+    const uint2 mw = mask[(size_t)w * maskStride + (g0 >> 5)];
+    const double u = U[energyValue];
+    const bool ldsU = v.V <= DENSE_ROWS_LDS_U;
+    if (ldsU)
+        for (int c = threadIdx.x; c < v.V; c += DENSE_ROWS_BLOCK) sU[c] = U[c];
+    __syncthreads();
+    if (!own) return;
+    const unsigned bits = (mw.x >> (g0 & 31)) & 0xFu;
+    long long q[3][4];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        q[c][0] = (long long)(u * (double)ef[c].x * FIXED_SCALE);
+        q[c][1] = (long long)(u * (double)ef[c].y * FIXED_SCALE);
+        q[c][2] = (long long)(u * (double)ef[c].z * FIXED_SCALE);
+        q[c][3] = (long long)(u * (double)ef[c].w * FIXED_SCALE);
+    }
+    if (bits) {
+        // the row ranges of this word's atoms with rows follow each other in
+        // the list
+        // the list (clamped to this group's entries: unconditional loads)
+        const int2* rl = rowList + (size_t)w * listStride + mw.y + __popc(mw.x & ((1u << (g0 & 31)) - 1u));
+        const int nr = __popc(bits);
+        int2 rr[4];
+#pragma unroll
+        for (int t = 0; t < 4; t++) rr[t] = rl[min(t, nr - 1)];
+#pragma unroll
+        for (int t = 0; t < 4; t++) {
+            const int e = __popc(bits & ((1u << t) - 1u));  // list entry of atom t
+            const int2 r = e == 0 ? rr[0] : e == 1 ? rr[1] : e == 2 ? rr[2] : rr[3];
+            const int kb = (bits >> t) & 1u ? r.x : 0, ke = (bits >> t) & 1u ? r.y : 0;
+            double fx = 0.0, fy = 0.0, fz = 0.0;
+            for (int k = kb; k < ke; k++) {
+                const int cv = sp.entryCv[k];
+                const float4 j = v.jrow[rbase + k];
+                const double uk = ldsU ? sU[cv] : U[cv];
+                fx -= uk * (double)j.x;
+                fy -= uk * (double)j.y;
+                fz -= uk * (double)j.z;
+            }
+            if ((bits >> t) & 1u) {
+                q[0][t] += (long long)(fx * FIXED_SCALE);
+                q[1][t] += (long long)(fy * FIXED_SCALE);
+                q[2][t] += (long long)(fz * FIXED_SCALE);
             }
         }
-        a.x += q[0];
-        a.y += q[1];
-        b.x += q[2];
-        b.y += q[3];
-        p[0] = a;
-        p[1] = b;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        longlong2* p = reinterpret_cast<longlong2*>(F + c * P + g0);
+        fa[c].x += q[c][0];
+        fa[c].y += q[c][1];
+        fb[c].x += q[c][2];
+        fb[c].y += q[c][3];
+        p[0] = fa[c];
+        p[1] = fb[c];
     }
 }
 
 void launch_scatter_dense_rows(const StepView& v, const ScatterPlan& sp, const float* const* fext, int energyValue,
-                               const int* slotOf, const unsigned* mask, int slotStride, int maskStride, int maxPadded,
+                               const int2* rowList, const uint2* mask, int listStride, int maskStride, int maxPadded,
                                int w0, int nw, hipStream_t s)
 {
-    const long long n4 = (3LL * maxPadded) / 4;
-    int nb = (int)((n4 + 255) / 256);
-    if (nb > 2048) nb = 2048;
-    hipLaunchKernelGGL(k_scatter_dense_rows, dim3(nb, nw), dim3(256), 0, s, v, sp, fext, energyValue, slotOf, mask,
-                       slotStride, maskStride, w0);
+    // one thread per 4 atoms of the largest walker (total atoms < 2^31)
+    const long long n4 = maxPadded / 4;
+    const int nb = (int)((n4 + DENSE_ROWS_BLOCK - 1) / DENSE_ROWS_BLOCK);
+    hipLaunchKernelGGL(k_scatter_dense_rows, dim3(nb, nw), dim3(DENSE_ROWS_BLOCK), 0, s, v, sp, fext, energyValue,
+                       rowList, mask, listStride, maskStride, w0);
 }
 
 void launch_scatter(const StepView& v, const ScatterPlan& sp, int w0, int nw, int includeForces, hipStream_t s)

@@ -45,9 +45,10 @@ __device__ inline void scatter_slot(const StepView& v, const ScatterPlan& sp, in
     for (int k = b; k < e; k++) {
         const int c = sp.entryCv[k];
         const double u = c < nLds ? sU[c] : v.dUdCV[(size_t)w * v.V + c];
-        fx -= u * (double)v.jx[base + k];
-        fy -= u * (double)v.jy[base + k];
-        fz -= u * (double)v.jz[base + k];
+        const float4 j = v.jrow[base + k];
+        fx -= u * (double)j.x;
+        fy -= u * (double)j.y;
+        fz -= u * (double)j.z;
     }
     const WalkerInfo wi = v.walkers[w];
     const int g = v.gidx[(size_t)w * v.S + s];
