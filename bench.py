@@ -55,16 +55,17 @@ FIXED_SCALE = 4294967296.0
 # averaged over 20 batched steps of `bench.py --workload W --batched-only
 # --steps 20 --warmup 2`).  On gfx950 FETCH_SIZE reads half the bytes that a
 # copy kernel of known size moves and WRITE_SIZE reads them in full, so
-# traffic = 2 * FETCH + 1 * WRITE.  Summaries under profiles/ (*_pmc.md).
+# traffic = 2 * FETCH + 1 * WRITE.  The per-kernel figures are in the
+# workload summaries under profiles/.
 PMC_FETCH_SCALE, PMC_WRITE_SCALE = 2.0, 1.0
 PMC_KIB = {
     "multiwalker": {"k_step_bias_chunks<1, 2>": (15052.2, 60.0)},
     "mixed-large": {"k_scatter_dense_rows": (41545.6, 49685.4)},
-    "coordination-opes": {"k_coord_rank": (278.5, 792.4), "k_coord_offsets": (21.1, 38.8),
-                          "k_coord_place": (359.4, 418.8), "k_coord_pairs<true, true>": (1656.8, 234.4),
-                          "k_coord_pairs<false, true>": (1151.6, 269.7), "k_cv_group_big": (95.5, 117.2)},
-    "path-batch": {"k_cv_fused (path MSD blocks)": (3377.4, 128.0), "k_path_rows": (2133.0, 1816.0)},
-    "small": {"k_step_bias_chunks<1, 2>": (21.7, 0.1)},
+    "coordination-opes": {"k_coord_rank": (278.4, 792.4), "k_coord_offsets": (20.8, 38.8),
+                          "k_coord_place": (357.3, 418.8), "k_coord_pairs_a": (1562.8, 250.0),
+                          "k_coord_pairs<false, true>": (835.8, 590.3), "k_cv_group_big": (84.2, 156.3)},
+    "path-batch": {"k_cv_fused (path MSD blocks)": (3368.2, 128.0), "k_path_rows": (2112.0, 2416.0)},
+    "small": {"k_step_tiny<1, 2>": (41.8, 0.9)},
 }
 
 
@@ -321,11 +322,18 @@ class Small(Workload):
             "parallelism": "one walker, direct kernel launches",
         }
 
+    def time_dominant(self, reps):
+        """The batched step of this plan is one launch (k_step_tiny): its duration."""
+        return time_fn(lambda: self.ctx.execute_batch(use_graph=False), reps)
+
     def roofline(self, t_launch):
         n = len(self.ctx.get_hills(0)[1])
-        nbytes = n * 24
-        return dict(bound="hbm", kernel="k_step_bias_chunks (hill list sum)", achieved=nbytes / t_launch / 1e9,
-                    peak=HBM_PEAK_GBS, unit="GB/s", per_unit="24 B per hill", units_per_launch=n,
+        # 7 atoms: 12 B position + 48 B fixed-point force read+write; 24 B per
+        # hill; 24 B of CV values and bias state
+        nbytes = 7 * (12 + 48) + n * 24 + 24
+        return dict(bound="hbm", kernel="k_step_tiny (whole step: CVs, hill sum, bias, scatter in one launch)",
+                    achieved=nbytes / t_launch / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
+                    per_unit="24 B per hill + 420 B per step (7 atoms x 60 B) + 24 B", units_per_launch=n,
                     bytes_per_launch=nbytes)
 
     def oracle_walker(self, w):
