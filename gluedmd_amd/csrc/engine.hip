@@ -13,10 +13,12 @@ namespace glued {
 void launch_cv_small(const StepView&, const SmallCVItem*, int, int, int, hipStream_t);
 void launch_cv_group(const StepView&, const GroupCVItem*, int, const int*, const float*, const float*, int, int,
                      hipStream_t);
-void launch_cv_path(const StepView&, const PathItem*, int, int, int, const int*, const float*, int, int, hipStream_t);
+void launch_cv_path(const StepView&, const PathItem*, int, int, int, int, const int*, const float*, int, int,
+                    hipStream_t);
 void launch_cv_group_big(const StepView&, const GroupCVItem*, int, const int*, const float*, const float*, int, int,
                          hipStream_t);
-void launch_path_rows(const StepView&, const PathItem*, int, int, const int*, const float*, int, int, hipStream_t);
+void launch_path_rows(const StepView&, const PathItem*, int, int, int, const int*, const float*, int, int,
+                      hipStream_t);
 
 void launch_coordination(const StepView&, const CoordBuffers&, const CoordItem*, const CoordItem&, int, int, int,
                          hipStream_t);
@@ -422,6 +424,7 @@ void Engine::buildPlan(int nMass, const double* masses)
                 msdStride_ += N;
                 pathMsdBlocks_ += (N + PATH_FPB - 1) / PATH_FPB;
                 pathRowBlocks_ += (M + PATH_TILE - 1) / PATH_TILE;
+                pathMaxN_ = std::max(pathMaxN_, N);
                 for (int k = 0; k < N * M * 3; k++) pathFrames.push_back((float)c.params[2 + k]);
                 path_.push_back(it);
                 break;
@@ -781,6 +784,10 @@ void Engine::buildPlan(int nMass, const double* masses)
         dOpesTable_.upload(ot);
         fplan_ = FusedBiasPlan{};
         fplan_.hills = dHillTable_.as<HillBiasDev>();
+        if (!ht.empty()) fplan_.hill0 = ht[0];
+        // one shared hill list and nothing else chunked: the chunk
+        // workgroups take CHUNK_WB walkers each
+        fplan_.walkersPerChunk = (ht.size() == 1 && ot.empty() && ht[0].shared && W_ > 1) ? CHUNK_WB : 1;
         fplan_.opes = dOpesTable_.as<OpesBiasDev>();
         fplan_.nHill = (int)ht.size();
         fplan_.nOpes = (int)ot.size();
@@ -1068,7 +1075,7 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         launch_cv_fused(v, cp, nw, s);
         launch_cv_group_big(v, dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupPos_.as<int>(),
                             dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
-        launch_path_rows(v, dPath_.as<PathItem>(), (int)path_.size(), pathRowBlocks_, dPathPos_.as<int>(),
+        launch_path_rows(v, dPath_.as<PathItem>(), (int)path_.size(), pathRowBlocks_, pathMaxN_, dPathPos_.as<int>(),
                          dPathFrames_.as<float>(), w0, nw, s);
     } else {
         launch_cv_small(v, dSmall_.as<SmallCVItem>(), (int)small_.size(), w0, nw, s);
@@ -1077,7 +1084,7 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         launch_cv_cm_chunks(v, cp, w0, nw, s);
         launch_cv_group_big(v, dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupPos_.as<int>(),
                             dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
-        launch_cv_path(v, dPath_.as<PathItem>(), (int)path_.size(), pathMsdBlocks_, pathRowBlocks_,
+        launch_cv_path(v, dPath_.as<PathItem>(), (int)path_.size(), pathMsdBlocks_, pathRowBlocks_, pathMaxN_,
                        dPathPos_.as<int>(), dPathFrames_.as<float>(), w0, nw, s);
     }
     if (!coord_.empty()) {
@@ -1525,9 +1532,15 @@ void Engine::setHillSigma(int bi, const double* sigma, int D)
         h->d.i2s2[d] = 1.0 / (2.0 * sigma[d] * sigma[d]);
         h->d.is2[d] = 1.0 / (sigma[d] * sigma[d]);
     }
-    // the batched stage reads the table from device memory (graph-safe)
+    // the batched stage reads the table from device memory, and the first
+    // hill bias also from its copy in the launch arguments (captured graphs
+    // hold that copy: they are rebuilt)
     hip_check(hipMemcpy(dHillTable_.as<HillBiasDev>() + k, &h->d, sizeof(HillBiasDev), hipMemcpyHostToDevice),
               "set_hill_sigma");
+    if (k == 0) {
+        fplan_.hill0 = h->d;
+        dropGraphs();
+    }
 }
 
 void Engine::hillAdjoint(hipStream_t s, int bi, const double* gV, double* gH, double* gSigma)

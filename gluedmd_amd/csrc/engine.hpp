@@ -221,6 +221,7 @@ class Engine {
     DBuf dSegStart_, dSegRow_ /* rowPos */, dRowCv_ /* entryCv */, dCombStart_, dCombSlot_, dTestGrad_, dEnergyU_, dEnergyFPtr_;
     DBuf dHillTable_, dOpesTable_, dFusedPart_, dFusedCounter_, dPathAux_, dAdjPart_, dPathMsd_, dPathCoef_;
     int msdStride_ = 0, pathMsdBlocks_ = 0, pathRowBlocks_ = 0;
+    int pathMaxN_ = 0;  // most frames of any path
     FusedBiasPlan fplan_{};
 
     // graph cache for the batched step

@@ -277,7 +277,17 @@ struct FusedBiasPlan {
     int scatter;             // 1: the finalizing workgroup also scatters
     int kind;                // chunk-kernel specialisation: 1 hills only, 2 OPES only (of one D), 0 mixed
     int kindD;               // that D
+    // a copy of hills[0] passed by value: with one hill bias the chunk
+    // workgroups read its fields from the kernel arguments, all in the first
+    // round trip, instead of through a chain of loads from the table
+    HillBiasDev hill0;
+    // walkers per chunk workgroup (1, or CHUNK_WB when the only chunked bias
+    // is one shared hill list: the chunk is loaded once for them all)
+    int walkersPerChunk;
+    int nWalkers;            // set at launch
 };
+
+constexpr int CHUNK_WB = 4;
 
 }  // namespace glued
 

@@ -235,6 +235,10 @@ struct HillChunkRegs {
     double hk[PER_THREAD];  // 0 past the end of the list: such entries add exactly zero
 };
 
+// The loads go to indices clamped into the table's capacity, issued together
+// with the load of the hill count: no load waits for the count.  Entries
+// past the count are then set to zero (centre and height), so they add
+// exactly zero as before.
 template <int D>
 __device__ inline void hill_chunk_load(const HillBiasDev& h, int w, int chunk, HillChunkRegs<D>& R)
 {
@@ -245,11 +249,114 @@ __device__ inline void hill_chunk_load(const HillBiasDev& h, int w, int chunk, H
     const int k0 = chunk * CHUNK + threadIdx.x;
 #pragma unroll
     for (int r = 0; r < PER_THREAD; r++) {
-        const int k = k0 + r * BIAS_BLOCK;
-        const bool ok = k < n;
+        const int kc = min(k0 + r * BIAS_BLOCK, h.cap - 1);
 #pragma unroll
-        for (int d = 0; d < D; d++) R.cx[r][d] = ok ? C[(size_t)d * h.cap + k] : 0.0;
-        R.hk[r] = ok ? H[k] : 0.0;
+        for (int d = 0; d < D; d++) R.cx[r][d] = C[(size_t)d * h.cap + kc];
+        R.hk[r] = H[kc];
+    }
+#pragma unroll
+    for (int r = 0; r < PER_THREAD; r++) {
+        const bool ok = k0 + r * BIAS_BLOCK < n;
+#pragma unroll
+        for (int d = 0; d < D; d++) R.cx[r][d] = ok ? R.cx[r][d] : 0.0;
+        R.hk[r] = ok ? R.hk[r] : 0.0;
+    }
+}
+
+// The walker's CV values of a hill bias (scalar loads).
+template <int D>
+__device__ inline void hill_chunk_cv(const StepView& v, const HillBiasDev& h, int w, double (&s)[D])
+{
+    const double* cv = v.cvVal + (size_t)w * v.V;
+#pragma unroll
+    for (int d = 0; d < D; d++) s[d] = cv[h.cv[d]];
+}
+
+// The thread's sum over its hills at CV values s: V and dV/ds_d.  The
+// exponent E is formed without contraction (its E <= 8 cut must fall where
+// the oracle's does); the gradient terms use fused multiply-adds.  PER: some
+// CV of the bias is periodic (a uniform branch picks the form, so that
+// non-periodic biases skip the minimum-image arithmetic).
+template <int D, bool PER>
+__device__ inline void hill_terms_impl(const HillBiasDev& h, const double (&s)[D], const HillChunkRegs<D>& R,
+                                       double* acc)
+{
+#pragma unroll
+    for (int d = 0; d <= D; d++) acc[d] = 0.0;
+#pragma unroll
+    for (int r = 0; r < PER_THREAD; r++) {
+        double e = 0.0, dc[D];
+        {
+#pragma clang fp contract(off)
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                double x = s[d] - R.cx[r][d];
+                if (PER && h.period[d] > 0.0) x = x - rint(x * h.invPeriod[d]) * h.period[d];
+                dc[d] = x;
+                e = e + x * x * h.i2s2[d];
+            }
+        }
+        const double t = (e <= 8.0) ? R.hk[r] * (double)expf(-(float)e) : 0.0;
+        acc[0] += t;
+#pragma unroll
+        for (int d = 0; d < D; d++) acc[1 + d] = fma(-t * dc[d], h.is2[d], acc[1 + d]);
+    }
+}
+
+template <int D>
+__device__ inline void hill_chunk_terms(const HillBiasDev& h, const double (&s)[D], const HillChunkRegs<D>& R,
+                                        double* acc)
+{
+    bool per = false;
+#pragma unroll
+    for (int d = 0; d < D; d++) per = per || h.period[d] > 0.0;
+    if (per)
+        hill_terms_impl<D, true>(h, s, R, acc);
+    else
+        hill_terms_impl<D, false>(h, s, R, acc);
+}
+
+template <int D>
+__device__ inline void hill_chunk_sum_at(const HillBiasDev& h, const double (&s)[D], const HillChunkRegs<D>& R,
+                                         double* sh, double* dst)
+{
+    double acc[D + 1];
+    hill_chunk_terms<D>(h, s, R, acc);
+    block_partial<D + 1>(acc, sh, dst);
+}
+
+// A shared hill list: one workgroup evaluates its chunk for up to CHUNK_WB
+// walkers (w0 ...), the chunk loaded once.  The walkers' sums go through one
+// butterfly (each value on its own, as block_partial's) and the same
+// wave-order sum, so every walker's partial equals a one-walker workgroup's
+// bit for bit.  sh >= CHUNK_WB (D + 1) 4.
+template <int D>
+__device__ void hill_chunk_walkers(const StepView& v, const HillBiasDev& h, int w0, int nw, int chunk,
+                                   int totalChunks, double* part, double* sh)
+{
+    constexpr int NV = CHUNK_WB * (D + 1);
+    double s[CHUNK_WB][D];
+#pragma unroll
+    for (int j = 0; j < CHUNK_WB; j++) hill_chunk_cv<D>(v, h, w0 + min(j, nw - 1), s[j]);
+    HillChunkRegs<D> R;
+    hill_chunk_load<D>(h, w0, chunk, R);
+    double acc[NV];
+#pragma unroll
+    for (int j = 0; j < CHUNK_WB; j++) hill_chunk_terms<D>(h, s[j], R, acc + j * (D + 1));
+    wave_sum_n<NV>(acc);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < NV; k++) sh[k * 4 + wid] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < nw) {
+        const int j = threadIdx.x;
+        double* dst = part + ((size_t)(w0 + j) * totalChunks + chunk) * PW;
+#pragma unroll
+        for (int d = 0; d <= D; d++) {
+            const int k = j * (D + 1) + d;
+            dst[d] = sh[k * 4] + sh[k * 4 + 1] + sh[k * 4 + 2] + sh[k * 4 + 3];
+        }
     }
 }
 
@@ -257,38 +364,22 @@ template <int D>
 __device__ inline void hill_chunk_sum(const StepView& v, const HillBiasDev& h, int w, const HillChunkRegs<D>& R,
                                       double* sh, double* dst)
 {
-#pragma clang fp contract(off)
-    const double* cv = v.cvVal + (size_t)w * v.V;
     double s[D];
-#pragma unroll
-    for (int d = 0; d < D; d++) s[d] = cv[h.cv[d]];
-    double acc[D + 1];
-#pragma unroll
-    for (int d = 0; d <= D; d++) acc[d] = 0.0;
-#pragma unroll
-    for (int r = 0; r < PER_THREAD; r++) {
-        double e = 0.0, dc[D];
-#pragma unroll
-        for (int d = 0; d < D; d++) {
-            double x = s[d] - R.cx[r][d];
-            if (h.period[d] > 0.0) x = x - rint(x * h.invPeriod[d]) * h.period[d];
-            dc[d] = x;
-            e = e + x * x * h.i2s2[d];
-        }
-        const double t = (e <= 8.0) ? R.hk[r] * (double)expf(-(float)e) : 0.0;
-        acc[0] += t;
-#pragma unroll
-        for (int d = 0; d < D; d++) acc[1 + d] -= t * dc[d] * h.is2[d];
-    }
-    block_partial<D + 1>(acc, sh, dst);
+    hill_chunk_cv<D>(v, h, w, s);
+    hill_chunk_sum_at<D>(h, s, R, sh, dst);
 }
 
+// The CV values are requested before the hill loads: both chains are then
+// in flight together (a wait for the scalar loads would otherwise also wait
+// for the hill loads, which count on the same counter).
 template <int D>
 __device__ void hill_chunk(const StepView& v, const HillBiasDev& h, int w, int chunk, double* sh, double* dst)
 {
+    double s[D];
+    hill_chunk_cv<D>(v, h, w, s);
     HillChunkRegs<D> R;
     hill_chunk_load<D>(h, w, chunk, R);
-    hill_chunk_sum<D>(v, h, w, R, sh, dst);
+    hill_chunk_sum_at<D>(h, s, R, sh, dst);
 }
 
 template <int D>
@@ -482,6 +573,10 @@ constexpr int MAX_LDS_VALUES = 1024;
 template <int KIND, int DD>
 __device__ inline void chunk_block(const StepView& v, const FusedBiasPlan& fp, int b, int w, double* sh)
 {
+    if (KIND == 1 && fp.nChunked == 1) {  // one hill bias: its descriptor from the kernel arguments
+        hill_chunk<DD>(v, fp.hill0, w, b, sh, fp.part + ((size_t)w * fp.totalChunks + b) * PW);
+        return;
+    }
     int k = 0;
     while (k + 1 < fp.nChunked && b >= fp.chunkStart[k + 1]) k++;
     const int chunk = b - fp.chunkStart[k];
@@ -516,8 +611,28 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_chunks(StepView v, Fus
     chunk_block<KIND, DD>(v, fp, blockIdx.x, blockIdx.y, sh);
 }
 
-static void launch_chunks(const StepView& v, const FusedBiasPlan& fp, int nw, hipStream_t s)
+// One shared hill list: CHUNK_WB walkers per workgroup (a kernel of its own,
+// so that the one-walker chunk kernel keeps its small register budget).
+template <int DD>
+__global__ __launch_bounds__(BIAS_BLOCK) void k_step_hill_chunks_shared(StepView v, FusedBiasPlan fp)
 {
+    __shared__ double sh[CHUNK_WB * (DD + 1) * 4];
+    const int w0 = blockIdx.y * CHUNK_WB;
+    hill_chunk_walkers<DD>(v, fp.hill0, w0, min(CHUNK_WB, fp.nWalkers - w0), blockIdx.x, fp.totalChunks, fp.part,
+                           sh);
+}
+
+static void launch_chunks(const StepView& v, const FusedBiasPlan& fpIn, int nw, hipStream_t s)
+{
+    FusedBiasPlan fp = fpIn;
+    fp.nWalkers = nw;
+    if (fp.kind == 1 && fp.walkersPerChunk > 1 && fp.kindD >= 1 && fp.kindD <= 3) {
+        const dim3 g(fp.totalChunks, (nw + CHUNK_WB - 1) / CHUNK_WB), b(BIAS_BLOCK);
+        if (fp.kindD == 1) hipLaunchKernelGGL(k_step_hill_chunks_shared<1>, g, b, 0, s, v, fp);
+        else if (fp.kindD == 2) hipLaunchKernelGGL(k_step_hill_chunks_shared<2>, g, b, 0, s, v, fp);
+        else hipLaunchKernelGGL(k_step_hill_chunks_shared<3>, g, b, 0, s, v, fp);
+        return;
+    }
     const dim3 g(fp.totalChunks, nw), b(BIAS_BLOCK);
     if (fp.kind == 1 && fp.kindD == 1) hipLaunchKernelGGL((k_step_bias_chunks<1, 1>), g, b, 0, s, v, fp);
     else if (fp.kind == 1 && fp.kindD == 2) hipLaunchKernelGGL((k_step_bias_chunks<1, 2>), g, b, 0, s, v, fp);
@@ -667,7 +782,7 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_tiny(StepView v, const Smal
     if constexpr (KIND == 1) {
         if (fp.nChunked == 1 && fp.totalChunks <= TINY_PREFETCH_CHUNKS) {
             // the hill loads do not depend on the CVs: in flight during the CV evaluation
-            const HillBiasDev& h = fp.hills[0];
+            const HillBiasDev& h = fp.hill0;
             HillChunkRegs<DD> R[TINY_PREFETCH_CHUNKS];
 #pragma unroll
             for (int c = 0; c < TINY_PREFETCH_CHUNKS; c++)

@@ -1459,12 +1459,13 @@ __global__ __launch_bounds__(PATH_MSD_THREADS) void k_path_msd(StepView v, const
     path_msd_block(v, items[p], slots, frames, blockIdx.x - items[p].msdBlk0, w0 + blockIdx.y);
 }
 
-__global__ __launch_bounds__(64 * PATH_PHASES) void k_path_rows(StepView v, const PathItem* __restrict__ items,
+template <int PH>
+__global__ __launch_bounds__(64 * PH) void k_path_rows(StepView v, const PathItem* __restrict__ items,
                                                                 int nItems, const int* __restrict__ slots,
                                                                 const float* __restrict__ frames, int w0)
 {
     __shared__ double shW[PATH_MAX_FRAMES];
-    __shared__ double shRed[6][64 * PATH_PHASES];
+    __shared__ double shRed[6][64 * PH];
     __shared__ double sh[4 * 16];
     int p = 0;
     while (p + 1 < nItems && (int)blockIdx.x >= items[p + 1].rowBlk0) p++;
@@ -1488,7 +1489,7 @@ __global__ __launch_bounds__(64 * PATH_PHASES) void k_path_rows(StepView v, cons
         t[0] += wv;
         t[1] += (f + 1) * wv;
     }
-    block_sum<2, PATH_PHASES>(t, sh);  // ends with a barrier: shW is complete
+    block_sum<2, PH>(t, sh);  // ends with a barrier: shW is complete
     const double S = t[0];
     const bool ok = S > 1e-300;
     const double s = ok ? t[1] / S : 0.0;
@@ -1499,7 +1500,7 @@ __global__ __launch_bounds__(64 * PATH_PHASES) void k_path_rows(StepView v, cons
             a[0] += ((f + 1) - s) * msd[f] * shW[f];
             a[1] += msd[f] * shW[f];
         }
-        block_sum<2, PATH_PHASES>(a, sh);
+        block_sum<2, PH>(a, sh);
         if (threadIdx.x == 0) {
             v.cvVal[(size_t)w * v.V + it.valueS] = s;
             v.cvVal[(size_t)w * v.V + it.valueS + 1] = z;
@@ -1510,21 +1511,21 @@ __global__ __launch_bounds__(64 * PATH_PHASES) void k_path_rows(StepView v, cons
     }
     double js0 = 0, js1 = 0, js2 = 0, jz0 = 0, jz1 = 0, jz2 = 0;
     if (j < M) {
-        // frames f = ph, ph + PATH_PHASES, ... in increasing order; the loads
+        // frames f = ph, ph + PH, ... in increasing order; the loads
         // of PATH_ROWS_UNROLL of them are issued before any is used
         const float* F = frames + it.frameOff + (size_t)j * 3;
-        for (int fb = ph; fb < N; fb += PATH_PHASES * PATH_ROWS_UNROLL) {
+        for (int fb = ph; fb < N; fb += PH * PATH_ROWS_UNROLL) {
             float r[PATH_ROWS_UNROLL][3];
 #pragma unroll
             for (int u = 0; u < PATH_ROWS_UNROLL; u++) {  // clamped frame: unconditional loads
-                const float* rp = F + (size_t)min(fb + u * PATH_PHASES, N - 1) * M * 3;
+                const float* rp = F + (size_t)min(fb + u * PH, N - 1) * M * 3;
                 r[u][0] = rp[0];
                 r[u][1] = rp[1];
                 r[u][2] = rp[2];
             }
 #pragma unroll
             for (int u = 0; u < PATH_ROWS_UNROLL; u++) {
-                const int f = fb + u * PATH_PHASES;
+                const int f = fb + u * PH;
                 if (f < N) {
                     const double wv = shW[f];
                     const double dx = (double)q.x - r[u][0], dy = (double)q.y - r[u][1], dz = (double)q.z - r[u][2];
@@ -1551,7 +1552,7 @@ __global__ __launch_bounds__(64 * PATH_PHASES) void k_path_rows(StepView v, cons
 #pragma unroll
         for (int k = 0; k < 6; k++) {
             double acc = 0.0;
-            for (int q = 0; q < PATH_PHASES; q++) acc += shRed[k][q * 64 + lane];
+            for (int q = 0; q < PH; q++) acc += shRed[k][q * 64 + lane];
             r6[k] = acc;
         }
         const double scS = ok ? -2.0 * lam / (M * S) : 0.0, scZ = ok ? 2.0 / (M * S) : 0.0;
@@ -1635,20 +1636,28 @@ void launch_path_frames_adjoint(const StepView& v, const PathItem* items, int pi
                        gF, W);
 }
 
-void launch_path_rows(const StepView& v, const PathItem* items, int n, int nRowBlocks, const int* slots,
+// Row blocks of 8 frame phases, or of 4 when there are more blocks than fit
+// on the chip at once (each lane then covers twice the frames, and twice as
+// many blocks are resident).  Up to 256 frames both forms form w_f, S and s
+// with the same additions (a thread per frame, the waves past the frames add
+// zeros), so the adjoint's coefficients match either.
+void launch_path_rows(const StepView& v, const PathItem* items, int n, int nRowBlocks, int maxN, const int* slots,
                       const float* frames, int w0, int nw, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_path_rows, dim3(nRowBlocks, nw), dim3(64 * PATH_PHASES), 0, s, v, items, n, slots, frames,
-                       w0);
+    if ((long long)nRowBlocks * nw > 512 && maxN <= 256)
+        hipLaunchKernelGGL(k_path_rows<4>, dim3(nRowBlocks, nw), dim3(64 * 4), 0, s, v, items, n, slots, frames, w0);
+    else
+        hipLaunchKernelGGL(k_path_rows<PATH_PHASES>, dim3(nRowBlocks, nw), dim3(64 * PATH_PHASES), 0, s, v, items, n,
+                           slots, frames, w0);
 }
 
-void launch_cv_path(const StepView& v, const PathItem* items, int n, int nMsdBlocks, int nRowBlocks,
+void launch_cv_path(const StepView& v, const PathItem* items, int n, int nMsdBlocks, int nRowBlocks, int maxN,
                     const int* slots, const float* frames, int w0, int nw, hipStream_t s)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_path_msd, dim3(nMsdBlocks, nw), dim3(PATH_MSD_THREADS), 0, s, v, items, n, slots, frames, w0);
-    launch_path_rows(v, items, n, nRowBlocks, slots, frames, w0, nw, s);
+    launch_path_rows(v, items, n, nRowBlocks, maxN, slots, frames, w0, nw, s);
 }
 
 // ---------------------------------------------------------------------------

@@ -74,3 +74,34 @@ def test_config5_reduced():
         for w in range(W):
             r = os_[w].execute(c["positions"][w])
             check(g, os_[w], r, g.getLastBias(w), w)
+
+
+def test_shared_hill_chunks_match_plain():
+    """A hill list shared by every configuration is evaluated by chunk
+    workgroups that take four walkers each (the chunk loaded once).  With 30
+    configurations (the last workgroup holds 2) the batched step must give
+    the plain per-walker launches' bias energies and fixed-point forces bit
+    for bit."""
+    from gluedmd_amd.context import Context
+    from oracle.oracle import OracleContext
+    c = wl.config3_system(batch=30, M=60, N=16)
+    f0 = Force()
+    f0.add_path(list(range(c["M"])), [fr.reshape(-1).tolist() for fr in c["frames"]], c["lam"])
+    sz = np.array([OracleContext(f0, c["M"]).eval_cvs(p)[0] for p in c["confs"]])
+    hc, hh, sig = wl.config3_hills(c["rng"], (sz[:, 0].min(), sz[:, 0].max()), (sz[:, 1].min(), sz[:, 1].max()),
+                                   n_hills=3000)
+    f, hb = config3_force(c, hc, hh, sig)
+    B = len(c["confs"])
+    g = Context(f, walkers=[c["M"]] * B)
+    g.set_hills(hb, hc, hh)
+    for b in range(B):
+        g.set_positions(c["confs"][b], b)
+    g.zero_forces()
+    for b in range(B):
+        g.execute(b)
+    plain = [(g.getBiasEnergies(b).copy(), g.get_forces_fixed(b).copy()) for b in range(B)]
+    g.zero_forces()
+    g.execute_batch(use_graph=False)
+    for b in range(B):
+        assert np.array_equal(g.getBiasEnergies(b), plain[b][0]), b
+        assert np.array_equal(g.get_forces_fixed(b), plain[b][1]), b
