@@ -64,8 +64,8 @@ PMC_KIB = {
     "coordination-opes": {"k_coord_rank": (278.4, 792.4), "k_coord_offsets": (20.8, 38.8),
                           "k_coord_place": (357.3, 418.8), "k_coord_pairs_a": (1562.8, 250.0),
                           "k_coord_pairs<false, true>": (835.8, 590.3), "k_cv_group_big": (84.2, 156.3)},
-    "path-batch": {"k_cv_fused (path MSD blocks)": (3368.2, 128.0), "k_path_rows": (2112.0, 2416.0)},
-    "small": {"k_step_tiny<1, 2>": (41.8, 0.9)},
+    "path-batch": {"k_cv_fused (path MSD blocks)": (3377.2, 128.0), "k_path_rows<4>": (2116.1, 2416.0)},
+    "small": {"k_step_tiny<1, 2>": (55.1, 0.9)},
 }
 
 

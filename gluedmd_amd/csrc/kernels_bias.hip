@@ -909,32 +909,51 @@ void launch_dep_grid(const StepView& v, const GridBiasDev& g, int w0, int nw, hi
 }
 
 // Hill deposit: centre = current CVs, height = h0 exp(-V hif) (the WT rule of
-// CommonGluedKernels.cpp:3662 with this bias's own V).  One thread per walker;
-// a shared list appends the walkers in walker order from one thread.
-__global__ void k_dep_hills(StepView v, HillBiasDev h, int w0, int nw)
+// CommonGluedKernels.cpp:3662 with this bias's own V).  Private lists: one
+// thread per walker.  A shared list takes the walkers in walker order, as the
+// reference's sequential deposits do: walker w0 + i goes to slot count + i,
+// and the walkers past the capacity are dropped.  A height depends only on
+// that walker's V of the last evaluation, so the appends are independent; one
+// workgroup reads the count before any thread advances it.
+constexpr int DEP_HILLS_SHARED_BLOCK = 256;
+
+__device__ inline void deposit_hill(const StepView& v, const HillBiasDev& h, int tb, int w, int slot)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int wFirst = h.shared ? w0 : w0 + t;
-    const int wLast = h.shared ? w0 + nw : wFirst + 1;
-    if (h.shared ? t != 0 : t >= nw) return;
-    const int tb = h.shared ? 0 : wFirst;
     double* C = h.centers + (size_t)tb * h.D * h.cap;
     double* H = h.heights + (size_t)tb * h.cap;
-    for (int w = wFirst; w < wLast; w++) {
-        int n = h.count[tb];
-        if (n >= h.cap) return;
-        const double* cv = v.cvVal + (size_t)w * v.V;
-        const double V = v.benergy[(size_t)w * v.B + h.eslot];
-        for (int d = 0; d < h.D; d++) C[(size_t)d * h.cap + n] = cv[h.cv[d]];
-        H[n] = h.hif > 0.0 ? h.h0 * exp(-V * h.hif) : h.h0;
-        h.count[tb] = n + 1;
+    const double* cv = v.cvVal + (size_t)w * v.V;
+    const double V = v.benergy[(size_t)w * v.B + h.eslot];
+    for (int d = 0; d < h.D; d++) C[(size_t)d * h.cap + slot] = cv[h.cv[d]];
+    H[slot] = h.hif > 0.0 ? h.h0 * exp(-V * h.hif) : h.h0;
+}
+
+__global__ void k_dep_hills(StepView v, HillBiasDev h, int w0, int nw)
+{
+    if (h.shared) {
+        __shared__ int sN;
+        if (threadIdx.x == 0) sN = h.count[0];
+        __syncthreads();
+        const int n0 = sN;
+        for (int i = threadIdx.x; i < nw && n0 + i < h.cap; i += blockDim.x) deposit_hill(v, h, 0, w0 + i, n0 + i);
+        __syncthreads();
+        if (threadIdx.x == 0) h.count[0] = min(n0 + nw, h.cap > n0 ? h.cap : n0);
+        return;
     }
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nw) return;
+    const int w = w0 + t;
+    const int n = h.count[w];
+    if (n >= h.cap) return;
+    deposit_hill(v, h, w, w, n);
+    h.count[w] = n + 1;
 }
 
 void launch_dep_hills(const StepView& v, const HillBiasDev& h, int w0, int nw, hipStream_t s)
 {
-    const int n = h.shared ? 1 : nw;
-    hipLaunchKernelGGL(k_dep_hills, dim3((n + 63) / 64), dim3(64), 0, s, v, h, w0, nw);
+    if (h.shared)
+        hipLaunchKernelGGL(k_dep_hills, dim3(1), dim3(DEP_HILLS_SHARED_BLOCK), 0, s, v, h, w0, nw);
+    else
+        hipLaunchKernelGGL(k_dep_hills, dim3((nw + 63) / 64), dim3(64), 0, s, v, h, w0, nw);
 }
 
 // ---------------------------------------------------------------------------

@@ -353,3 +353,46 @@ def test_tiny_step_matches_plain():
             assert np.array_equal(gA.getBiasEnergies(w), gP.getBiasEnergies(w)), (step, w)
     for w in range(W):
         assert gA.getBiasState(w) == gP.getBiasState(w)
+
+
+def test_shared_hill_deposits_fill_in_walker_order():
+    """Five walkers deposit every step into one shared hill list of capacity
+    12: each step appends the walkers in walker order (heights from each
+    walker's own V), and the third deposit step fills the list part-way
+    through the walkers.  Hills, count and energies follow the oracle's sequential
+    walker-order deposits."""
+    from gluedmd_amd.context import Context
+    from oracle.oracle import OracleContext, _Hills
+    W, N, cap = 5, 12, 12
+    f = Force(pbc=True, temperature=300.0)
+    d = f.add_distance([0, 1])
+    t = f.add_dihedral([2, 3, 4, 5])
+    hb = f.add_metad_hills([d, t], sigma=[0.1, 0.3], height=1.0, pace=1, gamma=8.0, max_hills=cap,
+                           period=[0.0, 2 * math.pi])
+    f.share_bias(hb)
+    g = Context(f, walkers=[N] * W)
+    g.set_box(2.0)
+    _, cvs, params, ip = f.biases[hb]
+    shared = _Hills(len(cvs), params, ip)
+    os_ = []
+    for w in range(W):
+        o = OracleContext(f, N, shared={hb: shared})
+        o.set_box(2.0)
+        os_.append(o)
+    rng = np.random.default_rng(17)
+    base = [rng.uniform(0, 2, size=(N, 3)) for _ in range(W)]
+    for step in range(5):
+        g.update_state(step)
+        for o in os_:
+            o.update_state(step)
+        pos = [(b + rng.normal(0, 0.03, size=b.shape)).astype(np.float32) for b in base]
+        g.set_all_positions(pos)
+        g.zero_forces()
+        g.execute_batch(use_graph=False)
+        for w in range(W):
+            r = os_[w].execute(pos[w])
+            check(g, os_[w], r, g.getLastBias(w), w)
+    hc, hh = g.get_hills(hb, 0)
+    assert len(hh) == len(shared.heights) == cap
+    assert np.allclose(hc, shared.centers, rtol=1e-6, atol=1e-9)
+    assert np.allclose(hh, shared.heights, rtol=1e-6, atol=0.0)
