@@ -198,6 +198,19 @@ void Engine::validate(int minAtoms) const
                 if (!ok) err("BIAS_OPES_MULTITHERMAL: cvIndices[0] must reference a CV_ENERGY value index");
                 break;
             }
+            case BIAS_HARMONIC:
+                if (D > MAX_BIAS_D) err("harmonic restraint supports at most 8 CVs");
+                if ((int)s.params.size() < 2 * D) err("HARMONIC params must be [k_0, s0_0, k_1, s0_1, ...]");
+                break;
+            case BIAS_LINEAR:
+                if (D > MAX_BIAS_D) err("linear coupling supports at most 8 CVs");
+                if ((int)s.params.size() < D) err("LINEAR params must be [k_0, k_1, ...]");
+                break;
+            case BIAS_UPPER_WALL:
+            case BIAS_LOWER_WALL:
+                if (D > MAX_BIAS_D) err("walls support at most 8 CVs");
+                if ((int)s.params.size() < 4 * D) err("wall params must be [at, kappa, eps, n] per CV");
+                break;
             default: err("bias type " + std::to_string(s.type) + " is not supported by this implementation");
         }
     }
@@ -725,6 +738,23 @@ void Engine::buildPlan(int nMass, const double* masses)
             scalCount++;
             smallBias_.push_back(it);
             smallBiasHost_.push_back({(int)b, pace, s.type});
+        } else if (s.type == BIAS_HARMONIC || s.type == BIAS_LINEAR || s.type == BIAS_UPPER_WALL ||
+                   s.type == BIAS_LOWER_WALL) {
+            // static restraints: parameters rounded to FP32 as the reference
+            // stores them (CommonBiasKernels.cpp:118-126, 676-678, 700-707)
+            SmallBiasItem it{};
+            it.type = s.type;
+            it.gslot = gslot;
+            it.eslot = (int)b;
+            it.D = D;
+            for (int d = 0; d < D; d++) it.cv[d] = s.cvs[d];
+            const int per = s.type == BIAS_HARMONIC ? 2 : s.type == BIAS_LINEAR ? 1 : 4;
+            it.stateOff = (int)smallConst_.size();
+            for (int k = 0; k < per * D; k++) smallConst_.push_back((double)(float)s.params[k]);
+            it.bias = s.type == BIAS_HARMONIC ? (s.iparams.empty() ? 0 : s.iparams[0] != 0)
+                                              : (s.type == BIAS_LOWER_WALL ? 1 : 0);
+            smallBias_.push_back(it);
+            smallBiasHost_.push_back({(int)b, 0, s.type});
         }
         gslot += D;
     }
@@ -805,6 +835,10 @@ void Engine::buildPlan(int nMass, const double* masses)
         fplan_.counter = dFusedCounter_.as<unsigned int>();
         fplan_.small = dSmallBias_.as<SmallBiasItem>();
         fplan_.nSmall = (int)smallBias_.size();
+        fplan_.restraints = 0;
+        for (auto& it : smallBias_)
+            fplan_.restraints |= it.type == BIAS_HARMONIC || it.type == BIAS_LINEAR || it.type == BIAS_UPPER_WALL ||
+                                 it.type == BIAS_LOWER_WALL;
         // one workgroup scatters a walker when its atom list is short
         fplan_.scatter = (S_ <= 4096 && V_ <= 1024) ? 1 : 0;
         // all chunked biases of one type and dimension: a specialised chunk
@@ -1769,8 +1803,17 @@ std::vector<char> Engine::getState(int w)
     }
     i32(0);  // PBMetaD
     i32(0);  // external
-    i32(0);  // linear
-    i32(0);  // walls
+    {
+        // linear and wall restraints are stateless: their counts only
+        // (CommonSerialization.cpp:104-108)
+        int nlin = 0, nwall = 0;
+        for (auto& it : smallBias_) {
+            nlin += it.type == BIAS_LINEAR;
+            nwall += it.type == BIAS_UPPER_WALL || it.type == BIAS_LOWER_WALL;
+        }
+        i32(nlin);
+        i32(nwall);
+    }
     int noe = 0, nmt = 0;
     for (auto& it : smallBias_) {
         noe += it.type == BIAS_OPES_EXPANDED;
@@ -1910,9 +1953,9 @@ void Engine::setState(int w, const std::vector<char>& blob)
     if (p >= end) return;
     count("external", 0);
     if (p >= end) return;
-    count("linear", 0);
+    (void)i32();  // linear restraints: stateless, count not checked (CommonSerialization.cpp:320-326)
     if (p >= end) return;
-    count("wall", 0);
+    (void)i32();  // walls
     if (p >= end) return;
     int noe = 0, nmt = 0;
     for (auto& it : smallBias_) {

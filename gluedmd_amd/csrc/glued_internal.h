@@ -205,9 +205,10 @@ struct OpesBiasDev {
 struct SmallBiasItem {
     int type, D, gslot, eslot;
     int cv[MAX_BIAS_D];
-    int bias;                 // index into the grid-bias table (METAD)
+    int bias;                 // index into the grid-bias table (METAD); restraints: flag
+                              // (HARMONIC: periodic; walls: 1 = lower)
     int nState;               // OPES_EXPANDED / MULTITHERMAL states
-    int stateOff;             // offset into the state arrays
+    int stateOff;             // offset into the state arrays (restraints: their parameters)
     double p[8];              // type-specific scalars (k, at, kT0, invKT, ...)
     GridBiasDev grid;         // METAD: a copy of the grid descriptor (one load less before the CV reads)
 };
@@ -285,6 +286,7 @@ struct FusedBiasPlan {
     // is one shared hill list: the chunk is loaded once for them all)
     int walkersPerChunk;
     int nWalkers;            // set at launch
+    int restraints;          // 1: some small bias is a harmonic / linear / wall restraint
 };
 
 constexpr int CHUNK_WB = 4;

@@ -118,6 +118,66 @@ __device__ inline double wave_max1(double x)
 }
 
 // bgL (optional, LDS): a copy of the walker's bias-gradient slots.
+// Static restraints, CV d of the bias at value s: adds to V, returns dV/ds.
+// Harmonic CommonGluedKernels.cpp:2506-2540 (periodic flag: the difference
+// folded to the nearest 2 pi image), linear :2542-2566, walls :2568-2616
+// (delta = s - at, or at - s for a lower wall; only delta > 0 counts;
+// n = 1, 2 in closed form, pow() otherwise).  Unfused, as the oracle.
+__device__ inline double restraint_term(const SmallBiasItem& it, const double* p, int d, double s, double& V)
+{
+#pragma clang fp contract(off)
+    const int flag = it.bias;
+    if (it.type == BIAS_HARMONIC) {
+        const double TWO_PI = 6.283185307179586;
+        const double k = p[2 * d];
+        double ds = s - p[2 * d + 1];
+        if (flag) ds -= TWO_PI * round(ds / TWO_PI);
+        V += 0.5 * k * ds * ds;
+        return k * ds;
+    }
+    if (it.type == BIAS_LINEAR) {
+        V += p[d] * s;
+        return p[d];
+    }
+    const double at = p[4 * d], kappa = p[4 * d + 1], eps = p[4 * d + 2], n = p[4 * d + 3];
+    const double delta = flag ? at - s : s - at;
+    if (delta <= 0.0) return 0.0;
+    double powered, dPow;
+    if (n == 1.0) {
+        powered = delta;
+        dPow = 1.0;
+    } else if (n == 2.0) {
+        powered = delta * delta;
+        dPow = 2.0 * delta;
+    } else {
+        powered = pow(delta, n);
+        dPow = n * pow(delta, n - 1.0);
+    }
+    const double e = exp(eps * delta);
+    V += kappa * powered * e;
+    return (flag ? -1.0 : 1.0) * kappa * (dPow + eps * powered) * e;
+}
+
+// Every CV of a restraint item: energy slot, gradient slots (and the LDS
+// mirror).
+__device__ inline void restraints_eval(const StepView& v, const SmallBiasState& st, const SmallBiasItem& it, int w,
+                                       double* bgL)
+{
+    const double* cv = v.cvVal + (size_t)w * v.V;
+    const double* p = st.stateConst + it.stateOff;  // parameters, FP32-rounded
+    double* bg = v.bgrad + (size_t)w * v.G + it.gslot;
+    double V = 0.0;
+    for (int d = 0; d < it.D; d++) {
+        const double g = restraint_term(it, p, d, cv[it.cv[d]], V);
+        bg[d] = g;
+        if (bgL) bgL[it.gslot + d] = g;
+    }
+    v.benergy[(size_t)w * v.B + it.eslot] = V;
+}
+
+// RESTRAINTS: the plan has restraint items (kernels of plans without them are
+// compiled without that code, which would otherwise cost the finalize time).
+template <bool RESTRAINTS = true>
 __device__ void small_bias_eval(const StepView& v, const SmallBiasState& st, const SmallBiasItem& it, int i, int w,
                                 double* bgL = nullptr)
 {
@@ -173,6 +233,8 @@ __device__ void small_bias_eval(const StepView& v, const SmallBiasState& st, con
             bg[0] = g0;
             if (bgL) bgL[it.gslot] = g0;
         }
+    } else if (RESTRAINTS && lane == 0) {  // harmonic, linear, walls
+        restraints_eval(v, st, it, w, bgL);
     }
 }
 
@@ -659,6 +721,7 @@ constexpr int FIN_OWNERS = 3 * 64;     // slot-owning threads (waves 0, 2, 3)
 constexpr int FIN_LDS_G = 512;         // bias-gradient slots held in LDS
 constexpr int FIN_LDS_COMB = 1024;     // combine entries held in LDS
 
+template <bool RESTRAINTS>
 __device__ void finalize_walker(const StepView& v, const FusedBiasPlan& fp, int w)
 {
     __shared__ double sU[MAX_LDS_VALUES];
@@ -711,7 +774,7 @@ __device__ void finalize_walker(const StepView& v, const FusedBiasPlan& fp, int 
             }
         }
     } else if (wave == 1) {
-        for (int i = 0; i < fp.nSmall; i++) small_bias_eval(v, fp.st, fp.small[i], i, w, bgL);
+        for (int i = 0; i < fp.nSmall; i++) small_bias_eval<RESTRAINTS>(v, fp.st, fp.small[i], i, w, bgL);
     }
     __syncthreads();
     if (!ldsComb) {
@@ -757,9 +820,10 @@ __device__ void finalize_walker(const StepView& v, const FusedBiasPlan& fp, int 
     for (int s = tid + FIN_SLOTS * FIN_OWNERS; s < v.S; s += BIAS_BLOCK) scatter_slot(v, sp, w, s, sU, MAX_LDS_VALUES);
 }
 
+template <bool RESTRAINTS>
 __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, FusedBiasPlan fp)
 {
-    finalize_walker(v, fp, blockIdx.x);
+    finalize_walker<RESTRAINTS>(v, fp, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -773,7 +837,7 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, F
 // ---------------------------------------------------------------------------
 constexpr int TINY_PREFETCH_CHUNKS = 4;  // one hill list of up to 4,096 entries: loads issued before the CVs
 
-template <int KIND, int DD>
+template <int KIND, int DD, bool RESTRAINTS = false>
 __global__ __launch_bounds__(BIAS_BLOCK) void k_step_tiny(StepView v, const SmallCVItem* __restrict__ cvItems, int nCv,
                                                           FusedBiasPlan fp)
 {
@@ -793,21 +857,22 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_tiny(StepView v, const Smal
             for (int c = 0; c < TINY_PREFETCH_CHUNKS; c++)
                 if (c < fp.totalChunks)
                     hill_chunk_sum<DD>(v, h, w, R[c], sh, fp.part + ((size_t)w * fp.totalChunks + c) * PW);
-            finalize_walker(v, fp, w);
+            finalize_walker<RESTRAINTS>(v, fp, w);
             return;
         }
     }
     for (int i = threadIdx.x; i < nCv; i += BIAS_BLOCK) small_item(v, cvItems[i], w);
     __syncthreads();
     for (int b = 0; b < fp.totalChunks; b++) chunk_block<KIND, DD>(v, fp, b, w, sh);  // each ends with a barrier
-    finalize_walker(v, fp, w);
+    finalize_walker<RESTRAINTS>(v, fp, w);
 }
 
 void launch_step_tiny(const StepView& v, const SmallCVItem* cvItems, int nCv, const FusedBiasPlan& fp, int nw,
                       hipStream_t s)
 {
     const dim3 g(nw), b(BIAS_BLOCK);
-    if (fp.kind == 1 && fp.kindD == 1) hipLaunchKernelGGL((k_step_tiny<1, 1>), g, b, 0, s, v, cvItems, nCv, fp);
+    if (fp.restraints) hipLaunchKernelGGL((k_step_tiny<0, 1, true>), g, b, 0, s, v, cvItems, nCv, fp);
+    else if (fp.kind == 1 && fp.kindD == 1) hipLaunchKernelGGL((k_step_tiny<1, 1>), g, b, 0, s, v, cvItems, nCv, fp);
     else if (fp.kind == 1 && fp.kindD == 2) hipLaunchKernelGGL((k_step_tiny<1, 2>), g, b, 0, s, v, cvItems, nCv, fp);
     else if (fp.kind == 1 && fp.kindD == 3) hipLaunchKernelGGL((k_step_tiny<1, 3>), g, b, 0, s, v, cvItems, nCv, fp);
     else if (fp.kind == 2 && fp.kindD == 1) hipLaunchKernelGGL((k_step_tiny<2, 1>), g, b, 0, s, v, cvItems, nCv, fp);
@@ -824,7 +889,10 @@ void launch_step_bias_chunks(const StepView& v, const FusedBiasPlan& fp, int nw,
 void launch_step_bias(const StepView& v, const FusedBiasPlan& fp, int nw, hipStream_t s)
 {
     if (fp.totalChunks > 0) launch_chunks(v, fp, nw, s);
-    hipLaunchKernelGGL(k_step_bias_finalize, dim3(nw), dim3(BIAS_BLOCK), 0, s, v, fp);
+    if (fp.restraints)
+        hipLaunchKernelGGL(k_step_bias_finalize<true>, dim3(nw), dim3(BIAS_BLOCK), 0, s, v, fp);
+    else
+        hipLaunchKernelGGL(k_step_bias_finalize<false>, dim3(nw), dim3(BIAS_BLOCK), 0, s, v, fp);
 }
 
 // ---------------------------------------------------------------------------

@@ -54,8 +54,12 @@ enum {
  * counterpart in the reference (params [h0, sigma*D, gamma, kT, period*D],
  * int params [pace, maxHills]; period 0 = non-periodic). */
 enum {
+    GLUED_BIAS_HARMONIC = 1,     /* params [k, s0] per CV; int params [periodic] (optional) */
     GLUED_BIAS_METAD = 3,
     GLUED_BIAS_OPES = 5,
+    GLUED_BIAS_UPPER_WALL = 8,   /* params [at, kappa, eps, n] per CV */
+    GLUED_BIAS_LOWER_WALL = 9,   /* params [at, kappa, eps, n] per CV */
+    GLUED_BIAS_LINEAR = 10,      /* params [k] per CV */
     GLUED_BIAS_OPES_EXPANDED = 11,
     GLUED_BIAS_OPES_MULTITHERMAL = 15,
     GLUED_BIAS_METAD_HILLS = 101

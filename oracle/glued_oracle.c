@@ -1065,3 +1065,67 @@ EXPORT void orc_multithermal_update(double U, double V, const double* dbeta, dou
     }
     rct[0] = r + inc + kT0 * log1p(-1.0 / cnt);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Static restraints (no state, no deposits).  Parameters arrive already    */
+/* rounded to FP32, as the reference stores them (CommonBiasKernels.cpp:    */
+/* 118-126, 676-678, 700-707).                                               */
+/* ------------------------------------------------------------------------ */
+
+/* Harmonic, CommonGluedKernels.cpp:2506-2540: V = sum k/2 (s - s0)^2; with  */
+/* the periodic flag the difference is folded to the nearest 2 pi image.     */
+/* p = [k_0, s0_0, k_1, s0_1, ...].                                          */
+EXPORT double orc_harmonic_eval(const double* s, int D, const double* p, int periodic, double* g)
+{
+    const double TWO_PI = 6.283185307179586;
+    double V = 0.0;
+    for (int d = 0; d < D; d++) {
+        double ds = s[d] - p[2 * d + 1];
+        if (periodic) ds -= TWO_PI * round(ds / TWO_PI);
+        V += 0.5 * p[2 * d] * ds * ds;
+        g[d] = p[2 * d] * ds;
+    }
+    return V;
+}
+
+/* Linear, CommonGluedKernels.cpp:2542-2566: V = sum k s, dV/ds = k.         */
+EXPORT double orc_linear_eval(const double* s, int D, const double* k, double* g)
+{
+    double V = 0.0;
+    for (int d = 0; d < D; d++) {
+        V += k[d] * s[d];
+        g[d] = k[d];
+    }
+    return V;
+}
+
+/* One-sided wall, CommonGluedKernels.cpp:2568-2616.  lower = 0: delta =    */
+/* s - at, else at - s; only delta > 0 counts: V = kappa delta^n e^(eps      */
+/* delta), dV/ds = +-kappa (n delta^(n-1) + eps delta^n) e^(eps delta), with */
+/* n = 1 and n = 2 in closed form and pow() otherwise.                       */
+/* p = [at, kappa, eps, n] per CV.                                           */
+EXPORT double orc_wall_eval(const double* s, int D, const double* p, int lower, double* g)
+{
+    double V = 0.0;
+    for (int d = 0; d < D; d++) {
+        const double at = p[4 * d], kappa = p[4 * d + 1], eps = p[4 * d + 2], n = p[4 * d + 3];
+        const double delta = lower ? at - s[d] : s[d] - at;
+        g[d] = 0.0;
+        if (delta <= 0.0) continue;
+        double powered, dPow;
+        if (n == 1.0) {
+            powered = delta;
+            dPow = 1.0;
+        } else if (n == 2.0) {
+            powered = delta * delta;
+            dPow = 2.0 * delta;
+        } else {
+            powered = pow(delta, n);
+            dPow = n * pow(delta, n - 1.0);
+        }
+        const double e = exp(eps * delta);
+        V += kappa * powered * e;
+        g[d] = (lower ? -1.0 : 1.0) * kappa * (dPow + eps * powered) * e;
+    }
+    return V;
+}

@@ -29,6 +29,7 @@ LIB = os.path.join(HERE, "libglued_oracle.so")
 CV_DISTANCE, CV_ANGLE, CV_DIHEDRAL, CV_COM_DISTANCE, CV_GYRATION, CV_COORDINATION, CV_RMSD = 1, 2, 3, 4, 5, 6, 7
 CV_PATH, CV_POSITION, CV_DRMSD, CV_CONTACTMAP, CV_ENERGY = 10, 11, 12, 13, 23
 BIAS_METAD, BIAS_OPES, BIAS_OPES_EXPANDED, BIAS_OPES_MULTITHERMAL, BIAS_METAD_HILLS = 3, 5, 11, 15, 101
+BIAS_HARMONIC, BIAS_UPPER_WALL, BIAS_LOWER_WALL, BIAS_LINEAR = 1, 8, 9, 10
 FIXED = 4294967296.0
 
 _lib = None
@@ -76,6 +77,9 @@ def lib():
             "orc_opes_expanded_update_logz": (None, [dp, i, dp, d, dp, ip]),
             "orc_multithermal_eval": (d, [d, dp, dp, i, d, dp]),
             "orc_multithermal_update": (None, [d, d, dp, dp, dp, dp, i, d]),
+            "orc_harmonic_eval": (d, [dp, i, dp, i, dp]),
+            "orc_linear_eval": (d, [dp, i, dp, dp]),
+            "orc_wall_eval": (d, [dp, i, dp, i, dp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -350,6 +354,29 @@ class _Multithermal:
                                       self.kT0)
 
 
+class _Restraint:
+    """Harmonic, linear and wall restraints: no state, no deposits.  The
+    parameters are rounded to FP32 as the reference stores them."""
+
+    def __init__(self, t, D, params, iparams):
+        self.t, self.D = t, D
+        per = {BIAS_HARMONIC: 2, BIAS_LINEAR: 1}.get(t, 4)
+        self.p = np.asarray(params[:per * D], dtype=np.float32).astype(np.float64)
+        self.periodic = int(iparams[0]) if (t == BIAS_HARMONIC and iparams) else 0
+
+    def eval(self, s):
+        s, sp = P(s)
+        g = np.zeros(self.D)
+        if self.t == BIAS_HARMONIC:
+            V = lib().orc_harmonic_eval(sp, self.D, self.p.ctypes.data_as(dp), self.periodic, g.ctypes.data_as(dp))
+        elif self.t == BIAS_LINEAR:
+            V = lib().orc_linear_eval(sp, self.D, self.p.ctypes.data_as(dp), g.ctypes.data_as(dp))
+        else:
+            V = lib().orc_wall_eval(sp, self.D, self.p.ctypes.data_as(dp), int(self.t == BIAS_LOWER_WALL),
+                                    g.ctypes.data_as(dp))
+        return V, g
+
+
 # ---------------------------------------------------------------------------
 # Spec-level evaluator
 # ---------------------------------------------------------------------------
@@ -383,6 +410,8 @@ class OracleContext:
                 self.states.append(_Expanded(D, params, iparams))
             elif t == BIAS_OPES_MULTITHERMAL:
                 self.states.append(_Multithermal(params, iparams))
+            elif t in (BIAS_HARMONIC, BIAS_LINEAR, BIAS_UPPER_WALL, BIAS_LOWER_WALL):
+                self.states.append(_Restraint(t, D, params, iparams))
             else:
                 raise ValueError(f"oracle: bias type {t} not covered")
         self.cv_ready = False

@@ -396,3 +396,55 @@ def test_shared_hill_deposits_fill_in_walker_order():
     assert len(hh) == len(shared.heights) == cap
     assert np.allclose(hc, shared.centers, rtol=1e-6, atol=1e-9)
     assert np.allclose(hh, shared.heights, rtol=1e-6, atol=0.0)
+
+
+def _restraint_force(with_group):
+    f = Force(pbc=True)
+    d = f.add_distance([0, 1])
+    a = f.add_angle([2, 3, 4])
+    t = f.add_dihedral([4, 5, 6, 7])
+    x = f.add_position(8, 0)
+    cvs = [d, a, t, x]
+    if with_group:
+        cvs.append(f.add_gyration(list(range(9, 30))))
+    f.add_harmonic([d, t], [150.0, 12.0], [0.45, 2.9], periodic=True)
+    f.add_harmonic([a], 40.0, 1.2)
+    f.add_upper_wall(d, 0.3, 80.0, n=2, eps=0.0)
+    f.add_lower_wall(a, 2.5, 20.0, n=3.5, eps=0.7)
+    f.add_upper_wall(x, 0.2, 10.0, n=1, eps=1.0)
+    f.add_linear([x, a], [3.0, -1.5])
+    if with_group:
+        f.add_lower_wall(cvs[-1], 5.0, 2.0)
+    return f
+
+
+@pytest.mark.parametrize("with_group", [False, True])
+def test_restraints_match_oracle_and_plain(with_group):
+    """Harmonic (periodic and not), upper and lower walls (n = 1, 2, 3.5;
+    eps = 0 and not) and linear restraints on few-atom CVs (the one-workgroup
+    step) and with a group CV (the three-launch step): each walker against
+    the oracle, and the batched step equal to the plain launches bit for
+    bit."""
+    from gluedmd_amd.context import Context
+    from oracle.oracle import OracleContext
+    W, N = 3, 32
+    f = _restraint_force(with_group)
+    g = Context(f, walkers=[N] * W)
+    g.set_box(1.6)
+    rng = np.random.default_rng(23)
+    pos = [rng.uniform(0, 1.6, size=(N, 3)).astype(np.float32) for _ in range(W)]
+    g.set_all_positions(pos)
+    g.zero_forces()
+    g.execute_batch(use_graph=False)
+    Fb = [g.get_forces_fixed(w).copy() for w in range(W)]
+    Eb = [g.getBiasEnergies(w).copy() for w in range(W)]
+    for w in range(W):
+        o = OracleContext(f, N)
+        o.set_box(1.6)
+        r = o.execute(pos[w])
+        check(g, o, r, g.getLastBias(w), w)
+    g.zero_forces()
+    g.execute_all_plain()
+    for w in range(W):
+        assert np.array_equal(g.get_forces_fixed(w), Fb[w]), w
+        assert np.array_equal(g.getBiasEnergies(w), Eb[w]), w
