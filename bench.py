@@ -59,13 +59,13 @@ FIXED_SCALE = 4294967296.0
 # workload summaries under profiles/.
 PMC_FETCH_SCALE, PMC_WRITE_SCALE = 2.0, 1.0
 PMC_KIB = {
-    "multiwalker": {"k_step_bias_chunks<1, 2>": (15052.2, 60.0)},
+    "multiwalker": {"k_step_bias_chunks<1, 2>": (15104.7, 60.0)},
     "mixed-large": {"k_scatter_dense_rows": (41545.6, 49685.4)},
     "coordination-opes": {"k_coord_rank": (278.4, 792.4), "k_coord_offsets": (20.8, 38.8),
                           "k_coord_place": (357.3, 418.8), "k_coord_pairs_a": (1562.8, 250.0),
                           "k_coord_pairs<false, true>": (835.8, 590.3), "k_cv_group_big": (84.2, 156.3)},
     "path-batch": {"k_cv_fused (path MSD blocks)": (3377.2, 128.0), "k_path_rows<4>": (2116.1, 2416.0)},
-    "small": {"k_step_tiny<1, 2>": (55.1, 0.9)},
+    "small": {"k_step_tiny<1, 2, false>": (54.9, 0.9)},
 }
 
 
