@@ -26,6 +26,7 @@ void launch_bias_small(const StepView&, const SmallBiasState&, const SmallBiasIt
 void launch_bias_hills(const StepView&, const HillBiasDev&, int, int, hipStream_t);
 void launch_bias_opes(const StepView&, const OpesBiasDev&, int, int, hipStream_t);
 void launch_dep_grid(const StepView&, const GridBiasDev&, int, int, hipStream_t);
+void launch_dep_pb(const StepView&, const SmallBiasState&, const SmallBiasItem&, int, int, hipStream_t);
 void launch_dep_hills(const StepView&, const HillBiasDev&, int, int, hipStream_t);
 void launch_dep_opes(const StepView&, const OpesBiasDev&, int, int, hipStream_t);
 void launch_opes_welford(const StepView&, const OpesBiasDev&, int, int, hipStream_t);
@@ -205,6 +206,16 @@ void Engine::validate(int minAtoms) const
             case BIAS_LINEAR:
                 if (D > MAX_BIAS_D) err("linear coupling supports at most 8 CVs");
                 if ((int)s.params.size() < D) err("LINEAR params must be [k_0, k_1, ...]");
+                break;
+            case BIAS_PBMETAD:
+                if (D > MAX_BIAS_D) err("PBMetaD supports at most 8 CVs");
+                if ((int)s.params.size() < 3 + 3 * D) err("PBMETAD params must be [h0, gamma, kT, (sigma, min, max)*D]");
+                if ((int)s.iparams.size() < 1 + 2 * D) err("PBMETAD int params must be [pace, (bins, periodic)*D]");
+                for (int d = 0; d < D; d++) {
+                    if (s.iparams[1 + 2 * d] < 1) err("PBMETAD needs at least one bin per CV");
+                    if (!(s.params[5 + 3 * d] > s.params[4 + 3 * d])) err("PBMETAD grid max must exceed min");
+                }
+                if (s.iparams[0] < 1) err("PBMETAD pace must be positive");
                 break;
             case BIAS_UPPER_WALL:
             case BIAS_LOWER_WALL:
@@ -738,6 +749,46 @@ void Engine::buildPlan(int nMass, const double* masses)
             scalCount++;
             smallBias_.push_back(it);
             smallBiasHost_.push_back({(int)b, pace, s.type});
+        } else if (s.type == BIAS_PBMETAD) {
+            // CommonBiasKernels.cpp:298-400: one 1-D grid per CV,
+            // spacing (max - min) / bins, points bins (+1 unless periodic)
+            SmallBiasItem it{};
+            it.type = s.type;
+            it.gslot = gslot;
+            it.eslot = (int)b;
+            it.D = D;
+            for (int d = 0; d < D; d++) it.cv[d] = s.cvs[d];
+            const double gamma = s.params[1], kT = s.params[2];
+            it.p[0] = s.params[0];
+            it.p[1] = kT;
+            it.p[2] = gamma <= 1.0 ? 0.0 : 1.0 / ((gamma - 1.0) * kT);
+            it.p[3] = s.shared ? 1.0 : 0.0;
+            it.stateOff = (int)smallConst_.size();
+            PbHost ph;
+            ph.item = (int)smallBias_.size();
+            ph.N = D;
+            ph.T = s.shared ? 1 : W_;
+            ph.pace = s.iparams[0];
+            long long sub = 0;
+            for (int d = 0; d < D; d++) {
+                const int bins = s.iparams[1 + 2 * d], per = s.iparams[2 + 2 * d] != 0;
+                const double sigma = s.params[3 + 3 * d], lo = s.params[4 + 3 * d], hi = s.params[5 + 3 * d];
+                const int np = bins + (per ? 0 : 1);
+                for (double x : {lo, bins / (hi - lo), (double)np, (double)per, sigma, (double)sub})
+                    smallConst_.push_back(x);
+                ph.npts.push_back(np);
+                sub += np;
+            }
+            ph.stride = sub;
+            ph.base = pbGridTotal_;
+            pbGridTotal_ += ph.T * sub;
+            if (pbGridTotal_ > (long long)INT32_MAX) fail("PBMetaD grids too large");
+            it.bias = (int)ph.base;
+            it.nState = (int)sub;
+            ph.numDeposited.assign(ph.T, 0);
+            pbs_.push_back(ph);
+            smallBias_.push_back(it);
+            smallBiasHost_.push_back({(int)b, ph.pace, s.type});
         } else if (s.type == BIAS_HARMONIC || s.type == BIAS_LINEAR || s.type == BIAS_UPPER_WALL ||
                    s.type == BIAS_LOWER_WALL) {
             // static restraints: parameters rounded to FP32 as the reference
@@ -765,6 +816,7 @@ void Engine::buildPlan(int nMass, const double* masses)
     dGridTable_.upload(gridTable);
     dSmallBias_.upload(smallBias_);
     dSmallConst_.upload(smallConst_);
+    dPbGrid_.alloc(sizeof(double) * (size_t)std::max(pbGridTotal_, 1LL));
     smallVarPerWalker_ = std::max((int)smallConst_.size(), 1);
     dSmallVar_.alloc(sizeof(double) * (size_t)W_ * smallVarPerWalker_);
     {
@@ -835,10 +887,10 @@ void Engine::buildPlan(int nMass, const double* masses)
         fplan_.counter = dFusedCounter_.as<unsigned int>();
         fplan_.small = dSmallBias_.as<SmallBiasItem>();
         fplan_.nSmall = (int)smallBias_.size();
-        fplan_.restraints = 0;
+        fplan_.extra = 0;
         for (auto& it : smallBias_)
-            fplan_.restraints |= it.type == BIAS_HARMONIC || it.type == BIAS_LINEAR || it.type == BIAS_UPPER_WALL ||
-                                 it.type == BIAS_LOWER_WALL;
+            fplan_.extra |= it.type == BIAS_HARMONIC || it.type == BIAS_LINEAR || it.type == BIAS_UPPER_WALL ||
+                            it.type == BIAS_LOWER_WALL || it.type == BIAS_PBMETAD;
         // one workgroup scatters a walker when its atom list is short
         fplan_.scatter = (S_ <= 4096 && V_ <= 1024) ? 1 : 0;
         // all chunked biases of one type and dimension: a specialised chunk
@@ -1074,6 +1126,7 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         fp.st.scal = dSmallScal_.as<double>();
         fp.st.stateVarStride = smallVarPerWalker_;
         fp.st.scalStride = 4 * (long long)std::max((int)smallBias_.size(), 1);
+        fp.st.pbGrid = dPbGrid_.as<double>();
         fp.sp = scatterPlan();
         fp.scatter = forces ? 1 : 0;
         launch_step_tiny(v, dSmall_.as<SmallCVItem>(), (int)small_.size(), fp, W_, s);
@@ -1151,6 +1204,7 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
     st.scal = dSmallScal_.as<double>();
     st.stateVarStride = smallVarPerWalker_;
     st.scalStride = 4 * (long long)std::max((int)smallBias_.size(), 1);
+    st.pbGrid = dPbGrid_.as<double>();
     const ScatterPlan sp = scatterPlan();
     const bool biases = (stageMask & 10) && testGrads_.empty();
     if (biases || (stageMask & 4)) {
@@ -1361,6 +1415,7 @@ uint64_t Engine::depositMask(int step) const
         const auto& hs = smallBiasHost_[i];
         const bool dep = hs.type == BIAS_OPES_EXPANDED ? step % hs.pace == 0
                          : hs.type == BIAS_OPES_MULTITHERMAL ? (hs.pace > 0 && step % hs.pace == 0)
+                         : hs.type == BIAS_PBMETAD           ? step % hs.pace == 0
                                                              : false;
         if (dep) m |= 1ull << bit;
         bit++;
@@ -1392,6 +1447,7 @@ void Engine::launchDeposits(hipStream_t s, uint64_t mask)
     st.scal = dSmallScal_.as<double>();
     st.stateVarStride = smallVarPerWalker_;
     st.scalStride = 4 * (long long)std::max(nS, 1);
+    st.pbGrid = dPbGrid_.as<double>();
     for (int pass = 0; pass < 2; pass++)  // expanded first, then multithermal
         for (int i = 0; i < nS; i++) {
             const int want = pass == 0 ? BIAS_OPES_EXPANDED : BIAS_OPES_MULTITHERMAL;
@@ -1400,6 +1456,8 @@ void Engine::launchDeposits(hipStream_t s, uint64_t mask)
         }
     for (int k = 0; k < nG; k++)
         if (mask >> (nO + nS + k) & 1) launch_dep_grid(v, grids_[k]->d, 0, W_, s);
+    for (auto& ph : pbs_)  // after the MetaD grids (CommonGluedKernels.cpp:5147-5154)
+        if (mask >> (nO + ph.item) & 1) launch_dep_pb(v, st, smallBias_[ph.item], 0, W_, s);
     for (int k = 0; k < (int)hills_.size(); k++)
         if (mask >> (nO + nS + nG + k) & 1) launch_dep_hills(v, hills_[k]->d, 0, W_, s);
     hip_check(hipGetLastError(), "deposit launch");
@@ -1419,6 +1477,9 @@ void Engine::commitDeposits(uint64_t mask)
     for (size_t k = 0; k < grids_.size(); k++)
         if (mask >> (nO + nS + k) & 1)
             for (auto& n : grids_[k]->numDeposited) n += grids_[k]->d.walkerStride ? 1 : W_;
+    for (auto& ph : pbs_)
+        if (mask >> (nO + ph.item) & 1)
+            for (auto& n : ph.numDeposited) n += ph.T == 1 ? W_ : 1;
 }
 
 void Engine::updateState(hipStream_t s, int step)
@@ -1801,7 +1862,18 @@ std::vector<char> Engine::getState(int w)
         auto gr = download(g->d.grid + (size_t)t * g->d.total, g->d.total);
         put(gr.data(), 8 * gr.size());
     }
-    i32(0);  // PBMetaD
+    i32((int32_t)pbs_.size());  // PBMetaD: per bias its sub-grids, each {numDeposited, values}
+    for (auto& ph : pbs_) {
+        const int t = ph.T == 1 ? 0 : w;
+        i32(ph.N);
+        auto gr = download(dPbGrid_.as<double>() + ph.base + (size_t)t * ph.stride, (size_t)ph.stride);
+        size_t off = 0;
+        for (int i = 0; i < ph.N; i++) {
+            i32(ph.numDeposited[t]);
+            put(gr.data() + off, 8 * (size_t)ph.npts[i]);
+            off += ph.npts[i];
+        }
+    }
     i32(0);  // external
     {
         // linear and wall restraints are stateless: their counts only
@@ -1949,7 +2021,23 @@ void Engine::setState(int w, const std::vector<char>& blob)
                   "restore");
     }
     if (p >= end) return;
-    count("PBMetaD", 0);
+    count("PBMetaD", (int)pbs_.size());
+    for (auto& ph : pbs_) {
+        const int t = ph.T == 1 ? 0 : w;
+        count("PBMetaD sub-grid", ph.N);
+        std::vector<double> gr((size_t)ph.stride);
+        size_t off = 0;
+        int nd = 0;
+        for (int i = 0; i < ph.N; i++) {
+            nd = i32();
+            get(gr.data() + off, 8 * (size_t)ph.npts[i]);
+            off += ph.npts[i];
+        }
+        ph.numDeposited[t] = nd;
+        hip_check(hipMemcpy(dPbGrid_.as<double>() + ph.base + (size_t)t * ph.stride, gr.data(), 8 * gr.size(),
+                            hipMemcpyHostToDevice),
+                  "restore");
+    }
     if (p >= end) return;
     count("external", 0);
     if (p >= end) return;

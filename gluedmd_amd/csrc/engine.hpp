@@ -189,6 +189,17 @@ class Engine {
     struct SmallHost {
         int spec, pace, type;
     };
+    // PBMetaD: N 1-D sub-grids per bias in dPbGrid_ (item base + table *
+    // stride + sub-grid offset); the deposit count per table
+    struct PbHost {
+        int item, N, T, pace;
+        std::vector<int> npts;
+        long long base, stride;
+        std::vector<int> numDeposited;
+    };
+    std::vector<PbHost> pbs_;
+    long long pbGridTotal_ = 0;
+    DBuf dPbGrid_;
     std::vector<GridHost*> grids_;
     std::vector<HillHost*> hills_;
     std::vector<OpesHost*> opes_;

@@ -38,6 +38,7 @@ enum CVType {
 enum BiasType {
     BIAS_HARMONIC = 1,
     BIAS_METAD = 3,
+    BIAS_PBMETAD = 4,
     BIAS_OPES = 5,
     BIAS_UPPER_WALL = 8,
     BIAS_LOWER_WALL = 9,
@@ -206,9 +207,9 @@ struct SmallBiasItem {
     int type, D, gslot, eslot;
     int cv[MAX_BIAS_D];
     int bias;                 // index into the grid-bias table (METAD); restraints: flag
-                              // (HARMONIC: periodic; walls: 1 = lower)
-    int nState;               // OPES_EXPANDED / MULTITHERMAL states
-    int stateOff;             // offset into the state arrays (restraints: their parameters)
+                              // (HARMONIC: periodic; walls: 1 = lower); PBMETAD: first grid value
+    int nState;               // OPES_EXPANDED / MULTITHERMAL states; PBMETAD: grid values per table
+    int stateOff;             // offset into the state arrays (restraints, PBMETAD: their constants)
     double p[8];              // type-specific scalars (k, at, kT0, invKT, ...)
     GridBiasDev grid;         // METAD: a copy of the grid descriptor (one load less before the CV reads)
 };
@@ -219,6 +220,7 @@ struct SmallBiasState {
     double* stateVar;         // [W or 1][...] deltaF (multithermal)
     double* scal;             // [W or 1][nSmall][4]: logZ, numUpdates, rct, counter
     long long stateVarStride, scalStride;
+    double* pbGrid;           // PBMETAD sub-grids: [item base + table * values per table + sub-grid offset]
 };
 
 }  // namespace glued
@@ -286,7 +288,7 @@ struct FusedBiasPlan {
     // is one shared hill list: the chunk is loaded once for them all)
     int walkersPerChunk;
     int nWalkers;            // set at launch
-    int restraints;          // 1: some small bias is a harmonic / linear / wall restraint
+    int extra;               // 1: some small bias is a restraint or PBMetaD (kernels with that code)
 };
 
 constexpr int CHUNK_WB = 4;

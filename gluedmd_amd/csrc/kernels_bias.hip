@@ -158,6 +158,66 @@ __device__ inline double restraint_term(const SmallBiasItem& it, const double* p
     return (flag ? -1.0 : 1.0) * kappa * (dPow + eps * powered) * e;
 }
 
+// PBMetaD, CommonGluedKernels.cpp:3697-3763 (one 1-D grid per CV, combined
+// by log-sum-exp): sub-grid i interpolated linearly at s_i (periodic wrap, or
+// clamped with lo <= N - 2), V_i and dV_i/ds_i; V = -kT (log sum_i
+// e^((Vmin - V_i)/kT) - log N) + Vmin, dV/ds_i = w_i dV_i/ds_i with the
+// softmax weights w_i, which are kept with V_i for the deposit.  Constants
+// per sub-grid: {origin, 1/spacing, points, periodic, sigma, offset}.
+constexpr int PB_CONST = 6;
+
+__device__ inline void pbmetad_eval(const StepView& v, const SmallBiasState& st, const SmallBiasItem& it, int w,
+                                    double* bgL)
+{
+#pragma clang fp contract(off)
+    const double* cv = v.cvVal + (size_t)w * v.V;
+    const double* c = st.stateConst + it.stateOff;
+    double* var = st.stateVar + (size_t)w * st.stateVarStride + it.stateOff;  // V_i, then w_i
+    const int tb = it.p[3] != 0.0 ? 0 : w;
+    const double* grid = st.pbGrid + it.bias + (size_t)tb * it.nState;
+    const int N = it.D;
+    const double kT = it.p[1];
+    double Vi[MAX_BIAS_D], gi[MAX_BIAS_D];
+    for (int i = 0; i < N; i++) {
+        const double* ci = c + PB_CONST * i;
+        const int np = (int)ci[2];
+        double frac = (cv[it.cv[i]] - ci[0]) * ci[1];
+        int lo, hi;
+        if (ci[3] != 0.0) {
+            frac = frac - floor(frac / (double)np) * (double)np;
+            if (frac < 0.0) frac += (double)np;
+            lo = (int)frac;
+            if (lo >= np) lo = np - 1;
+            hi = (lo + 1) % np;
+        } else {
+            if (frac < 0.0) frac = 0.0;
+            if (frac > (double)(np - 1)) frac = (double)(np - 1);
+            lo = (int)frac;
+            if (lo >= np - 1) lo = np - 2;
+            hi = lo + 1;
+        }
+        const double alpha = frac - lo;
+        const double* g = grid + (int)ci[5];
+        const double vlo = g[lo], vhi = g[hi];
+        Vi[i] = vlo * (1.0 - alpha) + vhi * alpha;
+        gi[i] = (vhi - vlo) * ci[1];
+    }
+    double bmin = Vi[0];
+    for (int i = 1; i < N; i++)
+        if (Vi[i] < bmin) bmin = Vi[i];
+    double ene = 0.0;
+    for (int i = 0; i < N; i++) ene += exp((bmin - Vi[i]) / kT);
+    v.benergy[(size_t)w * v.B + it.eslot] = -kT * (log(ene) - log((double)N)) + bmin;
+    double* bg = v.bgrad + (size_t)w * v.G + it.gslot;
+    for (int i = 0; i < N; i++) {
+        const double wi = exp((bmin - Vi[i]) / kT) / ene;
+        bg[i] = wi * gi[i];
+        if (bgL) bgL[it.gslot + i] = wi * gi[i];
+        var[i] = Vi[i];
+        var[N + i] = wi;
+    }
+}
+
 // Every CV of a restraint item: energy slot, gradient slots (and the LDS
 // mirror).
 __device__ inline void restraints_eval(const StepView& v, const SmallBiasState& st, const SmallBiasItem& it, int w,
@@ -175,9 +235,10 @@ __device__ inline void restraints_eval(const StepView& v, const SmallBiasState& 
     v.benergy[(size_t)w * v.B + it.eslot] = V;
 }
 
-// RESTRAINTS: the plan has restraint items (kernels of plans without them are
-// compiled without that code, which would otherwise cost the finalize time).
-template <bool RESTRAINTS = true>
+// EXTRA: the plan has restraint or PBMetaD items (kernels of plans without
+// them are compiled without that code, which would otherwise cost the
+// finalize time).
+template <bool EXTRA = true>
 __device__ void small_bias_eval(const StepView& v, const SmallBiasState& st, const SmallBiasItem& it, int i, int w,
                                 double* bgL = nullptr)
 {
@@ -233,8 +294,11 @@ __device__ void small_bias_eval(const StepView& v, const SmallBiasState& st, con
             bg[0] = g0;
             if (bgL) bgL[it.gslot] = g0;
         }
-    } else if (RESTRAINTS && lane == 0) {  // harmonic, linear, walls
-        restraints_eval(v, st, it, w, bgL);
+    } else if (EXTRA && lane == 0) {  // harmonic, linear, walls, PBMetaD
+        if (it.type == BIAS_PBMETAD)
+            pbmetad_eval(v, st, it, w, bgL);
+        else
+            restraints_eval(v, st, it, w, bgL);
     }
 }
 
@@ -721,7 +785,7 @@ constexpr int FIN_OWNERS = 3 * 64;     // slot-owning threads (waves 0, 2, 3)
 constexpr int FIN_LDS_G = 512;         // bias-gradient slots held in LDS
 constexpr int FIN_LDS_COMB = 1024;     // combine entries held in LDS
 
-template <bool RESTRAINTS>
+template <bool EXTRA>
 __device__ void finalize_walker(const StepView& v, const FusedBiasPlan& fp, int w)
 {
     __shared__ double sU[MAX_LDS_VALUES];
@@ -774,7 +838,7 @@ __device__ void finalize_walker(const StepView& v, const FusedBiasPlan& fp, int 
             }
         }
     } else if (wave == 1) {
-        for (int i = 0; i < fp.nSmall; i++) small_bias_eval<RESTRAINTS>(v, fp.st, fp.small[i], i, w, bgL);
+        for (int i = 0; i < fp.nSmall; i++) small_bias_eval<EXTRA>(v, fp.st, fp.small[i], i, w, bgL);
     }
     __syncthreads();
     if (!ldsComb) {
@@ -820,10 +884,10 @@ __device__ void finalize_walker(const StepView& v, const FusedBiasPlan& fp, int 
     for (int s = tid + FIN_SLOTS * FIN_OWNERS; s < v.S; s += BIAS_BLOCK) scatter_slot(v, sp, w, s, sU, MAX_LDS_VALUES);
 }
 
-template <bool RESTRAINTS>
+template <bool EXTRA>
 __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, FusedBiasPlan fp)
 {
-    finalize_walker<RESTRAINTS>(v, fp, blockIdx.x);
+    finalize_walker<EXTRA>(v, fp, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -837,7 +901,7 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, F
 // ---------------------------------------------------------------------------
 constexpr int TINY_PREFETCH_CHUNKS = 4;  // one hill list of up to 4,096 entries: loads issued before the CVs
 
-template <int KIND, int DD, bool RESTRAINTS = false>
+template <int KIND, int DD, bool EXTRA = false>
 __global__ __launch_bounds__(BIAS_BLOCK) void k_step_tiny(StepView v, const SmallCVItem* __restrict__ cvItems, int nCv,
                                                           FusedBiasPlan fp)
 {
@@ -857,21 +921,21 @@ __global__ __launch_bounds__(BIAS_BLOCK) void k_step_tiny(StepView v, const Smal
             for (int c = 0; c < TINY_PREFETCH_CHUNKS; c++)
                 if (c < fp.totalChunks)
                     hill_chunk_sum<DD>(v, h, w, R[c], sh, fp.part + ((size_t)w * fp.totalChunks + c) * PW);
-            finalize_walker<RESTRAINTS>(v, fp, w);
+            finalize_walker<EXTRA>(v, fp, w);
             return;
         }
     }
     for (int i = threadIdx.x; i < nCv; i += BIAS_BLOCK) small_item(v, cvItems[i], w);
     __syncthreads();
     for (int b = 0; b < fp.totalChunks; b++) chunk_block<KIND, DD>(v, fp, b, w, sh);  // each ends with a barrier
-    finalize_walker<RESTRAINTS>(v, fp, w);
+    finalize_walker<EXTRA>(v, fp, w);
 }
 
 void launch_step_tiny(const StepView& v, const SmallCVItem* cvItems, int nCv, const FusedBiasPlan& fp, int nw,
                       hipStream_t s)
 {
     const dim3 g(nw), b(BIAS_BLOCK);
-    if (fp.restraints) hipLaunchKernelGGL((k_step_tiny<0, 1, true>), g, b, 0, s, v, cvItems, nCv, fp);
+    if (fp.extra) hipLaunchKernelGGL((k_step_tiny<0, 1, true>), g, b, 0, s, v, cvItems, nCv, fp);
     else if (fp.kind == 1 && fp.kindD == 1) hipLaunchKernelGGL((k_step_tiny<1, 1>), g, b, 0, s, v, cvItems, nCv, fp);
     else if (fp.kind == 1 && fp.kindD == 2) hipLaunchKernelGGL((k_step_tiny<1, 2>), g, b, 0, s, v, cvItems, nCv, fp);
     else if (fp.kind == 1 && fp.kindD == 3) hipLaunchKernelGGL((k_step_tiny<1, 3>), g, b, 0, s, v, cvItems, nCv, fp);
@@ -889,7 +953,7 @@ void launch_step_bias_chunks(const StepView& v, const FusedBiasPlan& fp, int nw,
 void launch_step_bias(const StepView& v, const FusedBiasPlan& fp, int nw, hipStream_t s)
 {
     if (fp.totalChunks > 0) launch_chunks(v, fp, nw, s);
-    if (fp.restraints)
+    if (fp.extra)
         hipLaunchKernelGGL(k_step_bias_finalize<true>, dim3(nw), dim3(BIAS_BLOCK), 0, s, v, fp);
     else
         hipLaunchKernelGGL(k_step_bias_finalize<false>, dim3(nw), dim3(BIAS_BLOCK), 0, s, v, fp);
@@ -980,6 +1044,44 @@ void launch_dep_grid(const StepView& v, const GridBiasDev& g, int w0, int nw, hi
 {
     dim3 grid((g.total + DEP_NODES - 1) / DEP_NODES, g.walkerStride ? nw : 1);
     hipLaunchKernelGGL(k_dep_grid, grid, dim3(DEP_BLOCK), 0, s, v, g, w0, nw);
+}
+
+// PBMetaD deposit, CommonGluedKernels.cpp:3766-3795 and :5147-5154: every
+// sub-grid i takes a Gaussian at s_i of height w_i h0 exp(-V_i hif) (w_i,
+// V_i of the last evaluation).  One workgroup per (sub-grid, table), a thread
+// per node; a shared table takes the walkers in walker order, each node
+// adding them in that order.
+__global__ void k_dep_pb(StepView v, SmallBiasState st, SmallBiasItem it, int w0, int nw)
+{
+#pragma clang fp contract(off)
+    const int i = blockIdx.x;
+    const bool shared = it.p[3] != 0.0;
+    const int wFirst = shared ? w0 : w0 + blockIdx.y, wLast = shared ? w0 + nw : wFirst + 1;
+    const int tb = shared ? 0 : wFirst;
+    const double* c = st.stateConst + it.stateOff + PB_CONST * i;
+    const int np = (int)c[2], N = it.D;
+    double* grid = st.pbGrid + it.bias + (size_t)tb * it.nState + (int)c[5];
+    const double per = (double)np / c[1], sigma = c[4];
+    for (int idx = threadIdx.x; idx < np; idx += blockDim.x) {
+        const double pos = c[0] + (double)idx / c[1];
+        double acc = grid[idx];
+        for (int w = wFirst; w < wLast; w++) {
+            const double* var = st.stateVar + (size_t)w * st.stateVarStride + it.stateOff;
+            const double Vi = var[i], wi = var[N + i];
+            const double height = it.p[2] > 0.0 ? wi * it.p[0] * exp(-Vi * it.p[2]) : wi * it.p[0];
+            double dc = pos - v.cvVal[(size_t)w * v.V + it.cv[i]];
+            if (c[3] != 0.0) dc -= round(dc / per) * per;
+            const double e = dc * dc / (2.0 * sigma * sigma);
+            if (e <= 8.0) acc += height * exp(-e);
+        }
+        grid[idx] = acc;
+    }
+}
+
+void launch_dep_pb(const StepView& v, const SmallBiasState& st, const SmallBiasItem& it, int w0, int nw,
+                   hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dep_pb, dim3(it.D, it.p[3] != 0.0 ? 1 : nw), dim3(256), 0, s, v, st, it, w0, nw);
 }
 
 // Hill deposit: centre = current CVs, height = h0 exp(-V hif) (the WT rule of

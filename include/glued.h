@@ -56,6 +56,8 @@ enum {
 enum {
     GLUED_BIAS_HARMONIC = 1,     /* params [k, s0] per CV; int params [periodic] (optional) */
     GLUED_BIAS_METAD = 3,
+    GLUED_BIAS_PBMETAD = 4,      /* params [h0, gamma, kT, (sigma, min, max) per CV];
+                                    int params [pace, (bins, periodic) per CV] */
     GLUED_BIAS_OPES = 5,
     GLUED_BIAS_UPPER_WALL = 8,   /* params [at, kappa, eps, n] per CV */
     GLUED_BIAS_LOWER_WALL = 9,   /* params [at, kappa, eps, n] per CV */

@@ -1129,3 +1129,61 @@ EXPORT double orc_wall_eval(const double* s, int D, const double* p, int lower, 
     }
     return V;
 }
+
+/* ------------------------------------------------------------------------ */
+/* PBMetaD, CommonGluedKernels.cpp:3697-3795.  One 1-D grid per CV;         */
+/* c = {origin, 1/spacing, points, periodic, sigma, offset} per sub-grid,   */
+/* grid = the sub-grids back to back.                                        */
+/* ------------------------------------------------------------------------ */
+EXPORT double orc_pbmetad_eval(const double* s, int N, const double* c, const double* grid, double kT, double* Vi,
+                               double* wi, double* g)
+{
+    double gi[64];
+    for (int i = 0; i < N; i++) {
+        const double* ci = c + 6 * i;
+        const int np = (int)ci[2];
+        double frac = (s[i] - ci[0]) * ci[1];
+        int lo, hi;
+        if (ci[3] != 0.0) {
+            frac = frac - floor(frac / (double)np) * (double)np;
+            if (frac < 0.0) frac += (double)np;
+            lo = (int)frac;
+            if (lo >= np) lo = np - 1;
+            hi = (lo + 1) % np;
+        } else {
+            if (frac < 0.0) frac = 0.0;
+            if (frac > (double)(np - 1)) frac = (double)(np - 1);
+            lo = (int)frac;
+            if (lo >= np - 1) lo = np - 2;
+            hi = lo + 1;
+        }
+        const double alpha = frac - lo;
+        const double* gr = grid + (int)ci[5];
+        Vi[i] = gr[lo] * (1.0 - alpha) + gr[hi] * alpha;
+        gi[i] = (gr[hi] - gr[lo]) * ci[1];
+    }
+    double bmin = Vi[0];
+    for (int i = 1; i < N; i++)
+        if (Vi[i] < bmin) bmin = Vi[i];
+    double ene = 0.0;
+    for (int i = 0; i < N; i++) ene += exp((bmin - Vi[i]) / kT);
+    for (int i = 0; i < N; i++) {
+        wi[i] = exp((bmin - Vi[i]) / kT) / ene;
+        g[i] = wi[i] * gi[i];
+    }
+    return -kT * (log(ene) - log((double)N)) + bmin;
+}
+
+/* One sub-grid deposit: a Gaussian of the given height at `center`.        */
+EXPORT void orc_pbmetad_deposit(double* grid, const double* ci, double center, double height)
+{
+    const int np = (int)ci[2];
+    const double per = (double)np / ci[1], sigma = ci[4];
+    for (int idx = 0; idx < np; idx++) {
+        const double pos = ci[0] + (double)idx / ci[1];
+        double dc = pos - center;
+        if (ci[3] != 0.0) dc -= round(dc / per) * per;
+        const double e = dc * dc / (2.0 * sigma * sigma);
+        if (e <= 8.0) grid[idx] += height * exp(-e);
+    }
+}

@@ -29,7 +29,7 @@ LIB = os.path.join(HERE, "libglued_oracle.so")
 CV_DISTANCE, CV_ANGLE, CV_DIHEDRAL, CV_COM_DISTANCE, CV_GYRATION, CV_COORDINATION, CV_RMSD = 1, 2, 3, 4, 5, 6, 7
 CV_PATH, CV_POSITION, CV_DRMSD, CV_CONTACTMAP, CV_ENERGY = 10, 11, 12, 13, 23
 BIAS_METAD, BIAS_OPES, BIAS_OPES_EXPANDED, BIAS_OPES_MULTITHERMAL, BIAS_METAD_HILLS = 3, 5, 11, 15, 101
-BIAS_HARMONIC, BIAS_UPPER_WALL, BIAS_LOWER_WALL, BIAS_LINEAR = 1, 8, 9, 10
+BIAS_HARMONIC, BIAS_UPPER_WALL, BIAS_LOWER_WALL, BIAS_LINEAR, BIAS_PBMETAD = 1, 8, 9, 10, 4
 FIXED = 4294967296.0
 
 _lib = None
@@ -80,6 +80,8 @@ def lib():
             "orc_harmonic_eval": (d, [dp, i, dp, i, dp]),
             "orc_linear_eval": (d, [dp, i, dp, dp]),
             "orc_wall_eval": (d, [dp, i, dp, i, dp]),
+            "orc_pbmetad_eval": (d, [dp, i, dp, dp, d, dp, dp, dp]),
+            "orc_pbmetad_deposit": (None, [dp, dp, d, d]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -354,6 +356,45 @@ class _Multithermal:
                                       self.kT0)
 
 
+class _PBMetaD:
+    """PBMetaD state: one 1-D grid per CV; the last evaluation's V_i and
+    softmax weights w_i feed the deposit (heights w_i h0 e^(-V_i hif))."""
+
+    def __init__(self, D, params, iparams):
+        self.N, self.h0, gamma, self.kT = D, params[0], params[1], params[2]
+        self.hif = 0.0 if gamma <= 1.0 else 1.0 / ((gamma - 1.0) * self.kT)
+        self.pace = iparams[0]
+        c, off = [], 0
+        for d in range(D):
+            bins, per = iparams[1 + 2 * d], iparams[2 + 2 * d]
+            sigma, lo, hi = params[3 + 3 * d], params[4 + 3 * d], params[5 + 3 * d]
+            np_ = bins + (0 if per else 1)
+            c += [lo, bins / (hi - lo), float(np_), float(per != 0), sigma, float(off)]
+            off += np_
+        self.c = np.array(c, dtype=np.float64)
+        self.grid = np.zeros(off)
+        self.numDeposited = 0
+        self.Vi = np.zeros(D)
+        self.wi = np.full(D, 1.0 / D)
+
+    def eval(self, s):
+        s, sp = P(s)
+        g = np.zeros(self.N)
+        V = lib().orc_pbmetad_eval(sp, self.N, self.c.ctypes.data_as(dp), self.grid.ctypes.data_as(dp), self.kT,
+                                   self.Vi.ctypes.data_as(dp), self.wi.ctypes.data_as(dp), g.ctypes.data_as(dp))
+        return V, g
+
+    def deposit(self, s, Vi=None, wi=None):
+        Vi = self.Vi if Vi is None else Vi
+        wi = self.wi if wi is None else wi
+        for i in range(self.N):
+            off = int(self.c[6 * i + 5])
+            h = wi[i] * self.h0 * math.exp(-Vi[i] * self.hif) if self.hif > 0.0 else wi[i] * self.h0
+            sub = self.grid[off:off + int(self.c[6 * i + 2])]  # a view: deposited in place
+            lib().orc_pbmetad_deposit(sub.ctypes.data_as(dp), self.c[6 * i:].ctypes.data_as(dp), float(s[i]), h)
+        self.numDeposited += 1
+
+
 class _Restraint:
     """Harmonic, linear and wall restraints: no state, no deposits.  The
     parameters are rounded to FP32 as the reference stores them."""
@@ -410,6 +451,8 @@ class OracleContext:
                 self.states.append(_Expanded(D, params, iparams))
             elif t == BIAS_OPES_MULTITHERMAL:
                 self.states.append(_Multithermal(params, iparams))
+            elif t == BIAS_PBMETAD:
+                self.states.append(_PBMetaD(D, params, iparams))
             elif t in (BIAS_HARMONIC, BIAS_LINEAR, BIAS_UPPER_WALL, BIAS_LOWER_WALL):
                 self.states.append(_Restraint(t, D, params, iparams))
             else:
@@ -420,6 +463,7 @@ class OracleContext:
         self.benergy = None
         self.energy_input = None
         self.coord_pairs = []
+        self.pb_last = {}
 
     def set_box(self, box):
         self.box = box9(box)
@@ -576,6 +620,8 @@ class OracleContext:
                 if t == BIAS_OPES and st.adaptiveStride > 0 and welford:
                     st.welford(s)
                 V, g = st.eval(s)
+                if t == BIAS_PBMETAD:
+                    self.pb_last[b] = (st.Vi.copy(), st.wi.copy())
             be[b] = V
             for d, c in enumerate(cvs):
                 dU[c] += g[d]
@@ -624,7 +670,8 @@ class OracleContext:
         if step == 0:
             return
         biases = self.force.biases
-        order = ([BIAS_OPES], [BIAS_OPES_EXPANDED], [BIAS_OPES_MULTITHERMAL], [BIAS_METAD], [BIAS_METAD_HILLS])
+        order = ([BIAS_OPES], [BIAS_OPES_EXPANDED], [BIAS_OPES_MULTITHERMAL], [BIAS_METAD], [BIAS_PBMETAD],
+                 [BIAS_METAD_HILLS])
         for kinds in order:
             for b, (t, cvs, params, iparams) in enumerate(biases):
                 if t not in kinds:
@@ -644,6 +691,8 @@ class OracleContext:
                     st.update(s)
                 elif t == BIAS_OPES_MULTITHERMAL:
                     st.update(s[0], self.benergy[b])
+                elif t == BIAS_PBMETAD:  # this walker's V_i, w_i of its last evaluation
+                    st.deposit(s, *self.pb_last[b])
                 else:
                     st.deposit(s, self.benergy[b])
 

@@ -448,3 +448,62 @@ def test_restraints_match_oracle_and_plain(with_group):
     for w in range(W):
         assert np.array_equal(g.get_forces_fixed(w), Fb[w]), w
         assert np.array_equal(g.getBiasEnergies(w), Eb[w]), w
+
+
+@pytest.mark.parametrize("shared", [False, True])
+def test_pbmetad_deposits_match_oracle(shared):
+    """PBMetaD on a distance and a periodic dihedral, three walkers, a deposit
+    every step (private sub-grids, or one shared set taking the walkers in
+    walker order): energies, dU/dCV and forces follow the oracle every step;
+    the batched step equals the plain launches bit for bit; a bias-state blob
+    restores the sub-grids into a fresh context."""
+    from gluedmd_amd.context import Context
+    from oracle.oracle import OracleContext, _PBMetaD
+    W, N = 3, 16
+    f = Force(pbc=True, temperature=300.0)
+    d = f.add_distance([0, 1])
+    t = f.add_dihedral([2, 3, 4, 5])
+    pb = f.add_pbmetad([d, t], [0.08, 0.3], 1.5, 1, grid_min=[0.0, -math.pi], grid_max=[2.0, math.pi],
+                       bins=[100, 90], periodic=[False, True], gamma=8.0)
+    if shared:
+        f.share_bias(pb)
+    g = Context(f, walkers=[N] * W)
+    g.set_box(2.0)
+    _, cvs, params, ip = f.biases[pb]
+    common = {pb: _PBMetaD(len(cvs), params, ip)} if shared else None
+    os_ = []
+    for w in range(W):
+        o = OracleContext(f, N, shared=common)
+        o.set_box(2.0)
+        os_.append(o)
+    rng = np.random.default_rng(31)
+    base = [rng.uniform(0, 2, size=(N, 3)) for _ in range(W)]
+    for step in range(6):
+        g.update_state(step)
+        for o in os_:
+            o.update_state(step)
+        pos = [(b + rng.normal(0, 0.02, size=b.shape)).astype(np.float32) for b in base]
+        g.set_all_positions(pos)
+        g.zero_forces()
+        g.execute_batch(use_graph=False)
+        for w in range(W):
+            r = os_[w].execute(pos[w])
+            check(g, os_[w], r, g.getLastBias(w), w)
+    assert os_[0].energy_input is None and g.getLastBias(0) > 0.0
+    Fb = [g.get_forces_fixed(w).copy() for w in range(W)]
+    Eb = [g.getBiasEnergies(w).copy() for w in range(W)]
+    g.zero_forces()
+    g.execute_all_plain()
+    for w in range(W):
+        assert np.array_equal(g.get_forces_fixed(w), Fb[w]), w
+        assert np.array_equal(g.getBiasEnergies(w), Eb[w]), w
+    blobs = [g.getBiasState(w) for w in range(W)]
+    g2 = Context(f, walkers=[N] * W)
+    g2.set_box(2.0)
+    for w in range(W):
+        g2.setBiasState(blobs[w], w)
+    g2.set_all_positions(pos)
+    g2.zero_forces()
+    g2.execute_batch(use_graph=False)
+    for w in range(W):
+        assert np.array_equal(g2.getBiasEnergies(w), Eb[w]), w
