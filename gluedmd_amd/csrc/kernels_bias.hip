@@ -1404,16 +1404,22 @@ void launch_opes_welford(const StepView& v, const OpesBiasDev& o, int w0, int nw
 
 // OPES expanded logZ update (:3094-3125) and multithermal dF update
 // (:3174-3203); one thread per (item, walker).
+// One wave per (item, walker).  OPES-expanded: lane 0 (the log-sum-exp over
+// the states in state order).  Multithermal: the states' recursions are
+// independent, lanes over states (every lane forms the walker's scalars
+// from the values read at entry; lane 0 writes them back last).
 __global__ void k_dep_small(StepView v, SmallBiasState st, const SmallBiasItem* __restrict__ items, int n, int w0,
                             int onlyItem)
 {
     const int w = w0 + blockIdx.y;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.x;
+    const int lane = threadIdx.x;
     if (i >= n || (onlyItem >= 0 && i != onlyItem)) return;
     const SmallBiasItem it = items[i];
     const double* cv = v.cvVal + (size_t)w * v.V;
     double* sc = st.scal + (size_t)w * st.scalStride + 4 * i;
     if (it.type == BIAS_OPES_EXPANDED) {
+        if (lane != 0) return;
         const double invKT = it.p[0];
         const double* lw = st.stateConst + it.stateOff;
         double mx = -1e300;
@@ -1436,23 +1442,25 @@ __global__ void k_dep_small(StepView v, SmallBiasState st, const SmallBiasItem* 
         const double U = cv[it.cv[0]];
         const double V = v.benergy[(size_t)w * v.B + it.eslot];
         const double r = sc[2];
-        sc[3] += 1.0;
-        const double cnt = sc[3], logc = log(cnt - 1.0);
+        const double cnt = sc[3] + 1.0, logc = log(cnt - 1.0);
         const double arg = (V - r) / kT0 - logc;
         const double inc = arg > 0.0 ? kT0 * (arg + log1p(exp(-arg))) : kT0 * log1p(exp(arg));
-        for (int l = 0; l < it.nState; l++) {
+        for (int l = lane; l < it.nState; l += 64) {
             double dl = -db[l] * U + (V - r + dF[l]) / kT0 - logc;
             double dec = dl > 0.0 ? kT0 * (dl + log1p(exp(-dl))) : kT0 * log1p(exp(dl));
             dF[l] += inc - dec;
         }
-        sc[2] = r + inc + kT0 * log1p(-1.0 / cnt);
+        if (lane == 0) {
+            sc[3] = cnt;
+            sc[2] = r + inc + kT0 * log1p(-1.0 / cnt);
+        }
     }
 }
 
 void launch_dep_small(const StepView& v, const SmallBiasState& st, const SmallBiasItem* items, int n, int item,
                       int w0, int nw, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dep_small, dim3((n + 63) / 64, nw), dim3(64), 0, s, v, st, items, n, w0, item);
+    hipLaunchKernelGGL(k_dep_small, dim3(n, nw), dim3(64), 0, s, v, st, items, n, w0, item);
 }
 
 }  // namespace glued
