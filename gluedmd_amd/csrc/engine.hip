@@ -187,8 +187,6 @@ void Engine::validate(int minAtoms) const
                 break;
             case BIAS_OPES_EXPANDED:
                 if ((int)s.params.size() < 1 + D) err("OPES_EXPANDED params must be [kT, w*D]");
-                for (int d = 1; d < D; d++)
-                    if (s.cvs[d] != s.cvs[0] + d) err("OPES_EXPANDED CVs must be consecutive value indices");
                 break;
             case BIAS_OPES_MULTITHERMAL: {
                 if ((int)s.params.size() < 2)
@@ -737,6 +735,8 @@ void Engine::buildPlan(int nMass, const double* masses)
                 for (int l = 0; l < D; l++) ws += s.params[1 + l];
                 if (ws <= 0.0) ws = D;
                 for (int l = 0; l < D; l++) smallConst_.push_back(std::log(s.params[1 + l] / ws));
+                // the state's CV value index (any order, CommonBiasKernels.cpp:740-742)
+                for (int l = 0; l < D; l++) smallConst_.push_back((double)s.cvs[l]);
             } else {
                 const int N = (int)s.params.size() - 1;
                 it.D = 1;

@@ -259,17 +259,18 @@ __device__ void small_bias_eval(const StepView& v, const SmallBiasState& st, con
     } else if (it.type == BIAS_OPES_EXPANDED) {
         const double invKT = it.p[0];
         const double* lw = st.stateConst + it.stateOff;
+        const double* ci = lw + it.nState;  // CV value index of every state
         const double logZ = st.scal[(size_t)w * st.scalStride + 4 * i + 0];
         double mx = -1e300;
-        for (int l = lane; l < it.nState; l += 64) mx = fmax(mx, lw[l] - cv[it.cv[0] + l] * invKT);
+        for (int l = lane; l < it.nState; l += 64) mx = fmax(mx, lw[l] - cv[(int)ci[l]] * invKT);
         mx = wave_max1(mx);
         double se = 0.0;
-        for (int l = lane; l < it.nState; l += 64) se += exp(lw[l] - cv[it.cv[0] + l] * invKT - mx);
+        for (int l = lane; l < it.nState; l += 64) se += exp(lw[l] - cv[(int)ci[l]] * invKT - mx);
         se = wave_sum1(se);
         const double logQ = mx + log(se);
         if (lane == 0) *be = -(1.0 / invKT) * (logQ - logZ);
         for (int l = lane; l < it.nState; l += 64) {
-            const double pl = exp(lw[l] - cv[it.cv[0] + l] * invKT - logQ);
+            const double pl = exp(lw[l] - cv[(int)ci[l]] * invKT - logQ);
             bg[l] = pl;
             if (bgL) bgL[it.gslot + l] = pl;
         }
@@ -1422,10 +1423,11 @@ __global__ void k_dep_small(StepView v, SmallBiasState st, const SmallBiasItem* 
         if (lane != 0) return;
         const double invKT = it.p[0];
         const double* lw = st.stateConst + it.stateOff;
+        const double* ci = lw + it.nState;
         double mx = -1e300;
-        for (int l = 0; l < it.nState; l++) mx = fmax(mx, lw[l] - cv[it.cv[0] + l] * invKT);
+        for (int l = 0; l < it.nState; l++) mx = fmax(mx, lw[l] - cv[(int)ci[l]] * invKT);
         double se = 0.0;
-        for (int l = 0; l < it.nState; l++) se += exp(lw[l] - cv[it.cv[0] + l] * invKT - mx);
+        for (int l = 0; l < it.nState; l++) se += exp(lw[l] - cv[(int)ci[l]] * invKT - mx);
         const double logQ = mx + log(se);
         const int k = (int)sc[1];
         if (k == 0) {

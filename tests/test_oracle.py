@@ -630,20 +630,109 @@ def test_multithermal_eval_flat(oracle):
     assert abs(r["dUdCV"][0] - dVdU) < 1e-12
 
 
+def _lse(v):
+    v = np.asarray(v, dtype=np.float64)
+    m = v.max()
+    return float(m + math.log(math.fsum(np.exp(v - m))))
+
+
 def test_multithermal_update_matches_logmeanexp(oracle):
-    """dF recursion against a direct log-mean-exp of the reweighting factors:
-    after k updates, dF_l = -kT0 log( mean_i exp((V_i - rct...)) ) is what the
-    stabilised recursion tracks; checked via its invariant dF_l(beta_l=beta0)=0."""
+    """The stabilised Delta F / rct recursion (CommonGluedKernels.cpp:
+    3174-3203; start rct = 0, counter = 1, Delta F = 0 from
+    CommonBiasKernels.cpp:777-815) telescopes: with x = 1/kT0 and c_k = k + 1,
+    c_k e^{x rct_k} = c_{k-1} e^{x rct_{k-1}} + e^{x V_k} and
+    c_k e^{x (rct_k - dF_l,k)} = c_{k-1} e^{x (rct_{k-1} - dF_l,k-1)} + e^{x V_k - dbeta_l U_k}, so after K
+    updates  rct_K = kT0 log((1 + sum_i e^{x V_i}) / (K + 1))  and
+    dF_l,K = kT0 [log(1 + sum_i e^{x V_i}) - log(1 + sum_i e^{x V_i - dbeta_l U_i})].
+    Both sides are evaluated independently: the recursion by the oracle, the
+    closed form by numpy log-sum-exp over the recorded (U_i, V_i)."""
+    kT0 = 8.314462618e-3 * 300.0
+    temps = [300.0, 330.0, 370.0, 420.0, 500.0]
     f = Force(temperature=300.0)
-    f.add_multithermal(300.0, 500.0, 4, temps=[300.0, 350.0, 420.0, 500.0], temp0=300.0, pace=1)
+    f.add_multithermal(300.0, 500.0, len(temps), temps=temps, temp0=300.0, pace=1)
+    st = oracle.OracleContext(f, 2).states[0]
+    dbeta = np.array([1.0 / (8.314462618e-3 * t) for t in temps]) - 1.0 / kT0
+    assert np.allclose(st.dbeta, dbeta, rtol=1e-15, atol=0)
+    rng = np.random.default_rng(11)
+    K = 40
+    U = -1000.0 + rng.normal(0.0, 25.0, K)
+    V = rng.uniform(-30.0, 10.0, K)
+    for k in range(K):
+        st.update(U[k], V[k])
+        n = k + 1
+        lead = np.concatenate([[0.0], V[:n] / kT0])
+        rct = kT0 * (_lse(lead) - math.log(n + 1))
+        assert abs(st.rct[0] - rct) <= 1e-10 * max(1.0, abs(rct)), (k, st.rct[0], rct)
+        for l in range(len(temps)):
+            tail = np.concatenate([[0.0], V[:n] / kT0 - dbeta[l] * U[:n]])
+            dF = kT0 * (_lse(lead) - _lse(tail))
+            assert abs(st.dF[l] - dF) <= 1e-10 * max(1.0, abs(dF)), (k, l, st.dF[l], dF)
+    assert st.counter[0] == K + 1
+    # a planted sign error in the U expansion moves dF by O(kT0 |dbeta U|)
+    tail_bad = np.concatenate([[0.0], V / kT0 + dbeta[-1] * U])
+    assert abs(kT0 * (_lse(np.concatenate([[0.0], V / kT0])) - _lse(tail_bad)) - st.dF[-1]) > 1.0
+
+
+def test_multithermal_step_sequence_uses_previous_eval(oracle):
+    """Through the step semantics (updateState before execute, the deposit
+    uses the previous evaluation's U and V; CommonGluedKernels.cpp:5132-5135)
+    the state after K updates equals the closed form over the first K
+    evaluations, and each evaluation's V is the eval formula at the Delta F of
+    that moment."""
+    kT0 = 8.314462618e-3 * 300.0
+    temps = [300.0, 350.0, 420.0, 500.0]
+    f = Force(temperature=300.0)
+    f.add_multithermal(300.0, 500.0, len(temps), temps=temps, temp0=300.0, pace=1)
     c = oracle.OracleContext(f, 2)
-    rng = np.random.default_rng(0)
-    for step in range(20):
-        c.set_energy_input(-1000.0 + rng.normal(0, 20.0), np.zeros((2, 3)))
-        c.md_step(np.zeros((2, 3), dtype=np.float32), step)
-    st = c.states[0]
-    assert abs(st.dF[0]) < 1e-9  # the beta0 state never moves
-    assert np.all(np.isfinite(st.dF)) and st.counter[0] == 20.0
+    dbeta = np.array([1.0 / (8.314462618e-3 * t) for t in temps]) - 1.0 / kT0
+    rng = np.random.default_rng(5)
+    Us, Vs = [], []
+    for step in range(25):
+        U = -1200.0 + rng.normal(0, 30.0)
+        c.set_energy_input(U, np.zeros((2, 3)))
+        r = c.md_step(np.zeros((2, 3), dtype=np.float32), step)
+        n = len(Us)
+        lead = np.concatenate([[0.0], np.array(Vs) / kT0])
+        dF = np.array([kT0 * (_lse(lead) - _lse(np.concatenate([[0.0], np.array(Vs) / kT0 - dbeta[l] * np.array(Us)])))
+                       for l in range(len(temps))]) if n else np.zeros(len(temps))
+        diff = -dbeta * U + dF / kT0
+        V = -kT0 * (_lse(diff) - math.log(len(temps)))
+        assert abs(r["energy"] - V) <= 1e-10 * abs(V), (step, r["energy"], V)
+        Us.append(U)
+        Vs.append(r["energy"])
+
+
+def test_opes_expanded_logz_closed_form(oracle):
+    """The running log-mean of Q (CommonGluedKernels.cpp:3092-3125) after K
+    updates is logsumexp(logQ_1..K) - log K, each logQ_i being the
+    evaluation's log-sum-exp over the states, with the ECVs picked through a
+    non-consecutive, reordered CV-index list (the reference reads
+    cvValues[cvIdxList[l]])."""
+    kT = 2.5
+    f = Force()
+    cvs = [f.add_position(a, 0) for a in range(5)]
+    idx = [cvs[3], cvs[0], cvs[4]]
+    w = [1.0, 2.0, 3.0]
+    f.addBias(f.BIAS_OPES_EXPANDED, idx, [kT] + w, [1])
+    c = oracle.OracleContext(f, 5)
+    logW = np.log(np.array(w) / sum(w))
+    rng = np.random.default_rng(2)
+    logQs = []
+    for step in range(30):
+        pos = np.zeros((5, 3), dtype=np.float32)
+        pos[:, 0] = rng.uniform(0.0, 6.0, 5).astype(np.float32)
+        r = c.md_step(pos, step)
+        ecv = pos[[3, 0, 4], 0].astype(np.float64)
+        lq = _lse(logW - ecv / kT)
+        n = len(logQs)
+        logZ = (_lse(logQs) - math.log(n)) if n else 0.0
+        assert abs(c.states[0].logZ[0] - logZ) <= 1e-12 * max(1.0, abs(logZ)), (step, c.states[0].logZ[0], logZ)
+        assert abs(r["energy"] - (-kT * (lq - logZ))) <= 1e-12 * max(1.0, abs(r["energy"]))
+        p = np.exp(logW - ecv / kT - lq)
+        assert np.allclose(r["dUdCV"][[3, 0, 4]], p, rtol=1e-12, atol=0)
+        assert np.all(r["dUdCV"][[1, 2]] == 0.0)
+        logQs.append(lq)
+    assert int(c.states[0].nUpd[0]) == 29
 
 
 def test_opes_merge_variance_forms_agree():
