@@ -223,6 +223,8 @@ void Engine::validate(int minAtoms) const
             default: err("bias type " + std::to_string(s.type) + " is not supported by this implementation");
         }
     }
+    // every bias owns at most one bit of the 64-bit deposit mask
+    if (biases_.size() > 64) fail("GluedForce: at most 64 biases per force");
 }
 
 // ---------------------------------------------------------------------------
@@ -991,28 +993,28 @@ void Engine::setBox(int w, const double* b9)
     if (std::memcmp(b, &boxHost_[(size_t)w * 12], sizeof(b)) == 0) return;
     std::memcpy(&boxHost_[(size_t)w * 12], b, sizeof(b));
     hip_check(hipMemcpy(dBox_.as<double>() + (size_t)w * 12, b, sizeof(b), hipMemcpyHostToDevice), "upload box");
-    // cell lists depend on the box
-    bool ortho = b[1] == 0 && b[2] == 0 && b[3] == 0 && b[5] == 0 && b[6] == 0 && b[7] == 0;
+    // The cell grid of a coordination item is shared by every walker: it is
+    // used only when every walker's box is orthorhombic, and its per-axis
+    // cell count is the smallest over the walkers, so that every walker's
+    // cell edge is at least the cutoff (the pair passes visit the +-1
+    // neighbour cells only).  Recomputed over all boxes on every change.
     bool changed = false;
     for (size_t i = 0; i < coord_.size(); i++) {
         CoordItem& it = coord_[i];
-        int nc[3] = {1, 1, 1};
-        int use = 0;
-        if (pbc && ortho && it.rc2 > 0.0) {
-            const double rc = std::sqrt(it.rc2) * (1.0 + 1e-6);
-            use = 1;
-            for (int k = 0; k < 3; k++) {
-                nc[k] = (int)std::floor(b[4 * k] / rc);
-                if (nc[k] < 3) use = 0;
-                nc[k] = std::min(nc[k], 64);
+        int nc[3] = {64, 64, 64};
+        int use = pbc && it.rc2 > 0.0;
+        const double rc = std::sqrt(it.rc2) * (1.0 + 1e-6);
+        for (int v = 0; v < W_ && use; v++) {
+            const double* bv = &boxHost_[(size_t)v * 12];
+            const bool ortho = bv[1] == 0 && bv[2] == 0 && bv[3] == 0 && bv[5] == 0 && bv[6] == 0 && bv[7] == 0;
+            if (!ortho) use = 0;
+            for (int k = 0; k < 3 && use; k++) {
+                const int n = (int)std::floor(bv[4 * k] / rc);
+                if (n < 3) use = 0;
+                nc[k] = std::min(nc[k], n);
             }
         }
         if (!use) nc[0] = nc[1] = nc[2] = 1;
-        // one box geometry for all walkers: the smallest cell count wins
-        if (w > 0 && it.useCells) {
-            for (int k = 0; k < 3; k++) nc[k] = std::min(nc[k], it.nc[k]);
-            use = use && it.useCells;
-        }
         if (use != it.useCells || nc[0] != it.nc[0] || nc[1] != it.nc[1] || nc[2] != it.nc[2]) changed = true;
         it.useCells = use;
         for (int k = 0; k < 3; k++) it.nc[k] = nc[k];
@@ -1488,9 +1490,9 @@ void Engine::updateState(hipStream_t s, int step)
     if (!cvReady_) return;
     if (step == lastUpdateStep_) return;
     if (depositSites() == 0) return;
+    if (depositSites() > 64) fail("update_state: more than 64 deposit sites");
     lastUpdateStep_ = step;
     const uint64_t m = depositMask(step);
-    if (depositSites() > 64) fail("update_state: more than 64 deposit sites");
     launchDeposits(s, m);
     commitDeposits(m);
 }

@@ -989,18 +989,15 @@ __global__ __launch_bounds__(DEP_BLOCK) void k_dep_grid(StepView v, GridBiasDev 
     const bool priv = g.walkerStride != 0;
     const int wFirst = priv ? w0 + blockIdx.y : w0;
     const int wCount = priv ? 1 : nw;
-    // per node and dimension: position, period and its reciprocal, 1/(2 sigma^2)
-    // (the reference divides by 2 sigma^2 and by the period for every
-    // walker; the reciprocals are formed once here)
-    double pos[3] = {0.0, 0.0, 0.0}, per[3] = {0.0, 0.0, 0.0}, iper[3] = {0.0, 0.0, 0.0}, ii2s2[3], far2[3];
+    // per node and dimension: position, period, 2 sigma^2 (the divisions
+    // dc / period and dc^2 / (2 sigma^2) stay divisions, as in the reference
+    // and the oracle, so that the E <= 8 predicate is theirs bit for bit)
+    double pos[3] = {0.0, 0.0, 0.0}, per[3] = {0.0, 0.0, 0.0}, two_s2[3], far2[3];
     for (int d = 0; d < g.D; d++) {
         const int bi = (min(idx, g.total - 1) / g.strides[d]) % g.npts[d];
         pos[d] = g.origin[d] + (double)bi / g.invSpacing[d];
-        if (g.periodic[d]) {
-            per[d] = (double)g.npts[d] / g.invSpacing[d];
-            iper[d] = 1.0 / per[d];
-        }
-        ii2s2[d] = 1.0 / (2.0 * g.sigma[d] * g.sigma[d]);
+        if (g.periodic[d]) per[d] = (double)g.npts[d] / g.invSpacing[d];
+        two_s2[d] = 2.0 * g.sigma[d] * g.sigma[d];
         far2[d] = 16.0 * g.sigma[d] * g.sigma[d] * (1.0 + 1e-9);
     }
     double add = 0.0;
@@ -1020,11 +1017,11 @@ __global__ __launch_bounds__(DEP_BLOCK) void k_dep_grid(StepView v, GridBiasDev 
             bool near = true;
             for (int d = 0; d < g.D && near; d++) {
                 double dc = pos[d] - sC[t][d];
-                if (g.periodic[d]) dc -= round(dc * iper[d]) * per[d];
+                if (g.periodic[d]) dc -= round(dc / per[d]) * per[d];
                 if (dc * dc > far2[d]) {
                     near = false;
                 } else {
-                    e += dc * dc * ii2s2[d];
+                    e += dc * dc / two_s2[d];
                 }
             }
             if (near && e <= 8.0) add += sH[t] * exp(-e);
