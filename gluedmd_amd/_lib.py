@@ -56,6 +56,7 @@ SIGNATURES = {
     "glued_set_path_lambda": [vp, i, d],
     "glued_set_hill_sigma": [vp, i, c_double_p, i],
     "glued_hill_adjoint": [vp, vp, i, vp, vp, vp],
+    "glued_hill_adjoint_noalloc": [vp, vp, i, vp, vp, vp],
     "glued_set_path_frames": [vp, i, vp, i],
     "glued_path_frames_adjoint": [vp, vp, i, vp, vp],
 }

@@ -1,6 +1,7 @@
 // C-ABI of the GLUED step (declared in include/glued.h).  Converts every C++
 // exception into a status code and a thread-local message.
 #include <cstring>
+#include <mutex>
 #include <string>
 #include "../../include/glued.h"
 #include "engine.hpp"
@@ -12,6 +13,39 @@ struct glued_force {
 };
 
 static thread_local std::string g_err;
+
+// ---- workspace of glued_hill_adjoint_noalloc -------------------------------
+// The per-block sigma partials of the hill adjoint live in this file-scope
+// device array instead of engine-allocated memory, so that the entry point
+// never allocates.  Every GPU the library runs on holds its own copy (512 KiB)
+// until the process ends; its device address is looked up once per device
+// (hipGetSymbolAddress, with that device current) at the handle's
+// initialisation, or at the entry point's first call on a device.  In host
+// code the name `scratch` is only a host-side stand-in: kernels and copies get
+// the looked-up address, never the name.  Every call on one device shares the
+// array: calls that overlap on one GPU race.  Nothing in it needs clearing:
+// each call's first kernel writes every partial its second kernel reads.
+constexpr size_t kScratchDoubles = 65536;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+static double* scratch_for_current_device()
+{
+    int dev = -1;
+    glued::hip_check(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= kMaxDevices) throw glued::Error("hill_adjoint_noalloc: device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            throw glued::Error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    return static_cast<double*>(g_scratch_ptr[dev]);
+}
 
 template <class F>
 static int guard(F&& f)
@@ -104,13 +138,19 @@ int glued_share_bias(glued_handle h, int biasIdx, int shared)
 
 int glued_initialize(glued_handle h, int numParticles, int paddedNumParticles, const double* masses, int nMasses)
 {
-    return guard([&] { E(h).initialize(1, &numParticles, &paddedNumParticles, nMasses, masses); });
+    return guard([&] {
+        E(h).initialize(1, &numParticles, &paddedNumParticles, nMasses, masses);
+        (void)scratch_for_current_device();
+    });
 }
 
 int glued_initialize_batch(glued_handle h, int numWalkers, const int* numParticles, const int* paddedNumParticles,
                            const double* masses, int nMasses)
 {
-    return guard([&] { E(h).initialize(numWalkers, numParticles, paddedNumParticles, nMasses, masses); });
+    return guard([&] {
+        E(h).initialize(numWalkers, numParticles, paddedNumParticles, nMasses, masses);
+        (void)scratch_for_current_device();
+    });
 }
 
 int glued_set_atom_order(glued_handle h, int walker, const int* gpuToUser, int n)
@@ -260,6 +300,21 @@ int glued_hill_adjoint(glued_handle h, void* stream, int biasIdx, const double* 
                        double* dGradSigma)
 {
     return guard([&] { E(h).hillAdjoint((hipStream_t)stream, biasIdx, dGradV, dGradHeights, dGradSigma); });
+}
+
+int glued_hill_adjoint_noalloc(glued_handle h, void* stream, int biasIdx, const double* dGradV,
+                               double* dGradHeights, double* dGradSigma)
+{
+    return guard([&] {
+        Engine& e = E(h);
+        const size_t need = e.hillAdjointPartials(biasIdx);
+        if (need > kScratchDoubles)
+            throw glued::Error("hill_adjoint_noalloc: this bias needs " + std::to_string(need) +
+                               " doubles of workspace, more than the " + std::to_string(kScratchDoubles) +
+                               " of the library's fixed array; use glued_hill_adjoint");
+        e.hillAdjoint((hipStream_t)stream, biasIdx, dGradV, dGradHeights, dGradSigma, scratch_for_current_device(),
+                      kScratchDoubles);
+    });
 }
 
 int glued_set_path_frames(glued_handle h, int pathIdx, const double* frames, int n)

@@ -1640,17 +1640,34 @@ void Engine::setHillSigma(int bi, const double* sigma, int D)
     }
 }
 
-void Engine::hillAdjoint(hipStream_t s, int bi, const double* gV, double* gH, double* gSigma)
+size_t Engine::hillAdjointPartials(int bi) const
 {
+    for (auto* x : hills_)
+        if (x->spec == bi) return (size_t)hill_adjoint_blocks(x->d.cap) * (x->d.shared ? 1 : W_) * x->d.D;
+    fail("hill_adjoint: not a hill bias");
+    return 0;
+}
+
+void Engine::hillAdjoint(hipStream_t s, int bi, const double* gV, double* gH, double* gSigma, double* part,
+                         size_t partCap)
+{
+    if (!initialized_ || !cvReady_) fail("hill_adjoint: evaluate first");
     HillHost* h = nullptr;
     for (auto* x : hills_)
         if (x->spec == bi) h = x;
     if (!h) fail("hill_adjoint: not a hill bias");
-    const size_t need = sizeof(double) * (size_t)hill_adjoint_blocks(h->d.cap) * (h->d.shared ? 1 : W_) * h->d.D;
-    if (dAdjPart_.bytes < need) dAdjPart_.alloc(need);
+    const size_t need = hillAdjointPartials(bi);
+    if (part) {
+        if (need > partCap)
+            fail("hill_adjoint: needs " + std::to_string(need) + " doubles of partials, the buffer holds " +
+                 std::to_string(partCap));
+    } else {
+        if (dAdjPart_.bytes < sizeof(double) * need) dAdjPart_.alloc(sizeof(double) * need);
+        part = dAdjPart_.as<double>();
+    }
     StepView v = view(lastPos_, lastForce_);
     v.S = S_;
-    launch_hill_adjoint(v, h->d, gV, gH, dAdjPart_.as<double>(), gSigma, W_, s);
+    launch_hill_adjoint(v, h->d, gV, gH, part, gSigma, W_, s);
     hip_check(hipGetLastError(), "hill_adjoint launch");
 }
 

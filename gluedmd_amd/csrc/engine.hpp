@@ -109,7 +109,12 @@ class Engine {
     void setHillHeightsDevice(int w, int bias, const double* d, int n, hipStream_t s);
     void setPathLambda(int path, double lambda);
     void setHillSigma(int bias, const double* sigma, int D);
-    void hillAdjoint(hipStream_t s, int bias, const double* gV, double* gH, double* gSigma);
+    // part / partCap: a caller-owned buffer of per-block sigma partials
+    // (doubles); nullptr = the engine's own, allocated on first use.
+    void hillAdjoint(hipStream_t s, int bias, const double* gV, double* gH, double* gSigma, double* part = nullptr,
+                     size_t partCap = 0);
+    // doubles of per-block partials hillAdjoint needs for this bias
+    size_t hillAdjointPartials(int bias) const;
     void setPathFrames(int path, const double* frames, int n);
     void pathFramesAdjoint(hipStream_t s, int path, const double* gV, double* gF);
 

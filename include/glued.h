@@ -192,6 +192,17 @@ int glued_set_hill_sigma(glued_handle h, int biasIdx, const double* sigma, int D
  * dL/dsigma_d (device, [D]) for the CV values of the last evaluation. */
 int glued_hill_adjoint(glued_handle h, void* stream, int biasIdx, const double* dGradV, double* dGradHeights,
                        double* dGradSigma);
+/* glued_hill_adjoint without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc): the
+ * per-block partials live in a fixed 512 KiB device array of the library
+ * (65,536 doubles per GPU, held until the process ends), so this call makes
+ * no hipMalloc / hipFree.  A bias needing more, ceil(cap / 256) * T * D
+ * doubles (T = 1 shared or W private tables), is rejected with an error.
+ * Same arguments and results as glued_hill_adjoint, bit for bit.  Calls
+ * that overlap on one GPU share the array and race. */
+int glued_hill_adjoint_noalloc(glued_handle h, void* stream, int biasIdx, const double* dGradV,
+                               double* dGradHeights, double* dGradSigma);
 /* Reference frames of the pathIdx-th path CV (host, [N][M][3], stored as
  * FP32 like the CV's parameters); n = N*M*3.  No reference counterpart: the
  * reference fixes the frames at creation (CommonGluedKernels.cpp:4011-4024). */
