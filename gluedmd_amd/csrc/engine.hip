@@ -443,14 +443,35 @@ void Engine::buildPlan(int nMass, const double* masses)
                 }
                 it.frameOff = (int)pathFrames.size();
                 it.msdOff = msdStride_;
-                it.msdBlk0 = pathMsdBlocks_;
                 it.rowBlk0 = pathRowBlocks_;
                 msdStride_ += N;
-                pathMsdBlocks_ += (N + PATH_FPB - 1) / PATH_FPB;
                 pathRowBlocks_ += (M + PATH_TILE - 1) / PATH_TILE;
+                // MSD work: tiles of 16 walkers x 16 frames x K slices, about
+                // 512 blocks per path over the batch, at most 16 slices
+                {
+                    const int nChunks = (3 * M + PATH_KC - 1) / PATH_KC;
+                    const int wTiles = (W_ + PATH_TW - 1) / PATH_TW;
+                    it.frameTiles = (N + PATH_TF - 1) / PATH_TF;
+                    int sl = (512 + wTiles * it.frameTiles - 1) / (wTiles * it.frameTiles);
+                    sl = std::max(1, std::min(sl, std::min(nChunks, PATH_MAX_SLICES)));
+                    it.chunksPerSlice = (nChunks + sl - 1) / sl;
+                    it.slices = (nChunks + it.chunksPerSlice - 1) / it.chunksPerSlice;
+                    it.units = it.frameTiles * it.slices;
+                    it.msdBlk0 = pathMsdUnits_;
+                    pathMsdUnits_ += it.units;
+                    it.msdBlkF0 = pathMsdBlocks_;
+                    it.msdBlkF = (int)(((long long)wTiles * it.units + W_ - 1) / W_);
+                    pathMsdBlocks_ += it.msdBlkF;
+                    it.partOff = partStride_;
+                    partStride_ += it.slices * N + it.slices;
+                    it.centreOff = (int)pathCentreHost_.size();
+                    pathCentreHost_.resize(pathCentreHost_.size() + (size_t)3 * M);
+                    pathR2Host_.resize((size_t)msdStride_);
+                }
                 pathMaxN_ = std::max(pathMaxN_, N);
                 for (int k = 0; k < N * M * 3; k++) pathFrames.push_back((float)c.params[2 + k]);
                 path_.push_back(it);
+                centrePath((int)path_.size() - 1, pathFrames.data() + it.frameOff);
                 break;
             }
         }
@@ -535,6 +556,9 @@ void Engine::buildPlan(int nMass, const double* masses)
     dPathAux_.alloc(sizeof(double) * (size_t)W_ * std::max((int)path_.size(), 1) * 2);
     dPathMsd_.alloc(sizeof(double) * (size_t)W_ * std::max(msdStride_, 1));
     dPathFrames_.upload(pathFrames);
+    dPathCentre_.upload(pathCentreHost_);
+    dPathR2_.upload(pathR2Host_);
+    dPathPart_.alloc(sizeof(double) * (size_t)W_ * std::max(partStride_, 1));
     energyU_.assign(W_, 0.0);
     energyF_.assign(W_, nullptr);
     dEnergyU_.alloc(sizeof(double) * W_);
@@ -1088,6 +1112,10 @@ StepView Engine::view(const float4* posq, long long* force) const
     v.nPath = std::max((int)path_.size(), 1);
     v.pathMsd = dPathMsd_.as<double>();
     v.msdStride = std::max(msdStride_, 1);
+    v.pathPart = dPathPart_.as<double>();
+    v.partStride = std::max(partStride_, 1);
+    v.pathCentre = dPathCentre_.as<float>();
+    v.pathR2 = dPathR2_.as<double>();
     v.gStride = (int)groupSlots_.size();
     v.pStride = (int)pathSlots_.size();
     v.S = std::max(S_, 1);
@@ -1173,7 +1201,7 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         launch_cv_cm_chunks(v, cp, w0, nw, s);
         launch_cv_group_big(v, dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupPos_.as<int>(),
                             dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
-        launch_cv_path(v, dPath_.as<PathItem>(), (int)path_.size(), pathMsdBlocks_, pathRowBlocks_, pathMaxN_,
+        launch_cv_path(v, dPath_.as<PathItem>(), (int)path_.size(), pathMsdUnits_, pathRowBlocks_, pathMaxN_,
                        dPathPos_.as<int>(), dPathFrames_.as<float>(), w0, nw, s);
     }
     if (!coord_.empty()) {
@@ -1690,6 +1718,35 @@ void Engine::setPathFrames(int pi, const double* frames, int n)
             for (int k = 0; k < n; k++) c.params[2 + k] = (double)f[k];
     hip_check(hipMemcpy(dPathFrames_.as<float>() + it.frameOff, f.data(), sizeof(float) * n, hipMemcpyHostToDevice),
               "set_path_frames");
+    centrePath(pi, f.data());
+    hip_check(hipMemcpy(dPathCentre_.as<float>() + it.centreOff, pathCentreHost_.data() + it.centreOff,
+                        sizeof(float) * (size_t)3 * it.M, hipMemcpyHostToDevice),
+              "set_path_frames (centre)");
+    hip_check(hipMemcpy(dPathR2_.as<double>() + it.msdOff, pathR2Host_.data() + it.msdOff, sizeof(double) * it.N,
+                        hipMemcpyHostToDevice),
+              "set_path_frames (|r'|^2)");
+}
+
+// Centre of path pi (the mean frame, rounded to FP32) and |r_f - c|^2 in
+// FP64 (r - c is exact in FP64 for FP32 r and c, as in the MSD blocks).
+void Engine::centrePath(int pi, const float* frames)
+{
+    const PathItem& it = path_[pi];
+    const int M = it.M, N = it.N, K = 3 * M;
+    float* c = pathCentreHost_.data() + it.centreOff;
+    for (int k = 0; k < K; k++) {
+        double m = 0.0;
+        for (int f = 0; f < N; f++) m += (double)frames[(size_t)f * K + k];
+        c[k] = (float)(m / N);
+    }
+    for (int f = 0; f < N; f++) {
+        double r2 = 0.0;
+        for (int k = 0; k < K; k++) {
+            const double d = (double)frames[(size_t)f * K + k] - (double)c[k];
+            r2 += d * d;
+        }
+        pathR2Host_[it.msdOff + f] = r2;
+    }
 }
 
 void Engine::pathFramesAdjoint(hipStream_t s, int pi, const double* gV, double* gF)

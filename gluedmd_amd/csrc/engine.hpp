@@ -237,6 +237,12 @@ class Engine {
     DBuf dSegStart_, dSegRow_ /* rowPos */, dRowCv_ /* entryCv */, dCombStart_, dCombSlot_, dTestGrad_, dEnergyU_, dEnergyFPtr_;
     DBuf dHillTable_, dOpesTable_, dFusedPart_, dFusedCounter_, dPathAux_, dAdjPart_, dPathMsd_, dPathCoef_;
     int msdStride_ = 0, pathMsdBlocks_ = 0, pathRowBlocks_ = 0;
+    int pathMsdUnits_ = 0, partStride_ = 0;
+    // per path: centre [3M] (FP32) and |r_f - c|^2 (FP64) of the MSD contraction
+    std::vector<float> pathCentreHost_;
+    std::vector<double> pathR2Host_;
+    DBuf dPathPart_, dPathCentre_, dPathR2_;
+    void centrePath(int pi, const float* frames);
     int pathMaxN_ = 0;  // most frames of any path
     FusedBiasPlan fplan_{};
 

@@ -74,8 +74,12 @@ struct StepView {
     const double* energyU; // [W] value of the CV_ENERGY input
     double* pathAux;       // [W][nPath][2] ds/dlambda, dz/dlambda
     double* pathMsd;       // [W][msdStride] per-frame MSD of every path
+    double* pathPart;      // [W][partStride] MSD partials (cross terms, |x'|^2) per K slice
+    const float* pathCentre;   // per path: the mean frame [3M] (FP32), the centre of the MSD contraction
+    const double* pathR2;  // [msdStride] |r'_f|^2 of the centred frames (FP64)
     int nPath;
     int msdStride;
+    int partStride;
     int gStride;          // entries per walker of the resolved group / path gather tables
     int pStride;
     int S, V, R, G, B;
@@ -143,16 +147,29 @@ struct PathItem {
     int row0;             // rows interleaved s/z per atom
     int aux;              // index of this path in the [W][nPath][2] d(s,z)/d(lambda) buffer
     int msdOff;           // offset of frame 0 in a walker's msd row ([W][msdStride] FP64)
-    int msdBlk0;          // first MSD block of this path (PATH_FPB frames per block)
+    int msdBlk0;          // first unit of this path in the per-kind MSD launch
+    int msdBlkF0;         // first block of this path in the MSD range of the fused CV launch
+    int msdBlkF;          // blocks of this path per walker row of the fused CV launch
     int rowBlk0;          // first row block of this path (PATH_TILE atoms per block)
+    int frameTiles;       // ceil(N / PATH_TF)
+    int slices;           // K slices of the cross term (each PATH_KC * chunksPerSlice wide)
+    int chunksPerSlice;
+    int units;            // frameTiles * slices: MSD blocks per walker tile
+    int partOff;          // offset in a walker's partial row: [slices][N] cross terms, then [slices] |x'|^2
+    int centreOff;        // float offset of this path's centre (3M)
 };
 
 // Path CV work decomposition (shared by the host planner and the kernels).
-constexpr int PATH_FPB = 16;           // frames per MSD block (4 per wave)
+// MSD: the cross term sum_k x'_wk r'_fk of the centred coordinates is a
+// matrix-core contraction (v_mfma_f64_16x16x4f64) over tiles of PATH_TW
+// walkers x PATH_TF frames, K in chunks of PATH_KC values (PATH_KC / 3 atoms).
+constexpr int PATH_TW = 16;            // walkers per MSD tile (MFMA rows)
+constexpr int PATH_TF = 16;            // frames per MSD tile (MFMA columns)
+constexpr int PATH_KC = 96;            // K per chunk: 32 atoms x 3 components
+constexpr int PATH_MAX_SLICES = 16;    // K slices (partials per frame) of one path
 constexpr int PATH_TILE = 64;          // atoms per row block (one per lane)
 constexpr int PATH_PHASES = 8;         // waves per row block, frames f = phase mod 8
 constexpr int PATH_MAX_FRAMES = 4096;  // frames per path (LDS weight table)
-constexpr int PATH_LDS_ATOMS = 1024;   // positions staged in LDS per chunk by an MSD block
 
 }  // namespace glued
 

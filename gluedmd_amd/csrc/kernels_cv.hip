@@ -1351,112 +1351,159 @@ namespace glued {
 // Path CV (s, z), MSD metric, frames 1-indexed, no PBC:
 // CommonGluedKernels.cpp:667-794, in two phases so that a path of many frames
 // and atoms spreads over the whole chip:
-//   MSD blocks  (walker, PATH_FPB frames): the walker's path atoms are staged
-//               in LDS, each wave accumulates 4 frames with lanes striding
-//               over atoms (frame reads coalesce across lanes); msd_f goes to
-//               a [W][msdStride] buffer.
-//   row blocks  (walker, PATH_TILE atoms): every block rebuilds w_f, S and s
-//               from the msd row with the same fixed-order reduction (so all
-//               blocks agree bit for bit); lane = atom, wave = frame phase;
-//               the per-phase partial rows are summed in LDS in phase order.
-// x - r is formed in FP64 (exact for FP32 inputs); sums are FP64.  The row
-// blocks of tile 0 also write s, z and d(s, z)/d(lambda):
+//   MSD blocks  msd_wf = (|x'_w|^2 - 2 x'_w . r'_f + |r'_f|^2) / M on
+//               coordinates centred on the path's mean frame c: x' = x - c
+//               and r' = r - c are formed in FP64 from the FP32 values, where
+//               they are exact; |r'_f|^2 comes from the host.  The cross term
+//               is a contraction over K = 3M on the matrix cores: one block
+//               takes a tile of PATH_TW walkers x PATH_TF frames and one K
+//               slice, stages positions, frames and centre in LDS per chunk
+//               of PATH_KC values (FP32, as stored), and accumulates with
+//               v_mfma_f64_16x16x4f64 (exact products of the FP32-valued
+//               operands, FP64 sums); the four waves' partials are added in
+//               wave order and the slice's partial goes to a [W][slices][N]
+//               buffer (with |x'|^2 per slice).  Every output element has its
+//               own fixed accumulation order, so a walker's partials do not
+//               depend on the other walkers of its tile (the one-walker launch
+//               and the batched one give the same bits).  FP32 matrix cores
+//               were measured first: their rounded sums left s and z ~5e-8
+//               off, which a narrow hill width turns into 1e-4 gradient
+//               errors.
+//   row blocks  (walker, PATH_TILE atoms): every block rebuilds msd_f (slice
+//               partials added in slice order), w_f, S and s with the same
+//               fixed-order reduction (so all blocks agree bit for bit); lane
+//               = atom, wave = frame phase; the per-phase partial rows are
+//               summed in LDS in phase order.  x - r is formed in FP64 (exact
+//               for FP32 inputs); sums are FP64.  Tile 0 also stores msd_f and
+//               writes s, z and d(s, z)/d(lambda):
 //   ds/dlambda = -(1/S) sum_f (f+1-s) msd_f w_f,  dz/dlambda = (<msd>_w - z)/lambda.
 // With S <= 1e-300 (every frame's weight underflows) s = z = 0 as in the
 // reference, and the rows and lambda derivatives are 0 (the reference divides
 // by S there).  No frame cap at 128 (:3901); up to PATH_MAX_FRAMES.
 // ---------------------------------------------------------------------------
 constexpr int PATH_MSD_THREADS = 256;  // threads of an MSD block (k_path_msd, k_cv_fused)
-constexpr int PATH_MSD_UNROLL = 4;     // atoms per lane whose frame loads are issued together
 constexpr int PATH_ROWS_UNROLL = 8;    // frames per lane whose loads are issued together (row blocks)
+constexpr int PATH_LDS_ROW = PATH_KC + 4;  // padded LDS row of an operand tile (floats)
 
+typedef double path_v4d __attribute__((ext_vector_type(4)));
+
+// One MSD work unit: walker tile [wt0, wt0 + nwt) x frame tile x K slice.
 __device__ void path_msd_block(const StepView& v, const PathItem& it, const int* __restrict__ slots,
-                               const float* __restrict__ frames, int fb, int w)
+                               const float* __restrict__ frames, int unit, int wt0, int nwt)
 {
-    __shared__ float shX[3 * PATH_LDS_ATOMS];
-    constexpr int PER = PATH_LDS_ATOMS / PATH_MSD_THREADS;
-    const int M = it.M, N = it.N;
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int f0 = fb * PATH_FPB + wave * 4;
-    const int nf = f0 < N ? min(4, N - f0) : 0;
-    const float* F = frames + it.frameOff;
-    const int* tab = slots + (size_t)w * v.pStride + it.begin;
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int c0 = 0; c0 < M; c0 += PATH_LDS_ATOMS) {
-        const int mc = min(PATH_LDS_ATOMS, M - c0);
+    // operand tiles (walker positions, frames: FP32 as stored) and the
+    // chunk's centre; the tiles are reused for the FP64 reduction
+    __shared__ __align__(16) float sOp[2 * PATH_TW * PATH_LDS_ROW];
+    __shared__ float sC[PATH_KC];
+    float* sX = sOp;
+    float* sR = sOp + PATH_TW * PATH_LDS_ROW;
+    double* sRed = reinterpret_cast<double*>(sOp);  // [4][256] wave partials, then [16][32] |x'|^2
+    static_assert(sizeof(double) * (4 * 256 + PATH_TW * 32) <= sizeof(float) * 2 * PATH_TW * PATH_LDS_ROW,
+                  "the reduction buffers must fit the operand tiles");
+    const int M = it.M, N = it.N, K = 3 * M;
+    const int ft = unit % it.frameTiles, sl = unit / it.frameTiles;
+    const int f0 = ft * PATH_TF;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const float* C = v.pathCentre + it.centreOff;  // centre [K]
+    const float* R = frames + it.frameOff;         // frames [N][K]
+    const int* tab = slots + it.begin;
+    const int nChunks = (K + PATH_KC - 1) / PATH_KC;
+    const int cBegin = sl * it.chunksPerSlice, cEnd = min(nChunks, cBegin + it.chunksPerSlice);
+    path_v4d d = {0.0, 0.0, 0.0, 0.0};
+    double x2[2] = {0.0, 0.0};  // |x'|^2 of this thread's (walker row, atom) entries
+    // staging roles: thread t holds atom (t & 31) of walker rows (t >> 5) and (t >> 5) + 8
+    const int sa = t & 31, sr0 = t >> 5;
+    // MFMA roles: lane l supplies A[i = l % 16][k = l / 16] (walker row i) and
+    // B[k = l / 16][j = l % 16] (frame j) and ends with D[l / 16 + 4 e][l % 16]
+    // (the FP64 instruction's output rows are interleaved by 4)
+    const int mi = lane & 15, mk = lane >> 4;
+    const bool rowOk = mi < nwt, colOk = f0 + mi < N;
+    for (int ch = cBegin; ch < cEnd; ch++) {
+        const int k0 = ch * PATH_KC, kc = min(PATH_KC, K - k0);
+        const int j0 = k0 / 3, na = kc / 3;
         __syncthreads();
-        // staging: the table entries of all of this thread's atoms, then
-        // their positions (two memory round trips per chunk)
-        // (indices clamped into the chunk: every load is issued, none waits
-        // on a branch)
-        int gi[PER];
+        // walkers: table entries, then positions; frames: 16 x kc values,
+        // coalesced along K; the centre (clamped: every load issued, none
+        // waits on a branch)
+        int gi[2];
 #pragma unroll
-        for (int u = 0; u < PER; u++) gi[u] = tab[c0 + min((int)threadIdx.x + u * PATH_MSD_THREADS, mc - 1)];
-        float4 q[PER];
+        for (int u = 0; u < 2; u++) {
+            const int r = min(sr0 + 8 * u, nwt - 1);
+            gi[u] = tab[(size_t)(wt0 + r) * v.pStride + j0 + min(sa, na - 1)];
+        }
+        float4 q[2];
 #pragma unroll
-        for (int u = 0; u < PER; u++) q[u] = v.posq[gi[u]];
+        for (int u = 0; u < 2; u++) q[u] = v.posq[gi[u]];
+        float rv[PATH_TF * PATH_KC / PATH_MSD_THREADS];
 #pragma unroll
-        for (int u = 0; u < PER; u++) {
-            const int j = threadIdx.x + u * PATH_MSD_THREADS;
-            if (j < mc) {
-                shX[3 * j] = q[u].x;
-                shX[3 * j + 1] = q[u].y;
-                shX[3 * j + 2] = q[u].z;
+        for (int u = 0; u < PATH_TF * PATH_KC / PATH_MSD_THREADS; u++) {
+            const int e = t + u * PATH_MSD_THREADS;
+            const int fr = e / PATH_KC, k = e % PATH_KC;
+            rv[u] = R[(size_t)min(f0 + fr, N - 1) * K + k0 + min(k, kc - 1)];
+        }
+        const float cv = C[k0 + min(t, kc - 1)];
+        const float* cj = C + 3 * (j0 + min(sa, na - 1));
+        const double c0 = cj[0], c1 = cj[1], c2 = cj[2];
+#pragma unroll
+        for (int u = 0; u < 2; u++) {
+            const int r = sr0 + 8 * u;
+            float* dst = sX + r * PATH_LDS_ROW + 3 * sa;
+            dst[0] = q[u].x;
+            dst[1] = q[u].y;
+            dst[2] = q[u].z;
+            if (r < nwt && sa < na) {
+                const double xa = (double)q[u].x - c0, ya = (double)q[u].y - c1, za = (double)q[u].z - c2;
+                x2[u] += xa * xa + ya * ya + za * za;
             }
         }
+#pragma unroll
+        for (int u = 0; u < PATH_TF * PATH_KC / PATH_MSD_THREADS; u++) {
+            const int e = t + u * PATH_MSD_THREADS;
+            sR[(e / PATH_KC) * PATH_LDS_ROW + e % PATH_KC] = rv[u];
+        }
+        if (t < PATH_KC) sC[t] = cv;
         __syncthreads();
-        if (nf == 0) continue;
-        // lanes stride over atoms (frame reads coalesce across lanes); the
-        // frame values of PATH_MSD_UNROLL atoms are loaded before any is used,
-        // and each lane still adds its atoms in increasing order.  Frame and
-        // atom indices are clamped into range so that every load is issued
-        // unconditionally (a load under a lane condition is waited for at
-        // once); the clamped values are never added.
-        for (int jb = lane; jb < mc; jb += 64 * PATH_MSD_UNROLL) {
-            float r[PATH_MSD_UNROLL][4][3];
+        // wave `wave` takes K steps [wave * STEPS, wave * STEPS + STEPS) of
+        // the chunk; entries of invalid rows, frames or K are zero operands
+        constexpr int STEPS = PATH_KC / 4 / 4;
 #pragma unroll
-            for (int u = 0; u < PATH_MSD_UNROLL; u++) {
-                const int j = min(jb + 64 * u, mc - 1);
-#pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    const float* rp = F + ((size_t)(f0 + min(k, nf - 1)) * M + c0 + j) * 3;
-                    r[u][k][0] = rp[0];
-                    r[u][k][1] = rp[1];
-                    r[u][k][2] = rp[2];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < PATH_MSD_UNROLL; u++) {
-                const int j = jb + 64 * u;
-                if (j < mc) {
-                    const double x = shX[3 * j], y = shX[3 * j + 1], z = shX[3 * j + 2];
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        if (k < nf) {
-                            const double dx = x - (double)r[u][k][0], dy = y - (double)r[u][k][1],
-                                         dz = z - (double)r[u][k][2];
-                            acc[k] += dx * dx + dy * dy + dz * dz;
-                        }
-                    }
-                }
-            }
+        for (int st = 0; st < STEPS; st++) {
+            const int k = (wave * STEPS + st) * 4 + mk;
+            const double ck = sC[k];
+            const bool kOk = k < kc;
+            const double a = (rowOk && kOk) ? (double)sX[mi * PATH_LDS_ROW + k] - ck : 0.0;
+            const double b = (colOk && kOk) ? (double)sR[mi * PATH_LDS_ROW + k] - ck : 0.0;
+            d = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, d, 0, 0, 0);
         }
     }
-    if (nf == 0) return;
-    wave_sum_n<4>(acc);
-    if (lane < nf) {
-        const double a = lane == 0 ? acc[0] : lane == 1 ? acc[1] : lane == 2 ? acc[2] : acc[3];
-        v.pathMsd[(size_t)w * v.msdStride + it.msdOff + f0 + lane] = a / M;
+    __syncthreads();  // operand tiles free: reuse as the reduction buffer
+#pragma unroll
+    for (int e = 0; e < 4; e++) sRed[wave * 256 + ((lane >> 4) + 4 * e) * 16 + (lane & 15)] = d[e];
+#pragma unroll
+    for (int u = 0; u < 2; u++) sRed[4 * 256 + (sr0 + 8 * u) * 32 + sa] = x2[u];
+    __syncthreads();
+    double* part = v.pathPart + it.partOff;
+    {
+        const int e = t, wi = e / 16, fj = e % 16;
+        const double cr = sRed[e] + sRed[256 + e] + sRed[512 + e] + sRed[768 + e];
+        if (wi < nwt && f0 + fj < N) part[(size_t)(wt0 + wi) * v.partStride + sl * N + f0 + fj] = cr;
+    }
+    if (ft == 0 && t < nwt) {
+        double a = 0.0;
+        for (int k = 0; k < 32; k++) a += sRed[4 * 256 + t * 32 + k];
+        part[(size_t)(wt0 + t) * v.partStride + it.slices * N + sl] = a;
     }
 }
 
+// Per-kind MSD launch: grid (units of every path, walker tiles of the launch).
 __global__ __launch_bounds__(PATH_MSD_THREADS) void k_path_msd(StepView v, const PathItem* __restrict__ items, int nItems,
                                                   const int* __restrict__ slots, const float* __restrict__ frames,
-                                                  int w0)
+                                                  int w0, int nw)
 {
     int p = 0;
     while (p + 1 < nItems && (int)blockIdx.x >= items[p + 1].msdBlk0) p++;
-    path_msd_block(v, items[p], slots, frames, blockIdx.x - items[p].msdBlk0, w0 + blockIdx.y);
+    const int wt0 = w0 + blockIdx.y * PATH_TW;
+    path_msd_block(v, items[p], slots, frames, blockIdx.x - items[p].msdBlk0, wt0, min(PATH_TW, w0 + nw - wt0));
 }
 
 template <int PH>
@@ -1474,7 +1521,9 @@ __global__ __launch_bounds__(64 * PH) void k_path_rows(StepView v, const PathIte
     const int w = w0 + blockIdx.y;
     const int M = it.M, N = it.N;
     const double lam = it.lambda;
-    const double* msd = v.pathMsd + (size_t)w * v.msdStride + it.msdOff;
+    double* msd = v.pathMsd + (size_t)w * v.msdStride + it.msdOff;
+    const double* part = v.pathPart + (size_t)w * v.partStride + it.partOff;
+    const double* r2 = v.pathR2 + it.msdOff;
     const int lane = threadIdx.x & 63, ph = threadIdx.x >> 6;
     const int j = tile * PATH_TILE + lane;
     // this lane's position and row indices, loaded before the weight table
@@ -1482,9 +1531,15 @@ __global__ __launch_bounds__(64 * PH) void k_path_rows(StepView v, const PathIte
     const int jc = min(j, M - 1);
     const float4 q = gather_pos(v, slots, v.pStride, w, it.begin + jc);
     const size_t rowS = row_index(v, w, it.row0 + 2 * jc), rowZ = row_index(v, w, it.row0 + 2 * jc + 1);
+    double x2 = 0.0;
+    for (int sl = 0; sl < it.slices; sl++) x2 += part[it.slices * N + sl];
     double t[2] = {0.0, 0.0};
     for (int f = threadIdx.x; f < N; f += blockDim.x) {
-        const double wv = exp(-lam * msd[f]);
+        double cr = 0.0;
+        for (int sl = 0; sl < it.slices; sl++) cr += part[sl * N + f];
+        const double m = (x2 - 2.0 * cr + r2[f]) / M;
+        if (tile == 0) msd[f] = m;  // read back below by this thread, and by the frame adjoint
+        const double wv = exp(-lam * m);
         shW[f] = wv;
         t[0] += wv;
         t[1] += (f + 1) * wv;
@@ -1656,7 +1711,8 @@ void launch_cv_path(const StepView& v, const PathItem* items, int n, int nMsdBlo
                     const int* slots, const float* frames, int w0, int nw, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_path_msd, dim3(nMsdBlocks, nw), dim3(PATH_MSD_THREADS), 0, s, v, items, n, slots, frames, w0);
+    hipLaunchKernelGGL(k_path_msd, dim3(nMsdBlocks, (nw + PATH_TW - 1) / PATH_TW), dim3(PATH_MSD_THREADS), 0, s, v,
+                       items, n, slots, frames, w0, nw);
     launch_path_rows(v, items, n, nRowBlocks, maxN, slots, frames, w0, nw, s);
 }
 
@@ -1748,10 +1804,17 @@ __global__ __launch_bounds__(GROUP_BLOCK) void k_cv_fused(StepView v, CVFusedPla
     if (b < p.nGroup) {
         group_item<GROUP_BLOCK, GROUP_REG>(v, p.group[b], p.gslots, p.gmass, p.gref, w);
     } else if (b < p.nGroup + p.nPathMsdBlocks) {
+        // path q's (walker tile, unit) items spread over its msdBlkF blocks of
+        // every walker row: item = local block * W + walker row
         const int lb = b - p.nGroup;
         int q = 0;
-        while (q + 1 < p.nPath && lb >= p.path[q + 1].msdBlk0) q++;
-        path_msd_block(v, p.path[q], p.pslots, p.frames, lb - p.path[q].msdBlk0, w);
+        while (q + 1 < p.nPath && lb >= p.path[q + 1].msdBlkF0) q++;
+        const PathItem& it = p.path[q];
+        const int W = gridDim.y;
+        const int item = (lb - it.msdBlkF0) * W + w;
+        const int tile = item / it.units;
+        if (tile * PATH_TW < W)
+            path_msd_block(v, it, p.pslots, p.frames, item % it.units, tile * PATH_TW, min(PATH_TW, W - tile * PATH_TW));
     } else if (b < p.nGroup + p.nPathMsdBlocks + p.nCmBlocks) {
         cm_chunk_block(v, p, b - p.nGroup - p.nPathMsdBlocks, w);
     } else {
