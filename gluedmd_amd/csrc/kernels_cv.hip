@@ -1531,13 +1531,29 @@ __global__ __launch_bounds__(64 * PH) void k_path_rows(StepView v, const PathIte
     const int jc = min(j, M - 1);
     const float4 q = gather_pos(v, slots, v.pStride, w, it.begin + jc);
     const size_t rowS = row_index(v, w, it.row0 + 2 * jc), rowZ = row_index(v, w, it.row0 + 2 * jc + 1);
+    // the slice partials: all loads issued together (indices clamped into
+    // the slice count), then added in slice order
+    const int nsl = it.slices;
     double x2 = 0.0;
-    for (int sl = 0; sl < it.slices; sl++) x2 += part[it.slices * N + sl];
+    {
+        double pv[PATH_MAX_SLICES];
+#pragma unroll
+        for (int sl = 0; sl < PATH_MAX_SLICES; sl++) pv[sl] = part[nsl * N + min(sl, nsl - 1)];
+#pragma unroll
+        for (int sl = 0; sl < PATH_MAX_SLICES; sl++)
+            if (sl < nsl) x2 += pv[sl];
+    }
     double t[2] = {0.0, 0.0};
     for (int f = threadIdx.x; f < N; f += blockDim.x) {
+        double pv[PATH_MAX_SLICES];
+#pragma unroll
+        for (int sl = 0; sl < PATH_MAX_SLICES; sl++) pv[sl] = part[min(sl, nsl - 1) * N + f];
+        const double r2f = r2[f];
         double cr = 0.0;
-        for (int sl = 0; sl < it.slices; sl++) cr += part[sl * N + f];
-        const double m = (x2 - 2.0 * cr + r2[f]) / M;
+#pragma unroll
+        for (int sl = 0; sl < PATH_MAX_SLICES; sl++)
+            if (sl < nsl) cr += pv[sl];
+        const double m = (x2 - 2.0 * cr + r2f) / M;
         if (tile == 0) msd[f] = m;  // read back below by this thread, and by the frame adjoint
         const double wv = exp(-lam * m);
         shW[f] = wv;
