@@ -421,6 +421,8 @@ void Engine::buildPlan(int nMass, const double* masses)
                     it.preRes = rn / rm;
                     it.preDfunc = 0.5 * rn * (rn - rm) / rm;
                     it.preSecDev = rn * (rm * rm - 3.0 * rm * (rn - 1.0) + rn * (2.0 * rn - 3.0)) / (6.0 * rm);
+                    it.ir0sq = it.ir0 * it.ir0;
+                    it.rat612 = rn == 6.0 && rm == 12.0;
                 }
                 coord_.push_back(it);
                 break;
@@ -589,6 +591,7 @@ void Engine::buildPlan(int nMass, const double* masses)
         dCnAccLo_.alloc(sizeof(unsigned long long) * nci);
         dPairAcc_.alloc(sizeof(unsigned long long) * nci);
         dNpairs_.alloc(sizeof(long long) * nci);
+        dCoordTicket_.alloc(sizeof(unsigned int) * nci);
         dCellCount_.alloc(sizeof(int) * (size_t)W_ * cellStride_);
         dCellStart_.alloc(sizeof(int) * (size_t)W_ * (cellStride_ + coord_.size() + 1));
         dCoord_.upload(coord_);
@@ -1219,6 +1222,7 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         cb.cnAccLo = dCnAccLo_.as<unsigned long long>();
         cb.pairAcc = dPairAcc_.as<unsigned long long>();
         cb.npairs = dNpairs_.as<long long>();
+        cb.ticket = dCoordTicket_.as<unsigned int>();
         cb.cellOfStride = cellOfStride_;
         cb.cellStride = cellStride_;
         cb.sortStride = sortStride_;

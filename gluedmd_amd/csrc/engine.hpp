@@ -230,7 +230,7 @@ class Engine {
     DBuf dSmall_, dGroup_, dGroupBig_, dGroupPos_, dGroupMass_, dGroupRef_;
     DBuf dCm_, dCmPart_, dCmCount_;
     DBuf dCoord_, dCoordSlots_, dCellOf_, dCellCount_, dCellStart_, dSorted_, dUnsorted_, dCnAcc_, dCnAccLo_, dPairAcc_,
-        dNpairs_, dRank_, dBlockCnt_;
+        dNpairs_, dRank_, dBlockCnt_, dCoordTicket_;
     int cellOfStride_ = 0, cellStride_ = 0, sortStride_ = 0, blkStride_ = 0;
     DBuf dPath_, dPathPos_, dPathFrames_, dPathW_;
     DBuf dSmallBias_, dGridTable_, dSmallConst_, dSmallVar_, dSmallScal_;

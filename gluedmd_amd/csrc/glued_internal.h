@@ -128,6 +128,8 @@ struct CoordItem {
     // switching constants formed once on the host (FP64, unfused):
     // 1/r0 and the Taylor coefficients used for |r/r0 - 1| < 1e-4
     double ir0, preRes, preDfunc, preSecDev;
+    double ir0sq;         // 1/r0^2
+    int rat612;           // n = 6, m = 12: the switch is exactly 1/(1 + x^6)
     int value;
     int rowA, rowB;
     int useCells;         // cell list or brute force
@@ -258,6 +260,7 @@ struct CoordBuffers {
     unsigned long long* cnAccLo;    // [W][items] CN: low part, units of 2^-64
     unsigned long long* pairAcc;    // [W][items] pairs inside the cutoff (same life cycle)
     long long* npairs;              // [W][items]
+    unsigned int* ticket;           // [W][items] group-A cell workgroups done with their CN sums
     int cellOfStride, cellStride, sortStride, blkStride, nItems;
 };
 

@@ -682,7 +682,7 @@ void launch_cv_group_big(const StepView& v, const GroupCVItem* items, int n, con
 //
 // Cell list per group (cells of edge >= cutoff, at least 3 per dimension,
 // orthorhombic box), built by a counting sort of both groups into cell order
-// (k_coord_rank / k_coord_offsets / k_coord_place; the zero / bin / scan /
+// (k_coord_rank / k_coord_place; the zero / bin / scan /
 // fill kernels are the fallback for cell grids too large for LDS).
 // Without a usable cell list the partner group is scanned in chunks.
 //
@@ -783,13 +783,12 @@ __global__ void k_coord_fill(StepView v, CoordBuffers cb, const CoordItem* items
 constexpr int CELL_BLOCK = 1024;
 constexpr int CELL_LDS = 8192;  // cells (both groups) counted in LDS by the sort
 
-// Three-kernel counting sort (no global atomics, no single-CU scatter):
+// Two-kernel counting sort (no global atomics, no single-CU scatter):
 //   rank    one 1,024-thread workgroup per 1,024 atoms: gather, cell, rank of
 //           the atom among the workgroup's atoms of that cell (LDS counter),
 //           and the workgroup's count per cell (cell-major table);
-//   offsets one workgroup per walker: exclusive scan of the table (cell-major,
-//           workgroup-minor) -> where each workgroup's atoms of a cell start;
-//   place   every atom to its slot.
+//   place   every workgroup scans the table (cell-major, workgroup-minor) for
+//           where its atoms of each cell start, and places its atoms.
 int coord_blocks(int n) { return (n + CELL_BLOCK - 1) / CELL_BLOCK; }
 int coord_sort_fits(int ncellBoth, int n) { return ncellBoth <= CELL_LDS && ncellBoth * coord_blocks(n) <= 65536; }
 
@@ -818,53 +817,69 @@ __global__ __launch_bounds__(CELL_BLOCK) void k_coord_rank(StepView v, CoordBuff
     for (int c = tid; c < n2; c += CELL_BLOCK) tab[(size_t)c * nb + b] = lcnt[c];
 }
 
-__global__ __launch_bounds__(CELL_BLOCK) void k_coord_offsets(CoordBuffers cb, const CoordItem* items, int item, int nb,
-                                                              int w0)
-{
-    __shared__ int waveTot[CELL_BLOCK / 64];
-    const int w = w0 + blockIdx.y, tid = threadIdx.x;
-    const CoordItem it = items[item];
-    const int ncell = it.nc[0] * it.nc[1] * it.nc[2], n2 = 2 * ncell, len = n2 * nb;
-    int* tab = cb.blockCnt + (size_t)w * cb.blkStride + it.blkBase;
-    int* st = cb.cellStart + (size_t)w * (cb.cellStride + cb.nItems) + it.cellBase + item;
-    const int per = (len + CELL_BLOCK - 1) / CELL_BLOCK;
-    const int b0 = tid * per;
-    int tsum = 0;
-    for (int k = 0; k < per; k++)
-        if (b0 + k < len) tsum += tab[b0 + k];
-    const int lane = tid & 63, wid = tid >> 6;
-    int incl = tsum;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int t = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += t;
-    }
-    if (lane == 63) waveTot[wid] = incl;
-    __syncthreads();
-    int run = incl - tsum;
-    for (int q = 0; q < wid; q++) run += waveTot[q];
-    for (int k = 0; k < per; k++) {
-        const int e = b0 + k;
-        if (e >= len) break;
-        const int x = tab[e];
-        tab[e] = run;
-        if (e % nb == 0) st[e / nb] = run;  // start of cell e / nb
-        run += x;
-    }
-    if (tid == CELL_BLOCK - 1) st[n2] = run;
-}
+// Placement with the scan folded in: every placing workgroup scans the whole
+// (cell, workgroup) count table itself (cell-major, workgroup-minor; a few
+// thousand integers, all loads issued before the sums) and keeps the start of
+// its own atoms of every cell in LDS; workgroup 0 also writes the cell starts.
+// This replaces a one-workgroup scan launch between rank and place.
+constexpr int CELL_SCAN_PER = 16;  // table entries per thread held in registers by the scan
 
 __global__ __launch_bounds__(CELL_BLOCK) void k_coord_place(CoordBuffers cb, const CoordItem* items, int item, int w0)
 {
-    const int w = w0 + blockIdx.y, b = blockIdx.x, nb = gridDim.x;
+    __shared__ int sOff[CELL_LDS];
+    __shared__ int waveTot[CELL_BLOCK / 64];
+    const int w = w0 + blockIdx.y, b = blockIdx.x, nb = gridDim.x, tid = threadIdx.x;
     const CoordItem it = items[item];
     const int n = it.nA + it.nB;
-    const int i = b * CELL_BLOCK + threadIdx.x;
-    if (i >= n) return;
-    const size_t o = (size_t)w * cb.sortStride + it.sortBase + i;
-    const int c = cb.cellOf[(size_t)w * cb.cellOfStride + it.sortBase + i];
-    const int pos = cb.blockCnt[(size_t)w * cb.blkStride + it.blkBase + (size_t)c * nb + b] + cb.rank[o];
-    cb.sorted[(size_t)w * cb.sortStride + it.sortBase + pos] = cb.unsorted[o];
+    const int ncell = it.nc[0] * it.nc[1] * it.nc[2], n2 = 2 * ncell, len = n2 * nb;
+    const int* tab = cb.blockCnt + (size_t)w * cb.blkStride + it.blkBase;
+    int* st = cb.cellStart + (size_t)w * (cb.cellStride + cb.nItems) + it.cellBase + item;
+    // this atom's rank and cell, loaded before the scan (their round trip overlaps it)
+    const int i = b * CELL_BLOCK + tid;
+    const size_t o = (size_t)w * cb.sortStride + it.sortBase + min(i, n - 1);
+    const int cAtom = cb.cellOf[(size_t)w * cb.cellOfStride + it.sortBase + min(i, n - 1)];
+    const int rAtom = cb.rank[o];
+    const float4 qAtom = cb.unsorted[o];
+    // exclusive scan of the table, in rounds of CELL_BLOCK * CELL_SCAN_PER entries
+    int carry = 0;
+    for (int r0 = 0; r0 < len; r0 += CELL_BLOCK * CELL_SCAN_PER) {
+        const int e0 = r0 + tid * CELL_SCAN_PER;
+        int x[CELL_SCAN_PER];
+#pragma unroll
+        for (int k = 0; k < CELL_SCAN_PER; k++) x[k] = tab[min(e0 + k, len - 1)];
+        int tsum = 0;
+#pragma unroll
+        for (int k = 0; k < CELL_SCAN_PER; k++)
+            if (e0 + k < len) tsum += x[k];
+        const int lane = tid & 63, wid = tid >> 6;
+        int incl = tsum;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += t;
+        }
+        __syncthreads();  // waveTot of the previous round consumed
+        if (lane == 63) waveTot[wid] = incl;
+        __syncthreads();
+        int run = carry + incl - tsum;
+        for (int q = 0; q < wid; q++) run += waveTot[q];
+        int roundTot = 0;
+        for (int q = 0; q < CELL_BLOCK / 64; q++) roundTot += waveTot[q];
+#pragma unroll
+        for (int k = 0; k < CELL_SCAN_PER; k++) {
+            const int e = e0 + k;
+            if (e < len) {
+                const int c = e / nb, bb = e - c * nb;
+                if (bb == b) sOff[c] = run;
+                if (b == 0 && bb == 0) st[c] = run;  // start of cell c
+                run += x[k];
+            }
+        }
+        carry += roundTot;
+    }
+    if (b == 0 && tid == 0) st[n2] = carry;
+    __syncthreads();
+    if (i < n) cb.sorted[(size_t)w * cb.sortStride + it.sortBase + sOff[cAtom] + rAtom] = qAtom;
 }
 
 struct SwitchD {
@@ -917,6 +932,19 @@ __device__ inline void switch_eval(const SwitchD& s, double r, double& f, double
         const double dfx = (s.rm * xm * ix - s.rn * xn * ix + (s.rn - s.rm) * xn * xm * ix) / (den * den);
         dfdr = dfx * s.ir0;
     }
+}
+
+// n = 6, m = 12: (1 - x^6) / (1 - x^12) = 1 / (1 + x^6) for every x (no
+// cancellation, no Taylor branch), so from x^2 = r^2 / r0^2 alone:
+//   f = 1 / (1 + x^6),  (df/dr) / r = -6 x^4 f^2 / r0^2.
+// The reference's formula with its Taylor expansion for |x - 1| < 1e-4
+// (CommonGluedKernels.cpp:979-1007) agrees with it to ~1e-12 per pair.
+__device__ inline void switch_612(double r2, double ir0sq, double& f, double& s)
+{
+    const double x2 = r2 * ir0sq;
+    const double x4 = x2 * x2;
+    f = 1.0 / (1.0 + x4 * x2);
+    s = -6.0 * ir0sq * x4 * (f * f);
 }
 
 constexpr int COORD_BLOCK = 256;  // 4 waves
@@ -1312,6 +1340,292 @@ __global__ __launch_bounds__(COORD_A_BLOCK) void k_coord_pairs_a(StepView v, Coo
     }
 }
 
+// Cell-blocked pair pass (cell list): one 512-thread workgroup per home
+// cell of either group -- workgroups [0, ncell) take the group-A atoms of a
+// cell and form CN and the A rows, workgroups [ncell, 2 ncell) the group-B
+// atoms and the B rows -- so each pass visits every in-cutoff pair once, and
+// both passes are one launch.  A workgroup reads the cell starts of its home
+// cell and its 27 neighbour cells of the other group, then stages the homes
+// and the neighbours' atoms in LDS with coalesced loads (one round trip;
+// more than COORD_PCAP partners or COORD_HCAP homes are taken in chunks).
+// A task is one group-A home (or one of nSub interleaved ranges of its
+// partner blocks when the cell has few homes), or two group-B homes, one per
+// half-wave; lanes pre-check their partners in FP32 (1e-5 relative margin),
+// candidates are queued in LDS and evaluated 64 at a time on the exact FP64
+// predicate with FP64 switching.  Each lane sums its row terms of a task in
+// FP64 in its (fixed) queue order; a DPP wave sum (fixed order) gives the
+// task's row, converted once to 2^40 fixed point, and the tasks' integers
+// add exactly in the home's LDS accumulator, so rows do not depend on the
+// order of the waves.  Each lane's CN terms are summed in FP64 likewise and
+// converted once to 128-bit fixed point (units of 2^-64); CN and the pair
+// count are summed per workgroup and added with 64-bit atomics; the last
+// group-A workgroup of a walker to finish (ticket after a fence) converts
+// CN, stores it and the pair count, and clears the accumulators.
+constexpr int COORD_CB = 512;
+constexpr int COORD_CB_WAVES = COORD_CB / 64;
+constexpr int COORD_PCAP = 3072;  // partners staged in LDS per chunk
+constexpr int COORD_HCAP = 256;   // homes staged in LDS per chunk
+
+struct CoordCellShared {
+    float4 part[COORD_PCAP];
+    float4 home[COORD_HCAP];
+    long long g[COORD_HCAP][3];
+    short q[COORD_CB_WAVES][128];
+    int cellB[27], pre[28];
+    int homeB, homeN;
+    unsigned long long lo[COORD_CB_WAVES], np[COORD_CB_WAVES];
+    long long hi[COORD_CB_WAVES];
+};
+
+// One home cell.  SIDE_A: homes are group-A atoms (a task is one home, or
+// one of nSub interleaved ranges of its partner blocks); otherwise group-B
+// atoms, two homes per task, one per half-wave (a B atom has few partners,
+// so a task per home would end in a mostly empty evaluation round).
+template <bool SIDE_A>
+__device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const CoordItem& it, int item, int w, int hc,
+                           CoordCellShared& sh)
+{
+    constexpr int HPT = SIDE_A ? 1 : 2;  // homes per task
+    constexpr int LPH = 64 / HPT;        // lanes per home
+    const int nx = it.nc[0], ny = it.nc[1], nz = it.nc[2], ncell = nx * ny * nz;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int* st = cb.cellStart + (size_t)w * (cb.cellStride + cb.nItems) + it.cellBase + item;
+    const float4* sorted = cb.sorted + (size_t)w * cb.sortStride + it.sortBase;
+    if (tid < 27) {
+        int cx = hc % nx, cy = (hc / nx) % ny, cz = hc / (nx * ny);
+        cx = (cx + tid % 3 - 1 + nx) % nx;
+        cy = (cy + (tid / 3) % 3 - 1 + ny) % ny;
+        cz = (cz + tid / 9 - 1 + nz) % nz;
+        const int pc = (cz * ny + cy) * nx + cx + (SIDE_A ? ncell : 0);
+        const int b = st[pc], e = st[pc + 1];
+        sh.cellB[tid] = b;
+        sh.pre[tid + 1] = e - b;
+    } else if (tid == 32) {
+        const int c = hc + (SIDE_A ? 0 : ncell);
+        const int b = st[c], e = st[c + 1];
+        sh.homeB = b;
+        sh.homeN = e - b;
+        sh.pre[0] = 0;
+    }
+    const WalkerBox wb = load_box(v, w);
+    const double* bx = wb.b;
+    float bxf[12];
+    for (int k = 0; k < 12; k++) bxf[k] = (float)bx[k];
+    const SwitchD sw = make_switch(it);
+    const float rc2f = (float)(it.rc2 * (1.0 + 1e-5));
+    __syncthreads();
+    if (tid == 0)
+        for (int k = 0; k < 27; k++) sh.pre[k + 1] += sh.pre[k];
+    __syncthreads();
+    const int P = sh.pre[27], H = sh.homeN, homeB = sh.homeB;
+    double cnLane = 0.0;  // this lane's CN terms, in its fixed evaluation order
+    unsigned long long np = 0;
+    short* Q = sh.q[wv];
+    for (int h0 = 0; h0 < H; h0 += COORD_HCAP) {
+        const int hn = min(COORD_HCAP, H - h0);
+        __syncthreads();
+        for (int i = tid; i < hn; i += COORD_CB) {
+            sh.home[i] = sorted[homeB + h0 + i];
+            sh.g[i][0] = sh.g[i][1] = sh.g[i][2] = 0;
+        }
+        for (int p0 = 0; p0 < P; p0 += COORD_PCAP) {
+            const int pn = min(COORD_PCAP, P - p0);
+            __syncthreads();
+            {
+                // every source index first (neighbour cell of flat index q:
+                // pre[c] <= q < pre[c + 1]), then all loads, then the stores
+                constexpr int PER = COORD_PCAP / COORD_CB;
+                int src[PER];
+#pragma unroll
+                for (int u = 0; u < PER; u++) {
+                    const int q = p0 + min(tid + u * COORD_CB, pn - 1);
+                    int lo = 0, hi = 27;
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if (sh.pre[mid] <= q) lo = mid;
+                        else hi = mid;
+                    }
+                    src[u] = sh.cellB[lo] + (q - sh.pre[lo]);
+                }
+                float4 pv[PER];
+#pragma unroll
+                for (int u = 0; u < PER; u++) pv[u] = sorted[src[u]];
+#pragma unroll
+                for (int u = 0; u < PER; u++)
+                    if (tid + u * COORD_CB < pn) sh.part[tid + u * COORD_CB] = pv[u];
+            }
+            __syncthreads();
+            // tasks: (HPT homes, one of nSub interleaved ranges of their
+            // partner blocks), at least two tasks per wave when the cell has
+            // few homes
+            const int nBlk = (pn + LPH - 1) / LPH;
+            const int nGroups = (hn + HPT - 1) / HPT;
+            const int nSub = SIDE_A ? max(1, min(nBlk / 4, (2 * COORD_CB_WAVES + nGroups - 1) / nGroups)) : 1;
+            for (int t = wv; t < nGroups * nSub; t += COORD_CB_WAVES) {
+                const int grp = t / nSub, sub = t - grp * nSub;
+                const int myHome = grp * HPT + (HPT == 2 ? (lane >> 5) : 0);
+                const bool myOk = myHome < hn;
+                const float4 hp = sh.home[min(myHome, hn - 1)];
+                const float4 hp0 = sh.home[grp * HPT];
+                const float4 hp1 = sh.home[min(grp * HPT + HPT - 1, hn - 1)];
+                double acc[HPT][3];  // this lane's row terms of the task per home, in queue order
+#pragma unroll
+                for (int k = 0; k < HPT; k++) acc[k][0] = acc[k][1] = acc[k][2] = 0.0;
+                unsigned head = 0, tail = 0;
+                // FP64 evaluation of queue entry e, dr = r_A - r_B (exact predicate, reference switching)
+                auto evaluate = [&](unsigned e) {
+                    const int ent = Q[e & 127];
+                    const int half = HPT == 2 ? (ent >> 14) : 0;
+                    const float4 pp = sh.part[ent & 0x3FFF];
+                    const float4 hq = half ? hp1 : hp0;
+                    const float4 pa = SIDE_A ? hq : pp;
+                    const float4 pb = SIDE_A ? pp : hq;
+                    double dx = (double)pa.x - (double)pb.x, dy = (double)pa.y - (double)pb.y,
+                           dz = (double)pa.z - (double)pb.z;
+                    if (v.pbc) pbc_exact(dx, dy, dz, bx);
+                    const double r2 = dot3_exact(dx, dy, dz);
+                    if (r2 < 1e-20) return;
+                    if (it.rc2 > 0.0 && !(r2 < it.rc2)) return;
+                    double f, sc;
+                    if (it.rat612) {
+                        if (r2 * it.ir0sq > 1e6) return;  // x > 1000
+                        switch_612(r2, it.ir0sq, f, sc);
+                    } else {
+                        const double r = sqrt(r2);
+                        if (r * sw.ir0 > 1000.0) return;
+                        double dfdr;
+                        switch_eval(sw, r, f, dfdr);
+                        sc = dfdr / r;
+                    }
+                    if (!SIDE_A) sc = -sc;
+#pragma unroll
+                    for (int k = 0; k < HPT; k++) {
+                        const double m = k == half ? sc : 0.0;
+                        acc[k][0] += m * dx;
+                        acc[k][1] += m * dy;
+                        acc[k][2] += m * dz;
+                    }
+                    if (SIDE_A) {
+                        cnLane += f;
+                        np++;
+                    }
+                };
+                for (int pbk = sub; pbk < nBlk; pbk += nSub) {
+                    const int pi = pbk * LPH + (lane % LPH);
+                    const float4 pp = sh.part[min(pi, pn - 1)];
+                    bool cand = myOk && pi < pn;
+                    if (cand) {  // orthorhombic (cells need it), box >= 3 cutoffs: the FP32 image is never ambiguous
+                        float fx = hp.x - pp.x, fy = hp.y - pp.y, fz = hp.z - pp.z;
+                        if (v.pbc) {
+                            fx -= bxf[0] * rintf(fx * bxf[9]);
+                            fy -= bxf[4] * rintf(fy * bxf[10]);
+                            fz -= bxf[8] * rintf(fz * bxf[11]);
+                        }
+                        cand = fx * fx + fy * fy + fz * fz < rc2f;
+                    }
+                    const unsigned long long m = __ballot(cand);
+                    if (cand)
+                        Q[(tail + (unsigned)__popcll(m & ((1ull << lane) - 1ull))) & 127] =
+                            (short)(pi | (HPT == 2 ? (lane >> 5) << 14 : 0));
+                    tail += (unsigned)__popcll(m);
+                    if (tail - head >= 64) {
+                        __builtin_amdgcn_wave_barrier();
+                        evaluate(head + lane);
+                        head += 64;
+                        __builtin_amdgcn_wave_barrier();
+                    }
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (lane < (int)(tail - head)) evaluate(head + lane);
+                // the task's row of each home: a deterministic FP64 wave sum
+                // of the lanes' sums, converted once to 2^40 fixed point;
+                // the tasks' integers then add exactly in LDS
+                double tot[3 * HPT];
+#pragma unroll
+                for (int k = 0; k < HPT; k++) {
+                    tot[3 * k] = acc[k][0];
+                    tot[3 * k + 1] = acc[k][1];
+                    tot[3 * k + 2] = acc[k][2];
+                }
+                wave_sum_n<3 * HPT>(tot);
+                if (lane < 3 * HPT) {
+                    const int k = lane / 3, c = lane - 3 * k;
+                    const int h = grp * HPT + k;
+                    double val = tot[0];
+#pragma unroll
+                    for (int j = 1; j < 3 * HPT; j++)
+                        if (lane == j) val = tot[j];
+                    if (h < hn) atomicAdd((unsigned long long*)&sh.g[h][c], (unsigned long long)to_fix(val));
+                }
+            }
+        }
+        __syncthreads();
+        for (int i = tid; i < hn; i += COORD_CB) {
+            const int homeIdx = __float_as_int(sh.home[i].w);
+            store_row(v, w, (SIDE_A ? it.rowA : it.rowB) + homeIdx, (double)sh.g[i][0] * COORD_UNFIX,
+                      (double)sh.g[i][1] * COORD_UNFIX, (double)sh.g[i][2] * COORD_UNFIX);
+        }
+    }
+    if (!SIDE_A) return;
+    U128 cn{0ull, 0ll};
+    add_cn_term(cn, cnLane);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long lo = __shfl_xor(cn.lo, o, 64);
+        const long long hi = __shfl_xor(cn.hi, o, 64);
+        add128(cn, lo, hi);
+        np += __shfl_xor(np, o, 64);
+    }
+    if (lane == 0) {
+        sh.lo[wv] = cn.lo;
+        sh.hi[wv] = cn.hi;
+        sh.np[wv] = np;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        U128 t{0ull, 0ll};
+        unsigned long long tn = 0;
+        for (int q = 0; q < COORD_CB_WAVES; q++) {
+            add128(t, sh.lo[q], sh.hi[q]);
+            tn += sh.np[q];
+        }
+        // the workgroup's 128-bit CN sum V as (V >> 40) and (V mod 2^40): the
+        // low accumulator absorbs 2^24 workgroups without overflow, so two
+        // 64-bit atomics add V exactly
+        const size_t k = (size_t)w * cb.nItems + item;
+        atomicAdd(&cb.cnAccLo[k], t.lo & ((1ull << 40) - 1ull));
+        atomicAdd((unsigned long long*)&cb.cnAcc[k],
+                  (unsigned long long)((long long)((unsigned long long)t.hi << 24) + (long long)(t.lo >> 40)));
+        atomicAdd(&cb.pairAcc[k], tn);
+        __threadfence();
+        if (atomicAdd(&cb.ticket[k], 1u) == (unsigned)ncell - 1u) {
+            // every group-A workgroup of this walker has added its sums
+            __threadfence();
+            const long long Hc = (long long)atomicAdd((unsigned long long*)&cb.cnAcc[k], 0ull);
+            const unsigned long long Lc = atomicAdd(&cb.cnAccLo[k], 0ull);
+            const unsigned long long Pc = atomicAdd(&cb.pairAcc[k], 0ull);
+            // CN = (H 2^40 + L) 2^-64 = H 2^-24 + L 2^-64
+            v.cvVal[(size_t)w * v.V + it.value] = (double)Hc * 5.9604644775390625e-08 + (double)Lc * 5.421010862427522e-20;
+            cb.npairs[k] = (long long)Pc;
+            atomicExch((unsigned long long*)&cb.cnAcc[k], 0ull);
+            atomicExch(&cb.cnAccLo[k], 0ull);
+            atomicExch(&cb.pairAcc[k], 0ull);
+            atomicExch(&cb.ticket[k], 0u);
+        }
+    }
+}
+
+__global__ __launch_bounds__(COORD_CB) void k_coord_cells(StepView v, CoordBuffers cb,
+                                                          const CoordItem* __restrict__ items, int item, int w0)
+{
+    __shared__ CoordCellShared sh;
+    const int w = w0 + blockIdx.y;
+    const CoordItem it = items[item];
+    const int ncell = it.nc[0] * it.nc[1] * it.nc[2];
+    if ((int)blockIdx.x < ncell) coord_cell<true>(v, cb, it, item, w, blockIdx.x, sh);
+    else coord_cell<false>(v, cb, it, item, w, blockIdx.x - ncell, sh);
+}
+
 void launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordItem* dItems, const CoordItem& hIt,
                          int item, int w0, int nw, hipStream_t s)
 {
@@ -1321,7 +1635,6 @@ void launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordI
         if (coord_sort_fits(ncell2, n)) {
             const int nb = coord_blocks(n);
             hipLaunchKernelGGL(k_coord_rank, dim3(nb, nw), dim3(CELL_BLOCK), 0, s, v, cb, dItems, item, w0);
-            hipLaunchKernelGGL(k_coord_offsets, dim3(1, nw), dim3(CELL_BLOCK), 0, s, cb, dItems, item, nb, w0);
             hipLaunchKernelGGL(k_coord_place, dim3(nb, nw), dim3(CELL_BLOCK), 0, s, cb, dItems, item, w0);
         } else {
             hipLaunchKernelGGL(k_coord_zero, dim3((ncell2 + 255) / 256, nw), dim3(256), 0, s, cb, hIt.cellBase,
@@ -1334,9 +1647,8 @@ void launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordI
     const int wavesA = hIt.nA, wavesB = (hIt.nB + 1) / 2;
     const dim3 gA((wavesA + 3) / 4, nw), gB((wavesB + 3) / 4, nw);
     if (hIt.useCells) {
-        if (hIt.nA > 0)
-            hipLaunchKernelGGL(k_coord_pairs_a, dim3(hIt.nA, nw), dim3(COORD_A_BLOCK), 0, s, v, cb, dItems, item, w0);
-        hipLaunchKernelGGL((k_coord_pairs<false, true>), gB, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
+        const int ncell = hIt.nc[0] * hIt.nc[1] * hIt.nc[2];
+        hipLaunchKernelGGL(k_coord_cells, dim3(2 * ncell, nw), dim3(COORD_CB), 0, s, v, cb, dItems, item, w0);
     } else {
         hipLaunchKernelGGL((k_coord_pairs<true, false>), gA, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
         hipLaunchKernelGGL((k_coord_pairs<false, false>), gB, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
