@@ -17,19 +17,6 @@
 namespace glued {
 
 
-__device__ inline void pbc_f(float& x, float& y, float& z, const float* bx)
-{
-    float s3 = floorf(z * bx[11] + 0.5f);
-    x -= s3 * bx[6];
-    y -= s3 * bx[7];
-    z -= s3 * bx[8];
-    float s2 = floorf(y * bx[10] + 0.5f);
-    x -= s2 * bx[3];
-    y -= s2 * bx[4];
-    float s1 = floorf(x * bx[9] + 0.5f);
-    x -= s1 * bx[0];
-}
-
 __device__ inline double wave_sum(double v) { return wave_sum1(v); }
 
 // Block-wide FP64 sum of NV values per thread over NW waves (the block
@@ -976,15 +963,15 @@ __device__ inline void add_cn_term(U128& a, double f)
     add128(a, (unsigned long long)(frac * 18446744073709551616.0), (long long)ip);
 }
 
-// SIDE_A: one home atom per wave, lanes (cell, half) = (l / 2, l % 2) for l < 54.
-// SIDE_B: two home atoms per wave (lanes 0-31, 32-63), lane l % 32 = cell (< 27).
-template <bool SIDE_A, bool CELLS>
+// Pair passes without a cell list (no cutoff, or a box the cell grid does
+// not fit): the partner group is scanned in chunks.  SIDE_A: one home atom
+// per wave; SIDE_B: two home atoms per wave (lanes 0-31, 32-63).
+template <bool SIDE_A>
 __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBuffers cb,
                                                             const CoordItem* __restrict__ items, int item, int w0)
 {
     constexpr int HPW = SIDE_A ? 1 : 2;   // homes per wave
     constexpr int LPH = 64 / HPW;         // lanes per home
-    constexpr int LPC = SIDE_A ? 2 : 1;   // lanes per cell
     __shared__ float4 qPos[COORD_BLOCK / 64][128];
     __shared__ int qHome[COORD_BLOCK / 64][128];
     const int w = w0 + blockIdx.y;
@@ -1011,52 +998,20 @@ __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBu
     const bool homeOk = home < nHome;
     const WalkerBox wb = load_box(v, w);
     const double* bx = wb.b;
-    float bxf[12];
-    for (int k = 0; k < 12; k++) bxf[k] = (float)bx[k];
     const SwitchD sw = make_switch(it);
-    const float rc2f = (float)(it.rc2 * (1.0 + 1e-5));
 
-    // home positions of this wave (both halves), this lane's partner range.
-    // With cells the homes are taken in cell order: the sorted entry holds the
-    // position and the atom's index in its group, and its cell is recomputed
-    // as the sort computed it.
-    float4 ph;
-    int homeIdx = home;
-    if (CELLS) {
-        const float4 hq =
-            cb.sorted[(size_t)w * cb.sortStride + it.sortBase + (SIDE_A ? 0 : it.nA) + min(home, nHome - 1)];
-        ph = hq;
-        homeIdx = __float_as_int(hq.w);
-    } else {
-        ph = load_pos(v, w, cb.slots[SIDE_A ? it.aBegin + min(home, nHome - 1) : it.bBegin + min(home, nHome - 1)]);
-    }
+    // home positions of this wave (both halves), this lane's partner range
+    const float4 ph =
+        load_pos(v, w, cb.slots[SIDE_A ? it.aBegin + min(home, nHome - 1) : it.bBegin + min(home, nHome - 1)]);
+    const int homeIdx = home;
     float4 hp[HPW];
 #pragma unroll
     for (int k = 0; k < HPW; k++) {
         const int src = k * LPH;
         hp[k] = make_float4(__shfl(ph.x, src, 64), __shfl(ph.y, src, 64), __shfl(ph.z, src, 64), 0.f);
     }
-    const float4* part = cb.sorted + (size_t)w * cb.sortStride + it.sortBase + (SIDE_A ? it.nA : 0);
-    int pBegin = 0, pCount = 0, pStride = 1;
-    if (CELLS) {
-        const int cell = l / LPC, sub = l % LPC;
-        if (homeOk && cell < 27) {
-            const int ncell = it.nc[0] * it.nc[1] * it.nc[2];
-            const float4 hme = h == 0 ? hp[0] : hp[HPW - 1];
-            const int hc = cell_of(hme.x, hme.y, hme.z, bx, it.nc);
-            int cx = hc % it.nc[0], cy = (hc / it.nc[0]) % it.nc[1], cz = hc / (it.nc[0] * it.nc[1]);
-            cx = (cx + (cell % 3) - 1 + it.nc[0]) % it.nc[0];
-            cy = (cy + ((cell / 3) % 3) - 1 + it.nc[1]) % it.nc[1];
-            cz = (cz + (cell / 9) - 1 + it.nc[2]) % it.nc[2];
-            const int pc = (cz * it.nc[1] + cy) * it.nc[0] + cx + (SIDE_A ? ncell : 0);
-            const int* st = cb.cellStart + (size_t)w * (cb.cellStride + cb.nItems) + it.cellBase + item;
-            const int base = SIDE_A ? st[ncell] : 0;  // partner atoms start after the other group
-            const int b = st[pc] - base, e = st[pc + 1] - base;
-            pBegin = b + sub;
-            pCount = e > pBegin ? (e - pBegin + LPC - 1) / LPC : 0;
-            pStride = LPC;
-        }
-    } else if (homeOk) {
+    int pBegin = 0, pCount = 0;
+    if (homeOk) {
         const int chunk = (nPart + LPH - 1) / LPH;
         pBegin = l * chunk;
         pCount = max(0, min(nPart, pBegin + chunk) - pBegin);
@@ -1112,23 +1067,15 @@ __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBu
         float4 pf[COORD_PF];
 #pragma unroll
         for (int u = 0; u < COORD_PF; u++) {  // clamped to a valid partner: no divergent loads
-            const int j = pCount > 0 ? pBegin + min(k0 + u, pCount - 1) * pStride : 0;
-            pf[u] = CELLS ? part[j] : load_pos(v, w, cb.slots[SIDE_A ? it.bBegin + j : it.aBegin + j]);
+            const int j = pCount > 0 ? pBegin + min(k0 + u, pCount - 1) : 0;
+            pf[u] = load_pos(v, w, cb.slots[SIDE_A ? it.bBegin + j : it.aBegin + j]);
         }
 #pragma unroll
         for (int u = 0; u < COORD_PF; u++) {
             const int k = k0 + u;
             if (k >= maxCount) break;
             const float4 pp = pf[u];
-            const bool have = k < pCount;
-            bool cand = have;
-            if (CELLS && have) {  // cells guarantee box >= 3 cutoffs: the FP32 image is never ambiguous
-                const float4 pa = SIDE_A ? hp[h] : pp;
-                const float4 pb = SIDE_A ? pp : hp[h];
-                float fx = pa.x - pb.x, fy = pa.y - pb.y, fz = pa.z - pb.z;
-                if (v.pbc) pbc_f(fx, fy, fz, bxf);
-                cand = fx * fx + fy * fy + fz * fz < rc2f;
-            }
+            const bool cand = k < pCount;  // every partner: the exact predicate decides
             const unsigned long long m = __ballot(cand);
             if (cand) {
                 const unsigned slot = tail + (unsigned)__popcll(m & ((1ull << lane) - 1ull));
@@ -1183,160 +1130,6 @@ __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBu
             atomicAdd((unsigned long long*)&cb.cnAcc[k], (unsigned long long)H);
             atomicAdd(&cb.pairAcc[k], np);
         }
-    }
-}
-
-// Pass A over the cell list with one 192-thread workgroup per A atom: wave q
-// takes the 3x3 neighbour cells of the slab dz = q - 1, seven lanes per cell,
-// so each wave walks a third of the candidates of k_coord_pairs<true, true>
-// (shorter dependent chains, three times the waves in flight).  The waves'
-// gradient, CN and pair-count sums are integers, added in LDS: the results
-// equal those of the one-wave-per-atom form bit for bit.
-constexpr int COORD_A_BLOCK = 192;
-constexpr int COORD_A_LPC = 7;  // lanes per cell: 9 cells x 7 = 63 lanes
-constexpr int COORD_A_PF = 4;   // partner loads in flight per lane (register budget: 4 waves per SIMD)
-
-__global__ __launch_bounds__(COORD_A_BLOCK) void k_coord_pairs_a(StepView v, CoordBuffers cb,
-                                                                 const CoordItem* __restrict__ items, int item, int w0)
-{
-    __shared__ float4 qPos[COORD_A_BLOCK / 64][128];
-    __shared__ long long sG[COORD_A_BLOCK / 64][3];
-    __shared__ unsigned long long sLo[COORD_A_BLOCK / 64], sNp[COORD_A_BLOCK / 64];
-    __shared__ long long sHi[COORD_A_BLOCK / 64];
-    const int w = w0 + blockIdx.y;
-    const CoordItem it = items[item];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const WalkerBox wb = load_box(v, w);
-    const double* bx = wb.b;
-    float bxf[12];
-    for (int k = 0; k < 12; k++) bxf[k] = (float)bx[k];
-    const SwitchD sw = make_switch(it);
-    const float rc2f = (float)(it.rc2 * (1.0 + 1e-5));
-    // homes in cell order: the sorted entry holds the position and the atom's
-    // index in group A; its cell is recomputed as the sort computed it
-    const float4 hq = cb.sorted[(size_t)w * cb.sortStride + it.sortBase + blockIdx.x];  // grid.x = nA
-    const float4 hp = make_float4(hq.x, hq.y, hq.z, 0.f);
-    const int home = __float_as_int(hq.w);
-    const float4* part = cb.sorted + (size_t)w * cb.sortStride + it.sortBase + it.nA;
-    int pBegin = 0, pCount = 0;
-    {
-        const int ci = lane / COORD_A_LPC, sub = lane % COORD_A_LPC;
-        if (ci < 9) {
-            const int ncell = it.nc[0] * it.nc[1] * it.nc[2];
-            const int hc = cell_of(hq.x, hq.y, hq.z, bx, it.nc);
-            int cx = hc % it.nc[0], cy = (hc / it.nc[0]) % it.nc[1], cz = hc / (it.nc[0] * it.nc[1]);
-            cx = (cx + (ci % 3) - 1 + it.nc[0]) % it.nc[0];
-            cy = (cy + (ci / 3) - 1 + it.nc[1]) % it.nc[1];
-            cz = (cz + wv - 1 + it.nc[2]) % it.nc[2];
-            const int pc = (cz * it.nc[1] + cy) * it.nc[0] + cx + ncell;
-            const int* st = cb.cellStart + (size_t)w * (cb.cellStride + cb.nItems) + it.cellBase + item;
-            const int base = st[ncell];  // B atoms start after group A
-            const int b = st[pc] - base, e = st[pc + 1] - base;
-            pBegin = b + sub;
-            pCount = e > pBegin ? (e - pBegin + COORD_A_LPC - 1) / COORD_A_LPC : 0;
-        }
-    }
-    int maxCount = pCount;
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) maxCount = max(maxCount, __shfl_xor(maxCount, o, 64));
-
-    long long gx = 0, gy = 0, gz = 0;
-    U128 cn{0ull, 0ll};
-    unsigned long long np = 0;
-    float4* Q = qPos[wv];
-    unsigned head = 0, tail = 0;
-    // FP64 evaluation of queue entry e (exact predicate, reference switching), dr = r_A - r_B
-    auto evaluate = [&](int e) {
-        const float4 pp = Q[e & 127];
-        double dx = (double)hp.x - (double)pp.x, dy = (double)hp.y - (double)pp.y, dz = (double)hp.z - (double)pp.z;
-        if (v.pbc) pbc_exact(dx, dy, dz, bx);
-        const double r2 = dot3_exact(dx, dy, dz);
-        if (r2 < 1e-20) return;
-        if (it.rc2 > 0.0 && !(r2 < it.rc2)) return;
-        const double r = sqrt(r2);
-        if (r * sw.ir0 > 1000.0) return;
-        double f, dfdr;
-        switch_eval(sw, r, f, dfdr);
-        const double sc = dfdr / r;
-        gx += to_fix(sc * dx);
-        gy += to_fix(sc * dy);
-        gz += to_fix(sc * dz);
-        add_cn_term(cn, f);
-        np++;
-    };
-    for (int k0 = 0; k0 < maxCount; k0 += COORD_A_PF) {
-        float4 pf[COORD_A_PF];
-#pragma unroll
-        for (int u = 0; u < COORD_A_PF; u++) {  // clamped to a valid partner: no divergent loads
-            const int j = pCount > 0 ? pBegin + min(k0 + u, pCount - 1) * COORD_A_LPC : 0;
-            pf[u] = part[j];
-        }
-#pragma unroll
-        for (int u = 0; u < COORD_A_PF; u++) {
-            const int k = k0 + u;
-            if (k >= maxCount) break;
-            const float4 pp = pf[u];
-            bool cand = k < pCount;
-            if (cand) {  // cells guarantee box >= 3 cutoffs: the FP32 image is never ambiguous
-                float fx = hp.x - pp.x, fy = hp.y - pp.y, fz = hp.z - pp.z;
-                if (v.pbc) pbc_f(fx, fy, fz, bxf);
-                cand = fx * fx + fy * fy + fz * fz < rc2f;
-            }
-            const unsigned long long m = __ballot(cand);
-            if (cand) Q[(tail + (unsigned)__popcll(m & ((1ull << lane) - 1ull))) & 127] = pp;
-            tail += (unsigned)__popcll(m);
-            if (tail - head >= 64) {
-                __builtin_amdgcn_wave_barrier();
-                evaluate((int)(head + lane));
-                head += 64;
-                __builtin_amdgcn_wave_barrier();
-            }
-        }
-    }
-    __builtin_amdgcn_wave_barrier();
-    if (lane < (int)(tail - head)) evaluate((int)(head + lane));
-
-    // integer (exact) sums: wave butterfly, then the three waves in LDS
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        gx += __shfl_xor(gx, o, 64);
-        gy += __shfl_xor(gy, o, 64);
-        gz += __shfl_xor(gz, o, 64);
-        const unsigned long long lo = __shfl_xor(cn.lo, o, 64);
-        const long long hi = __shfl_xor(cn.hi, o, 64);
-        add128(cn, lo, hi);
-        np += __shfl_xor(np, o, 64);
-    }
-    if (lane == 0) {
-        sG[wv][0] = gx;
-        sG[wv][1] = gy;
-        sG[wv][2] = gz;
-        sLo[wv] = cn.lo;
-        sHi[wv] = cn.hi;
-        sNp[wv] = np;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        long long tx = 0, ty = 0, tz = 0;
-        U128 t{0ull, 0ll};
-        unsigned long long tn = 0;
-        for (int q = 0; q < COORD_A_BLOCK / 64; q++) {
-            tx += sG[q][0];
-            ty += sG[q][1];
-            tz += sG[q][2];
-            add128(t, sLo[q], sHi[q]);
-            tn += sNp[q];
-        }
-        store_row(v, w, it.rowA + home, (double)tx * COORD_UNFIX, (double)ty * COORD_UNFIX, (double)tz * COORD_UNFIX);
-        // the workgroup's 128-bit CN sum V as (V >> 40) and (V mod 2^40): the low
-        // accumulator absorbs 2^24 workgroups without overflow, so two 64-bit
-        // atomics add V exactly
-        const size_t k = (size_t)w * cb.nItems + item;
-        const unsigned long long L = t.lo & ((1ull << 40) - 1ull);
-        const long long H = (long long)((unsigned long long)t.hi << 24) + (long long)(t.lo >> 40);
-        atomicAdd(&cb.cnAccLo[k], L);
-        atomicAdd((unsigned long long*)&cb.cnAcc[k], (unsigned long long)H);
-        atomicAdd(&cb.pairAcc[k], tn);
     }
 }
 
@@ -1650,8 +1443,8 @@ void launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordI
         const int ncell = hIt.nc[0] * hIt.nc[1] * hIt.nc[2];
         hipLaunchKernelGGL(k_coord_cells, dim3(2 * ncell, nw), dim3(COORD_CB), 0, s, v, cb, dItems, item, w0);
     } else {
-        hipLaunchKernelGGL((k_coord_pairs<true, false>), gA, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
-        hipLaunchKernelGGL((k_coord_pairs<false, false>), gB, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
+        hipLaunchKernelGGL((k_coord_pairs<true>), gA, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
+        hipLaunchKernelGGL((k_coord_pairs<false>), gB, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
     }
 }
 
