@@ -133,9 +133,9 @@ class MultiWalker(Workload):
         hb = self.c["hill_bias"]
         n = sum(len(self.ctx.get_hills(hb, w)[1]) for w in range(self.W))
         D = 2
-        nbytes = n * 8 * (D + 1)  # FP64 centre (D) + height per hill, streamed once
+        nbytes = n * 4 * (D + 1)  # FP32 centre (D) + height per hill, streamed once
         return dict(bound="hbm", kernel="k_step_bias_chunks (hill sums, D=2)", achieved=nbytes / t_launch / 1e9,
-                    peak=HBM_PEAK_GBS, unit="GB/s", per_unit="24 B per hill (2 FP64 centres + FP64 height)",
+                    peak=HBM_PEAK_GBS, unit="GB/s", per_unit="12 B per hill (2 FP32 centres + FP32 height)",
                     units_per_launch=n, bytes_per_launch=nbytes)
 
     def oracle_walker(self, w):
@@ -328,12 +328,12 @@ class Small(Workload):
 
     def roofline(self, t_launch):
         n = len(self.ctx.get_hills(0)[1])
-        # 7 atoms: 12 B position + 48 B fixed-point force read+write; 24 B per
-        # hill; 24 B of CV values and bias state
-        nbytes = 7 * (12 + 48) + n * 24 + 24
+        # 7 atoms: 12 B position + 48 B fixed-point force read+write; 12 B per
+        # hill (FP32 centres and height); 24 B of CV values and bias state
+        nbytes = 7 * (12 + 48) + n * 12 + 24
         return dict(bound="hbm", kernel="k_step_tiny (whole step: CVs, hill sum, bias, scatter in one launch)",
                     achieved=nbytes / t_launch / 1e9, peak=HBM_PEAK_GBS, unit="GB/s",
-                    per_unit="24 B per hill + 420 B per step (7 atoms x 60 B) + 24 B", units_per_launch=n,
+                    per_unit="12 B per hill + 420 B per step (7 atoms x 60 B) + 24 B", units_per_launch=n,
                     bytes_per_launch=nbytes)
 
     def oracle_walker(self, w):

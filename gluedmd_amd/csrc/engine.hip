@@ -28,6 +28,7 @@ void launch_bias_opes(const StepView&, const OpesBiasDev&, int, int, hipStream_t
 void launch_dep_grid(const StepView&, const GridBiasDev&, int, int, hipStream_t);
 void launch_dep_pb(const StepView&, const SmallBiasState&, const SmallBiasItem&, int, int, hipStream_t);
 void launch_dep_hills(const StepView&, const HillBiasDev&, int, int, hipStream_t);
+void launch_f64_to_f32(const double*, float*, int, hipStream_t);
 void launch_dep_opes(const StepView&, const OpesBiasDev&, int, int, hipStream_t);
 void launch_opes_welford(const StepView&, const OpesBiasDev&, int, int, hipStream_t);
 void launch_dep_small(const StepView&, const SmallBiasState&, const SmallBiasItem*, int, int, int, int, hipStream_t);
@@ -673,13 +674,13 @@ void Engine::buildPlan(int nMass, const double* masses)
             d.chunks = hill_chunks_for(d.cap);
             h->pace = s.iparams[0];
             h->spec = (int)b;
-            h->centers.alloc(sizeof(double) * (size_t)T * D * d.cap);
-            h->heights.alloc(sizeof(double) * (size_t)T * d.cap);
+            h->centers.alloc(sizeof(float) * (size_t)T * D * d.cap);
+            h->heights.alloc(sizeof(float) * (size_t)T * d.cap);
             h->count.alloc(sizeof(int) * T);
             h->part.alloc(sizeof(double) * (size_t)W_ * d.chunks * (D + 1));
             h->counter.alloc(sizeof(unsigned int) * W_);
-            d.centers = h->centers.as<double>();
-            d.heights = h->heights.as<double>();
+            d.centers = h->centers.as<float>();
+            d.heights = h->heights.as<float>();
             d.count = h->count.as<int>();
             d.part = h->part.as<double>();
             d.counter = h->counter.as<unsigned int>();
@@ -1628,10 +1629,8 @@ void Engine::setHillHeightsDevice(int w, int bi, const double* d, int n, hipStre
     if (w < 0 || w >= W_) fail("set_hill_heights: walker out of range");
     if (n < 0 || n > h->d.cap) fail("set_hill_heights: count exceeds the capacity");
     const int t = h->d.shared ? 0 : w;
-    if (n)
-        hip_check(hipMemcpyAsync(h->d.heights + (size_t)t * h->d.cap, d, sizeof(double) * n, hipMemcpyDeviceToDevice,
-                                 s),
-                  "set_hill_heights");
+    launch_f64_to_f32(d, h->d.heights + (size_t)t * h->d.cap, n, s);
+    hip_check(hipGetLastError(), "set_hill_heights");
 }
 
 void Engine::setPathLambda(int pi, double lambda)
@@ -1852,13 +1851,16 @@ void Engine::setHills(int w, int bi, const double* centers, const double* height
     if (w < 0 || w >= W_) fail("set_hills: walker out of range");
     if (n < 0 || n > h->d.cap) fail("set_hills: hill count exceeds the capacity");
     const int t = h->d.shared ? 0 : w, D = h->d.D;
-    std::vector<double> c((size_t)D * h->d.cap, 0.0);
-    for (int k = 0; k < n; k++)
-        for (int d = 0; d < D; d++) c[(size_t)d * h->d.cap + k] = centers[(size_t)k * D + d];
-    hip_check(hipMemcpy(h->d.centers + (size_t)t * D * h->d.cap, c.data(), sizeof(double) * c.size(),
+    // stored in FP32 (rounded to nearest), kernel-major -> SoA
+    std::vector<float> c((size_t)D * h->d.cap, 0.0f), hh((size_t)n);
+    for (int k = 0; k < n; k++) {
+        for (int d = 0; d < D; d++) c[(size_t)d * h->d.cap + k] = (float)centers[(size_t)k * D + d];
+        hh[k] = (float)heights[k];
+    }
+    hip_check(hipMemcpy(h->d.centers + (size_t)t * D * h->d.cap, c.data(), sizeof(float) * c.size(),
                         hipMemcpyHostToDevice),
               "set_hills");
-    if (n) hip_check(hipMemcpy(h->d.heights + (size_t)t * h->d.cap, heights, sizeof(double) * n, hipMemcpyHostToDevice),
+    if (n) hip_check(hipMemcpy(h->d.heights + (size_t)t * h->d.cap, hh.data(), sizeof(float) * n, hipMemcpyHostToDevice),
                      "set_hills");
     hip_check(hipMemcpy(h->d.count + t, &n, sizeof(int), hipMemcpyHostToDevice), "set_hills");
 }
@@ -1878,8 +1880,8 @@ int Engine::getHills(int w, int bi, double* centers, double* heights, int cap)
         auto c = download(h->d.centers + (size_t)t * D * h->d.cap, (size_t)D * h->d.cap);
         auto hh = download(h->d.heights + (size_t)t * h->d.cap, n);
         for (int k = 0; k < n; k++) {
-            for (int d = 0; d < D; d++) centers[(size_t)k * D + d] = c[(size_t)d * h->d.cap + k];
-            heights[k] = hh[k];
+            for (int d = 0; d < D; d++) centers[(size_t)k * D + d] = (double)c[(size_t)d * h->d.cap + k];
+            heights[k] = (double)hh[k];
         }
     }
     return n;
@@ -2003,8 +2005,8 @@ std::vector<char> Engine::getState(int w)
         i32(n);
         i32(D);
         for (int k = 0; k < n; k++)
-            for (int d = 0; d < D; d++) f64(c[(size_t)d * h->d.cap + k]);
-        put(hh.data(), 8 * (size_t)n);
+            for (int d = 0; d < D; d++) f64((double)c[(size_t)d * h->d.cap + k]);
+        for (int k = 0; k < n; k++) f64((double)hh[k]);
     }
     return buf;
 }

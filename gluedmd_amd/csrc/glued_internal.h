@@ -195,8 +195,8 @@ struct HillBiasDev {
     // host-computed reciprocals (the oracle forms the same expressions):
     // i2s2 = 1/(2 sigma^2), is2 = 1/sigma^2, invPeriod = 1/period
     double i2s2[MAX_BIAS_D], is2[MAX_BIAS_D], invPeriod[MAX_BIAS_D];
-    double* centers;          // [W or 1][D][cap]  (SoA, coalesced per CV)
-    double* heights;          // [W or 1][cap]
+    float* centers;           // [W or 1][D][cap]  (SoA, coalesced per CV; FP32, widened before use)
+    float* heights;           // [W or 1][cap]     (FP32)
     int* count;               // [W or 1]
     int cap;
     int shared;

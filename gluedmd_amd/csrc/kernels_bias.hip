@@ -370,16 +370,16 @@ template <int D>
 __device__ inline void hill_chunk_load(const HillBiasDev& h, int w, int chunk, HillChunkRegs<D>& R)
 {
     const int tb = h.shared ? 0 : w;
-    const double* __restrict__ C = h.centers + (size_t)tb * D * h.cap;
-    const double* __restrict__ H = h.heights + (size_t)tb * h.cap;
+    const float* __restrict__ C = h.centers + (size_t)tb * D * h.cap;
+    const float* __restrict__ H = h.heights + (size_t)tb * h.cap;
     const int n = h.count[tb];
     const int k0 = chunk * CHUNK + threadIdx.x;
 #pragma unroll
     for (int r = 0; r < PER_THREAD; r++) {
         const int kc = min(k0 + r * BIAS_BLOCK, h.cap - 1);
 #pragma unroll
-        for (int d = 0; d < D; d++) R.cx[r][d] = C[(size_t)d * h.cap + kc];
-        R.hk[r] = H[kc];
+        for (int d = 0; d < D; d++) R.cx[r][d] = (double)C[(size_t)d * h.cap + kc];
+        R.hk[r] = (double)H[kc];
     }
 #pragma unroll
     for (int r = 0; r < PER_THREAD; r++) {
@@ -397,6 +397,18 @@ __device__ inline void hill_chunk_cv(const StepView& v, const HillBiasDev& h, in
     const double* cv = v.cvVal + (size_t)w * v.V;
 #pragma unroll
     for (int d = 0; d < D; d++) s[d] = cv[h.cv[d]];
+}
+
+// exp(-e) for 0 <= e <= 8 from the FP32 exponential: e is rounded to FP32
+// (an argument error of up to 4.8e-7 at e = 8, which the exponential would
+// carry into the result) and the remainder d = e - e32, exact in FP64, is put
+// back to first order, exp(-e) = exp(-e32) (1 - d) (second order < 1e-13).
+// The relative error left is the FP32 exponential's own, ~1e-7.
+__device__ inline double hill_exp(double e)
+{
+    const float ef = (float)e;
+    const double d = e - (double)ef;
+    return (double)expf(-ef) * (1.0 - d);
 }
 
 // The thread's sum over its hills at CV values s: V and dV/ds_d.  The
@@ -423,7 +435,7 @@ __device__ inline void hill_terms_impl(const HillBiasDev& h, const double (&s)[D
                 e = e + x * x * h.i2s2[d];
             }
         }
-        const double t = (e <= 8.0) ? R.hk[r] * (double)expf(-(float)e) : 0.0;
+        const double t = (e <= 8.0) ? R.hk[r] * hill_exp(e) : 0.0;
         acc[0] += t;
 #pragma unroll
         for (int d = 0; d < D; d++) acc[1 + d] = fma(-t * dc[d], h.is2[d], acc[1 + d]);
@@ -1093,12 +1105,12 @@ constexpr int DEP_HILLS_SHARED_BLOCK = 256;
 
 __device__ inline void deposit_hill(const StepView& v, const HillBiasDev& h, int tb, int w, int slot)
 {
-    double* C = h.centers + (size_t)tb * h.D * h.cap;
-    double* H = h.heights + (size_t)tb * h.cap;
+    float* C = h.centers + (size_t)tb * h.D * h.cap;
+    float* H = h.heights + (size_t)tb * h.cap;
     const double* cv = v.cvVal + (size_t)w * v.V;
     const double V = v.benergy[(size_t)w * v.B + h.eslot];
-    for (int d = 0; d < h.D; d++) C[(size_t)d * h.cap + slot] = cv[h.cv[d]];
-    H[slot] = h.hif > 0.0 ? h.h0 * exp(-V * h.hif) : h.h0;
+    for (int d = 0; d < h.D; d++) C[(size_t)d * h.cap + slot] = (float)cv[h.cv[d]];
+    H[slot] = (float)(h.hif > 0.0 ? h.h0 * exp(-V * h.hif) : h.h0);
 }
 
 __global__ void k_dep_hills(StepView v, HillBiasDev h, int w0, int nw)
@@ -1490,12 +1502,12 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_hill_adjoint(StepView v, HillBias
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     const int tb = h.shared ? 0 : blockIdx.y;
     const int n = h.count[tb];
-    const double* C = h.centers + (size_t)tb * D * h.cap;
+    const float* C = h.centers + (size_t)tb * D * h.cap;
     double c[D], hk = 0.0;
     const bool ok = k < n;
 #pragma unroll
-    for (int d = 0; d < D; d++) c[d] = ok ? C[(size_t)d * h.cap + k] : 0.0;
-    if (ok) hk = h.heights[(size_t)tb * h.cap + k];
+    for (int d = 0; d < D; d++) c[d] = ok ? (double)C[(size_t)d * h.cap + k] : 0.0;
+    if (ok) hk = (double)h.heights[(size_t)tb * h.cap + k];
     double gh = 0.0, gs[D];
 #pragma unroll
     for (int d = 0; d < D; d++) gs[d] = 0.0;
@@ -1521,7 +1533,7 @@ __global__ __launch_bounds__(ADJ_BLOCK) void k_hill_adjoint(StepView v, HillBias
                     e = e + x * x * h.i2s2[d];
                 }
                 if (!(e <= 8.0)) continue;
-                const double t = sG[j] * (double)expf(-(float)e);
+                const double t = sG[j] * hill_exp(e);
                 gh += t;
 #pragma unroll
                 for (int d = 0; d < D; d++) gs[d] += t * hk * dc[d] * dc[d];
@@ -1563,6 +1575,19 @@ void launch_hill_adjoint(const StepView& v, const HillBiasDev& h, const double* 
         default: hipLaunchKernelGGL(k_hill_adjoint<4>, g, b, 0, s, v, h, gV, gH, sigPart, W); break;
     }
     hipLaunchKernelGGL(k_hill_adjoint_sigma, dim3(1), dim3(64), 0, s, h, sigPart, (int)(g.x * g.y), gSigma);
+}
+
+// FP64 device values to FP32 storage (hill heights given by the autograd
+// wrapper), round to nearest.
+__global__ void k_f64_to_f32(const double* __restrict__ src, float* __restrict__ dst, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i];
+}
+
+void launch_f64_to_f32(const double* src, float* dst, int n, hipStream_t s)
+{
+    if (n > 0) hipLaunchKernelGGL(k_f64_to_f32, dim3((n + 255) / 256), dim3(256), 0, s, src, dst, n);
 }
 
 }  // namespace glued

@@ -51,6 +51,12 @@ class SplitMix64:
         return (lo + np.floor(self.uniform(n) * (hi - lo + 1))).astype(np.int64)
 
 
+def fp32_values(a):
+    """FP64 array of FP32-representable values (hill centres and heights are
+    stored in FP32)."""
+    return np.asarray(a, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
 def rng(config):
     return SplitMix64(SEED_BASE + int(config))
 
@@ -66,8 +72,8 @@ def config1(n_frames=1000, n_prior=1000):
     r = rng(1)
     base = r.uniform(22 * 3, 0.0, 3.0).reshape(22, 3)
     frames = np.stack([base + r.normal(22 * 3, 0.01).reshape(22, 3) for _ in range(n_frames)]).astype(np.float32)
-    hc = r.uniform(2 * n_prior, -math.pi, math.pi).reshape(n_prior, 2)
-    hh = r.uniform(n_prior, 0.1, 1.2)
+    hc = fp32_values(r.uniform(2 * n_prior, -math.pi, math.pi).reshape(n_prior, 2))
+    hh = fp32_values(r.uniform(n_prior, 0.1, 1.2))
     f = Force(pbc=True, temperature=300.0)
     f.add_distance([0, 21])
     phi = f.add_dihedral([4, 6, 8, 14])
@@ -163,8 +169,8 @@ def config3_system(batch=256, M=300, N=64, n_hills=20000):
 def config3_hills(r, s_range, z_range, n_hills=20000):
     """Builder choice: hill centres uniform over the (s, z) bounding box of the
     configurations, heights U[0.1,1.2], sigma_s = 0.5, sigma_z = 0.1 z-range."""
-    c = np.stack([r.uniform(n_hills, *s_range), r.uniform(n_hills, *z_range)], axis=1)
-    h = r.uniform(n_hills, 0.1, 1.2)
+    c = fp32_values(np.stack([r.uniform(n_hills, *s_range), r.uniform(n_hills, *z_range)], axis=1))
+    h = fp32_values(r.uniform(n_hills, 0.1, 1.2))
     sig = np.array([0.5, 0.1 * max(z_range[1] - z_range[0], 1e-3)])
     return c, h, sig
 
@@ -207,7 +213,7 @@ def config4(n_walkers=64, n_hills=20000, extra_hills=4096):
     hills = []
     for w in range(n_walkers):
         c = np.stack([r.uniform(n_hills, 0.0, 0.6), r.uniform(n_hills, -math.pi, math.pi)], axis=1)
-        hills.append((c, r.uniform(n_hills, 0.1, 1.2)))
+        hills.append((fp32_values(c), fp32_values(r.uniform(n_hills, 0.1, 1.2))))
     f = Force(pbc=True, temperature=300.0)
     rmsd = f.add_rmsd(list(range(500)), ref.tolist(), fit=True)
     phi = f.add_dihedral([500, 501, 502, 503])

@@ -199,7 +199,16 @@ class _Grid:
         self.numDeposited += 1
 
 
+def _fp32(a):
+    """FP64 array of the FP32-rounded values (round to nearest)."""
+    return np.asarray(a, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
 class _Hills:
+    """Hill-list metadynamics state.  Centres and heights are kept as FP32
+    values (set or deposited values are rounded to nearest), the precision
+    the engine stores them in; the sums over them are FP64."""
+
     def __init__(self, D, params, iparams):
         self.D = D
         self.h0 = params[0]
@@ -211,6 +220,22 @@ class _Hills:
         self.centers = np.zeros((0, D))
         self.heights = np.zeros(0)
 
+    @property
+    def centers(self):
+        return self._centers
+
+    @centers.setter
+    def centers(self, c):
+        self._centers = _fp32(c).reshape(-1, self.D)
+
+    @property
+    def heights(self):
+        return self._heights
+
+    @heights.setter
+    def heights(self, h):
+        self._heights = _fp32(h).reshape(-1)
+
     def eval(self, s):
         if len(self.heights) == 0:
             return 0.0, np.zeros(self.D)
@@ -220,7 +245,7 @@ class _Hills:
         if len(self.heights) >= self.cap:
             return
         h = self.h0 * math.exp(-V * self.hif) if self.hif > 0.0 else self.h0
-        self.centers = np.vstack([self.centers, np.asarray(center)[None, :]])
+        self.centers = np.vstack([self.centers, np.asarray(center, dtype=np.float64)[None, :]])
         self.heights = np.append(self.heights, h)
 
 
