@@ -442,10 +442,10 @@ def main():
     for _ in range(args.warmup):
         ctx.execute_batch(use_graph=False)
     t_batched = time_loop(lambda k: ctx.execute_batch(use_graph=False), args.steps, sync)
-    log(f"batched {t_batched * 1e6:.1f} us/step")
+    log(f"batched {t_batched * 1e6:.1f} us/step; launch shapes: {ctx.tuning()}")
     if args.batched_only:
         print(json.dumps({"metric": "batched step only", "workload": wk.name, "ms_per_step": round(t_batched * 1e3, 4),
-                          "value": round(W / t_batched, 1), "unit": "evals/s"}))
+                          "value": round(W / t_batched, 1), "unit": "evals/s", "tuning": ctx.tuning()}))
         return 0
 
     # ---- plain: one walker per launch sequence ----
@@ -507,6 +507,7 @@ def main():
         "eval_plus_deposit_evals_per_s": round(W / t_dep, 1),
         "eval_plus_deposit_graph_evals_per_s": round(W / t_dep_graph, 1),
         "stage_us": stage_t,
+        "tuning": ctx.tuning(),
         "roofline": rf,
     }
     if cpu is not None:

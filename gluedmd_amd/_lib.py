@@ -57,6 +57,7 @@ SIGNATURES = {
     "glued_set_hill_sigma": [vp, i, c_double_p, i],
     "glued_hill_adjoint": [vp, vp, i, vp, vp, vp],
     "glued_hill_adjoint_noalloc": [vp, vp, i, vp, vp, vp],
+    "glued_get_tuning": [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)],
     "glued_set_path_frames": [vp, i, vp, i],
     "glued_path_frames_adjoint": [vp, vp, i, vp, vp],
 }

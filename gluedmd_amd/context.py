@@ -286,6 +286,14 @@ class Context:
         g, gp = _f64(np.asarray(grid).reshape(-1))
         call("glued_set_grid", self.h, int(walker), int(bias_index), gp, len(g))
 
+    def tuning(self):
+        """Launch shapes the autotuner chose for this plan, and how."""
+        n = ctypes.c_size_t()
+        call("glued_get_tuning", self.h, None, 0, ctypes.byref(n))
+        buf = ctypes.create_string_buffer(n.value + 1)
+        call("glued_get_tuning", self.h, buf, n.value + 1, ctypes.byref(n))
+        return buf.value.decode()
+
     def plan_info(self):
         r, a = ctypes.c_int(), ctypes.c_int()
         call("glued_plan_info", self.h, ctypes.byref(r), ctypes.byref(a))

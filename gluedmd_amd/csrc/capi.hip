@@ -1,5 +1,6 @@
 // C-ABI of the GLUED step (declared in include/glued.h).  Converts every C++
 // exception into a status code and a thread-local message.
+#include <algorithm>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -325,6 +326,19 @@ int glued_set_path_frames(glued_handle h, int pathIdx, const double* frames, int
 int glued_path_frames_adjoint(glued_handle h, void* stream, int pathIdx, const double* dGradV, double* dGradFrames)
 {
     return guard([&] { E(h).pathFramesAdjoint((hipStream_t)stream, pathIdx, dGradV, dGradFrames); });
+}
+
+int glued_get_tuning(glued_handle h, char* buf, size_t cap, size_t* len)
+{
+    return guard([&] {
+        const std::string r = E(h).tuningReport();
+        if (len) *len = r.size();
+        if (buf && cap) {
+            const size_t n = std::min(cap - 1, r.size());
+            std::memcpy(buf, r.data(), n);
+            buf[n] = 0;
+        }
+    });
 }
 
 int glued_plan_info(glued_handle h, int* rows, int* atoms)

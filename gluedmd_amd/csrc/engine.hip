@@ -1,5 +1,8 @@
 // Host engine of the GLUED enhanced-sampling step (see engine.hpp).
 #include "engine.hpp"
+#include <cstdio>
+#include <cstdlib>
+#include <mutex>
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -13,11 +16,11 @@ namespace glued {
 void launch_cv_small(const StepView&, const SmallCVItem*, int, int, int, hipStream_t);
 void launch_cv_group(const StepView&, const GroupCVItem*, int, const int*, const float*, const float*, int, int,
                      hipStream_t);
-void launch_cv_path(const StepView&, const PathItem*, int, int, int, int, const int*, const float*, int, int,
+void launch_cv_path(const StepView&, const PathItem*, int, int, int, int, const int*, const float*, int, int, int,
                     hipStream_t);
 void launch_cv_group_big(const StepView&, const GroupCVItem*, int, const int*, const float*, const float*, int, int,
                          hipStream_t);
-void launch_path_rows(const StepView&, const PathItem*, int, int, int, const int*, const float*, int, int,
+void launch_path_rows(const StepView&, const PathItem*, int, int, int, const int*, const float*, int, int, int,
                       hipStream_t);
 
 void launch_coordination(const StepView&, const CoordBuffers&, const CoordItem*, const CoordItem&, int, int, int,
@@ -423,6 +426,7 @@ void Engine::buildPlan(int nMass, const double* masses)
                     it.preDfunc = 0.5 * rn * (rn - rm) / rm;
                     it.preSecDev = rn * (rm * rm - 3.0 * rm * (rn - 1.0) + rn * (2.0 * rn - 3.0)) / (6.0 * rm);
                     it.ir0sq = it.ir0 * it.ir0;
+                    it.cellBlock = tune_.coordBlock;
                     it.rat612 = rn == 6.0 && rm == 12.0;
                 }
                 coord_.push_back(it);
@@ -449,24 +453,7 @@ void Engine::buildPlan(int nMass, const double* masses)
                 it.rowBlk0 = pathRowBlocks_;
                 msdStride_ += N;
                 pathRowBlocks_ += (M + PATH_TILE - 1) / PATH_TILE;
-                // MSD work: tiles of 16 walkers x 16 frames x K slices, about
-                // 512 blocks per path over the batch, at most 16 slices
                 {
-                    const int nChunks = (3 * M + PATH_KC - 1) / PATH_KC;
-                    const int wTiles = (W_ + PATH_TW - 1) / PATH_TW;
-                    it.frameTiles = (N + PATH_TF - 1) / PATH_TF;
-                    int sl = (512 + wTiles * it.frameTiles - 1) / (wTiles * it.frameTiles);
-                    sl = std::max(1, std::min(sl, std::min(nChunks, PATH_MAX_SLICES)));
-                    it.chunksPerSlice = (nChunks + sl - 1) / sl;
-                    it.slices = (nChunks + it.chunksPerSlice - 1) / it.chunksPerSlice;
-                    it.units = it.frameTiles * it.slices;
-                    it.msdBlk0 = pathMsdUnits_;
-                    pathMsdUnits_ += it.units;
-                    it.msdBlkF0 = pathMsdBlocks_;
-                    it.msdBlkF = (int)(((long long)wTiles * it.units + W_ - 1) / W_);
-                    pathMsdBlocks_ += it.msdBlkF;
-                    it.partOff = partStride_;
-                    partStride_ += it.slices * N + it.slices;
                     it.centreOff = (int)pathCentreHost_.size();
                     pathCentreHost_.resize(pathCentreHost_.size() + (size_t)3 * M);
                     pathR2Host_.resize((size_t)msdStride_);
@@ -555,13 +542,22 @@ void Engine::buildPlan(int nMass, const double* masses)
     dCmCount_.alloc(sizeof(unsigned int) * (size_t)W_ * std::max((int)cm_.size(), 1));
     dGroupMass_.upload(groupMass);
     dGroupRef_.upload(groupRef);
-    dPath_.upload(path_);
     dPathAux_.alloc(sizeof(double) * (size_t)W_ * std::max((int)path_.size(), 1) * 2);
     dPathMsd_.alloc(sizeof(double) * (size_t)W_ * std::max(msdStride_, 1));
     dPathFrames_.upload(pathFrames);
     dPathCentre_.upload(pathCentreHost_);
     dPathR2_.upload(pathR2Host_);
-    dPathPart_.alloc(sizeof(double) * (size_t)W_ * std::max(partStride_, 1));
+    planPathWork();
+    // row phases: one choice for every launch of this engine (one walker or
+    // all), 4 when the batch's row blocks exceed one residency round and the
+    // paths have at most 256 frames, else 8 (the autotuner may change it)
+    if (tune_.pathPhases == 0) tune_.pathPhases = (long long)pathRowBlocks_ * W_ > 512 && pathMaxN_ <= 256 ? 4 : 8;
+    {
+        // the partial buffer holds the most slices any tuned shape can use
+        long long most = 0;
+        for (const auto& it : path_) most += (long long)PATH_MAX_SLICES * (it.N + 1);
+        dPathPart_.alloc(sizeof(double) * (size_t)W_ * std::max<long long>(most, 1));
+    }
     energyU_.assign(W_, 0.0);
     energyF_.assign(W_, nullptr);
     dEnergyU_.alloc(sizeof(double) * W_);
@@ -939,6 +935,149 @@ void Engine::buildPlan(int nMass, const double* masses)
     }
 }
 
+// MSD work of every path: tiles of PATH_TW walkers x PATH_TF frames x K
+// slices, about tune_.pathBlocks blocks per path over the batch, at most
+// PATH_MAX_SLICES slices; the per-kind launch takes one unit per block and
+// walker tile, the fused CV launch spreads (walker tile, unit) items over
+// msdBlkF blocks of every walker row.
+void Engine::planPathWork()
+{
+    pathMsdUnits_ = pathMsdBlocks_ = partStride_ = 0;
+    const int wTiles = (W_ + PATH_TW - 1) / PATH_TW;
+    for (auto& it : path_) {
+        const int nChunks = (3 * it.M + PATH_KC - 1) / PATH_KC;
+        it.frameTiles = (it.N + PATH_TF - 1) / PATH_TF;
+        int sl = (tune_.pathBlocks + wTiles * it.frameTiles - 1) / (wTiles * it.frameTiles);
+        sl = std::max(1, std::min(sl, std::min(nChunks, PATH_MAX_SLICES)));
+        it.chunksPerSlice = (nChunks + sl - 1) / sl;
+        it.slices = (nChunks + it.chunksPerSlice - 1) / it.chunksPerSlice;
+        it.units = it.frameTiles * it.slices;
+        it.msdBlk0 = pathMsdUnits_;
+        pathMsdUnits_ += it.units;
+        it.msdBlkF0 = pathMsdBlocks_;
+        it.msdBlkF = (int)(((long long)wTiles * it.units + W_ - 1) / W_);
+        pathMsdBlocks_ += it.msdBlkF;
+        it.partOff = partStride_;
+        partStride_ += it.slices * it.N + it.slices;
+    }
+    dPath_.upload(path_);
+}
+
+// ---------------------------------------------------------------------------
+// Launch-shape autotuning.  Candidates: the path MSD block target
+// {128, 256, 512, 1024}, the path row phases {4, 8} and the coordination cell
+// workgroup {256, 512, 1024}.  At an engine's first evaluation, when it is a
+// batched one, each candidate's CV stage (every walker) is timed with HIP
+// events on the caller's stream (3 launches to warm, the mean of 8), one
+// group at a time, the others at their current best; the choice is cached
+// per plan shape for the process, so later engines of the same shape take
+// it without timing.  An engine whose first evaluation is a one-walker
+// launch keeps the defaults (or a cached choice), so that all of its
+// evaluations, plain or batched, run the same shapes and give the same bits.
+// GLUED_AUTOTUNE=0 in the environment keeps the defaults.
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_tuneMu;
+std::map<std::string, Engine::Tuning> g_tuneCache;
+}  // namespace
+
+std::string Engine::shapeKey() const
+{
+    std::string k = "W" + std::to_string(W_);
+    for (int n : natoms_) k += "," + std::to_string(n);
+    for (const auto& it : path_) k += ";path" + std::to_string(it.N) + "x" + std::to_string(it.M);
+    for (const auto& it : coord_)
+        k += ";coord" + std::to_string(it.nA) + "x" + std::to_string(it.nB) + ":" + std::to_string(it.useCells) + ":" +
+             std::to_string(it.nc[0]) + "," + std::to_string(it.nc[1]) + "," + std::to_string(it.nc[2]);
+    return k;
+}
+
+void Engine::applyTuning()
+{
+    if (!path_.empty()) planPathWork();
+    if (!coord_.empty()) {
+        for (auto& it : coord_) it.cellBlock = tune_.coordBlock;
+        dCoord_.upload(coord_);
+    }
+    dropGraphs();
+}
+
+void Engine::adoptCachedTuning()
+{
+    std::lock_guard<std::mutex> lock(g_tuneMu);
+    auto f = g_tuneCache.find(shapeKey());
+    if (f == g_tuneCache.end()) return;
+    tune_ = f->second;
+    tuneHow_ = "cached";
+    applyTuning();
+}
+
+void Engine::autotune(hipStream_t s)
+{
+    if (tuned_) return;
+    tuned_ = true;
+    if (evaluated_ || capturing_) return;
+    const char* env = std::getenv("GLUED_AUTOTUNE");
+    if (env && env[0] == '0') return;
+    bool cells = false;
+    for (const auto& it : coord_) cells |= it.useCells != 0;
+    if (path_.empty() && !cells) return;
+    adoptCachedTuning();
+    if (tuneHow_ == "cached") return;
+    hipEvent_t e0, e1;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    auto timeCv = [&]() {
+        applyTuning();
+        for (int r = 0; r < 3; r++) launchStep(s, 0, W_, true, 1, true);
+        hip_check(hipEventRecord(e0, s), "event");
+        for (int r = 0; r < 8; r++) launchStep(s, 0, W_, true, 1, true);
+        hip_check(hipEventRecord(e1, s), "event");
+        hip_check(hipEventSynchronize(e1), "event");
+        float ms = 0.0f;
+        hip_check(hipEventElapsedTime(&ms, e0, e1), "event");
+        return 1000.0 * ms / 8.0;  // us per CV stage
+    };
+    std::string log;
+    auto search = [&](int Tuning::*field, std::initializer_list<int> values, const char* name) {
+        int best = tune_.*field;
+        double bestT = 1e300;
+        log += std::string(name) + ":";
+        for (int val : values) {
+            tune_.*field = val;
+            const double t = timeCv();
+            char b[48];
+            std::snprintf(b, sizeof(b), " %d=%.1fus", val, t);
+            log += b;
+            if (t < bestT) {
+                bestT = t;
+                best = val;
+            }
+        }
+        tune_.*field = best;
+        log += "; ";
+    };
+    if (cells) search(&Tuning::coordBlock, {256, 512, 1024}, "coordBlock");
+    if (!path_.empty()) {
+        search(&Tuning::pathBlocks, {128, 256, 512, 1024}, "pathBlocks");
+        search(&Tuning::pathPhases, {4, 8}, "pathPhases");
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    applyTuning();
+    tuneHow_ = "measured";
+    tuneLog_ = log;
+    std::lock_guard<std::mutex> lock(g_tuneMu);
+    g_tuneCache[shapeKey()] = tune_;
+}
+
+std::string Engine::tuningReport() const
+{
+    return "pathBlocks=" + std::to_string(tune_.pathBlocks) + " pathPhases=" + std::to_string(tune_.pathPhases) +
+           " coordBlock=" + std::to_string(tune_.coordBlock) + " (" + tuneHow_ + ")" +
+           (tuneLog_.empty() ? "" : " " + tuneLog_);
+}
+
 void Engine::uploadGidx(int w)
 {
     std::vector<int> g(std::max(S_, 1), 0);
@@ -1197,7 +1336,7 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         launch_cv_group_big(v, dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupPos_.as<int>(),
                             dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
         launch_path_rows(v, dPath_.as<PathItem>(), (int)path_.size(), pathRowBlocks_, pathMaxN_, dPathPos_.as<int>(),
-                         dPathFrames_.as<float>(), w0, nw, s);
+                         dPathFrames_.as<float>(), w0, nw, tune_.pathPhases, s);
     } else {
         launch_cv_small(v, dSmall_.as<SmallCVItem>(), (int)small_.size(), w0, nw, s);
         launch_cv_group(v, dGroup_.as<GroupCVItem>(), (int)group_.size(), dGroupPos_.as<int>(),
@@ -1206,7 +1345,7 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         launch_cv_group_big(v, dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupPos_.as<int>(),
                             dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
         launch_cv_path(v, dPath_.as<PathItem>(), (int)path_.size(), pathMsdUnits_, pathRowBlocks_, pathMaxN_,
-                       dPathPos_.as<int>(), dPathFrames_.as<float>(), w0, nw, s);
+                       dPathPos_.as<int>(), dPathFrames_.as<float>(), w0, nw, tune_.pathPhases, s);
     }
     if (!coord_.empty()) {
         CoordBuffers cb;
@@ -1323,8 +1462,13 @@ double Engine::execute(hipStream_t s, int w, const float4* posq, long long* forc
     // posq / force point at this walker's block; present them as the batch base
     lastPos_ = posq - posOff_[w];
     lastForce_ = force - forceOff_[w];
+    if (!tuned_) {  // a one-walker first evaluation: no timing, a cached choice if there is one
+        tuned_ = true;
+        if (!evaluated_) adoptCachedTuning();
+    }
     launchStep(s, w, 1, forces);
     cvReady_ = true;
+    evaluated_ = true;
     double e = 0.0;
     if (energy && testGrads_.empty() && !biases_.empty()) {
         std::vector<double> be((size_t)B_);
@@ -1371,6 +1515,11 @@ void Engine::executeStage(hipStream_t s, const float4* posq, long long* force, i
         hip_check(hipGetLastError(), "kernel launch");
         return;
     }
+    if (!tuned_) {
+        tuned_ = true;
+        if (!evaluated_) adoptCachedTuning();
+    }
+    evaluated_ = true;
     launchStep(s, 0, W_, true, 1 << stage);
 }
 
@@ -1382,6 +1531,8 @@ void Engine::executeBatch(hipStream_t s, const float4* posq, long long* force, b
     if (V_ == 0) return;
     lastPos_ = posq;
     lastForce_ = force;
+    autotune(s);
+    evaluated_ = true;
     if (!useGraph) {
         launchStep(s, 0, W_, forces, 7, true);
         cvReady_ = true;
@@ -1559,6 +1710,8 @@ void Engine::stepBatch(hipStream_t s, int step, const float4* posq, long long* f
     }
     lastPos_ = posq;
     lastForce_ = force;
+    autotune(s);
+    evaluated_ = true;
     if (stepGraphPos_ != posq || stepGraphForce_ != force || stepGraphForces_ != forces) {
         for (auto& kv : stepGraphs_) {
             (void)hipGraphExecDestroy(kv.second.second);

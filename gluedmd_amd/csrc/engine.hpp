@@ -130,6 +130,14 @@ class Engine {
     void setHills(int w, int bias, const double* centers, const double* heights, int n);
     int getHills(int w, int bias, double* centers, double* heights, int cap);
     void setGrid(int w, int bias, const double* grid, int n);
+    // Launch shapes picked by the autotuner (path MSD blocks per batch, path
+    // row phases, coordination cell workgroup size) and how they were picked.
+    struct Tuning {
+        int pathBlocks = 512;  // target MSD blocks per path over the batch
+        int pathPhases = 0;    // 4 or 8 waves per path row block (0 until planned)
+        int coordBlock = 512;  // threads per coordination cell workgroup
+    };
+    std::string tuningReport() const;
     int numWalkers() const { return W_; }
     int jacobianRows() const { return R_; }
     int atomSlots() const { return S_; }
@@ -137,6 +145,14 @@ class Engine {
 
    private:
     void validate(int minAtoms) const;
+    Tuning tune_;
+    bool tuned_ = false, evaluated_ = false;
+    std::string tuneHow_ = "default", tuneLog_;
+    void planPathWork();
+    void applyTuning();
+    void autotune(hipStream_t s);
+    void adoptCachedTuning();
+    std::string shapeKey() const;
     void buildPlan(int nMass, const double* masses);
     void launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMask = 7, bool fused = false);
     StepView view(const float4* posq, long long* force) const;

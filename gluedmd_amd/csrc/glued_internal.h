@@ -128,6 +128,7 @@ struct CoordItem {
     // switching constants formed once on the host (FP64, unfused):
     // 1/r0 and the Taylor coefficients used for |r/r0 - 1| < 1e-4
     double ir0, preRes, preDfunc, preSecDev;
+    int cellBlock;        // threads per cell workgroup (256, 512, 1024: a tuned launch shape)
     double ir0sq;         // 1/r0^2
     int rat612;           // n = 6, m = 12: the switch is exactly 1/(1 + x^6)
     int value;

@@ -1133,7 +1133,7 @@ __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBu
     }
 }
 
-// Cell-blocked pair pass (cell list): one 512-thread workgroup per home
+// Cell-blocked pair pass (cell list): one workgroup per home
 // cell of either group -- workgroups [0, ncell) take the group-A atoms of a
 // cell and form CN and the A rows, workgroups [ncell, 2 ncell) the group-B
 // atoms and the B rows -- so each pass visits every in-cutoff pair once, and
@@ -1154,30 +1154,34 @@ __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBu
 // count are summed per workgroup and added with 64-bit atomics; the last
 // group-A workgroup of a walker to finish (ticket after a fence) converts
 // CN, stores it and the pair count, and clears the accumulators.
-constexpr int COORD_CB = 512;
-constexpr int COORD_CB_WAVES = COORD_CB / 64;
+// The workgroup size CB (256, 512 or 1,024 threads) is a launch-shape choice
+// of the autotuner; every size gives the same pairs and the same exact
+// integer sums, the FP64 per-lane partial sums differ in rounding only.
 constexpr int COORD_PCAP = 3072;  // partners staged in LDS per chunk
 constexpr int COORD_HCAP = 256;   // homes staged in LDS per chunk
 
+template <int CB>
 struct CoordCellShared {
+    static constexpr int WAVES = CB / 64;
     float4 part[COORD_PCAP];
     float4 home[COORD_HCAP];
     long long g[COORD_HCAP][3];
-    short q[COORD_CB_WAVES][128];
+    short q[WAVES][128];
     int cellB[27], pre[28];
     int homeB, homeN;
-    unsigned long long lo[COORD_CB_WAVES], np[COORD_CB_WAVES];
-    long long hi[COORD_CB_WAVES];
+    unsigned long long lo[WAVES], np[WAVES];
+    long long hi[WAVES];
 };
 
 // One home cell.  SIDE_A: homes are group-A atoms (a task is one home, or
 // one of nSub interleaved ranges of its partner blocks); otherwise group-B
 // atoms, two homes per task, one per half-wave (a B atom has few partners,
 // so a task per home would end in a mostly empty evaluation round).
-template <bool SIDE_A>
+template <bool SIDE_A, int CB>
 __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const CoordItem& it, int item, int w, int hc,
-                           CoordCellShared& sh)
+                           CoordCellShared<CB>& sh)
 {
+    constexpr int COORD_CB = CB, COORD_CB_WAVES = CB / 64;
     constexpr int HPT = SIDE_A ? 1 : 2;  // homes per task
     constexpr int LPH = 64 / HPT;        // lanes per home
     const int nx = it.nc[0], ny = it.nc[1], nz = it.nc[2], ncell = nx * ny * nz;
@@ -1408,15 +1412,16 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
     }
 }
 
-__global__ __launch_bounds__(COORD_CB) void k_coord_cells(StepView v, CoordBuffers cb,
-                                                          const CoordItem* __restrict__ items, int item, int w0)
+template <int CB>
+__global__ __launch_bounds__(CB) void k_coord_cells(StepView v, CoordBuffers cb, const CoordItem* __restrict__ items,
+                                                    int item, int w0)
 {
-    __shared__ CoordCellShared sh;
+    __shared__ CoordCellShared<CB> sh;
     const int w = w0 + blockIdx.y;
     const CoordItem it = items[item];
     const int ncell = it.nc[0] * it.nc[1] * it.nc[2];
-    if ((int)blockIdx.x < ncell) coord_cell<true>(v, cb, it, item, w, blockIdx.x, sh);
-    else coord_cell<false>(v, cb, it, item, w, blockIdx.x - ncell, sh);
+    if ((int)blockIdx.x < ncell) coord_cell<true, CB>(v, cb, it, item, w, blockIdx.x, sh);
+    else coord_cell<false, CB>(v, cb, it, item, w, blockIdx.x - ncell, sh);
 }
 
 void launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordItem* dItems, const CoordItem& hIt,
@@ -1441,7 +1446,13 @@ void launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordI
     const dim3 gA((wavesA + 3) / 4, nw), gB((wavesB + 3) / 4, nw);
     if (hIt.useCells) {
         const int ncell = hIt.nc[0] * hIt.nc[1] * hIt.nc[2];
-        hipLaunchKernelGGL(k_coord_cells, dim3(2 * ncell, nw), dim3(COORD_CB), 0, s, v, cb, dItems, item, w0);
+        const dim3 g(2 * ncell, nw);
+        if (hIt.cellBlock == 256)
+            hipLaunchKernelGGL(k_coord_cells<256>, g, dim3(256), 0, s, v, cb, dItems, item, w0);
+        else if (hIt.cellBlock == 1024)
+            hipLaunchKernelGGL(k_coord_cells<1024>, g, dim3(1024), 0, s, v, cb, dItems, item, w0);
+        else
+            hipLaunchKernelGGL(k_coord_cells<512>, g, dim3(512), 0, s, v, cb, dItems, item, w0);
     } else {
         hipLaunchKernelGGL((k_coord_pairs<true>), gA, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
         hipLaunchKernelGGL((k_coord_pairs<false>), gB, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
@@ -1817,11 +1828,14 @@ void launch_path_frames_adjoint(const StepView& v, const PathItem* items, int pi
 // many blocks are resident).  Up to 256 frames both forms form w_f, S and s
 // with the same additions (a thread per frame, the waves past the frames add
 // zeros), so the adjoint's coefficients match either.
+// phases: 4 or 8 waves per row block (the autotuner's choice; 0 = 4 when the
+// blocks exceed one residency round and the paths have at most 256 frames)
 void launch_path_rows(const StepView& v, const PathItem* items, int n, int nRowBlocks, int maxN, const int* slots,
-                      const float* frames, int w0, int nw, hipStream_t s)
+                      const float* frames, int w0, int nw, int phases, hipStream_t s)
 {
     if (n <= 0) return;
-    if ((long long)nRowBlocks * nw > 512 && maxN <= 256)
+    if (phases == 0) phases = (long long)nRowBlocks * nw > 512 && maxN <= 256 ? 4 : PATH_PHASES;
+    if (phases == 4)
         hipLaunchKernelGGL(k_path_rows<4>, dim3(nRowBlocks, nw), dim3(64 * 4), 0, s, v, items, n, slots, frames, w0);
     else
         hipLaunchKernelGGL(k_path_rows<PATH_PHASES>, dim3(nRowBlocks, nw), dim3(64 * PATH_PHASES), 0, s, v, items, n,
@@ -1829,12 +1843,12 @@ void launch_path_rows(const StepView& v, const PathItem* items, int n, int nRowB
 }
 
 void launch_cv_path(const StepView& v, const PathItem* items, int n, int nMsdBlocks, int nRowBlocks, int maxN,
-                    const int* slots, const float* frames, int w0, int nw, hipStream_t s)
+                    const int* slots, const float* frames, int w0, int nw, int phases, hipStream_t s)
 {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_path_msd, dim3(nMsdBlocks, (nw + PATH_TW - 1) / PATH_TW), dim3(PATH_MSD_THREADS), 0, s, v,
                        items, n, slots, frames, w0, nw);
-    launch_path_rows(v, items, n, nRowBlocks, maxN, slots, frames, w0, nw, s);
+    launch_path_rows(v, items, n, nRowBlocks, maxN, slots, frames, w0, nw, phases, s);
 }
 
 // ---------------------------------------------------------------------------

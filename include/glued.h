@@ -213,6 +213,13 @@ int glued_set_path_frames(glued_handle h, int pathIdx, const double* frames, int
  * counterpart (the reference has no frame derivatives). */
 int glued_path_frames_adjoint(glued_handle h, void* stream, int pathIdx, const double* dGradV, double* dGradFrames);
 
+/* Launch shapes chosen by the autotuner (no reference counterpart): the
+ * path MSD block target, the path row phases and the coordination cell
+ * workgroup size, how they were chosen (measured at the first batched
+ * evaluation, cached for the plan shape, or default) and the measured CV
+ * stage times; a NUL-terminated text of *len characters (buf may be NULL to
+ * ask for the length). */
+int glued_get_tuning(glued_handle h, char* buf, size_t cap, size_t* len);
 /* Plan sizes: Jacobian rows and distinct atoms of one walker. */
 int glued_plan_info(glued_handle h, int* rows, int* atoms);
 
