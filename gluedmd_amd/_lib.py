@@ -58,6 +58,8 @@ SIGNATURES = {
     "glued_hill_adjoint": [vp, vp, i, vp, vp, vp],
     "glued_hill_adjoint_noalloc": [vp, vp, i, vp, vp, vp],
     "glued_get_tuning": [vp, ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)],
+    "glued_opes_adjoint": [vp, vp, i, vp, vp, vp, vp, vp],
+    "glued_set_opes_kernels": [vp, i, i, c_double_p, c_double_p, c_double_p, i, ctypes.c_double],
     "glued_set_path_frames": [vp, i, vp, i],
     "glued_path_frames_adjoint": [vp, vp, i, vp, vp],
 }

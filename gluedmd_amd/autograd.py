@@ -20,6 +20,19 @@ the batch.  ``PathMetadFunction`` is its ``torch.autograd.Function``:
                           (k_path_frames_coef / k_path_frames_grad).
 If the engine was used for another evaluation between forward and backward,
 backward recomputes the forward first (nothing per pair is stored).
+
+``OpesPipeline`` evaluates one OPES bias (opesEvalBias,
+CommonGluedKernels.cpp:2642-2723) for a batch of configurations sharing one
+kernel table, and ``OpesFunction`` differentiates it with respect to the
+coordinates and the table itself.  With A = kT/invGF, P_b = sum_k h_k
+(G_bk - eps_c)[n2_bk < cutoff2], h_k = exp(logw_k), G_bk = exp(-n2_bk/2),
+n2_bk = sum_d (s_bd - c_kd)^2/sigma_kd^2 and V_b = A log(P_b n/U + eps):
+  dV_b/dlogw_k  = F_b h_k (G_bk - eps_c),  F_b = A n/(U (P_b n/U + eps));
+  dV_b/dc_kd    = F_b h_k G_bk (s_bd - c_kd)/sigma_kd^2;
+  dV_b/dsigma_kd = F_b h_k G_bk (s_bd - c_kd)^2/sigma_kd^3;
+  dV_b/dU       = -A (P_b n/U)/(U (P_b n/U + eps)),
+U = sum_uprob, held as the stored input it is in the evaluation
+(k_opes_adjoint).
 """
 import ctypes
 
@@ -153,3 +166,98 @@ class PathMetadFunction(torch.autograd.Function):
             call("glued_path_frames_adjoint", e.h, e.stream, 0, _p(g), _p(gf))
             grad_fr = gf.view(pipe.N, pipe.M, 3).to(frames.dtype).to(frames.device)
         return grad_x, grad_h, grad_lam, grad_sig, grad_fr, None
+
+
+class OpesPipeline:
+    """V_b of the OPES bias ``bias_index`` of ``force`` for B configurations of
+    ``natoms`` atoms (the walkers of one engine, one kernel table for all).
+    The force holds this one bias; its CVs are any the engine evaluates."""
+
+    def __init__(self, force, bias_index, natoms, batch, *, box=None, device="cuda:0"):
+        if force.getNumBiases() != 1:
+            raise ValueError("OpesPipeline: the force must hold exactly one bias, the OPES one")
+        if force.biases[bias_index][0] != force.BIAS_OPES:
+            raise ValueError("OpesPipeline: bias_index must name an OPES bias")
+        force.share_bias(bias_index)
+        self.force = force
+        self.bias = int(bias_index)
+        self.B, self.natoms = int(batch), int(natoms)
+        self.D = len(force.biases[bias_index][1])
+        self.cap = int(force.biases[bias_index][3][2])
+        self.ctx = Context(force, walkers=[self.natoms] * self.B, device=device)
+        if box is not None:
+            self.ctx.set_box(box)
+        self.device = self.ctx.device
+        self.generation = 0
+        self.n = 0
+
+    def __call__(self, x, centers, sigmas, logw, sum_uprob):
+        """x [B, natoms, 3]; centers, sigmas [n, D]; logw [n]; sum_uprob
+        scalar.  Every argument may require grad."""
+        return OpesFunction.apply(x, centers, sigmas, logw, sum_uprob, self)
+
+    def _evaluate(self, x, centers, sigmas, logw, sum_uprob):
+        e = self.ctx
+        n = int(logw.shape[0])
+        if n > self.cap:
+            raise ValueError(f"OpesPipeline: {n} kernels exceed the capacity {self.cap}")
+        c = np.ascontiguousarray(centers.detach().to("cpu", torch.float64).numpy().reshape(n, self.D))
+        sg = np.ascontiguousarray(sigmas.detach().to("cpu", torch.float64).numpy().reshape(n, self.D))
+        lw = np.ascontiguousarray(logw.detach().to("cpu", torch.float64).numpy())
+        e.set_opes_kernels(self.bias, c, sg, lw, float(sum_uprob))
+        self.n = n
+        P = e.padded[0]
+        e.posq.view(self.B, P, 4)[:, :self.natoms, :3].copy_(x.detach().to(torch.float32))
+        e.zero_forces()
+        e.execute_batch(use_graph=True)
+        self.generation += 1
+        return self.generation
+
+    def energies(self):
+        out = torch.empty(self.B, dtype=torch.float64, device=self.device)
+        call("glued_copy_buffer", self.ctx.h, 2, _p(out), out.numel() * 8, self.ctx.stream)
+        return out
+
+
+class OpesFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x, centers, sigmas, logw, sum_uprob, pipe):
+        gen = pipe._evaluate(x, centers, sigmas, logw, sum_uprob)
+        V = pipe.energies().clone()
+        ctx.pipe, ctx.gen = pipe, gen
+        ctx.u_shape = sum_uprob.shape if torch.is_tensor(sum_uprob) else None
+        ctx.u_device = sum_uprob.device if torch.is_tensor(sum_uprob) else None
+        ctx.save_for_backward(x.detach(), centers.detach(), sigmas.detach(), logw.detach(),
+                              torch.as_tensor(float(sum_uprob), dtype=torch.float64))
+        return V
+
+    @staticmethod
+    def backward(ctx, gV):
+        pipe = ctx.pipe
+        x, centers, sigmas, logw, su = ctx.saved_tensors
+        if pipe.generation != ctx.gen:  # the engine moved on: recompute this forward
+            pipe._evaluate(x, centers, sigmas, logw, su)
+        e = pipe.ctx
+        B, N, P, D, n, cap = pipe.B, pipe.natoms, e.padded[0], pipe.D, pipe.n, pipe.cap
+        g = gV.detach().to(torch.float64).contiguous()
+        grad_x = grad_c = grad_s = grad_l = grad_u = None
+        if ctx.needs_input_grad[0]:
+            F = e.forces.view(B, 3, P)[:, :, :N].to(torch.float64) / FIXED_SCALE  # = -dV/dx
+            grad_x = (-F.transpose(1, 2) * g[:, None, None]).to(x.dtype)
+        if any(ctx.needs_input_grad[1:5]):
+            gc = torch.empty(cap * D, dtype=torch.float64, device=pipe.device)
+            gs = torch.empty(cap * D, dtype=torch.float64, device=pipe.device)
+            gl = torch.empty(cap, dtype=torch.float64, device=pipe.device)
+            gu = torch.empty(1, dtype=torch.float64, device=pipe.device)
+            call("glued_opes_adjoint", e.h, e.stream, pipe.bias, _p(g), _p(gc), _p(gs), _p(gl), _p(gu))
+            if ctx.needs_input_grad[1]:
+                grad_c = gc.view(cap, D)[:n].reshape(centers.shape).to(centers.dtype).to(centers.device)
+            if ctx.needs_input_grad[2]:
+                grad_s = gs.view(cap, D)[:n].reshape(sigmas.shape).to(sigmas.dtype).to(sigmas.device)
+            if ctx.needs_input_grad[3]:
+                grad_l = gl[:n].to(logw.dtype).to(logw.device)
+            if ctx.needs_input_grad[4]:
+                grad_u = gu[0]
+                if ctx.u_shape is not None:
+                    grad_u = grad_u.reshape(ctx.u_shape).to(ctx.u_device)
+        return grad_x, grad_c, grad_s, grad_l, grad_u, None

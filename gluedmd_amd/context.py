@@ -286,6 +286,14 @@ class Context:
         g, gp = _f64(np.asarray(grid).reshape(-1))
         call("glued_set_grid", self.h, int(walker), int(bias_index), gp, len(g))
 
+    def set_opes_kernels(self, bias_index, centers, sigmas, logw, sum_uprob, walker=0):
+        """An OPES kernel table (kernel-major centres and sigmas [n][D],
+        log-weights [n]) and its sum_uprob."""
+        c, cp = _f64(np.asarray(centers, dtype=np.float64).reshape(-1))
+        s, sp = _f64(np.asarray(sigmas, dtype=np.float64).reshape(-1))
+        lw, lp = _f64(logw)
+        call("glued_set_opes_kernels", self.h, int(walker), int(bias_index), cp, sp, lp, len(lw), float(sum_uprob))
+
     def tuning(self):
         """Launch shapes the autotuner chose for this plan, and how."""
         n = ctypes.c_size_t()

@@ -318,6 +318,20 @@ int glued_hill_adjoint_noalloc(glued_handle h, void* stream, int biasIdx, const 
     });
 }
 
+int glued_opes_adjoint(glued_handle h, void* stream, int biasIdx, const double* dGradV, double* dGradCenters,
+                       double* dGradSigmas, double* dGradLogW, double* dGradSumUprob)
+{
+    return guard([&] {
+        E(h).opesAdjoint((hipStream_t)stream, biasIdx, dGradV, dGradCenters, dGradSigmas, dGradLogW, dGradSumUprob);
+    });
+}
+
+int glued_set_opes_kernels(glued_handle h, int walker, int biasIdx, const double* centers, const double* sigmas,
+                           const double* logw, int n, double sumUprob)
+{
+    return guard([&] { E(h).setOpesKernels(walker, biasIdx, centers, sigmas, logw, n, sumUprob); });
+}
+
 int glued_set_path_frames(glued_handle h, int pathIdx, const double* frames, int n)
 {
     return guard([&] { E(h).setPathFrames(pathIdx, frames, n); });

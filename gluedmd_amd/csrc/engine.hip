@@ -32,6 +32,8 @@ void launch_dep_grid(const StepView&, const GridBiasDev&, int, int, hipStream_t)
 void launch_dep_pb(const StepView&, const SmallBiasState&, const SmallBiasItem&, int, int, hipStream_t);
 void launch_dep_hills(const StepView&, const HillBiasDev&, int, int, hipStream_t);
 void launch_f64_to_f32(const double*, float*, int, hipStream_t);
+void launch_opes_adjoint(const StepView&, const OpesBiasDev&, const double*, double*, double*, double*, double*, int,
+                         hipStream_t);
 void launch_dep_opes(const StepView&, const OpesBiasDev&, int, int, hipStream_t);
 void launch_opes_welford(const StepView&, const OpesBiasDev&, int, int, hipStream_t);
 void launch_dep_small(const StepView&, const SmallBiasState&, const SmallBiasItem*, int, int, int, int, hipStream_t);
@@ -1957,6 +1959,57 @@ long long Engine::coordinationPairs(int w, int ci)
     if (ci < 0 || ci >= (int)coord_.size()) fail("coordination_pairs: index out of range");
     hip_check(hipDeviceSynchronize(), "sync");
     return download(dNpairs_.as<long long>() + (size_t)w * coord_.size() + ci, 1)[0];
+}
+
+// Adjoint of an OPES bias with respect to its kernel table (centres,
+// widths, log-weights) and sum_uprob, at the CV values and energies of the
+// last evaluation; g = dL/dV_w on the device, [W].
+void Engine::opesAdjoint(hipStream_t s, int bi, const double* gV, double* gC, double* gS, double* gL, double* gU)
+{
+    if (!initialized_ || !cvReady_) fail("opes_adjoint: evaluate first");
+    OpesHost* o = nullptr;
+    for (auto* x : opes_)
+        if (x->spec == bi) o = x;
+    if (!o) fail("opes_adjoint: not an OPES bias");
+    StepView v = view(lastPos_, lastForce_);
+    v.S = S_;
+    launch_opes_adjoint(v, o->d, gV, gC, gS, gL, gU, W_, s);
+    hip_check(hipGetLastError(), "opes_adjoint launch");
+}
+
+// A kernel table set directly (kernel-major centres and widths [n][D],
+// log-weights [n]) with its sum_uprob; the count and the slot-claim counter
+// follow n, the other accumulators are kept.
+void Engine::setOpesKernels(int w, int bi, const double* centers, const double* sigmas, const double* logw, int n,
+                            double sumUprob)
+{
+    if (!initialized_) fail("set_opes_kernels: initialize first");
+    OpesHost* o = nullptr;
+    for (auto* x : opes_)
+        if (x->spec == bi) o = x;
+    if (!o) fail("set_opes_kernels: not an OPES bias");
+    if (w < 0 || w >= W_) fail("set_opes_kernels: walker out of range");
+    if (n < 0 || n > o->d.cap) fail("set_opes_kernels: kernel count exceeds the capacity");
+    hip_check(hipDeviceSynchronize(), "sync");  // no evaluation in flight reads the table
+    const int t = o->d.shared ? 0 : w, D = o->d.D;
+    std::vector<double> cs((size_t)D * o->d.cap, 0.0), ss((size_t)D * o->d.cap, 1.0);
+    for (int k = 0; k < n; k++)
+        for (int d = 0; d < D; d++) {
+            cs[(size_t)d * o->d.cap + k] = centers[(size_t)k * D + d];
+            ss[(size_t)d * o->d.cap + k] = sigmas[(size_t)k * D + d];
+        }
+    hip_check(hipMemcpy(o->d.centers + (size_t)t * D * o->d.cap, cs.data(), 8 * cs.size(), hipMemcpyHostToDevice),
+              "set_opes_kernels");
+    hip_check(hipMemcpy(o->d.sigmas + (size_t)t * D * o->d.cap, ss.data(), 8 * ss.size(), hipMemcpyHostToDevice),
+              "set_opes_kernels");
+    if (n) hip_check(hipMemcpy(o->d.logw + (size_t)t * o->d.cap, logw, 8 * (size_t)n, hipMemcpyHostToDevice),
+                     "set_opes_kernels");
+    auto I = download(o->d.ints + 4 * t, 4);
+    I[0] = n;
+    I[1] = n;
+    hip_check(hipMemcpy(o->d.ints + 4 * t, I.data(), 16, hipMemcpyHostToDevice), "set_opes_kernels");
+    hip_check(hipMemcpy(o->d.dbl + (size_t)t * (4 + 2 * D), &sumUprob, 8, hipMemcpyHostToDevice), "set_opes_kernels");
+    o->depositMirror[t] = n;
 }
 
 // getOPESMetrics, CommonGluedKernels.cpp:5164-5210: [zed, rct, nker, neff].

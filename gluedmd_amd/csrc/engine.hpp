@@ -116,6 +116,9 @@ class Engine {
     // doubles of per-block partials hillAdjoint needs for this bias
     size_t hillAdjointPartials(int bias) const;
     void setPathFrames(int path, const double* frames, int n);
+    void opesAdjoint(hipStream_t s, int bias, const double* gV, double* gC, double* gS, double* gL, double* gU);
+    void setOpesKernels(int w, int bias, const double* centers, const double* sigmas, const double* logw, int n,
+                        double sumUprob);
     void pathFramesAdjoint(hipStream_t s, int path, const double* gV, double* gF);
 
     // ---- read-backs ----

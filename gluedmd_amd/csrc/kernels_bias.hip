@@ -1590,4 +1590,120 @@ void launch_f64_to_f32(const double* src, float* dst, int n, hipStream_t s)
     if (n > 0) hipLaunchKernelGGL(k_f64_to_f32, dim3((n + 255) / 256), dim3(256), 0, s, src, dst, n);
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of an OPES bias with respect to its kernel parameters (an
+// extension: the reference has no parameter derivatives).  For the eval of
+// CommonGluedKernels.cpp:2642-2723, V = A log(max(pz + eps, 1e-30)),
+// A = invGF kT, pz = P n / U, P = sum_k h_k (G_k - Gfloor) over the kernels
+// it includes (n2 < cutoff2, G_k > Gfloor, finite h_k):
+//   F_w            = g_w dV_w/dP = g_w A n / (U (pz + eps))   (pz / P = n / U),
+//                    with pz + eps = exp(V_w / A) from the walker's last V,
+//   dL/dlogw_k    += F_w h_k (G_k - Gfloor),
+//   dL/dc_kd      += F_w h_k G_k (s_d - c_kd) / sigma_kd^2,
+//   dL/dsigma_kd  += F_w h_k G_k (s_d - c_kd)^2 / sigma_kd^3,
+//   dL/dU         += -g_w A pz / (U (pz + eps))   (sum_uprob, the stored normaliser),
+// summed over the walkers that read the table; zero where the eval's
+// log-argument floor or the no-kernel branch applies.  One thread per kernel,
+// the walkers' CV values and factors staged in LDS, walkers in index order.
+// Outputs: gC, gS [T][cap][D] kernel-major, gL [T][cap], gU [T].
+constexpr int OPES_ADJ_BLOCK = 256;
+
+template <int D>
+__global__ __launch_bounds__(OPES_ADJ_BLOCK) void k_opes_adjoint(StepView v, OpesBiasDev o,
+                                                                 const double* __restrict__ gV, double* gC,
+                                                                 double* gS, double* gL, double* gU, int W)
+{
+#pragma clang fp contract(off)
+    __shared__ double sS[OPES_ADJ_BLOCK][D];
+    __shared__ double sF[OPES_ADJ_BLOCK], sFU[OPES_ADJ_BLOCK];
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tb = o.shared ? 0 : blockIdx.y;
+    const int nk = o.ints[tb * 4 + 0];
+    const double su = o.dbl[(size_t)tb * (4 + 2 * D) + 0];
+    const double A = o.invGF * o.kT;
+    const double kfloor = exp(-0.5 * o.cutoff2);
+    const bool ok = k < nk;
+    const int kc = min(k, max(nk - 1, 0));
+    double c[D], sg[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        c[d] = o.centers[((size_t)tb * D + d) * o.cap + kc];
+        sg[d] = o.sigmas[((size_t)tb * D + d) * o.cap + kc];
+    }
+    const double lw = o.logw[(size_t)tb * o.cap + kc];
+    double gc[D], gs[D], gl = 0.0, gu = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) gc[d] = gs[d] = 0.0;
+    const int wBegin = o.shared ? 0 : blockIdx.y;
+    const int wEnd = o.shared ? W : blockIdx.y + 1;
+    for (int w0 = wBegin; w0 < wEnd; w0 += OPES_ADJ_BLOCK) {
+        const int m = min(OPES_ADJ_BLOCK, wEnd - w0);
+        __syncthreads();
+        if ((int)threadIdx.x < m) {
+            const int w = w0 + threadIdx.x;
+            const double* cv = v.cvVal + (size_t)w * v.V;
+#pragma unroll
+            for (int d = 0; d < D; d++) sS[threadIdx.x][d] = cv[o.cv[d]];
+            double F = 0.0, FU = 0.0;
+            if (nk > 0 && su > 0.0) {
+                const double varg = exp(v.benergy[(size_t)w * v.B + o.eslot] / A);  // pz + eps of the last eval
+                if (varg > 1e-30 * (1.0 + 1e-9)) {
+                    F = gV[w] * A * (double)nk / (su * varg);
+                    FU = -gV[w] * A * (varg - o.eps) / (su * varg);
+                }
+            }
+            sF[threadIdx.x] = F;
+            sFU[threadIdx.x] = FU;
+        }
+        __syncthreads();
+        if (blockIdx.x == 0 && threadIdx.x == 0)
+            for (int j = 0; j < m; j++) gu += sFU[j];
+        if (!ok) continue;
+        for (int j = 0; j < m; j++) {
+            const double F = sF[j];
+            if (F == 0.0) continue;
+            double n2 = 0.0, dc[D];
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                dc[d] = sS[j][d] - c[d];
+                n2 = n2 + dc[d] * dc[d] / (sg[d] * sg[d]);
+            }
+            if (!(n2 < o.cutoff2)) continue;
+            const double G = exp(-0.5 * n2), kern = G - kfloor;
+            if (kern <= 0.0) continue;
+            const double hk = exp(lw);
+            if (!(hk > 0.0 && hk < 1e308)) continue;
+            const double t = F * hk;
+            gl += t * kern;
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                const double q = t * G * dc[d] / (sg[d] * sg[d]);
+                gc[d] += q;
+                gs[d] += q * dc[d] / sg[d];
+            }
+        }
+    }
+    if (k < o.cap) {
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            gC[((size_t)tb * o.cap + k) * D + d] = ok ? gc[d] : 0.0;
+            gS[((size_t)tb * o.cap + k) * D + d] = ok ? gs[d] : 0.0;
+        }
+        gL[(size_t)tb * o.cap + k] = ok ? gl : 0.0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) gU[tb] = gu;
+}
+
+void launch_opes_adjoint(const StepView& v, const OpesBiasDev& o, const double* gV, double* gC, double* gS,
+                         double* gL, double* gU, int W, hipStream_t s)
+{
+    const dim3 g((o.cap + OPES_ADJ_BLOCK - 1) / OPES_ADJ_BLOCK, o.shared ? 1 : W), b(OPES_ADJ_BLOCK);
+    switch (o.D) {
+        case 1: hipLaunchKernelGGL(k_opes_adjoint<1>, g, b, 0, s, v, o, gV, gC, gS, gL, gU, W); break;
+        case 2: hipLaunchKernelGGL(k_opes_adjoint<2>, g, b, 0, s, v, o, gV, gC, gS, gL, gU, W); break;
+        case 3: hipLaunchKernelGGL(k_opes_adjoint<3>, g, b, 0, s, v, o, gV, gC, gS, gL, gU, W); break;
+        default: hipLaunchKernelGGL(k_opes_adjoint<4>, g, b, 0, s, v, o, gV, gC, gS, gL, gU, W); break;
+    }
+}
+
 }  // namespace glued

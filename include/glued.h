@@ -203,6 +203,21 @@ int glued_hill_adjoint(glued_handle h, void* stream, int biasIdx, const double* 
  * that overlap on one GPU share the array and race. */
 int glued_hill_adjoint_noalloc(glued_handle h, void* stream, int biasIdx, const double* dGradV,
                                double* dGradHeights, double* dGradSigma);
+/* Adjoint of an OPES bias with respect to its kernel table (no reference
+ * counterpart; the function differentiated is opesEvalBias,
+ * CommonGluedKernels.cpp:2642-2723): given g_w = dL/dV_w (device, [W]) writes
+ * dL/d(centres) and dL/d(sigmas) (device, [T][cap][D] kernel-major),
+ * dL/d(log-weights) ([T][cap]) and dL/d(sum_uprob) ([T]), T = 1 shared or W
+ * private tables, for the CV values and energies of the last evaluation;
+ * sum_uprob is treated as the stored input it is in the eval. */
+int glued_opes_adjoint(glued_handle h, void* stream, int biasIdx, const double* dGradV, double* dGradCenters,
+                       double* dGradSigmas, double* dGradLogW, double* dGradSumUprob);
+/* The kernel table of an OPES bias set directly (host, kernel-major centres
+ * and sigmas [n][D], log-weights [n]) with its sum_uprob; kernel count and
+ * slot-claim counter follow n.  No reference counterpart (the reference loads
+ * tables only from the state blob, CommonSerialization.cpp:180-410). */
+int glued_set_opes_kernels(glued_handle h, int walker, int biasIdx, const double* centers, const double* sigmas,
+                           const double* logw, int n, double sumUprob);
 /* Reference frames of the pathIdx-th path CV (host, [N][M][3], stored as
  * FP32 like the CV's parameters); n = N*M*3.  No reference counterpart: the
  * reference fixes the frames at creation (CommonGluedKernels.cpp:4011-4024). */
