@@ -746,6 +746,14 @@ void Engine::buildPlan(int nMass, const double* masses)
             d.dbl = o->dbl.as<double>();
             d.part = o->part.as<double>();
             d.counter = o->counter.as<unsigned int>();
+            // two-launch deposit of private tables: the workgroups' partials,
+            // the change record and two tickets per table
+            o->depPart.alloc(sizeof(double) * (size_t)T * OPES_DEP_G * OPES_PART);
+            o->depRec.alloc(sizeof(double) * (size_t)T * OPES_REC);
+            o->depCount.alloc(sizeof(unsigned int) * 2 * T);
+            d.depPart = o->depPart.as<double>();
+            d.depRec = o->depRec.as<double>();
+            d.depCount = o->depCount.as<unsigned int>();
             opes_.push_back(o);
         } else if (s.type == BIAS_OPES_EXPANDED || s.type == BIAS_OPES_MULTITHERMAL) {
             SmallBiasItem it{};

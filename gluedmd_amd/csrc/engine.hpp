@@ -208,7 +208,7 @@ class Engine {
         int pace, spec;
         std::vector<int> depositMirror;  // per table, the reference's CPU numKernels mirror
         std::vector<int> nSamplesMirror;
-        DBuf centers, sigmas, logw, ints, dbl, part, counter;
+        DBuf centers, sigmas, logw, ints, dbl, part, counter, depPart, depRec, depCount;
     };
     struct SmallHost {
         int spec, pace, type;

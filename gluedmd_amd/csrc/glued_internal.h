@@ -204,9 +204,20 @@ struct HillBiasDev {
     int chunks;               // blocks per walker in the eval launch
     double* part;             // [W][chunks][D+1]
     unsigned int* counter;    // [W]
+    double* depPart;          // [T][workgroups][OPES_PART]: deposit partials (private tables)
+    double* depRec;           // [T][record]: the deposit's change, for the sum_uprob launch
+    unsigned int* depCount;   // [T][2]: deposit tickets
 };
 
 // OPES (BIAS_OPES).  CommonBiasKernels.cpp:489-664.
+// Private OPES tables deposit over OPES_DEP_G workgroups in two launches:
+// per workgroup a partial of OPES_PART doubles (best n2, its index, that
+// kernel's centre, width and log-weight), and a record of OPES_REC doubles
+// per table carries the change between the launches.
+constexpr int OPES_DEP_G = 16;
+constexpr int OPES_PART = 3 + 2 * MAX_BIAS_D;
+constexpr int OPES_REC = 6 + 6 * MAX_BIAS_D;
+
 struct OpesBiasDev {
     int D, cv[MAX_BIAS_D], gslot, eslot;
     int variant, adaptiveStride, cap, shared, chunks;
@@ -219,6 +230,9 @@ struct OpesBiasDev {
     double* dbl;              // [T][4 + 2D] = sum_uprob, sumWeights, sum_w, sum_w2, mean[D], M2[D]
     double* part;             // [W][chunks][D+1]
     unsigned int* counter;    // [W]
+    double* depPart;          // [T][workgroups][OPES_PART]: deposit partials (private tables)
+    double* depRec;           // [T][record]: the deposit's change, for the sum_uprob launch
+    unsigned int* depCount;   // [T][2]: deposit tickets
 };
 
 // Few-parameter biases evaluated by one thread (grid MetaD eval, restraints,

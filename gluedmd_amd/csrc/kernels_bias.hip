@@ -1151,14 +1151,38 @@ void launch_dep_hills(const StepView& v, const HillBiasDev& h, int w0, int nw, h
 // O(N) on both append and merge: on a merge the old row and column of the
 // merged kernel are subtracted and the new ones added, which equals the
 // reference's O(N^2) recomputation up to FP64 rounding.
+//
+// Latency layout.  The table's first opes_lds_kernels<D>() kernels (centres,
+// widths, log-weights; 4,000 at D = 2) are staged in LDS once per launch and
+// kept in step with every merge and append, so the search, the merge and the
+// sum_uprob update of a walker read no global memory; kernels past that are
+// read from global memory where needed.  The scalar recursion (weights,
+// n_eff, widths, height) and the merge arithmetic are formed redundantly by
+// every thread from the same values, held in registers for the whole launch,
+// so no broadcast is needed; thread 0 alone stores.  Two barriers per walker.
+// The arithmetic and the summation order (kernel index ascending per thread,
+// waves in order) are those of the one-lane scalar form, so results are
+// unchanged.
 // ---------------------------------------------------------------------------
-constexpr int OPES_DEP_BLOCK = 1024;
+constexpr int OPES_DEP_BLOCK = 512;
+constexpr int OPES_DEP_LDS_BYTES = 160 * 1024 - 1024;  // the table stage; the rest holds the reductions
 
-__device__ inline double opes_T(const double* ci, const double* ck, const double* sk, double hk, int D, double cutoff2,
-                                double epsd)
+template <int D>
+constexpr int opes_lds_kernels()
+{
+    return OPES_DEP_LDS_BYTES / (8 * (2 * D + 1));
+}
+
+// exp(-n2/2) - eps, scaled by the kernel height, for n2 below the cutoff
+// (partial sums of n2 only grow, so stopping at the first one past the
+// cutoff is the reference's test on the full sum).
+template <int D>
+__device__ __forceinline__ double opes_T(const double* ci, const double* ck, const double* sk, double hk,
+                                         double cutoff2, double epsd)
 {
 #pragma clang fp contract(off)
     double n2 = 0.0;
+#pragma unroll
     for (int d = 0; d < D; d++) {
         double dc = ci[d] - ck[d];
         n2 = n2 + dc * dc / (sk[d] * sk[d]);
@@ -1167,16 +1191,31 @@ __device__ inline double opes_T(const double* ci, const double* ck, const double
     return hk * (exp(-0.5 * n2) - epsd);
 }
 
+// The same term with the kernel height given as its logarithm; the height
+// exp(lk) is formed only inside the cutoff.
+template <int D>
+__device__ __forceinline__ double opes_Tl(const double* ci, const double* ck, const double* sk, double lk,
+                                          double cutoff2, double epsd)
+{
+#pragma clang fp contract(off)
+    double n2 = 0.0;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        double dc = ci[d] - ck[d];
+        n2 = n2 + dc * dc / (sk[d] * sk[d]);
+        if (n2 >= cutoff2) return 0.0;
+    }
+    return exp(lk) * (exp(-0.5 * n2) - epsd);
+}
+
+template <int D>
 __global__ __launch_bounds__(OPES_DEP_BLOCK) void k_dep_opes(StepView v, OpesBiasDev o, int w0, int nw)
 {
 #pragma clang fp contract(off)
-    __shared__ double shN2[OPES_DEP_BLOCK / 64];
-    __shared__ int shIdx[OPES_DEP_BLOCK / 64];
-    __shared__ double shSum[OPES_DEP_BLOCK / 64];
-    __shared__ double sC[MAX_BIAS_D], sS[MAX_BIAS_D], sOldC[MAX_BIAS_D], sOldS[MAX_BIAS_D];
-    __shared__ double sH, sLw, sOldH;
-    __shared__ int sMerge, sAppend, sSkip, sNBefore;
-    const int D = o.D;
+    constexpr int B = OPES_DEP_BLOCK, NWV = OPES_DEP_BLOCK / 64, LK = opes_lds_kernels<D>();
+    __shared__ double tC[D * LK], tS[D * LK], tL[LK];  // staged table, [d][k]
+    __shared__ double shN2[NWV], shSum[NWV];
+    __shared__ int shIdx[NWV];
     const int tb = o.shared ? 0 : w0 + blockIdx.x;
     const int wFirst = o.shared ? w0 : w0 + blockIdx.x;
     const int wLast = o.shared ? w0 + nw : wFirst + 1;
@@ -1185,72 +1224,106 @@ __global__ __launch_bounds__(OPES_DEP_BLOCK) void k_dep_opes(StepView v, OpesBia
     double* L = o.logw + (size_t)tb * o.cap;
     int* I = o.ints + tb * 4;
     double* Q = o.dbl + (size_t)tb * (4 + 2 * D);
-    double* mean = Q + 4;
-    double* M2 = Q + 4 + D;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int lk = min(o.cap, LK);
+
+    // ---- the table scalars and the first walker's inputs, loaded together ----
+    int i0 = I[0], i1 = I[1], i2 = I[2], i3 = I[3];
+    double q0 = Q[0], q1 = Q[1], q2 = Q[2], q3 = Q[3], m2[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) m2[d] = Q[4 + D + d];
+    double Vb = v.benergy[(size_t)wFirst * v.B + o.eslot], cvx[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) cvx[d] = v.cvVal[(size_t)wFirst * v.V + o.cv[d]];
+    // ---- stage the kernels present into LDS ----
+    const int nStage = min(i0, lk);
+#pragma unroll 4
+    for (int k = tid; k < nStage; k += B) {
+        double c[D], sg[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            c[d] = C[(size_t)d * o.cap + k];
+            sg[d] = Sg[(size_t)d * o.cap + k];
+        }
+        const double l = L[k];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            tC[d * LK + k] = c[d];
+            tS[d * LK + k] = sg[d];
+        }
+        tL[k] = l;
+    }
     const double epsd = exp(-0.5 * o.cutoff2);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
 
     for (int w = wFirst; w < wLast; w++) {
-        const double* cv = v.cvVal + (size_t)w * v.V;
-        // ---- scalar part: weights, neff, sigma, height (thread 0) ----
-        if (threadIdx.x == 0) {
-            sSkip = 0;
-            if (o.adaptiveStride > 0 && I[3] < o.adaptiveStride) sSkip = 1;
-            if (!sSkip) {
-                sNBefore = I[0];
-                if (o.variant != 1 && o.adaptiveStride == 0) I[3]++;
-                const int n = I[3];
-                const double V = v.benergy[(size_t)w * v.B + o.eslot];
-                double lw_raw, h_raw;
-                if (o.variant == 2) {
-                    lw_raw = 0.0;
-                    h_raw = 1.0;
-                } else {
-                    lw_raw = V * (1.0 / o.kT);
-                    h_raw = exp(lw_raw);
-                }
-                Q[2] += h_raw;
-                Q[3] += h_raw * h_raw;
-                I[2]++;
-                const double neff = (1.0 + Q[2]) * (1.0 + Q[2]) / (1.0 + Q[3]);
-                const double size = o.variant == 2 ? (double)I[2] : neff;
-                double lw = lw_raw;
-                for (int d = 0; d < D; d++) {
-                    double sg;
-                    if (o.variant == 1) {
-                        sg = o.sigma0[d];
-                    } else if (o.adaptiveStride > 0) {
-                        double factor = o.variant == 2 ? 1.0 : o.gamma;
-                        double var = n > 1 ? M2[d] / factor / (double)(n - 1) : 0.0;
-                        sg = var > 0.0 ? sqrt(var) : 1.0;
-                        if (sg < o.sigmaMin) sg = o.sigmaMin;
-                    } else {
-                        sg = o.sigma0[d] * pow(size * (D + 2) / 4.0, -1.0 / (D + 4.0));
-                        if (sg < o.sigmaMin) sg = o.sigmaMin;
-                    }
-                    sS[d] = sg;
-                    sC[d] = cv[o.cv[d]];
-                }
-                if (o.variant != 1 && o.adaptiveStride == 0)
-                    for (int d = 0; d < D; d++) lw += log(o.sigma0[d] / sS[d]);
-                sLw = lw;
-                sH = exp(lw);
-                Q[1] += h_raw;
-            }
+        if (w > wFirst) {
+            Vb = v.benergy[(size_t)w * v.B + o.eslot];
+#pragma unroll
+            for (int d = 0; d < D; d++) cvx[d] = v.cvVal[(size_t)w * v.V + o.cv[d]];
         }
-        __syncthreads();
-        if (sSkip) continue;
-        const int nB = sNBefore;
+        // ---- weights, n_eff, widths, height (every thread, same values) ----
+        if (o.adaptiveStride > 0 && i3 < o.adaptiveStride) continue;
+        const int nB = i0;
+        if (o.variant != 1 && o.adaptiveStride == 0) i3++;
+        const int n = i3;
+        double lw_raw, h_raw;
+        if (o.variant == 2) {
+            lw_raw = 0.0;
+            h_raw = 1.0;
+        } else {
+            lw_raw = Vb * (1.0 / o.kT);
+            h_raw = exp(lw_raw);
+        }
+        q2 += h_raw;
+        q3 += h_raw * h_raw;
+        i2++;
+        const double neff = (1.0 + q2) * (1.0 + q2) / (1.0 + q3);
+        const double size = o.variant == 2 ? (double)i2 : neff;
+        double sC[D], sS[D], lw = lw_raw;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            double sg;
+            if (o.variant == 1) {
+                sg = o.sigma0[d];
+            } else if (o.adaptiveStride > 0) {
+                double factor = o.variant == 2 ? 1.0 : o.gamma;
+                double var = n > 1 ? m2[d] / factor / (double)(n - 1) : 0.0;
+                sg = var > 0.0 ? sqrt(var) : 1.0;
+                if (sg < o.sigmaMin) sg = o.sigmaMin;
+            } else {
+                sg = o.sigma0[d] * pow(size * (D + 2) / 4.0, -1.0 / (D + 4.0));
+                if (sg < o.sigmaMin) sg = o.sigmaMin;
+            }
+            sS[d] = sg;
+            sC[d] = cvx[d];
+        }
+        if (o.variant != 1 && o.adaptiveStride == 0)
+#pragma unroll
+            for (int d = 0; d < D; d++) lw += log(o.sigma0[d] / sS[d]);
+        const double h = exp(lw);
+        q1 += h_raw;
+
         // ---- compression search: arg-min (n2, k) over n2 < 1 ----
         double best = 1.0;
         int bestK = -1;
-        for (int k = threadIdx.x; k < nB; k += blockDim.x) {
+        const int nS = min(nB, lk);
+        for (int k = tid; k < nB; k += B) {  // k increases within a thread: first index wins ties
             double n2 = 0.0;
-            for (int d = 0; d < D; d++) {
-                double x = (sC[d] - C[(size_t)d * o.cap + k]) / Sg[(size_t)d * o.cap + k];
-                n2 = n2 + x * x;
+            if (k < nS) {
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    double x = (sC[d] - tC[d * LK + k]) / tS[d * LK + k];
+                    n2 = n2 + x * x;
+                }
+            } else {
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    double x = (sC[d] - C[(size_t)d * o.cap + k]) / Sg[(size_t)d * o.cap + k];
+                    n2 = n2 + x * x;
+                }
             }
-            if (n2 < best) {  // k increases within a thread: first index wins ties
+            if (n2 < best) {
                 best = n2;
                 bestK = k;
             }
@@ -1268,114 +1341,529 @@ __global__ __launch_bounds__(OPES_DEP_BLOCK) void k_dep_opes(StepView v, OpesBia
             shIdx[wid] = bestK;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            double b = 1.0;
-            int bk = -1;
-            for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
-                int ok = shIdx[i];
-                if (ok >= 0 && (bk < 0 || shN2[i] < b || (shN2[i] == b && ok < bk))) {
-                    b = shN2[i];
-                    bk = ok;
-                }
-            }
-            sMerge = bk;
-            sAppend = 0;
-            if (bk >= 0) {
-                for (int d = 0; d < D; d++) {
-                    sOldC[d] = C[(size_t)d * o.cap + bk];
-                    sOldS[d] = Sg[(size_t)d * o.cap + bk];
-                }
-                sOldH = exp(L[bk]);
-                const double lo = L[bk], lwn = sLw;
-                const double ho = sOldH, ht = ho + sH;
-                const double lmax = lo > lwn ? lo : lwn, lmin = lo < lwn ? lo : lwn;
-                const double lm = lmax + log1p(exp(lmin - lmax));
-                const double f1 = ho / ht, f2 = sH / ht;
-                for (int d = 0; d < D; d++) {
-                    const double c1 = sOldC[d], s1 = sOldS[d];
-                    const double cm = f1 * c1 + f2 * sC[d];
-                    // the reference's f1(s1^2+c1^2) + f2(s2^2+c2^2) - cm^2
-                    // (CommonGluedKernels.cpp:2898-2905) in the algebraically
-                    // identical form without its cancellation at large |c|
-                    const double dcm = c1 - sC[d];
-                    double ss = f1 * s1 * s1 + f2 * sS[d] * sS[d] + f1 * f2 * dcm * dcm;
-                    if (ss < 0.0) ss = 0.0;
-                    double sm = sqrt(ss);
-                    if (sm < o.sigmaMin) sm = o.sigmaMin;
-                    C[(size_t)d * o.cap + bk] = cm;
-                    Sg[(size_t)d * o.cap + bk] = sm;
-                }
-                L[bk] = lm;
-            } else {
-                const int k = I[1]++;
-                if (k < o.cap) {
-                    for (int d = 0; d < D; d++) {
-                        C[(size_t)d * o.cap + k] = sC[d];
-                        Sg[(size_t)d * o.cap + k] = sS[d];
-                    }
-                    L[k] = sLw;
-                    I[0]++;
-                    sAppend = 1;
-                } else {
-                    I[1]--;
-                }
+
+        // ---- merge into the nearest kernel, or append (every thread forms
+        // the same values; thread 0 stores) ----
+        double b = 1.0;
+        int mk = -1;
+        for (int i = 0; i < NWV; i++) {
+            const int ok = shIdx[i];
+            if (ok >= 0 && (mk < 0 || shN2[i] < b || (shN2[i] == b && ok < mk))) {
+                b = shN2[i];
+                mk = ok;
             }
         }
-        __syncthreads();
-        // ---- sum_uprob update, O(N) ----
-        const int mk = sMerge;
+        double oldC[D], oldS[D], newC[D], newS[D], newL = 0.0, oldH = 0.0, hNew = 0.0;
+        int ka = -1;
+        if (mk >= 0) {
+            double oldL;
+            if (mk < lk) {
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    oldC[d] = tC[d * LK + mk];
+                    oldS[d] = tS[d * LK + mk];
+                }
+                oldL = tL[mk];
+            } else {
+#pragma unroll
+                for (int d = 0; d < D; d++) {
+                    oldC[d] = C[(size_t)d * o.cap + mk];
+                    oldS[d] = Sg[(size_t)d * o.cap + mk];
+                }
+                oldL = L[mk];
+            }
+            oldH = exp(oldL);
+            const double lmax = oldL > lw ? oldL : lw, lmin = oldL < lw ? oldL : lw;
+            newL = lmax + log1p(exp(lmin - lmax));
+            const double ht = oldH + h;
+            const double f1 = oldH / ht, f2 = h / ht;
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                const double c1 = oldC[d], s1 = oldS[d];
+                newC[d] = f1 * c1 + f2 * sC[d];
+                // the reference's f1(s1^2+c1^2) + f2(s2^2+c2^2) - cm^2
+                // (CommonGluedKernels.cpp:2898-2905) in the algebraically
+                // identical form without its cancellation at large |c|
+                const double dcm = c1 - sC[d];
+                double ss = f1 * s1 * s1 + f2 * sS[d] * sS[d] + f1 * f2 * dcm * dcm;
+                if (ss < 0.0) ss = 0.0;
+                double sm = sqrt(ss);
+                if (sm < o.sigmaMin) sm = o.sigmaMin;
+                newS[d] = sm;
+            }
+            hNew = exp(newL);
+        } else if (i1 < o.cap) {
+            ka = i1++;
+            i0++;
+        }
+        // ---- sum_uprob update, O(N), against the table before this deposit ----
         double acc = 0.0;
         if (mk >= 0) {
-            double cNew[MAX_BIAS_D], sNew[MAX_BIAS_D], cK[MAX_BIAS_D], sK[MAX_BIAS_D];
-            for (int d = 0; d < D; d++) {
-                cNew[d] = C[(size_t)d * o.cap + mk];
-                sNew[d] = Sg[(size_t)d * o.cap + mk];
-            }
-            const double hNew = exp(L[mk]);
-            for (int i = threadIdx.x; i < nB; i += blockDim.x) {
-                double ci[MAX_BIAS_D], si[MAX_BIAS_D];
-                for (int d = 0; d < D; d++) {
-                    ci[d] = C[(size_t)d * o.cap + i];
-                    si[d] = Sg[(size_t)d * o.cap + i];
-                }
-                const double hi = exp(L[i]);
+            for (int i = tid; i < nB; i += B) {
                 if (i == mk) {
                     // diagonal: row and column coincide
-                    acc += opes_T(cNew, cNew, sNew, hNew, D, o.cutoff2, epsd) -
-                           opes_T(sOldC, sOldC, sOldS, sOldH, D, o.cutoff2, epsd);
+                    acc += opes_T<D>(newC, newC, newS, hNew, o.cutoff2, epsd) -
+                           opes_T<D>(oldC, oldC, oldS, oldH, o.cutoff2, epsd);
                     continue;
                 }
-                for (int d = 0; d < D; d++) {
-                    cK[d] = sOldC[d];
-                    sK[d] = sOldS[d];
+                double ci[D], si[D], li;
+                if (i < nS) {
+#pragma unroll
+                    for (int d = 0; d < D; d++) {
+                        ci[d] = tC[d * LK + i];
+                        si[d] = tS[d * LK + i];
+                    }
+                    li = tL[i];
+                } else {
+#pragma unroll
+                    for (int d = 0; d < D; d++) {
+                        ci[d] = C[(size_t)d * o.cap + i];
+                        si[d] = Sg[(size_t)d * o.cap + i];
+                    }
+                    li = L[i];
                 }
                 // row mk (j = mk, kernel i) and column mk (j = i, kernel mk)
-                acc += opes_T(cNew, ci, si, hi, D, o.cutoff2, epsd) - opes_T(cK, ci, si, hi, D, o.cutoff2, epsd);
-                acc += opes_T(ci, cNew, sNew, hNew, D, o.cutoff2, epsd) -
-                       opes_T(ci, cK, sK, sOldH, D, o.cutoff2, epsd);
+                acc += opes_Tl<D>(newC, ci, si, li, o.cutoff2, epsd) - opes_Tl<D>(oldC, ci, si, li, o.cutoff2, epsd);
+                acc += opes_T<D>(ci, newC, newS, hNew, o.cutoff2, epsd) -
+                       opes_T<D>(ci, oldC, oldS, oldH, o.cutoff2, epsd);
             }
-        } else if (sAppend) {
-            for (int j = threadIdx.x; j < nB; j += blockDim.x) {
-                double cj[MAX_BIAS_D], sj[MAX_BIAS_D];
-                for (int d = 0; d < D; d++) {
-                    cj[d] = C[(size_t)d * o.cap + j];
-                    sj[d] = Sg[(size_t)d * o.cap + j];
+        } else if (ka >= 0) {
+            for (int i = tid; i < nB; i += B) {
+                double cj[D], sj[D], lj;
+                if (i < nS) {
+#pragma unroll
+                    for (int d = 0; d < D; d++) {
+                        cj[d] = tC[d * LK + i];
+                        sj[d] = tS[d * LK + i];
+                    }
+                    lj = tL[i];
+                } else {
+#pragma unroll
+                    for (int d = 0; d < D; d++) {
+                        cj[d] = C[(size_t)d * o.cap + i];
+                        sj[d] = Sg[(size_t)d * o.cap + i];
+                    }
+                    lj = L[i];
                 }
-                acc += opes_T(cj, sC, sS, sH, D, o.cutoff2, epsd);
-                acc += opes_T(sC, cj, sj, exp(L[j]), D, o.cutoff2, epsd);
+                acc += opes_T<D>(cj, sC, sS, h, o.cutoff2, epsd);
+                acc += opes_Tl<D>(sC, cj, sj, lj, o.cutoff2, epsd);
             }
         }
         acc = wsum(acc);
         if (lane == 0) shSum[wid] = acc;
         __syncthreads();
-        if (threadIdx.x == 0 && (mk >= 0 || sAppend)) {
+        // ---- store the changed kernel (global table and LDS stage) ----
+        if (tid == 0 && (mk >= 0 || ka >= 0)) {
+            const int kc = mk >= 0 ? mk : ka;
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                const double c = mk >= 0 ? newC[d] : sC[d], sg = mk >= 0 ? newS[d] : sS[d];
+                C[(size_t)d * o.cap + kc] = c;
+                Sg[(size_t)d * o.cap + kc] = sg;
+                if (kc < lk) {
+                    tC[d * LK + kc] = c;
+                    tS[d * LK + kc] = sg;
+                }
+            }
+            const double l = mk >= 0 ? newL : lw;
+            L[kc] = l;
+            if (kc < lk) tL[kc] = l;
+        }
+        if (mk >= 0 || ka >= 0) {
             double t = 0.0;
-            for (int i = 0; i < (int)(blockDim.x >> 6); i++) t += shSum[i];
-            if (mk < 0) t += sH * (1.0 - epsd);
-            Q[0] += t;
+            for (int i = 0; i < NWV; i++) t += shSum[i];
+            if (mk < 0) t += h * (1.0 - epsd);
+            q0 += t;
         }
         __syncthreads();
     }
+    if (tid == 0) {
+        I[0] = i0;
+        I[1] = i1;
+        I[2] = i2;
+        I[3] = i3;
+        Q[0] = q0;
+        Q[1] = q1;
+        Q[2] = q2;
+        Q[3] = q3;
+    }
+}
+
+// Private tables: the same deposit spread over OPES_DEP_G workgroups per
+// table in two launches.  k_dep_opes_find: every workgroup forms the scalar
+// recursion (same values everywhere) and the arg-min over its kernels
+// (kernel k belongs to workgroup (k / 256) mod G: lanes read consecutive
+// kernels; the first two of a thread's kernels are loaded before anything
+// else), and writes its best (n2, index) with that kernel's centre, width
+// and log-weight; the last workgroup to finish (ticket) takes the
+// lexicographic minimum of the G partials -- the same (n2, index) the
+// one-workgroup search finds --, merges or appends, stores the table
+// scalars, and records the change for the second launch.  k_dep_opes_sum:
+// the O(N) sum_uprob change over the table as it was before the deposit, G
+// partials added in workgroup order by the last workgroup.  Same terms as
+// the one-workgroup form; only the grouping of the FP64 sum differs.
+// Record per table (doubles): flag (0 none, 1 merge, 2 append), the changed
+// kernel's index, nB, old height, new height, deposit height, then old
+// centre, old width, new centre, new width, deposit centre, deposit width.
+constexpr int OPES_SPLIT_BLOCK = 256, OPES_SPLIT_PF = 2;
+
+template <int D>
+__global__ __launch_bounds__(OPES_SPLIT_BLOCK) void k_dep_opes_find(StepView v, OpesBiasDev o, int w0)
+{
+#pragma clang fp contract(off)
+    constexpr int G = OPES_DEP_G, NWV = OPES_SPLIT_BLOCK / 64, PF = OPES_SPLIT_PF, NV = 2 * D + 1;
+    constexpr int STRIDE = G * OPES_SPLIT_BLOCK;
+    __shared__ double shN2[NWV], shVal[NWV][NV];
+    __shared__ int shIdx[NWV];
+    const int tb = w0 + blockIdx.y, w = tb, g = blockIdx.x;
+    double* C = o.centers + (size_t)tb * D * o.cap;
+    double* Sg = o.sigmas + (size_t)tb * D * o.cap;
+    double* L = o.logw + (size_t)tb * o.cap;
+    int* I = o.ints + tb * 4;
+    double* Q = o.dbl + (size_t)tb * (4 + 2 * D);
+    double* part = o.depPart + (size_t)tb * G * OPES_PART;
+    double* rec = o.depRec + (size_t)tb * OPES_REC;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int k0 = g * OPES_SPLIT_BLOCK + tid;
+
+    // ---- every load the search needs first: this thread's first kernels
+    // (clamped into the capacity), the table scalars, the walker's inputs ----
+    double pc[PF][D], ps[PF][D], pl[PF];
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+        const int k = min(k0 + j * STRIDE, o.cap - 1);
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            pc[j][d] = C[(size_t)d * o.cap + k];
+            ps[j][d] = Sg[(size_t)d * o.cap + k];
+        }
+        pl[j] = L[k];
+    }
+    int i0 = I[0], i1 = I[1], i2 = I[2], i3 = I[3];
+    double q1 = Q[1], q2 = Q[2], q3 = Q[3], m2[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) m2[d] = Q[4 + D + d];
+    const double Vb = v.benergy[(size_t)w * v.B + o.eslot];
+    double cvx[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) cvx[d] = v.cvVal[(size_t)w * v.V + o.cv[d]];
+    if (o.adaptiveStride > 0 && i3 < o.adaptiveStride) {
+        if (g == 0 && tid == 0) rec[0] = 0.0;
+        return;
+    }
+    const int nB = i0;
+    if (o.variant != 1 && o.adaptiveStride == 0) i3++;
+    const int n = i3;
+    double lw_raw, h_raw;
+    if (o.variant == 2) {
+        lw_raw = 0.0;
+        h_raw = 1.0;
+    } else {
+        lw_raw = Vb * (1.0 / o.kT);
+        h_raw = exp(lw_raw);
+    }
+    q2 += h_raw;
+    q3 += h_raw * h_raw;
+    i2++;
+    const double neff = (1.0 + q2) * (1.0 + q2) / (1.0 + q3);
+    const double size = o.variant == 2 ? (double)i2 : neff;
+    double sC[D], sS[D], lw = lw_raw;
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        double sg;
+        if (o.variant == 1) {
+            sg = o.sigma0[d];
+        } else if (o.adaptiveStride > 0) {
+            double factor = o.variant == 2 ? 1.0 : o.gamma;
+            double var = n > 1 ? m2[d] / factor / (double)(n - 1) : 0.0;
+            sg = var > 0.0 ? sqrt(var) : 1.0;
+            if (sg < o.sigmaMin) sg = o.sigmaMin;
+        } else {
+            sg = o.sigma0[d] * pow(size * (D + 2) / 4.0, -1.0 / (D + 4.0));
+            if (sg < o.sigmaMin) sg = o.sigmaMin;
+        }
+        sS[d] = sg;
+        sC[d] = cvx[d];
+    }
+    if (o.variant != 1 && o.adaptiveStride == 0)
+#pragma unroll
+        for (int d = 0; d < D; d++) lw += log(o.sigma0[d] / sS[d]);
+    q1 += h_raw;
+
+    // ---- this workgroup's share of the arg-min (n2, k) over n2 < 1 ----
+    double best = 1.0, bv[NV];
+    int bestK = -1;
+#pragma unroll
+    for (int e = 0; e < NV; e++) bv[e] = 0.0;
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+        double n2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            double x = (sC[d] - pc[j][d]) / ps[j][d];
+            n2 = n2 + x * x;
+        }
+        if (k0 + j * STRIDE < nB && n2 < best) {  // k increases within a thread: first index wins ties
+            best = n2;
+            bestK = k0 + j * STRIDE;
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                bv[d] = pc[j][d];
+                bv[D + d] = ps[j][d];
+            }
+            bv[2 * D] = pl[j];
+        }
+    }
+    for (int k = k0 + PF * STRIDE; k < nB; k += STRIDE) {
+        double c[D], sg[D], n2 = 0.0;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            c[d] = C[(size_t)d * o.cap + k];
+            sg[d] = Sg[(size_t)d * o.cap + k];
+        }
+        const double l = L[k];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            double x = (sC[d] - c[d]) / sg[d];
+            n2 = n2 + x * x;
+        }
+        if (n2 < best) {
+            best = n2;
+            bestK = k;
+#pragma unroll
+            for (int d = 0; d < D; d++) {
+                bv[d] = c[d];
+                bv[D + d] = sg[d];
+            }
+            bv[2 * D] = l;
+        }
+    }
+    const int myK = bestK;
+    for (int off = 32; off > 0; off >>= 1) {
+        double ob = __shfl_xor(best, off, 64);
+        int ok = __shfl_xor(bestK, off, 64);
+        if (ok >= 0 && (bestK < 0 || ob < best || (ob == best && ok < bestK))) {
+            best = ob;
+            bestK = ok;
+        }
+    }
+    if (lane == 0) {
+        shN2[wid] = best;
+        shIdx[wid] = bestK;
+    }
+    if (bestK >= 0 && myK == bestK)  // the lane holding the wave's best kernel
+#pragma unroll
+        for (int e = 0; e < NV; e++) shVal[wid][e] = bv[e];
+    __syncthreads();
+    if (tid == 0) {
+        double b = 1.0;
+        int bk = -1, bw = 0;
+        for (int i = 0; i < NWV; i++) {
+            const int ok = shIdx[i];
+            if (ok >= 0 && (bk < 0 || shN2[i] < b || (shN2[i] == b && ok < bk))) {
+                b = shN2[i];
+                bk = ok;
+                bw = i;
+            }
+        }
+        double* pp = part + g * OPES_PART;
+        pp[0] = b;
+        pp[1] = (double)bk;
+#pragma unroll
+        for (int e = 0; e < NV; e++) pp[2 + e] = shVal[bw][e];
+    }
+    if (!last_block(&o.depCount[2 * tb], G)) return;
+    if (tid >= 64) return;
+
+    // ---- last workgroup, wave 0: lane i reads partial i; the minimum ----
+    double b = 1.0, mv[NV];
+    int mk = -1;
+#pragma unroll
+    for (int e = 0; e < NV; e++) mv[e] = 0.0;
+    if (lane < G) {
+        const double* pp = part + lane * OPES_PART;
+        b = __hip_atomic_load(pp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        mk = (int)__hip_atomic_load(pp + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+#pragma unroll
+        for (int e = 0; e < NV; e++) mv[e] = __hip_atomic_load(pp + 2 + e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (mk < 0) b = 1.0;
+    }
+    const int ownK = mk;
+    for (int off = 32; off > 0; off >>= 1) {
+        double ob = __shfl_xor(b, off, 64);
+        int ok = __shfl_xor(mk, off, 64);
+        if (ok >= 0 && (mk < 0 || ob < b || (ob == b && ok < mk))) {
+            b = ob;
+            mk = ok;
+        }
+    }
+    // the winning lane's kernel values, to every lane
+    const unsigned long long own = __ballot(mk >= 0 && ownK == mk);
+    const int src = own ? __ffsll(own) - 1 : 0;
+#pragma unroll
+    for (int e = 0; e < NV; e++) mv[e] = __shfl(mv[e], src, 64);
+    if (lane != 0) return;
+    const double h = exp(lw);
+    double flag = 0.0, kc = -1.0, oldH = 0.0, hNew = 0.0;
+    if (mk >= 0) {
+        double oldC[D], oldS[D], newC[D], newS[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            oldC[d] = mv[d];
+            oldS[d] = mv[D + d];
+        }
+        const double oldL = mv[2 * D];
+        oldH = exp(oldL);
+        const double lmax = oldL > lw ? oldL : lw, lmin = oldL < lw ? oldL : lw;
+        const double newL = lmax + log1p(exp(lmin - lmax));
+        const double ht = oldH + h;
+        const double f1 = oldH / ht, f2 = h / ht;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            const double c1 = oldC[d], s1 = oldS[d];
+            newC[d] = f1 * c1 + f2 * sC[d];
+            // the reference's f1(s1^2+c1^2) + f2(s2^2+c2^2) - cm^2
+            // (CommonGluedKernels.cpp:2898-2905) in the algebraically
+            // identical form without its cancellation at large |c|
+            const double dcm = c1 - sC[d];
+            double ss = f1 * s1 * s1 + f2 * sS[d] * sS[d] + f1 * f2 * dcm * dcm;
+            if (ss < 0.0) ss = 0.0;
+            double sm = sqrt(ss);
+            if (sm < o.sigmaMin) sm = o.sigmaMin;
+            newS[d] = sm;
+        }
+        hNew = exp(newL);
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            C[(size_t)d * o.cap + mk] = newC[d];
+            Sg[(size_t)d * o.cap + mk] = newS[d];
+            rec[6 + d] = oldC[d];
+            rec[6 + D + d] = oldS[d];
+            rec[6 + 2 * D + d] = newC[d];
+            rec[6 + 3 * D + d] = newS[d];
+        }
+        L[mk] = newL;
+        flag = 1.0;
+        kc = mk;
+    } else if (i1 < o.cap) {
+        const int ka = i1++;
+        i0++;
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            C[(size_t)d * o.cap + ka] = sC[d];
+            Sg[(size_t)d * o.cap + ka] = sS[d];
+        }
+        L[ka] = lw;
+        flag = 2.0;
+        kc = ka;
+    }
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        rec[6 + 4 * D + d] = sC[d];
+        rec[6 + 5 * D + d] = sS[d];
+    }
+    rec[0] = flag;
+    rec[1] = kc;
+    rec[2] = nB;
+    rec[3] = oldH;
+    rec[4] = hNew;
+    rec[5] = h;
+    I[0] = i0;
+    I[1] = i1;
+    I[2] = i2;
+    I[3] = i3;
+    Q[1] = q1;
+    Q[2] = q2;
+    Q[3] = q3;
+    o.depCount[2 * tb] = 0u;
+}
+
+template <int D>
+__global__ __launch_bounds__(OPES_SPLIT_BLOCK) void k_dep_opes_sum(OpesBiasDev o, int w0)
+{
+#pragma clang fp contract(off)
+    constexpr int G = OPES_DEP_G, NWV = OPES_SPLIT_BLOCK / 64, PF = OPES_SPLIT_PF;
+    constexpr int STRIDE = G * OPES_SPLIT_BLOCK;
+    __shared__ double shSum[NWV];
+    const int tb = w0 + blockIdx.y, g = blockIdx.x;
+    const double* C = o.centers + (size_t)tb * D * o.cap;
+    const double* Sg = o.sigmas + (size_t)tb * D * o.cap;
+    const double* L = o.logw + (size_t)tb * o.cap;
+    double* part = o.depPart + (size_t)tb * G * OPES_PART;
+    const double* rec = o.depRec + (size_t)tb * OPES_REC;
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int k0 = g * OPES_SPLIT_BLOCK + tid;
+    // this thread's first kernels (clamped into the capacity) with the record
+    double pc[PF][D], ps[PF][D], pl[PF];
+#pragma unroll
+    for (int j = 0; j < PF; j++) {
+        const int k = min(k0 + j * STRIDE, o.cap - 1);
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            pc[j][d] = C[(size_t)d * o.cap + k];
+            ps[j][d] = Sg[(size_t)d * o.cap + k];
+        }
+        pl[j] = L[k];
+    }
+    const double flag = rec[0];
+    const int mk = (int)rec[1], nB = (int)rec[2];
+    const double oldH = rec[3], hNew = rec[4], h = rec[5];
+    double oldC[D], oldS[D], newC[D], newS[D], sC[D], sS[D];
+#pragma unroll
+    for (int d = 0; d < D; d++) {
+        oldC[d] = rec[6 + d];
+        oldS[d] = rec[6 + D + d];
+        newC[d] = rec[6 + 2 * D + d];
+        newS[d] = rec[6 + 3 * D + d];
+        sC[d] = rec[6 + 4 * D + d];
+        sS[d] = rec[6 + 5 * D + d];
+    }
+    if (flag == 0.0) return;
+    const double epsd = exp(-0.5 * o.cutoff2);
+    // the change's terms against kernel i (before the deposit)
+    auto terms = [&](int i, const double* ci, const double* si, double li) -> double {
+        if (flag == 1.0) {
+            if (i == mk)  // diagonal: row and column coincide
+                return opes_T<D>(newC, newC, newS, hNew, o.cutoff2, epsd) -
+                       opes_T<D>(oldC, oldC, oldS, oldH, o.cutoff2, epsd);
+            // row mk (j = mk, kernel i) and column mk (j = i, kernel mk)
+            double a = opes_Tl<D>(newC, ci, si, li, o.cutoff2, epsd) - opes_Tl<D>(oldC, ci, si, li, o.cutoff2, epsd);
+            return a + (opes_T<D>(ci, newC, newS, hNew, o.cutoff2, epsd) -
+                        opes_T<D>(ci, oldC, oldS, oldH, o.cutoff2, epsd));
+        }
+        double a = opes_T<D>(ci, sC, sS, h, o.cutoff2, epsd);
+        return a + opes_Tl<D>(sC, ci, si, li, o.cutoff2, epsd);
+    };
+    double acc = 0.0;
+#pragma unroll
+    for (int j = 0; j < PF; j++)
+        if (k0 + j * STRIDE < nB) acc += terms(k0 + j * STRIDE, pc[j], ps[j], pl[j]);
+    for (int i = k0 + PF * STRIDE; i < nB; i += STRIDE) {
+        double ci[D], si[D];
+#pragma unroll
+        for (int d = 0; d < D; d++) {
+            ci[d] = C[(size_t)d * o.cap + i];
+            si[d] = Sg[(size_t)d * o.cap + i];
+        }
+        acc += terms(i, ci, si, L[i]);
+    }
+    acc = wsum(acc);
+    if (lane == 0) shSum[wid] = acc;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int i = 0; i < NWV; i++) t += shSum[i];
+        part[g * OPES_PART] = t;
+    }
+    if (!last_block(&o.depCount[2 * tb + 1], G)) return;
+    if (tid != 0) return;
+    double t = 0.0;
+    for (int i = 0; i < G; i++)
+        t += __hip_atomic_load(part + i * OPES_PART, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (flag == 2.0) t += h * (1.0 - epsd);
+    double* Q = o.dbl + (size_t)tb * (4 + 2 * D);
+    Q[0] += t;
+    o.depCount[2 * tb + 1] = 0u;
 }
 
 // Every-step Welford accumulator (adaptive sigma), :3009-3027.
@@ -1403,7 +1891,24 @@ __global__ void k_opes_welford(StepView v, OpesBiasDev o, int w0, int nw)
 
 void launch_dep_opes(const StepView& v, const OpesBiasDev& o, int w0, int nw, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dep_opes, dim3(o.shared ? 1 : nw), dim3(OPES_DEP_BLOCK), 0, s, v, o, w0, nw);
+#define GLUED_DEP_OPES(DD)                                                                                          \
+    if (o.shared) {                                                                                                  \
+        hipLaunchKernelGGL(k_dep_opes<DD>, dim3(1), dim3(OPES_DEP_BLOCK), 0, s, v, o, w0, nw);                        \
+    } else {                                                                                                         \
+        hipLaunchKernelGGL(k_dep_opes_find<DD>, dim3(OPES_DEP_G, nw), dim3(OPES_SPLIT_BLOCK), 0, s, v, o, w0);       \
+        hipLaunchKernelGGL(k_dep_opes_sum<DD>, dim3(OPES_DEP_G, nw), dim3(OPES_SPLIT_BLOCK), 0, s, o, w0);           \
+    }
+    switch (o.D) {
+    case 1: GLUED_DEP_OPES(1) break;
+    case 2: GLUED_DEP_OPES(2) break;
+    case 3: GLUED_DEP_OPES(3) break;
+    case 4: GLUED_DEP_OPES(4) break;
+    case 5: GLUED_DEP_OPES(5) break;
+    case 6: GLUED_DEP_OPES(6) break;
+    case 7: GLUED_DEP_OPES(7) break;
+    default: GLUED_DEP_OPES(8) break;
+    }
+#undef GLUED_DEP_OPES
 }
 
 void launch_opes_welford(const StepView& v, const OpesBiasDev& o, int w0, int nw, hipStream_t s)
