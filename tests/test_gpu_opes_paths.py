@@ -186,3 +186,68 @@ def test_opes_shared_table_four_walkers(adaptive):
         assert st.nk < deposits, "no merges"
     _, s, _ = st.table()
     assert np.allclose(g.getKernelSigmas(ob, 0), s.reshape(-1), rtol=1e-9)
+
+
+@pytest.mark.parametrize("shared", [False, True])
+def test_opes_large_table_deposits(shared):
+    """Tables past the deposit kernels' staged prefixes: 9,000 kernels in a
+    private table (the 16 deposit workgroups load two kernels per thread,
+    8,192 in all, before the search; the rest stream) and 4,500 kernels
+    shared by two walkers (the one-workgroup form stages 4,000 kernels of a
+    2-CV table in LDS; the rest stream).  Each step puts the CVs 0.1 sigma
+    from a kernel past the prefix (a merge there) or far from every kernel
+    (an append).  Kernel count exact, metrics within 1e-6, energies and
+    forces every step."""
+    from gluedmd_amd import workloads as wl
+    from gluedmd_amd.context import Context
+    from oracle.oracle import _Opes
+    W = 2 if shared else 1
+    n = 4500 if shared else 9000
+    sig = np.array([0.05, 0.04])
+    f = Force(pbc=True, temperature=300.0)
+    a = f.add_distance([0, 1])
+    b = f.add_distance([0, 2])
+    ob = f.add_opes([a, b], sig.tolist(), gamma=10.0, pace=1, max_kernels=n + 64)
+    if shared:
+        f.share_bias(ob)
+    rng = np.random.default_rng(41)
+    centers, sigmas, logw = wl.opes_prior_kernels(wl.SplitMix64(41), [1.0, 1.0], sig, 10.0, n=n, spread=6.0)
+    _, cvs, params, ip = f.biases[ob]
+    common = _Opes(len(cvs), params, ip)
+    su = wl.sum_uprob_np(centers, sigmas, logw, common.cutoff2)
+    common.load(centers, sigmas, logw, su)
+    g = Context(f, walkers=[3] * W)
+    g.set_box(8.0)
+    g.setBiasState(wl.opes_state_blob(centers, sigmas, logw, su))
+    if shared:
+        os_ = _walker_oracles(f, 3, W, box=8.0, shared={ob: common})
+    else:
+        os_ = _walker_oracles(f, 3, W, box=8.0)
+        os_[0].states[ob] = common
+    lo = 4000 if shared else 8192
+    targets = [k for k in range(lo, n) if min(centers[k]) > 0.3]
+    for step in range(6):
+        pos = []
+        for w in range(W):
+            if (step + w) % 3 == 2:
+                d1, d2 = 3.2 + 0.1 * step, 3.2 + 0.07 * w  # far from every prior kernel
+            else:
+                d1, d2 = centers[targets[int(rng.integers(len(targets)))]] + 0.1 * sig
+            pos.append(np.array([(0, 0, 0), (d1, 0, 0), (0, d2, 0)], dtype=np.float32))
+        g.update_state(step)
+        g.set_all_positions(pos)
+        g.zero_forces()
+        g.execute_batch(use_graph=False)
+        for o in os_:
+            o.update_state(step)
+        for w, o in enumerate(os_):
+            r = o.execute(pos[w])
+            check(g, o, r, g.getLastBias(w), w)
+    for w in range(W):
+        m = g.getOPESMetrics(ob, w)
+        assert int(m[2]) == common.nk, (w, m[2], common.nk)
+        assert_close_rel(m, common.metrics(), 1e-6, "OPES metrics")
+    assert common.nk > n, "no appends"
+    assert common.nk < n + 6 * W, "no merges"
+    _, s, _ = common.table()
+    assert np.allclose(g.getKernelSigmas(ob, 0), s.reshape(-1), rtol=1e-9)
