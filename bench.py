@@ -50,30 +50,41 @@ FP64_VALU_TFLOPS = 78.6
 FP32_VALU_TFLOPS = 157.3
 FIXED_SCALE = 4294967296.0
 
-# HBM traffic per launch of each workload's roofline kernel(s), from two
+# HBM traffic per launch of each kernel of a workload's batched step, from two
 # rocprofv3 --pmc passes (FETCH_SIZE, then WRITE_SIZE; KiB per dispatch,
-# averaged over 20 batched steps of `bench.py --workload W --batched-only
+# averaged over the batched steps of `bench.py --workload W --batched-only
 # --steps 20 --warmup 2`).  On gfx950 FETCH_SIZE reads half the bytes that a
 # copy kernel of known size moves and WRITE_SIZE reads them in full, so
-# traffic = 2 * FETCH + 1 * WRITE.  The per-kernel figures are in the
-# workload summaries under profiles/.
+# traffic = 2 * FETCH + 1 * WRITE.  The per-kernel figures and the commands
+# are in the workload summaries under profiles/.
 PMC_FETCH_SCALE, PMC_WRITE_SCALE = 2.0, 1.0
 PMC_KIB = {
-    "multiwalker": {"k_step_bias_chunks<1, 2>": (15104.7, 60.0)},
-    "mixed-large": {"k_scatter_dense_rows": (41545.6, 49685.4)},
-    "coordination-opes": {"k_coord_rank": (278.4, 792.4), "k_coord_offsets": (20.8, 38.8),
-                          "k_coord_place": (357.3, 418.8), "k_coord_pairs_a": (1562.8, 250.0),
-                          "k_coord_pairs<false, true>": (835.8, 590.3), "k_cv_group_big": (84.2, 156.3)},
-    "path-batch": {"k_cv_fused (path MSD blocks)": (3377.2, 128.0), "k_path_rows<4>": (2116.1, 2416.0)},
-    "small": {"k_step_tiny<1, 2, false>": (54.9, 0.9)},
+    "multiwalker": {"k_cv_fused": (440.4, 512.5), "k_step_bias_chunks<1, 2>": (7585.6, 60.0),
+                    "k_step_bias_finalize<false>": (540.3, 769.0)},
+    "mixed-large": {"k_cv_fused": (11506.8, 2883.7), "k_cv_group_big": (2950.5, 815.5),
+                    "k_path_rows<8>": (3867.2, 554.9), "k_step_bias_finalize<false>": (87.0, 2.0),
+                    "k_scatter_dense_rows": (41545.6, 49685.2)},
+    "coordination-opes": {"k_coord_rank": (279.4, 792.4), "k_coord_place": (461.9, 420.6),
+                          "k_coord_cells<512>": (1916.9, 705.9), "k_cv_group_big": (83.5, 156.3),
+                          "k_step_bias_chunks<2, 2>": (147.3, 3.0), "k_step_bias_finalize<false>": (13.7, 0.1),
+                          "k_scatter": (379.7, 515.7)},
+    "path-batch": {"k_cv_fused": (3798.3, 770.7), "k_path_rows<4>": (3974.4, 2544.0),
+                   "k_step_hill_chunks_shared<2>": (448.9, 336.0), "k_step_bias_finalize<false>": (1749.0, 1823.3)},
+    "small": {"k_step_tiny<1, 2, false>": (42.4, 0.9)},
+}
+# the kernel(s) each workload's roofline line is about
+ROOFLINE_KERNELS = {
+    "multiwalker": ["k_step_bias_chunks<1, 2>"],
+    "mixed-large": ["k_scatter_dense_rows"],
+    "coordination-opes": ["k_coord_rank", "k_coord_place", "k_coord_cells<512>", "k_cv_group_big"],
+    "path-batch": ["k_cv_fused", "k_path_rows<4>"],
+    "small": ["k_step_tiny<1, 2, false>"],
 }
 
 
-def pmc_traffic(workload):
-    t = PMC_KIB.get(workload)
-    if not t:
-        return None
-    return round(sum(PMC_FETCH_SCALE * f + PMC_WRITE_SCALE * w for f, w in t.values()) * 1024)
+def pmc_traffic(workload, kernels):
+    t = PMC_KIB[workload]
+    return round(sum(PMC_FETCH_SCALE * t[k][0] + PMC_WRITE_SCALE * t[k][1] for k in kernels) * 1024)
 
 
 def log(msg):
@@ -87,9 +98,13 @@ class Workload:
     name = ""
     walkers = 1
 
+    def dominant(self):
+        """One launch of the roofline kernel(s)."""
+        self.ctx.execute_stage(self.stage)
+
     def time_dominant(self, reps):
         """Average duration of one launch of the roofline kernel(s) (s)."""
-        return time_stage(self.ctx, self.stage, reps)
+        return time_fn(self.dominant, reps)
 
     def roofline(self, t_launch):
         raise NotImplementedError
@@ -322,9 +337,9 @@ class Small(Workload):
             "parallelism": "one walker, direct kernel launches",
         }
 
-    def time_dominant(self, reps):
-        """The batched step of this plan is one launch (k_step_tiny): its duration."""
-        return time_fn(lambda: self.ctx.execute_batch(use_graph=False), reps)
+    def dominant(self):
+        """The batched step of this plan is one launch (k_step_tiny)."""
+        self.ctx.execute_batch(use_graph=False)
 
     def roofline(self, t_launch):
         n = len(self.ctx.get_hills(0)[1])
@@ -376,39 +391,131 @@ def time_fn(fn, reps):
     return e0.elapsed_time(e1) / reps * 1e-3
 
 
+def time_cold(fn, reps, flush_mib=512):
+    """Average GPU duration of one call with nothing of its working set in the
+    caches: before each call a 512 MiB buffer is overwritten on the same
+    stream (more than the 256 MB memory-side cache), then the call is timed
+    alone between two HIP events.  flush_mib=0: the same single-call timing
+    without the flush."""
+    import torch
+    stream = torch.cuda.current_stream()
+    buf = torch.empty(max(flush_mib, 1) * 2**18, dtype=torch.float32, device="cuda")
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
+    fn()
+    for k in range(reps):
+        if flush_mib:
+            buf.fill_(float(k))
+        ev[k][0].record(stream)
+        fn()
+        ev[k][1].record(stream)
+    torch.cuda.synchronize()
+    del buf
+    return sum(a.elapsed_time(b) for a, b in ev) / reps * 1e-3
+
+
 def time_stage(ctx, stage, reps):
     """Average duration of one launch of a stage."""
     return time_fn(lambda: ctx.execute_stage(stage), reps)
 
 
-def cpu_baseline(wk, seconds):
-    """FP64 oracle on the host: one core, walkers evaluated in turn until the
-    time budget is used; also the checker of the GPU forces for those walkers."""
+def host_info():
+    """The host the CPU baseline ran on."""
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu": model, "nproc": os.cpu_count(), "usable": len(os.sched_getaffinity(0))}
+
+
+def _cpu_worker(items, reps, start_at):
+    """Process of the CPU baseline's parallel leg (started with the spawn
+    method: a fresh interpreter that imports no GPU library): evaluate the
+    given oracle contexts `reps` times from `start_at` on."""
+    while time.time() < start_at:
+        time.sleep(0.001)
+    t0 = time.time()
+    n = 0
+    for _ in range(reps):
+        for o, p in items:
+            o.execute(p)
+            n += 1
+    return n, t0, time.time()
+
+
+def cpu_baseline(wk, seconds, procs):
+    """FP64 oracle on the host.
+
+    The oracle contexts of walkers 0..n-1 (n = max(P, min(W, 4P))) are built
+    first; their first evaluation checks the GPU's forces (untimed).  Leg 1,
+    one thread, about seconds/2: the contexts evaluated in turn, over and
+    over.  Leg 2, `procs` processes (one per core, each single-threaded), as
+    many passes as leg 1's rate fills in about seconds/2: context i in
+    process i mod P; the rate is all processes' evals over the span from the
+    first start to the last end.  `value` is leg 2's rate."""
+    import concurrent.futures
+    import multiprocessing
     import numpy as np
     import torch
     ctx = wk.ctx
     ctx.zero_forces()
     ctx.execute_batch(use_graph=False)
     torch.cuda.synchronize()
-    evals, t_used, num, den, amax, w = 0, 0.0, 0.0, 0.0, 0.0, 0
-    while t_used < seconds and evals < 1000 * wk.W:
-        o = wk.oracle_walker(w)
-        pos = wk.positions(w)
-        t0 = time.perf_counter()
-        r = o.execute(pos)
-        t_used += time.perf_counter() - t0
-        evals += 1
-        Fg = ctx.get_forces(w)
-        num += float(np.sum((Fg - r["forces"]) ** 2))
-        den += float(np.sum(r["forces"] ** 2))
-        amax = max(amax, float(np.abs(Fg).max()))
-        w = (w + 1) % wk.W
-    rate = evals / t_used
-    return ({"value": round(rate, 2), "unit": "evals/s", "cores": 1, "kind": "port",
-             "sample": f"{evals} walker evals (walkers 0..{min(evals, wk.W) - 1}, cycled) of the same workload, "
-                       "FP64 C oracle driven from Python, single thread"},
-            # all-zero FP64 forces (a flat bias): report the largest GPU force instead
-            float(math.sqrt(num / den)) if den > 0 else {"fp64_forces": "all zero", "gpu_abs_max": amax})
+    P = max(1, procs)
+    items = []
+    num = den = amax = 0.0
+    for i in range(max(P, min(wk.W, 4 * P))):
+        w = i % wk.W
+        o, pos = wk.oracle_walker(w), wk.positions(w)
+        if i < wk.W:
+            r = o.execute(pos)
+            Fg = ctx.get_forces(w)
+            num += float(np.sum((Fg - r["forces"]) ** 2))
+            den += float(np.sum(r["forces"] ** 2))
+            amax = max(amax, float(np.abs(Fg).max()))
+        items.append((o, pos))
+    err = float(math.sqrt(num / den)) if den > 0 else {"fp64_forces": "all zero", "gpu_abs_max": amax}
+    half = seconds / 2
+    n1, t0 = 0, time.perf_counter()
+    while time.perf_counter() - t0 < half:
+        o, pos = items[n1 % len(items)]
+        o.execute(pos)
+        n1 += 1
+    rate1 = n1 / (time.perf_counter() - t0)
+    one = {"value": round(rate1, 2), "unit": "evals/s", "cores": 1, "kind": "port",
+           "sample": f"{n1} walker evals cycling over the oracle contexts of walkers 0..{min(len(items), wk.W) - 1} "
+                     "of the same workload, FP64 C oracle (gcc -O3 -march=x86-64-v3) driven from Python, one thread"}
+    if P == 1:
+        one["host"] = host_info()
+        return one, err
+    work = [items[i::P] for i in range(P)]
+    reps = max(1, int(math.ceil(rate1 * half / max(len(work[0]), 1))))
+    saved = {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS")}
+    for k in saved:
+        os.environ[k] = "1"
+    try:
+        mp = multiprocessing.get_context("spawn")
+        with concurrent.futures.ProcessPoolExecutor(P, mp_context=mp) as ex:
+            start_at = time.time() + 8.0
+            res = [f.result() for f in [ex.submit(_cpu_worker, work[i], reps, start_at) for i in range(P)]]
+    finally:
+        for k, v in saved.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    n_par = sum(r[0] for r in res)
+    span = max(r[2] for r in res) - min(r[1] for r in res)
+    out = {"value": round(n_par / span, 2), "unit": "evals/s", "cores": P, "kind": "port",
+           "sample": f"{n_par} walker evals over {P} single-threaded processes ({len(items)} oracle contexts of "
+                     f"walkers 0..{min(len(items), wk.W) - 1}, {reps} passes each) of the same workload, "
+                     "FP64 C oracle (gcc -O3 -march=x86-64-v3) driven from Python",
+           "single_thread": one, "host": host_info()}
+    return out, err
 
 
 def main():
@@ -421,6 +528,8 @@ def main():
     ap.add_argument("--walkers", type=int, default=64, help="multiwalker: walker count")
     ap.add_argument("--hills", type=int, default=20000, help="multiwalker: private hills per walker")
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-procs", type=int, default=min(16, len(os.sched_getaffinity(0))),
+                    help="processes of the CPU baseline's parallel leg (the GPU box's CPU share is 16)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--batched-only", action="store_true",
                     help="time only the batched step (for profiling its kernels)")
@@ -448,22 +557,44 @@ def main():
                           "value": round(W / t_batched, 1), "unit": "evals/s", "tuning": ctx.tuning()}))
         return 0
 
-    # ---- plain: one walker per launch sequence ----
-    for _ in range(2):
-        ctx.execute_all_plain()
-    t_plain = time_loop(lambda k: ctx.execute_all_plain(), args.plain_steps, sync)
-    log(f"plain {t_plain * 1e6:.1f} us/step")
-
-    # ---- CPU baseline and force check, on the state before any deposit ----
-    cpu = err = None
-    if not args.no_cpu_baseline:
-        cpu, err = cpu_baseline(wk, args.cpu_seconds)
-        log(f"cpu baseline {cpu['value']} evals/s, force rel RMS {err}")
-
-    # ---- the same batched step replayed as one captured hipGraph ----
-    for _ in range(10):
+    # ---- the same batched step replayed as one captured hipGraph (timed
+    # right after the direct launches, with the same warm-up) ----
+    for _ in range(args.warmup):
         ctx.execute_batch(use_graph=True)
     t_graph = time_loop(lambda k: ctx.execute_batch(use_graph=True), args.steps, sync)
+    log(f"graph {t_graph * 1e6:.1f} us/step")
+
+    # ---- roofline of the dominant kernel: back to back (its working set
+    # stays in the caches across launches, as across the timed steps) and
+    # with the caches flushed before every launch ----
+    t_launch = wk.time_dominant(200)
+    t_cold = time_cold(wk.dominant, 50)
+    t_alone = time_cold(wk.dominant, 50, flush_mib=0)
+    rf = wk.roofline(t_launch)
+    rf_cold = wk.roofline(t_cold)
+    rf["frac"] = round(rf["achieved"] / rf["peak"], 4)
+    rf["achieved"] = round(rf["achieved"], 3)
+    rf["avg_launch_us"] = round(t_launch * 1e6, 3)
+    rf["cold"] = {"achieved": round(rf_cold["achieved"], 3), "frac": round(rf_cold["achieved"] / rf["peak"], 4),
+                  "avg_launch_us": round(t_cold * 1e6, 3), "warm_alone_us": round(t_alone * 1e6, 3),
+                  "how": "512 MiB overwritten on the stream before each of 50 launches, each launch timed alone "
+                         "(warm_alone_us: the same single-launch timing without the flush)"}
+    rf["traffic"] = pmc_traffic(wk.name, ROOFLINE_KERNELS[wk.name])
+    rf["traffic_kernels"] = ROOFLINE_KERNELS[wk.name]
+    stage_t = {}
+    for st, name in ((0, "cv"), (1, "bias"), (2, "scatter")):
+        stage_t[name] = round(time_stage(ctx, st, 100) * 1e6, 2)
+    step_traffic = pmc_traffic(wk.name, sorted(PMC_KIB[wk.name]))
+    step = {"us": round(t_batched * 1e6, 3), "kernels": sorted(PMC_KIB[wk.name]), "traffic": step_traffic,
+            "traffic_GBps": round(step_traffic / t_batched / 1e9, 1),
+            "hbm_frac": round(step_traffic / t_batched / 1e9 / HBM_PEAK_GBS, 4)}
+
+    # ---- CPU baseline and force check, on the state before any deposit
+    # (after the GPU timings above, so the host legs do not overlap them) ----
+    cpu = err = None
+    if not args.no_cpu_baseline:
+        cpu, err = cpu_baseline(wk, args.cpu_seconds, args.cpu_procs)
+        log(f"cpu baseline {cpu['value']} evals/s ({cpu['cores']} cores), force rel RMS {err}")
 
     # ---- eval + deposit (pace 1: every walker deposits every step) ----
     ctx.update_state(1)
@@ -474,18 +605,11 @@ def main():
     ctx.step_batch(100)
     t_dep_graph = time_loop(lambda k: ctx.step_batch(k + 101), 10, sync)
 
-    # ---- roofline of the dominant kernel ----
-    t_launch = wk.time_dominant(200)
-    rf = wk.roofline(t_launch)
-    rf["frac"] = round(rf["achieved"] / rf["peak"], 4)
-    rf["achieved"] = round(rf["achieved"], 3)
-    rf["traffic"] = pmc_traffic(wk.name)
-    if rf["traffic"] is not None:
-        rf["traffic_kernels"] = sorted(PMC_KIB[wk.name])
-    rf["avg_launch_us"] = round(t_launch * 1e6, 3)
-    stage_t = {}
-    for st, name in ((0, "cv"), (1, "bias"), (2, "scatter")):
-        stage_t[name] = round(time_stage(ctx, st, 100) * 1e6, 2)
+    # ---- plain: one walker per launch sequence ----
+    for _ in range(2):
+        ctx.execute_all_plain()
+    t_plain = time_loop(lambda k: ctx.execute_all_plain(), args.plain_steps, sync)
+    log(f"plain {t_plain * 1e6:.1f} us/step")
 
     out = {
         "metric": "CV+bias+force evals/sec, batched vs plain on 1 GPU; force RMS err vs FP64",
@@ -509,6 +633,7 @@ def main():
         "stage_us": stage_t,
         "tuning": ctx.tuning(),
         "roofline": rf,
+        "step": step,
     }
     if cpu is not None:
         out["cpu_baseline"], out["force_rel_rms_vs_fp64"] = cpu, err

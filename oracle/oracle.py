@@ -103,6 +103,11 @@ def f32(x):
     return float(np.float32(x))
 
 
+def f32v(xs):
+    """Values as the reference stores them (float), back in FP64."""
+    return np.asarray(xs, dtype=np.float64).astype(np.float32).astype(np.float64)
+
+
 # ---------------------------------------------------------------------------
 # thin wrappers (arrays in, numbers out)
 # ---------------------------------------------------------------------------
@@ -585,7 +590,7 @@ class OracleContext:
             elif t == CV_RMSD:
                 n = len(atoms)
                 a, ap = P(atoms, np.int32)
-                ref, rp = P(np.array([f32(x) for x in params[:3 * n]]))
+                ref, rp = P(f32v(params[:3 * n]))
                 fit = 1 if (len(params) > 3 * n and params[3 * n] != 0.0) else 0
                 g = np.zeros(3 * n)
                 cv[v] = L.orc_cv_rmsd(pp, ap, n, rp, fit, g.ctypes.data_as(dp), None)
@@ -594,7 +599,7 @@ class OracleContext:
             elif t == CV_PATH:
                 M, N = len(atoms), int(params[1])
                 a, ap = P(atoms, np.int32)
-                fr, frp = P(np.array([f32(x) for x in params[2:2 + N * M * 3]]))
+                fr, frp = P(f32v(params[2:2 + N * M * 3]))
                 sz = np.zeros(2)
                 gs = np.zeros(3 * M)
                 gz = np.zeros(3 * M)
@@ -607,7 +612,7 @@ class OracleContext:
             elif t == CV_DRMSD:
                 np_ = len(atoms) // 2
                 a, ap = P(atoms[:2 * np_], np.int32)
-                d0, d0p = P(np.array([f32(x) for x in params[:np_]]))
+                d0, d0p = P(f32v(params[:np_]))
                 g = np.zeros(6 * np_)
                 cv[v] = L.orc_cv_drmsd(pp, ap, np_, d0p, bp, pbc, g.ctypes.data_as(dp))
                 rows.append((v, np.array(atoms[:2 * np_]), g.reshape(-1, 3)))
@@ -615,7 +620,7 @@ class OracleContext:
             elif t == CV_CONTACTMAP:
                 np_ = len(atoms) // 2
                 a, ap = P(atoms[:2 * np_], np.int32)
-                pr, prp = P(np.array([f32(x) for x in params[:4 * np_]]))
+                pr, prp = P(f32v(params[:4 * np_]))
                 g = np.zeros(6 * np_)
                 cv[v] = L.orc_cv_contactmap(pp, ap, np_, prp, bp, pbc, g.ctypes.data_as(dp))
                 rows.append((v, np.array(atoms[:2 * np_]), g.reshape(-1, 3)))
