@@ -1,10 +1,12 @@
 """MI355X-native GLUED enhanced-sampling step: CV evaluation, metadynamics /
 OPES bias evaluation and deposition, and the chain-rule force scatter, as
 hand-written HIP kernels for gfx950 behind a C-ABI (include/glued.h)."""
-from . import glued
+from . import bias_state, glued
+from .bias_state import BiasStateMerger
 from .glued import Force, GluedForce, multithermal_temperature_ladder
 
-__all__ = ["glued", "Force", "GluedForce", "multithermal_temperature_ladder", "Context"]
+__all__ = ["bias_state", "glued", "BiasStateMerger", "Force", "GluedForce", "multithermal_temperature_ladder",
+           "Context"]
 
 
 def __getattr__(name):
