@@ -770,38 +770,94 @@ __global__ void k_coord_fill(StepView v, CoordBuffers cb, const CoordItem* items
 constexpr int CELL_BLOCK = 1024;
 constexpr int CELL_LDS = 8192;  // cells (both groups) counted in LDS by the sort
 
-// Two-kernel counting sort (no global atomics, no single-CU scatter):
+// Two-kernel counting sort (no global atomics, no single-CU scatter), stable:
+// within a cell the atoms keep their list order, so the sorted array -- and
+// with it the order in which the pair kernels meet the pairs -- is the same
+// on every evaluation.
 //   rank    one 1,024-thread workgroup per 1,024 atoms: gather, cell, rank of
-//           the atom among the workgroup's atoms of that cell (LDS counter),
+//           the atom among the workgroup's earlier atoms of that cell (within
+//           a wave from a ballot over the lanes of the same cell; across
+//           waves from LDS counters that the waves advance in wave order),
 //           and the workgroup's count per cell (cell-major table);
 //   place   every workgroup scans the table (cell-major, workgroup-minor) for
 //           where its atoms of each cell start, and places its atoms.
 int coord_blocks(int n) { return (n + CELL_BLOCK - 1) / CELL_BLOCK; }
 int coord_sort_fits(int ncellBoth, int n) { return ncellBoth <= CELL_LDS && ncellBoth * coord_blocks(n) <= 65536; }
 
+constexpr int CELL_WAVE_TAB = 2048;  // cells (both groups) up to which the waves' counts go to a per-wave table
+
 __global__ __launch_bounds__(CELL_BLOCK) void k_coord_rank(StepView v, CoordBuffers cb, const CoordItem* items,
                                                            int item, int w0)
 {
+    constexpr int WAVES = CELL_BLOCK / 64;
     __shared__ int lcnt[CELL_LDS];
+    __shared__ unsigned short wcnt[WAVES][CELL_WAVE_TAB];  // per wave, its atoms per cell
     const int w = w0 + blockIdx.y, tid = threadIdx.x, b = blockIdx.x;
     const CoordItem it = items[item];
     const int n = it.nA + it.nB, nb = gridDim.x;
     const int ncell = it.nc[0] * it.nc[1] * it.nc[2], n2 = 2 * ncell;
-    for (int c = tid; c < n2; c += CELL_BLOCK) lcnt[c] = 0;
+    const bool table = n2 <= CELL_WAVE_TAB;  // uniform
+    if (table)
+        for (int k = tid; k < WAVES * n2; k += CELL_BLOCK) wcnt[k / n2][k % n2] = 0;
+    else
+        for (int c = tid; c < n2; c += CELL_BLOCK) lcnt[c] = 0;
     __syncthreads();
-    const int i = b * CELL_BLOCK + tid;
-    if (i < n) {
+    const int i = b * CELL_BLOCK + tid, lane = tid & 63, wv = tid >> 6;
+    const bool valid = i < n;
+    int c = -1;
+    const size_t o = (size_t)w * cb.sortStride + it.sortBase + i;
+    if (valid) {
         const bool a = i < it.nA;
         const float4 q = load_pos(v, w, cb.slots[a ? it.aBegin + i : it.bBegin + (i - it.nA)]);
-        const int c = cell_of(q.x, q.y, q.z, v.box + (size_t)w * 12, it.nc) + (a ? 0 : ncell);
-        const size_t o = (size_t)w * cb.sortStride + it.sortBase + i;
+        c = cell_of(q.x, q.y, q.z, v.box + (size_t)w * 12, it.nc) + (a ? 0 : ncell);
         cb.cellOf[(size_t)w * cb.cellOfStride + it.sortBase + i] = c;
-        cb.rank[o] = atomicAdd(&lcnt[c], 1);
         cb.unsorted[o] = make_float4(q.x, q.y, q.z, __int_as_float(a ? i : i - it.nA));
     }
-    __syncthreads();
+    // rank among the wave's lower lanes of the same cell, and the group size
+    // (each lane compares its cell with every lane's; the lowest lane of a
+    // group is its leader)
+    int inWave = 0, group = 0;
+#pragma unroll 16
+    for (int j = 0; j < 64; j++) {
+        const int cj = __builtin_amdgcn_readlane(c, j);
+        inWave += (cj == c) & (j < lane);
+        group += cj == c;
+    }
+    const bool leader = inWave == 0;
+    int rank = 0;
     int* tab = cb.blockCnt + (size_t)w * cb.blkStride + it.blkBase;
-    for (int c = tid; c < n2; c += CELL_BLOCK) tab[(size_t)c * nb + b] = lcnt[c];
+    if (table) {
+        // every wave's group sizes, then each atom adds the counts of the
+        // earlier waves of its cell
+        if (valid && leader) wcnt[wv][c] = (unsigned short)group;
+        __syncthreads();
+        if (valid) {
+            int base = 0;
+            for (int k = 0; k < wv; k++) base += wcnt[k][c];
+            rank = base + inWave;
+            cb.rank[o] = rank;
+        }
+        for (int cc = tid; cc < n2; cc += CELL_BLOCK) {
+            int t = 0;
+#pragma unroll
+            for (int k = 0; k < WAVES; k++) t += wcnt[k][cc];
+            tab[(size_t)cc * nb + b] = t;
+        }
+        return;
+    }
+    // many cells: the waves take their cells' counters in wave order
+    for (int k = 0; k < WAVES; k++) {
+        __syncthreads();
+        if (wv == k && valid) {
+            const int base = lcnt[c];
+            __builtin_amdgcn_wave_barrier();
+            if (leader) lcnt[c] = base + group;
+            rank = base + inWave;
+        }
+    }
+    if (valid) cb.rank[o] = rank;
+    __syncthreads();
+    for (int cc = tid; cc < n2; cc += CELL_BLOCK) tab[(size_t)cc * nb + b] = lcnt[cc];
 }
 
 // Placement with the scan folded in: every placing workgroup scans the whole

@@ -72,6 +72,39 @@ def test_coordination_random(cutoff, pbc, n):
         assert g.coordination_pairs(0) == o.coord_pairs[0]
 
 
+def test_coordination_cells_bit_reproducible():
+    """The cell-list sort keeps list order within a cell, so CN, the pair
+    count and every Jacobian row come out with the same bits on every
+    evaluation, in the batched and the one-walker launches alike (3 walkers,
+    ~40 atoms per cell, 12 evaluations each)."""
+    from gluedmd_amd.context import Context
+    rng = np.random.default_rng(17)
+    n, L, W = 8000, 4.4, 3
+    nA = 1000
+    f = Force(pbc=True)
+    f.add_coordination(list(range(nA)), list(range(nA, n)), r0=0.4, n=6, m=12, cutoff=1.2)
+    f.setTestBiasGradients([1.0])
+    g = Context(f, walkers=[n] * W)
+    g.set_box(L)
+    g.set_all_positions([rng.uniform(0, L, size=(n, 3)).astype(np.float32) for _ in range(W)])
+    ref = None
+    for rep in range(12):
+        g.zero_forces()
+        if rep % 2:
+            g.execute_all_plain()
+        else:
+            g.execute_batch(use_graph=False)
+        got = [(g.getLastCVValues(w).copy(), g.get_forces_fixed(w).copy(), g.coordination_pairs(0, w))
+               for w in range(W)]
+        if ref is None:
+            ref = got
+            continue
+        for w in range(W):
+            assert np.array_equal(got[w][0], ref[w][0]), (rep, w)
+            assert np.array_equal(got[w][1], ref[w][1]), (rep, w)
+            assert got[w][2] == ref[w][2]
+
+
 def test_group_cvs_random():
     rng = np.random.default_rng(3)
     n = 700
