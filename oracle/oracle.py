@@ -525,6 +525,7 @@ class OracleContext:
     def eval_cvs(self, pos, f64=False, exact_params=False):
         L = lib()
         f32 = (lambda x: float(x)) if exact_params else globals()["f32"]
+        f32v = (lambda xs: np.asarray(xs, dtype=np.float64)) if exact_params else globals()["f32v"]
         pos = np.asarray(pos, dtype=np.float64 if f64 else np.float32).astype(np.float64).reshape(-1)
         pos, pp = P(pos)
         bx, bp = P(self.box)
