@@ -23,8 +23,8 @@ void launch_cv_group_big(const StepView&, const GroupCVItem*, int, const int*, c
 void launch_path_rows(const StepView&, const PathItem*, int, int, int, const int*, const float*, int, int, int,
                       hipStream_t);
 
-void launch_coordination(const StepView&, const CoordBuffers&, const CoordItem*, const CoordItem&, int, int, int,
-                         hipStream_t);
+bool launch_coordination(const StepView&, const CoordBuffers&, const CoordItem*, const CoordItem&, int, int, int,
+                         hipStream_t, const GroupExtra*);
 void launch_bias_small(const StepView&, const SmallBiasState&, const SmallBiasItem*, int, int, int, hipStream_t);
 void launch_bias_hills(const StepView&, const HillBiasDev&, int, int, hipStream_t);
 void launch_bias_opes(const StepView&, const OpesBiasDev&, int, int, hipStream_t);
@@ -1341,10 +1341,18 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
     cp.nCmBlocks = cmBlocks_;
     cp.cmPart = dCmPart_.as<double>();
     cp.cmCount = dCmCount_.as<unsigned int>();
+    // batched form: the wide group CVs ride in the first coordination
+    // item's placement launch when it has one (no data between them)
+    const GroupExtra ge{dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupPos_.as<int>(),
+                        dGroupMass_.as<float>(), dGroupRef_.as<float>()};
+    bool bigPending = false;
     if (fused && w0 == 0 && nw == W_) {
         launch_cv_fused(v, cp, nw, s);
-        launch_cv_group_big(v, dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupPos_.as<int>(),
-                            dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
+        bigPending = !groupBig_.empty();
+        if (bigPending && coord_.empty()) {
+            launch_cv_group_big(v, ge.items, ge.n, ge.slots, ge.mass, ge.ref, w0, nw, s);
+            bigPending = false;
+        }
         launch_path_rows(v, dPath_.as<PathItem>(), (int)path_.size(), pathRowBlocks_, pathMaxN_, dPathPos_.as<int>(),
                          dPathFrames_.as<float>(), w0, nw, tune_.pathPhases, s);
     } else {
@@ -1378,8 +1386,11 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         cb.sortStride = sortStride_;
         cb.nItems = (int)coord_.size();
         for (size_t i = 0; i < coord_.size(); i++)
-            launch_coordination(v, cb, dCoord_.as<CoordItem>(), coord_[i], (int)i, w0, nw, s);
+            if (launch_coordination(v, cb, dCoord_.as<CoordItem>(), coord_[i], (int)i, w0, nw, s,
+                                    bigPending ? &ge : nullptr))
+                bigPending = false;
     }
+    if (bigPending) launch_cv_group_big(v, ge.items, ge.n, ge.slots, ge.mass, ge.ref, w0, nw, s);
     }
     SmallBiasState st;
     st.grids = dGridTable_.as<GridBiasDev>();

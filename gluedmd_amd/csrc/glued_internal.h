@@ -279,6 +279,16 @@ struct CoordBuffers {
     int cellOfStride, cellStride, sortStride, blkStride, nItems;
 };
 
+// Wide group CVs launched as extra workgroups of another launch (the
+// coordination placement kernel): items, atom slots, masses, references.
+struct GroupExtra {
+    const GroupCVItem* items;
+    int n;
+    const int* slots;
+    const float* mass;
+    const float* ref;
+};
+
 // Scatter plan: Jacobian rows grouped by atom slot (CSR), CV of every row,
 // and the gradient slots that sum into each dU/dCV.
 struct ScatterPlan {

@@ -867,11 +867,23 @@ __global__ __launch_bounds__(CELL_BLOCK) void k_coord_rank(StepView v, CoordBuff
 // This replaces a one-workgroup scan launch between rank and place.
 constexpr int CELL_SCAN_PER = 16;  // table entries per thread held in registers by the scan
 
-__global__ __launch_bounds__(CELL_BLOCK) void k_coord_place(CoordBuffers cb, const CoordItem* items, int item, int w0)
+// The launch may carry the wide group CVs of the step as ge.n extra
+// workgroups after the placing ones (they share no data with the sort, and
+// so take no launch of their own in the batched step).
+static_assert(CELL_BLOCK == GROUP_BIG_BLOCK, "wide group CVs ride in the placement launch");
+
+__global__ __launch_bounds__(CELL_BLOCK) void k_coord_place(CoordBuffers cb, const CoordItem* items, int item, int w0,
+                                                            StepView v, GroupExtra ge)
 {
     __shared__ int sOff[CELL_LDS];
     __shared__ int waveTot[CELL_BLOCK / 64];
-    const int w = w0 + blockIdx.y, b = blockIdx.x, nb = gridDim.x, tid = threadIdx.x;
+    const int nb = gridDim.x - ge.n;
+    if ((int)blockIdx.x >= nb) {
+        group_item<GROUP_BIG_BLOCK, GROUP_BIG_REG>(v, ge.items[blockIdx.x - nb], ge.slots, ge.mass, ge.ref,
+                                                    w0 + blockIdx.y);
+        return;
+    }
+    const int w = w0 + blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
     const CoordItem it = items[item];
     const int n = it.nA + it.nB;
     const int ncell = it.nc[0] * it.nc[1] * it.nc[2], n2 = 2 * ncell, len = n2 * nb;
@@ -1480,16 +1492,23 @@ __global__ __launch_bounds__(CB) void k_coord_cells(StepView v, CoordBuffers cb,
     else coord_cell<false, CB>(v, cb, it, item, w, blockIdx.x - ncell, sh);
 }
 
-void launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordItem* dItems, const CoordItem& hIt,
-                         int item, int w0, int nw, hipStream_t s)
+bool launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordItem* dItems, const CoordItem& hIt,
+                         int item, int w0, int nw, hipStream_t s, const GroupExtra* extra)
 {
     const int n = hIt.nA + hIt.nB;
+    bool carried = false;
     if (hIt.useCells) {
         const int ncell2 = 2 * hIt.nc[0] * hIt.nc[1] * hIt.nc[2];
         if (coord_sort_fits(ncell2, n)) {
             const int nb = coord_blocks(n);
+            GroupExtra ge{};
+            if (extra && extra->n > 0) {
+                ge = *extra;
+                carried = true;
+            }
             hipLaunchKernelGGL(k_coord_rank, dim3(nb, nw), dim3(CELL_BLOCK), 0, s, v, cb, dItems, item, w0);
-            hipLaunchKernelGGL(k_coord_place, dim3(nb, nw), dim3(CELL_BLOCK), 0, s, cb, dItems, item, w0);
+            hipLaunchKernelGGL(k_coord_place, dim3(nb + ge.n, nw), dim3(CELL_BLOCK), 0, s, cb, dItems, item, w0, v,
+                               ge);
         } else {
             hipLaunchKernelGGL(k_coord_zero, dim3((ncell2 + 255) / 256, nw), dim3(256), 0, s, cb, hIt.cellBase,
                                ncell2, w0);
@@ -1513,6 +1532,7 @@ void launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordI
         hipLaunchKernelGGL((k_coord_pairs<true>), gA, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
         hipLaunchKernelGGL((k_coord_pairs<false>), gB, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
     }
+    return carried;
 }
 
 }  // namespace glued
