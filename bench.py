@@ -59,16 +59,16 @@ FIXED_SCALE = 4294967296.0
 # are in the workload summaries under profiles/.
 PMC_FETCH_SCALE, PMC_WRITE_SCALE = 2.0, 1.0
 PMC_KIB = {
-    "multiwalker": {"k_cv_fused": (440.4, 512.5), "k_step_bias_chunks<1, 2>": (7585.6, 60.0),
-                    "k_step_bias_finalize<false>": (540.3, 769.0)},
-    "mixed-large": {"k_cv_fused": (11506.8, 2883.7), "k_cv_group_big": (2950.5, 815.5),
-                    "k_path_rows<8>": (3867.2, 554.9), "k_step_bias_finalize<false>": (87.0, 2.0),
-                    "k_scatter_dense_rows": (41545.6, 49685.2)},
-    "coordination-opes": {"k_coord_rank": (279.4, 792.4), "k_coord_place": (461.9, 420.6),
-                          "k_coord_cells<512>": (1916.9, 705.9), "k_cv_group_big": (83.5, 156.3),
-                          "k_step_bias_chunks<2, 2>": (147.3, 3.0), "k_step_bias_finalize<false>": (13.7, 0.1),
-                          "k_scatter": (379.7, 515.7)},
-    "path-batch": {"k_cv_fused": (3798.3, 770.7), "k_path_rows<4>": (3974.4, 2544.0),
+    "multiwalker": {"k_cv_fused": (448.6, 512.5), "k_step_bias_chunks<1, 2>": (7585.5, 60.0),
+                    "k_step_bias_finalize<false>": (540.2, 769.0)},
+    "mixed-large": {"k_cv_fused": (11516.2, 2892.4), "k_cv_group_big": (2954.5, 815.5),
+                    "k_path_rows<8>": (3869.4, 554.9), "k_step_bias_finalize<false>": (87.0, 2.0),
+                    "k_scatter_dense_rows": (41542.5, 49685.2)},
+    # k_coord_place carries the Rg(5,000) workgroup
+    "coordination-opes": {"k_coord_rank": (297.9, 792.4), "k_coord_place": (550.6, 576.8),
+                          "k_coord_cells<512>": (1904.5, 705.3), "k_step_bias_chunks<2, 2>": (147.3, 3.0),
+                          "k_step_bias_finalize<false>": (13.5, 0.1), "k_scatter": (379.7, 515.7)},
+    "path-batch": {"k_cv_fused": (3470.8, 1132.3), "k_path_rows<4>": (3476.9, 2544.0),
                    "k_step_hill_chunks_shared<2>": (448.9, 336.0), "k_step_bias_finalize<false>": (1749.0, 1823.3)},
     "small": {"k_step_tiny<1, 2, false>": (42.4, 0.9)},
 }
@@ -76,7 +76,7 @@ PMC_KIB = {
 ROOFLINE_KERNELS = {
     "multiwalker": ["k_step_bias_chunks<1, 2>"],
     "mixed-large": ["k_scatter_dense_rows"],
-    "coordination-opes": ["k_coord_rank", "k_coord_place", "k_coord_cells<512>", "k_cv_group_big"],
+    "coordination-opes": ["k_coord_rank", "k_coord_place", "k_coord_cells<512>"],
     "path-batch": ["k_cv_fused", "k_path_rows<4>"],
     "small": ["k_step_tiny<1, 2, false>"],
 }
@@ -287,7 +287,7 @@ class PathBatch(Workload):
         f = Force(temperature=300.0)
         s = f.add_path(list(range(c["M"])), [fr.reshape(-1).tolist() for fr in c["frames"]], c["lam"])
         hb = f.add_metad_hills([s, s + 1], sigma=sig.tolist(), height=1.2, pace=1, gamma=None,
-                               max_hills=len(hh) + B * 32)
+                               max_hills=len(hh) + B * 48)
         f.share_bias(hb)
         ctx = Context(f, walkers=[c["M"]] * B)
         ctx.set_hills(hb, hc, hh)
@@ -304,10 +304,13 @@ class PathBatch(Workload):
 
     def roofline(self, t_launch):
         c = self.c
-        flops = self.W * c["N"] * c["M"] * (3 * 3 + 3 * 4)  # MSD pass (sub, fma) + gradient pass per frame-atom
-        return dict(bound="fp32-valu", kernel="CV stage (path CV, 256 configurations x 64 frames x 300 atoms)",
-                    achieved=flops / t_launch / 1e12, peak=FP32_VALU_TFLOPS, unit="TFLOP/s",
-                    per_unit="21 flop per frame-atom (MSD 9, row gradient 12)",
+        # per frame-atom: the MSD cross term x'.r' (3 multiply-adds, FP64 matrix
+        # cores) and the row pass (x - r, then two weighted accumulations, FP64)
+        flops = self.W * c["N"] * c["M"] * (6 + 3 + 12)
+        return dict(bound="fp64 (matrix cores + VALU)",
+                    kernel="CV stage (path CV, 256 configurations x 64 frames x 300 atoms)",
+                    achieved=flops / t_launch / 1e12, peak=FP64_VALU_TFLOPS, unit="TFLOP/s",
+                    per_unit="21 flop per frame-atom (MSD cross term 6 on v_mfma_f64_16x16x4f64, row pass 15)",
                     units_per_launch=self.W * c["N"] * c["M"], flops_per_launch=flops)
 
     def oracle_walker(self, w):
@@ -467,7 +470,7 @@ def cpu_baseline(wk, seconds, procs):
     torch.cuda.synchronize()
     P = max(1, procs)
     items = []
-    num = den = amax = 0.0
+    num = den = amax = cverr = 0.0
     for i in range(max(P, min(wk.W, 4 * P))):
         w = i % wk.W
         o, pos = wk.oracle_walker(w), wk.positions(w)
@@ -477,8 +480,13 @@ def cpu_baseline(wk, seconds, procs):
             num += float(np.sum((Fg - r["forces"]) ** 2))
             den += float(np.sum(r["forces"] ** 2))
             amax = max(amax, float(np.abs(Fg).max()))
+            cv = ctx.getLastCVValues(w)
+            cverr = max(cverr, float(np.max(np.abs(cv - r["cv"]) / np.maximum(np.abs(r["cv"]), 1e-12))))
         items.append((o, pos))
-    err = float(math.sqrt(num / den)) if den > 0 else {"fp64_forces": "all zero", "gpu_abs_max": amax}
+    err = {"forces_rel_rms": float(math.sqrt(num / den)) if den > 0 else None, "cv_max_rel": cverr,
+           "walkers": min(len(items), wk.W)}
+    if den == 0:
+        err["note"] = f"FP64 forces all zero (no bias acts on the atoms yet); largest GPU force {amax}"
     half = seconds / 2
     n1, t0 = 0, time.perf_counter()
     while time.perf_counter() - t0 < half:
@@ -589,27 +597,33 @@ def main():
             "traffic_GBps": round(step_traffic / t_batched / 1e9, 1),
             "hbm_frac": round(step_traffic / t_batched / 1e9 / HBM_PEAK_GBS, 4)}
 
-    # ---- CPU baseline and force check, on the state before any deposit
-    # (after the GPU timings above, so the host legs do not overlap them) ----
-    cpu = err = None
-    if not args.no_cpu_baseline:
-        cpu, err = cpu_baseline(wk, args.cpu_seconds, args.cpu_procs)
-        log(f"cpu baseline {cpu['value']} evals/s ({cpu['cores']} cores), force rel RMS {err}")
-
-    # ---- eval + deposit (pace 1: every walker deposits every step) ----
+    # ---- eval + deposit (pace 1: every walker deposits every step), from a
+    # snapshot of the bias state that is restored afterwards ----
+    snap = [ctx.getBiasState(w) for w in range(W)]
     ctx.update_state(1)
     ctx.execute_batch(use_graph=False)
-    # (10 steps per form: the smallest hill capacity, path-batch's, holds 32 steps of 256 deposits)
-    t_dep = time_loop(lambda k: (ctx.update_state(k + 2), ctx.execute_batch(use_graph=False)), 10, sync)
+    # (20 steps per form: the smallest hill capacity, path-batch's, holds 48 steps of 256 deposits)
+    t_dep = time_loop(lambda k: (ctx.update_state(k + 2), ctx.execute_batch(use_graph=False)), 20, sync)
     # as one graph per deposit pattern (glued_step_batch)
     ctx.step_batch(100)
-    t_dep_graph = time_loop(lambda k: ctx.step_batch(k + 101), 10, sync)
+    t_dep_graph = time_loop(lambda k: ctx.step_batch(k + 101), 20, sync)
+    for w in range(W):
+        ctx.setBiasState(snap[w], w)
+    ctx.execute_batch(use_graph=False)
+    sync()
 
     # ---- plain: one walker per launch sequence ----
     for _ in range(2):
         ctx.execute_all_plain()
     t_plain = time_loop(lambda k: ctx.execute_all_plain(), args.plain_steps, sync)
     log(f"plain {t_plain * 1e6:.1f} us/step")
+
+    # ---- CPU baseline and force check, on the restored pre-deposit state
+    # (last, so the host legs do not overlap the GPU timings) ----
+    cpu = err = None
+    if not args.no_cpu_baseline:
+        cpu, err = cpu_baseline(wk, args.cpu_seconds, args.cpu_procs)
+        log(f"cpu baseline {cpu['value']} evals/s ({cpu['cores']} cores), error vs FP64 {err}")
 
     out = {
         "metric": "CV+bias+force evals/sec, batched vs plain on 1 GPU; force RMS err vs FP64",
@@ -636,7 +650,7 @@ def main():
         "step": step,
     }
     if cpu is not None:
-        out["cpu_baseline"], out["force_rel_rms_vs_fp64"] = cpu, err
+        out["cpu_baseline"], out["error_vs_fp64"] = cpu, err
     print(json.dumps(out))
     return 0
 

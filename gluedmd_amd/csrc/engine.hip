@@ -1027,6 +1027,11 @@ void Engine::autotune(hipStream_t s)
     if (tuned_) return;
     tuned_ = true;
     if (evaluated_ || capturing_) return;
+    {
+        // under the caller's own graph capture nothing can be timed: keep the defaults
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return;
+    }
     const char* env = std::getenv("GLUED_AUTOTUNE");
     if (env && env[0] == '0') return;
     bool cells = false;
