@@ -72,6 +72,42 @@ def test_coordination_random(cutoff, pbc, n):
         assert g.coordination_pairs(0) == o.coord_pairs[0]
 
 
+def test_coordination_per_atom_neighbour_counts():
+    """Neighbour counts per group-A atom, exact: 48 coordination CVs that
+    share one B group of 3,000 atoms (cell list, cutoff 0.9 nm) and each hold
+    one A atom, so every CV's in-cutoff pair count is one A atom's neighbour
+    count; every count and every CN against the oracle, batched and plain."""
+    from gluedmd_amd.context import Context
+    from oracle.oracle import OracleContext
+    rng = np.random.default_rng(23)
+    nB, nA, L = 3000, 48, 3.1
+    n = nA + nB
+    f = Force(pbc=True)
+    B = list(range(nA, n))
+    for a in range(nA):
+        f.add_coordination([a], B, r0=0.3, n=6, m=12, cutoff=0.9)
+    f.setTestBiasGradients([1.0] * nA)
+    pos = rng.uniform(0, L, size=(n, 3)).astype(np.float32)
+    o = OracleContext(f, n)
+    o.set_box(L)
+    r = o.execute(pos)
+    assert len(set(o.coord_pairs)) > 5  # the counts differ from atom to atom
+    g = Context(f, walkers=[n, n])
+    g.set_box(L)
+    g.set_all_positions([pos, pos])
+    for form in ("batched", "plain"):
+        g.zero_forces()
+        if form == "batched":
+            g.execute_batch(use_graph=False)
+        else:
+            g.execute_all_plain()
+        for w in range(2):
+            got = [g.coordination_pairs(k, w) for k in range(nA)]
+            assert got == list(o.coord_pairs), (form, w)
+            cv = g.getLastCVValues(w)
+            assert np.all(np.abs(cv - r["cv"]) <= 1e-6 * np.abs(r["cv"]) + 1e-12), form
+
+
 def test_coordination_cells_bit_reproducible():
     """The cell-list sort keeps list order within a cell, so CN, the pair
     count and every Jacobian row come out with the same bits on every
