@@ -29,6 +29,8 @@ SIGNATURES = {
     "glued_num_cv_values": [vp, c_int_p],
     "glued_set_test_bias_gradients": [vp, c_double_p, i],
     "glued_share_bias": [vp, i, i],
+    "glued_get_multiwalker_ptrs": [vp, i, vp, i, vp],
+    "glued_redirect_to_primary_bias": [vp, i, vp, i],
     "glued_initialize": [vp, i, i, c_double_p, i],
     "glued_initialize_batch": [vp, i, c_int_p, c_int_p, c_double_p, i],
     "glued_set_atom_order": [vp, i, c_int_p, i],
@@ -36,6 +38,7 @@ SIGNATURES = {
     "glued_set_energy_input": [vp, i, d, vp, vp],
     "glued_execute": [vp, vp, i, vp, vp, i, i, c_double_p],
     "glued_execute_batch": [vp, vp, vp, vp, i, i],
+    "glued_execute_plain_all": [vp, vp, vp, vp, i],
     "glued_step_batch": [vp, vp, i, vp, vp, i],
     "glued_update_state": [vp, vp, i],
     "glued_execute_stage": [vp, vp, vp, vp, i],

@@ -173,9 +173,9 @@ class Context:
 
     def execute_all_plain(self, include_forces=True):
         """Plain form for every walker: the reference's one-walker-per-launch
-        structure, looping over walkers on the host."""
-        for w in range(self.W):
-            self.execute(w, include_forces, False)
+        structure, walker after walker, looped inside the library."""
+        call("glued_execute_plain_all", self.h, self.stream, ctypes.c_void_p(self.posq.data_ptr()),
+             ctypes.c_void_p(self.forces.data_ptr()), int(include_forces))
 
     def execute_batch(self, include_forces=True, use_graph=True):
         call("glued_execute_batch", self.h, self.stream, ctypes.c_void_p(self.posq.data_ptr()),
@@ -253,6 +253,22 @@ class Context:
 
     def setBiasState(self, blob, walker=0):
         call("glued_set_bias_state", self.h, int(walker), blob, len(blob))
+
+    def getMultiWalkerPtrs(self, bias_index):
+        """Device addresses of a single-table bias (MetaD grid, hill list or
+        OPES table) for setMultiWalkerPtrs on another context of the device."""
+        n = ctypes.c_int()
+        buf = (ctypes.c_int64 * 8)()
+        call("glued_get_multiwalker_ptrs", self.h, int(bias_index), buf, 8, ctypes.byref(n))
+        return [int(buf[k]) for k in range(n.value)]
+
+    def setMultiWalkerPtrs(self, bias_index, ptrs, primary=None):
+        """Share another context's table of this bias (its getMultiWalkerPtrs);
+        `primary`, when given, is kept alive as long as this context."""
+        arr = (ctypes.c_int64 * max(len(ptrs), 1))(*[int(p) for p in ptrs])
+        call("glued_redirect_to_primary_bias", self.h, int(bias_index), arr, len(ptrs))
+        if primary is not None:
+            self._primaries = getattr(self, "_primaries", []) + [primary]
 
     def getOPESMetrics(self, bias_index=0, walker=0):
         out = np.zeros(4)

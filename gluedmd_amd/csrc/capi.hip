@@ -137,6 +137,24 @@ int glued_share_bias(glued_handle h, int biasIdx, int shared)
     return guard([&] { E(h).setShared(biasIdx, shared); });
 }
 
+int glued_get_multiwalker_ptrs(glued_handle h, int biasIdx, int64_t* ptrs, int capacity, int* count)
+{
+    return guard([&] {
+        const std::vector<long long> p = E(h).multiWalkerPtrs(biasIdx);
+        if (count) *count = (int)p.size();
+        for (int k = 0; k < (int)p.size() && k < capacity; k++) ptrs[k] = p[k];
+    });
+}
+
+int glued_redirect_to_primary_bias(glued_handle h, int biasIdx, const int64_t* ptrs, int count)
+{
+    return guard([&] {
+        if (count < 0 || (count > 0 && !ptrs)) throw std::invalid_argument("redirect_bias: no pointers");
+        std::vector<long long> p(ptrs, ptrs + count);
+        E(h).redirectBias(biasIdx, p.data(), count);
+    });
+}
+
 int glued_initialize(glued_handle h, int numParticles, int paddedNumParticles, const double* masses, int nMasses)
 {
     return guard([&] {
@@ -177,6 +195,12 @@ int glued_execute(glued_handle h, void* stream, int walker, const void* posq, in
                                 includeForces != 0, includeEnergy != 0);
         if (energy) *energy = e;
     });
+}
+
+int glued_execute_plain_all(glued_handle h, void* stream, const void* posq, int64_t* forces, int includeForces)
+{
+    return guard([&] { E(h).executePlainAll((hipStream_t)stream, (const float4*)posq, (long long*)forces,
+                                            includeForces != 0); });
 }
 
 int glued_execute_batch(glued_handle h, void* stream, const void* posq, int64_t* forces, int includeForces,

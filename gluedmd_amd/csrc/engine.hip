@@ -112,6 +112,117 @@ void Engine::setShared(int bias, int shared)
     biases_[bias].shared = shared ? 1 : 0;
 }
 
+// Cross-handle sharing (CommonGluedKernels.cpp:5234-5262 and the kernel
+// interface's getMultiWalkerPtrs, GluedKernels.h:89-104).  A table is shared
+// whole: for OPES the kernels, the counts and the normalisation sums (sum_uprob,
+// KDNorm, sample count), so one-walker handles pointed at one primary and
+// stepped in turn on one stream give the bits of one handle whose walkers
+// share the table.  Only single-table biases take part (one walker, or a bias
+// shared by all walkers of the handle).
+namespace {
+void require_device_range(long long ptr, size_t bytes, const char* what)
+{
+    hipDeviceptr_t base = nullptr;
+    size_t size = 0;
+    if (ptr == 0 || hipMemGetAddressRange(&base, &size, (hipDeviceptr_t)(uintptr_t)ptr) != hipSuccess) {
+        (void)hipGetLastError();
+        fail(std::string("redirect_bias: ") + what + " is not device memory of this process");
+    }
+    const uintptr_t lo = (uintptr_t)base, p = (uintptr_t)ptr;
+    if (p < lo || p + bytes > lo + size)
+        fail(std::string("redirect_bias: ") + what + " is smaller than this bias's table (different shape?)");
+}
+}  // namespace
+
+std::vector<long long> Engine::multiWalkerPtrs(int bias)
+{
+    if (!initialized_) fail("multiwalker_ptrs: initialize first");
+    if (bias < 0 || bias >= (int)biases_.size()) fail("multiwalker_ptrs: bias index out of range");
+    const int T = biases_[bias].shared ? 1 : W_;
+    if (T != 1) fail("multiwalker_ptrs: the bias has one table per walker; share it or use one walker");
+    for (auto* g : grids_)
+        if (g->spec == bias) return {(long long)(uintptr_t)g->d.grid};
+    for (auto* h : hills_)
+        if (h->spec == bias)
+            return {(long long)(uintptr_t)h->d.centers, (long long)(uintptr_t)h->d.heights,
+                    (long long)(uintptr_t)h->d.count};
+    for (auto* o : opes_)
+        if (o->spec == bias)
+            return {(long long)(uintptr_t)o->d.centers, (long long)(uintptr_t)o->d.sigmas,
+                    (long long)(uintptr_t)o->d.logw, (long long)(uintptr_t)o->d.ints,
+                    (long long)(uintptr_t)o->d.dbl};
+    fail("multiwalker_ptrs: only METAD, METAD_HILLS and OPES tables can be shared between handles");
+}
+
+void Engine::redirectBias(int bias, const long long* p, int n)
+{
+    if (!initialized_) fail("redirect_bias: initialize first");
+    if (bias < 0 || bias >= (int)biases_.size()) fail("redirect_bias: bias index out of range");
+    const int T = biases_[bias].shared ? 1 : W_;
+    if (T != 1) fail("redirect_bias: the bias has one table per walker; share it or use one walker");
+    auto want = [&](int k) {
+        if (n != k) fail("redirect_bias: expected " + std::to_string(k) + " pointers, got " + std::to_string(n));
+    };
+    bool done = false;
+    for (size_t k = 0; k < grids_.size() && !done; k++) {
+        GridHost* g = grids_[k];
+        if (g->spec != bias) continue;
+        want(1);
+        require_device_range(p[0], sizeof(double) * (size_t)g->d.total, "grid");
+        g->d.grid = (double*)(uintptr_t)p[0];
+        hip_check(hipMemcpy(dGridTable_.as<GridBiasDev>() + k, &g->d, sizeof(GridBiasDev), hipMemcpyHostToDevice),
+                  "redirect_bias: grid table");
+        for (size_t i = 0; i < smallBias_.size(); i++) {
+            SmallBiasItem& it = smallBias_[i];
+            if (it.type != BIAS_METAD || it.bias != (int)k) continue;
+            it.grid = g->d;
+            hip_check(hipMemcpy(dSmallBias_.as<SmallBiasItem>() + i, &it, sizeof(SmallBiasItem),
+                                hipMemcpyHostToDevice),
+                      "redirect_bias: small-bias table");
+        }
+        done = true;
+    }
+    for (auto* h : hills_) {
+        if (done || h->spec != bias) continue;
+        want(3);
+        require_device_range(p[0], sizeof(float) * (size_t)h->d.D * h->d.cap, "hill centres");
+        require_device_range(p[1], sizeof(float) * (size_t)h->d.cap, "hill heights");
+        require_device_range(p[2], sizeof(int), "hill count");
+        h->d.centers = (float*)(uintptr_t)p[0];
+        h->d.heights = (float*)(uintptr_t)p[1];
+        h->d.count = (int*)(uintptr_t)p[2];
+        done = true;
+    }
+    for (auto* o : opes_) {
+        if (done || o->spec != bias) continue;
+        want(5);
+        const size_t D = (size_t)o->d.D, cap = (size_t)o->d.cap;
+        require_device_range(p[0], sizeof(double) * D * cap, "OPES centres");
+        require_device_range(p[1], sizeof(double) * D * cap, "OPES widths");
+        require_device_range(p[2], sizeof(double) * cap, "OPES log-weights");
+        require_device_range(p[3], sizeof(int) * 4, "OPES counts");
+        require_device_range(p[4], sizeof(double) * (4 + 2 * D), "OPES sums");
+        o->d.centers = (double*)(uintptr_t)p[0];
+        o->d.sigmas = (double*)(uintptr_t)p[1];
+        o->d.logw = (double*)(uintptr_t)p[2];
+        o->d.ints = (int*)(uintptr_t)p[3];
+        o->d.dbl = (double*)(uintptr_t)p[4];
+        done = true;
+    }
+    if (!done) fail("redirect_bias: only METAD, METAD_HILLS and OPES tables can be shared between handles");
+    // the chunked-bias tables and the graphs hold the old addresses
+    for (size_t k = 0; k < hills_.size(); k++)
+        hip_check(hipMemcpy(dHillTable_.as<HillBiasDev>() + k, &hills_[k]->d, sizeof(HillBiasDev),
+                            hipMemcpyHostToDevice),
+                  "redirect_bias: hill table");
+    if (!hills_.empty()) fplan_.hill0 = hills_[0]->d;
+    for (size_t k = 0; k < opes_.size(); k++)
+        hip_check(hipMemcpy(dOpesTable_.as<OpesBiasDev>() + k, &opes_[k]->d, sizeof(OpesBiasDev),
+                            hipMemcpyHostToDevice),
+                  "redirect_bias: OPES table");
+    dropGraphs();
+}
+
 void Engine::setTestBiasGradients(const double* g, int n)
 {
     if (initialized_) fail("setTestBiasGradients: must be called before initialization");
@@ -1505,6 +1616,23 @@ double Engine::execute(hipStream_t s, int w, const float4* posq, long long* forc
         for (size_t b = 0; b < biases_.size(); b++) e += be[b];
     }
     return e;
+}
+
+// The plain form for every walker, looped on the host side of the library:
+// walker w's launch sequence is the one execute(w) issues, back to back on s.
+void Engine::executePlainAll(hipStream_t s, const float4* posq, long long* force, bool forces)
+{
+    if (!initialized_) fail("execute_plain_all: initialize first");
+    if (V_ == 0) return;
+    lastPos_ = posq;
+    lastForce_ = force;
+    if (!tuned_) {
+        tuned_ = true;
+        if (!evaluated_) adoptCachedTuning();
+    }
+    for (int w = 0; w < W_; w++) launchStep(s, w, 1, forces);
+    cvReady_ = true;
+    evaluated_ = true;
 }
 
 void Engine::executeStage(hipStream_t s, const float4* posq, long long* force, int stage)

@@ -84,6 +84,10 @@ class Engine {
     int addCV(int type, const int* atoms, int na, const double* params, int np);
     int addBias(int type, const int* cvs, int nc, const double* params, int np, const int* ip, int ni);
     void setShared(int bias, int shared);
+    // one bias table shared by several handles: the device addresses of a
+    // bias's table, and pointing this handle's table at another's
+    std::vector<long long> multiWalkerPtrs(int bias);
+    void redirectBias(int bias, const long long* ptrs, int n);
     void setTestBiasGradients(const double* g, int n);
     int numValues() const { return numValues_; }
     bool pbc = false;
@@ -98,6 +102,7 @@ class Engine {
     // ---- step ----
     double execute(hipStream_t s, int w, const float4* posq, long long* force, bool forces, bool energy);
     void executeBatch(hipStream_t s, const float4* posq, long long* force, bool forces, bool useGraph);
+    void executePlainAll(hipStream_t s, const float4* posq, long long* force, bool forces);
     void updateState(hipStream_t s, int step);
     // One MD step for every walker: updateState(step) then the batched
     // evaluation, replayed as one hipGraph per deposit pattern.

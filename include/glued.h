@@ -92,10 +92,31 @@ int glued_num_cv_values(glued_handle h, int* n);
 /* Replaces GluedForce::setTestBiasGradients (GluedForce.cpp:186-189): inject
  * dU/dCV, skipping bias evaluation (CommonGluedKernels.cpp:4670-4671). */
 int glued_set_test_bias_gradients(glued_handle h, const double* grads, int n);
-/* Replaces GluedForce::setMultiWalkerPtrs / redirectToPrimaryBias
- * (GluedForce.cpp:257-270, CommonGluedKernels.cpp:5234-5262): one bias state
- * shared by every walker of the handle.  Call before initialization. */
+/* One bias state shared by every walker of the handle (the within-context
+ * form of the reference's multi-walker sharing, CommonGluedKernels.cpp
+ * :2912-2933).  Call before initialization. */
 int glued_share_bias(glued_handle h, int biasIdx, int shared);
+/* Replaces GluedForce::getMultiWalkerPtrs (GluedForce.cpp:257-262,
+ * GluedKernels.h:89-96): the device addresses of bias biasIdx's table, for
+ * glued_redirect_to_primary_bias on another handle of the same device.
+ * METAD: {grid}; METAD_HILLS: {centres, heights, count}; OPES: {centres,
+ * widths, log-weights, counts (numKernels, numAllocated, steps, samples),
+ * sums (sum_uprob, KDNorm, sum_w, sum_w2, Welford mean and M2)}.  *count
+ * receives the number of addresses (at most capacity are written).  The bias
+ * must have one table (a one-walker handle, or glued_share_bias).  After
+ * initialization. */
+int glued_get_multiwalker_ptrs(glued_handle h, int biasIdx, int64_t* ptrs, int capacity, int* count);
+/* Replaces GluedForce::setMultiWalkerPtrs / redirectToPrimaryBias
+ * (GluedForce.cpp:264-270, CommonGluedKernels.cpp:5234-5262): this handle's
+ * table of bias biasIdx becomes the table at ptrs (from
+ * glued_get_multiwalker_ptrs of a handle with the same bias definition).
+ * The whole table is shared, its OPES sums included, so handles stepped in
+ * turn on one stream give the results of one handle whose walkers share the
+ * table.  Each address is checked to lie in a device allocation large enough
+ * for this bias's table.  The primary handle must outlive this one; handles
+ * sharing a table must not run deposits concurrently (one stream, or ordered
+ * streams). */
+int glued_redirect_to_primary_bias(glued_handle h, int biasIdx, const int64_t* ptrs, int count);
 
 /* ---- initialization ---------------------------------------------------- */
 
@@ -126,6 +147,12 @@ int glued_set_energy_input(glued_handle h, int walker, double U, const float* fo
  * includeEnergy (one host synchronisation), else 0. */
 int glued_execute(glued_handle h, void* stream, int walker, const void* posq, int64_t* forces, int includeForces,
                   int includeEnergy, double* energy);
+/* glued_execute (without the energy read-back) for walkers 0..W-1 in turn,
+ * looped inside the library: posq and forces are the batch bases.  The
+ * reference's structure for several contexts (one CalcGluedForceKernel::
+ * execute per context, CommonGluedKernels.cpp:4646-5048) with no per-walker
+ * foreign-call overhead; results equal W glued_execute calls. */
+int glued_execute_plain_all(glued_handle h, void* stream, const void* posq, int64_t* forces, int includeForces);
 /* Batched form: every walker in one launch per stage; useGraph = 1 replays
  * the step as one captured hipGraph. */
 int glued_execute_batch(glued_handle h, void* stream, const void* posq, int64_t* forces, int includeForces,

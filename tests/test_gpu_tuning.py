@@ -60,6 +60,12 @@ def test_autotune_measures_caches_and_keeps_parity():
     for w in range(W):
         assert np.array_equal(g.get_forces_fixed(w), Fb[w]), w
         assert np.array_equal(g.getLastCVValues(w), cvb[w]), w
+    # the library's walker loop equals one glued_execute call per walker
+    g.zero_forces()
+    for w in range(W):
+        g.execute(w, True, False)
+    for w in range(W):
+        assert np.array_equal(g.get_forces_fixed(w), Fb[w]), w
     # a second engine of the same plan shape takes the cached choice
     g2 = Context(f, walkers=[n] * W)
     g2.set_box(L)
