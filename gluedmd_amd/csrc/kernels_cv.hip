@@ -1225,15 +1225,26 @@ __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBu
 // The workgroup size CB (256, 512 or 1,024 threads) is a launch-shape choice
 // of the autotuner; every size gives the same pairs and the same exact
 // integer sums, the FP64 per-lane partial sums differ in rounding only.
-constexpr int COORD_PCAP = 3072;  // partners staged in LDS per chunk
-constexpr int COORD_HCAP = 256;   // homes staged in LDS per chunk
+constexpr int COORD_PCAP = 2560;  // partners staged in LDS per chunk
+constexpr int COORD_HCAP = 128;   // homes staged in LDS per chunk
 
+// Staged atoms are held twice: the positions as given (the exact FP64
+// evaluation), and for the FP32 pre-check their image nearest the home
+// cell's centre, relative to that centre (SoA).  A pair within the cutoff
+// is within 1.5 cell edges of the centre (the home is within half an edge),
+// and a cell edge exceeds the cutoff with at least 3 cells per axis, so the
+// nearest images of both atoms are the pair's minimum image: the pre-check
+// needs no per-pair image arithmetic.  Two workgroups of 512 threads fit in
+// a CU's LDS.
 template <int CB>
 struct CoordCellShared {
     static constexpr int WAVES = CB / 64;
     float4 part[COORD_PCAP];
+    float pw[3][COORD_PCAP];
     float4 home[COORD_HCAP];
+    float hw[3][COORD_HCAP];
     long long g[COORD_HCAP][3];
+    double cen[3];
     short q[WAVES][128];
     int cellB[27], pre[28];
     int homeB, homeN;
@@ -1274,8 +1285,19 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
     }
     const WalkerBox wb = load_box(v, w);
     const double* bx = wb.b;
-    float bxf[12];
-    for (int k = 0; k < 12; k++) bxf[k] = (float)bx[k];
+    // the home cell's centre (in LDS: it is needed only while staging) and
+    // the nearest image relative to it
+    if (tid == 64) {
+        const int hcx = hc % nx, hcy = (hc / nx) % ny, hcz = hc / (nx * ny);
+        sh.cen[0] = ((double)hcx + 0.5) * bx[0] / (double)nx;
+        sh.cen[1] = ((double)hcy + 0.5) * bx[4] / (double)ny;
+        sh.cen[2] = ((double)hcz + 0.5) * bx[8] / (double)nz;
+    }
+    auto rel = [&](float x, int d) -> float {
+        double t = (double)x - sh.cen[d];
+        if (v.pbc) t -= bx[4 * d] * rint(t * bx[9 + d]);
+        return (float)t;
+    };
     const SwitchD sw = make_switch(it);
     const float rc2f = (float)(it.rc2 * (1.0 + 1e-5));
     __syncthreads();
@@ -1290,7 +1312,11 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
         const int hn = min(COORD_HCAP, H - h0);
         __syncthreads();
         for (int i = tid; i < hn; i += COORD_CB) {
-            sh.home[i] = sorted[homeB + h0 + i];
+            const float4 hq = sorted[homeB + h0 + i];
+            sh.home[i] = hq;
+            sh.hw[0][i] = rel(hq.x, 0);
+            sh.hw[1][i] = rel(hq.y, 1);
+            sh.hw[2][i] = rel(hq.z, 2);
             sh.g[i][0] = sh.g[i][1] = sh.g[i][2] = 0;
         }
         for (int p0 = 0; p0 < P; p0 += COORD_PCAP) {
@@ -1299,7 +1325,7 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
             {
                 // every source index first (neighbour cell of flat index q:
                 // pre[c] <= q < pre[c + 1]), then all loads, then the stores
-                constexpr int PER = COORD_PCAP / COORD_CB;
+                constexpr int PER = (COORD_PCAP + COORD_CB - 1) / COORD_CB;
                 int src[PER];
 #pragma unroll
                 for (int u = 0; u < PER; u++) {
@@ -1316,8 +1342,15 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
 #pragma unroll
                 for (int u = 0; u < PER; u++) pv[u] = sorted[src[u]];
 #pragma unroll
-                for (int u = 0; u < PER; u++)
-                    if (tid + u * COORD_CB < pn) sh.part[tid + u * COORD_CB] = pv[u];
+                for (int u = 0; u < PER; u++) {
+                    const int i = tid + u * COORD_CB;
+                    if (i < pn) {
+                        sh.part[i] = pv[u];
+                        sh.pw[0][i] = rel(pv[u].x, 0);
+                        sh.pw[1][i] = rel(pv[u].y, 1);
+                        sh.pw[2][i] = rel(pv[u].z, 2);
+                    }
+                }
             }
             __syncthreads();
             // tasks: (HPT homes, one of nSub interleaved ranges of their
@@ -1330,7 +1363,8 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
                 const int grp = t / nSub, sub = t - grp * nSub;
                 const int myHome = grp * HPT + (HPT == 2 ? (lane >> 5) : 0);
                 const bool myOk = myHome < hn;
-                const float4 hp = sh.home[min(myHome, hn - 1)];
+                const int hcl = min(myHome, hn - 1);
+                const float hx = sh.hw[0][hcl], hy = sh.hw[1][hcl], hz = sh.hw[2][hcl];
                 const float4 hp0 = sh.home[grp * HPT];
                 const float4 hp1 = sh.home[min(grp * HPT + HPT - 1, hn - 1)];
                 double acc[HPT][3];  // this lane's row terms of the task per home, in queue order
@@ -1377,17 +1411,9 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
                 };
                 for (int pbk = sub; pbk < nBlk; pbk += nSub) {
                     const int pi = pbk * LPH + (lane % LPH);
-                    const float4 pp = sh.part[min(pi, pn - 1)];
-                    bool cand = myOk && pi < pn;
-                    if (cand) {  // orthorhombic (cells need it), box >= 3 cutoffs: the FP32 image is never ambiguous
-                        float fx = hp.x - pp.x, fy = hp.y - pp.y, fz = hp.z - pp.z;
-                        if (v.pbc) {
-                            fx -= bxf[0] * rintf(fx * bxf[9]);
-                            fy -= bxf[4] * rintf(fy * bxf[10]);
-                            fz -= bxf[8] * rintf(fz * bxf[11]);
-                        }
-                        cand = fx * fx + fy * fy + fz * fz < rc2f;
-                    }
+                    const int pcl = min(pi, pn - 1);
+                    const float fx = hx - sh.pw[0][pcl], fy = hy - sh.pw[1][pcl], fz = hz - sh.pw[2][pcl];
+                    const bool cand = myOk && pi < pn && fx * fx + fy * fy + fz * fz < rc2f;
                     const unsigned long long m = __ballot(cand);
                     if (cand)
                         Q[(tail + (unsigned)__popcll(m & ((1ull << lane) - 1ull))) & 127] =
@@ -1481,7 +1507,7 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
 }
 
 template <int CB>
-__global__ __launch_bounds__(CB) void k_coord_cells(StepView v, CoordBuffers cb, const CoordItem* __restrict__ items,
+__global__ __launch_bounds__(CB) __attribute__((amdgpu_waves_per_eu(CB == 256 ? 2 : 4))) void k_coord_cells(StepView v, CoordBuffers cb, const CoordItem* __restrict__ items,
                                                     int item, int w0)
 {
     __shared__ CoordCellShared<CB> sh;
