@@ -394,20 +394,26 @@ def time_fn(fn, reps):
     return e0.elapsed_time(e1) / reps * 1e-3
 
 
-def time_cold(fn, reps, flush_mib=512):
+def time_cold(fn, reps, flush_mib=512, write=False):
     """Average GPU duration of one call with nothing of its working set in the
-    caches: before each call a 512 MiB buffer is overwritten on the same
-    stream (more than the 256 MB memory-side cache), then the call is timed
-    alone between two HIP events.  flush_mib=0: the same single-call timing
-    without the flush."""
+    caches: before each call a 512 MiB buffer (more than the 256 MB
+    memory-side cache) is read on the same stream, then the call is timed
+    alone between two HIP events.  The flush reads, so the caches it leaves
+    hold clean lines; write=True overwrites the buffer instead, and the call
+    then also pays for writing back the flush's dirty lines it evicts.
+    flush_mib=0: the same single-call timing without the flush."""
     import torch
     stream = torch.cuda.current_stream()
-    buf = torch.empty(max(flush_mib, 1) * 2**18, dtype=torch.float32, device="cuda")
+    buf = torch.ones(max(flush_mib, 1) * 2**18, dtype=torch.float32, device="cuda")
+    sink = torch.zeros(reps, dtype=torch.float32, device="cuda")
+    torch.cuda.synchronize()
     ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(reps)]
     fn()
     for k in range(reps):
-        if flush_mib:
+        if flush_mib and write:
             buf.fill_(float(k))
+        elif flush_mib:
+            torch.sum(buf, dim=0, out=sink[k])
         ev[k][0].record(stream)
         fn()
         ev[k][1].record(stream)
@@ -577,6 +583,7 @@ def main():
     # with the caches flushed before every launch ----
     t_launch = wk.time_dominant(200)
     t_cold = time_cold(wk.dominant, 50)
+    t_cold_w = time_cold(wk.dominant, 50, write=True)
     t_alone = time_cold(wk.dominant, 50, flush_mib=0)
     rf = wk.roofline(t_launch)
     rf_cold = wk.roofline(t_cold)
@@ -585,8 +592,11 @@ def main():
     rf["avg_launch_us"] = round(t_launch * 1e6, 3)
     rf["cold"] = {"achieved": round(rf_cold["achieved"], 3), "frac": round(rf_cold["achieved"] / rf["peak"], 4),
                   "avg_launch_us": round(t_cold * 1e6, 3), "warm_alone_us": round(t_alone * 1e6, 3),
-                  "how": "512 MiB overwritten on the stream before each of 50 launches, each launch timed alone "
-                         "(warm_alone_us: the same single-launch timing without the flush)"}
+                  "after_writes_us": round(t_cold_w * 1e6, 3),
+                  "how": "512 MiB read on the stream before each of 50 launches, each launch timed alone "
+                         "(warm_alone_us: the same single-launch timing without the flush; after_writes_us: "
+                         "the flush overwrites the 512 MiB instead, and the launch also writes back the dirty "
+                         "lines it evicts)"}
     rf["traffic"] = pmc_traffic(wk.name, ROOFLINE_KERNELS[wk.name])
     rf["traffic_kernels"] = ROOFLINE_KERNELS[wk.name]
     stage_t = {}
