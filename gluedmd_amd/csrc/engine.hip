@@ -600,6 +600,8 @@ void Engine::buildPlan(int nMass, const double* masses)
     dGidx_.alloc(sizeof(int) * (size_t)W_ * std::max(S_, 1));
     groupSlots_ = groupSlots;
     pathSlots_ = pathSlots;
+    coordSlots_ = coordSlots;
+    dCoordPos_.alloc(sizeof(int) * (size_t)W_ * std::max((int)coordSlots_.size(), 1));
     dGroupPos_.alloc(sizeof(int) * (size_t)W_ * std::max((int)groupSlots_.size(), 1));
     dPathPos_.alloc(sizeof(int) * (size_t)W_ * std::max((int)pathSlots_.size(), 1));
     if (energyValue_ >= 0) {
@@ -1225,6 +1227,7 @@ void Engine::uploadGidx(int w)
     };
     resolve(groupSlots_, dGroupPos_);
     resolve(pathSlots_, dPathPos_);
+    resolve(coordSlots_, dCoordPos_);
     if (dSlotMask_.bytes) {
         // per 32-atom word: the bits of the atoms with Jacobian rows and the
         // count of such atoms in the earlier words; the row ranges of those
@@ -1484,6 +1487,8 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
     if (!coord_.empty()) {
         CoordBuffers cb;
         cb.slots = dCoordSlots_.as<int>();
+        cb.pos = dCoordPos_.as<int>();
+        cb.posStride = std::max((int)coordSlots_.size(), 1);
         cb.cellOf = dCellOf_.as<int>();
         cb.cellCount = dCellCount_.as<int>();
         cb.cellStart = dCellStart_.as<int>();

@@ -180,7 +180,7 @@ class Engine {
     std::vector<int> natoms_, padded_;
     std::vector<long long> posOff_, forceOff_;
     std::vector<int> slotAtom_;          // user atom of every slot
-    std::vector<int> groupSlots_, pathSlots_;  // slot of every group / path gather entry
+    std::vector<int> groupSlots_, pathSlots_, coordSlots_;  // slot of every group / path / coordination gather entry
     std::vector<std::vector<int>> userToGpu_;
     std::vector<double> boxHost_;        // [W][12]
     std::vector<int> rowSlot_, rowCv_;
@@ -253,7 +253,7 @@ class Engine {
     DBuf dWalkers_, dGidx_, dBox_, dCv_, dJ_, dBgrad_, dBener_, dUd_;
     DBuf dSmall_, dGroup_, dGroupBig_, dGroupPos_, dGroupMass_, dGroupRef_;
     DBuf dCm_, dCmPart_, dCmCount_;
-    DBuf dCoord_, dCoordSlots_, dCellOf_, dCellCount_, dCellStart_, dSorted_, dUnsorted_, dCnAcc_, dCnAccLo_, dPairAcc_,
+    DBuf dCoord_, dCoordSlots_, dCoordPos_, dCellOf_, dCellCount_, dCellStart_, dSorted_, dUnsorted_, dCnAcc_, dCnAccLo_, dPairAcc_,
         dNpairs_, dRank_, dBlockCnt_, dCoordTicket_;
     int cellOfStride_ = 0, cellStride_ = 0, sortStride_ = 0, blkStride_ = 0;
     DBuf dPath_, dPathPos_, dPathFrames_, dPathW_;

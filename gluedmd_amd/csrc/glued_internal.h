@@ -264,6 +264,7 @@ namespace glued {
 // Coordination scratch (cell lists, sorted atoms, per-block partial sums).
 struct CoordBuffers {
     const int* slots;      // coordSlots
+    const int* pos;        // [W][posStride] posq index of every coordSlots entry (walker offset included)
     int* cellOf;           // [W][cellOfStride]
     int* cellCount;        // [W][cellStride]
     int* cellStart;        // [W][cellStride + items]  (scan output, 1 extra per item)
@@ -276,7 +277,7 @@ struct CoordBuffers {
     unsigned long long* pairAcc;    // [W][items] pairs inside the cutoff (same life cycle)
     long long* npairs;              // [W][items]
     unsigned int* ticket;           // [W][items] group-A cell workgroups done with their CN sums
-    int cellOfStride, cellStride, sortStride, blkStride, nItems;
+    int cellOfStride, cellStride, sortStride, blkStride, nItems, posStride;
 };
 
 // Wide group CVs launched as extra workgroups of another launch (the

@@ -786,29 +786,30 @@ int coord_sort_fits(int ncellBoth, int n) { return ncellBoth <= CELL_LDS && ncel
 
 constexpr int CELL_WAVE_TAB = 2048;  // cells (both groups) up to which the waves' counts go to a per-wave table
 
-__global__ __launch_bounds__(CELL_BLOCK) void k_coord_rank(StepView v, CoordBuffers cb, const CoordItem* items,
-                                                           int item, int w0)
+__global__ __launch_bounds__(CELL_BLOCK) void k_coord_rank(StepView v, CoordBuffers cb, CoordItem it, int item,
+                                                           int w0)
 {
     constexpr int WAVES = CELL_BLOCK / 64;
     __shared__ int lcnt[CELL_LDS];
     __shared__ unsigned short wcnt[WAVES][CELL_WAVE_TAB];  // per wave, its atoms per cell
     const int w = w0 + blockIdx.y, tid = threadIdx.x, b = blockIdx.x;
-    const CoordItem it = items[item];
     const int n = it.nA + it.nB, nb = gridDim.x;
     const int ncell = it.nc[0] * it.nc[1] * it.nc[2], n2 = 2 * ncell;
     const bool table = n2 <= CELL_WAVE_TAB;  // uniform
-    if (table)
-        for (int k = tid; k < WAVES * n2; k += CELL_BLOCK) wcnt[k / n2][k % n2] = 0;
-    else
-        for (int c = tid; c < n2; c += CELL_BLOCK) lcnt[c] = 0;
-    __syncthreads();
     const int i = b * CELL_BLOCK + tid, lane = tid & 63, wv = tid >> 6;
+    if (table) {
+        // each wave clears its own row (its LDS operations complete in order)
+        for (int k = lane; k < n2; k += 64) wcnt[wv][k] = 0;
+    } else {
+        for (int c = tid; c < n2; c += CELL_BLOCK) lcnt[c] = 0;
+        __syncthreads();
+    }
     const bool valid = i < n;
     int c = -1;
     const size_t o = (size_t)w * cb.sortStride + it.sortBase + i;
     if (valid) {
         const bool a = i < it.nA;
-        const float4 q = load_pos(v, w, cb.slots[a ? it.aBegin + i : it.bBegin + (i - it.nA)]);
+        const float4 q = v.posq[cb.pos[(size_t)w * cb.posStride + (a ? it.aBegin + i : it.bBegin + (i - it.nA))]];
         c = cell_of(q.x, q.y, q.z, v.box + (size_t)w * 12, it.nc) + (a ? 0 : ncell);
         cb.cellOf[(size_t)w * cb.cellOfStride + it.sortBase + i] = c;
         cb.unsorted[o] = make_float4(q.x, q.y, q.z, __int_as_float(a ? i : i - it.nA));
@@ -872,7 +873,7 @@ constexpr int CELL_SCAN_PER = 16;  // table entries per thread held in registers
 // so take no launch of their own in the batched step).
 static_assert(CELL_BLOCK == GROUP_BIG_BLOCK, "wide group CVs ride in the placement launch");
 
-__global__ __launch_bounds__(CELL_BLOCK) void k_coord_place(CoordBuffers cb, const CoordItem* items, int item, int w0,
+__global__ __launch_bounds__(CELL_BLOCK) void k_coord_place(CoordBuffers cb, CoordItem it, int item, int w0,
                                                             StepView v, GroupExtra ge)
 {
     __shared__ int sOff[CELL_LDS];
@@ -884,7 +885,6 @@ __global__ __launch_bounds__(CELL_BLOCK) void k_coord_place(CoordBuffers cb, con
         return;
     }
     const int w = w0 + blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
-    const CoordItem it = items[item];
     const int n = it.nA + it.nB;
     const int ncell = it.nc[0] * it.nc[1] * it.nc[2], n2 = 2 * ncell, len = n2 * nb;
     const int* tab = cb.blockCnt + (size_t)w * cb.blkStride + it.blkBase;
@@ -1209,9 +1209,9 @@ __global__ __launch_bounds__(COORD_BLOCK) void k_coord_pairs(StepView v, CoordBu
 // cell and its 27 neighbour cells of the other group, then stages the homes
 // and the neighbours' atoms in LDS with coalesced loads (one round trip;
 // more than COORD_PCAP partners or COORD_HCAP homes are taken in chunks).
-// A task is one group-A home (or one of nSub interleaved ranges of its
-// partner blocks when the cell has few homes), or two group-B homes, one per
-// half-wave; lanes pre-check their partners in FP32 (1e-5 relative margin),
+// A task is one group-A home and a contiguous range of its partner blocks
+// (each wave takes an equal share of the cell's (home, block) units), or
+// two group-B homes, one per half-wave; lanes pre-check their partners in FP32 (1e-5 relative margin),
 // candidates are queued in LDS and evaluated 64 at a time on the exact FP64
 // predicate with FP64 switching.  Each lane sums its row terms of a task in
 // FP64 in its (fixed) queue order; a DPP wave sum (fixed order) gives the
@@ -1252,8 +1252,8 @@ struct CoordCellShared {
     long long hi[WAVES];
 };
 
-// One home cell.  SIDE_A: homes are group-A atoms (a task is one home, or
-// one of nSub interleaved ranges of its partner blocks); otherwise group-B
+// One home cell.  SIDE_A: homes are group-A atoms (a task is one home and
+// a contiguous range of its partner blocks); otherwise group-B
 // atoms, two homes per task, one per half-wave (a B atom has few partners,
 // so a task per home would end in a mostly empty evaluation round).
 template <bool SIDE_A, int CB>
@@ -1353,14 +1353,15 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
                 }
             }
             __syncthreads();
-            // tasks: (HPT homes, one of nSub interleaved ranges of their
-            // partner blocks), at least two tasks per wave when the cell has
-            // few homes
+            // tasks: HPT homes and a range of their partner blocks.  Group
+            // A: the (home, block) units of the chunk in home-major order,
+            // split into one contiguous, equal range per wave, so that the
+            // waves finish together and each ends at most a few tasks (one
+            // per home it touches) with a partly filled evaluation round.
+            // Group B: two homes per task, all of their partner blocks.
             const int nBlk = (pn + LPH - 1) / LPH;
             const int nGroups = (hn + HPT - 1) / HPT;
-            const int nSub = SIDE_A ? max(1, min(nBlk / 4, (2 * COORD_CB_WAVES + nGroups - 1) / nGroups)) : 1;
-            for (int t = wv; t < nGroups * nSub; t += COORD_CB_WAVES) {
-                const int grp = t / nSub, sub = t - grp * nSub;
+            auto run_task = [&](int grp, int pbBeg, int pbEnd) {
                 const int myHome = grp * HPT + (HPT == 2 ? (lane >> 5) : 0);
                 const bool myOk = myHome < hn;
                 const int hcl = min(myHome, hn - 1);
@@ -1409,7 +1410,7 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
                         np++;
                     }
                 };
-                for (int pbk = sub; pbk < nBlk; pbk += nSub) {
+                for (int pbk = pbBeg; pbk < pbEnd; pbk++) {
                     const int pi = pbk * LPH + (lane % LPH);
                     const int pcl = min(pi, pn - 1);
                     const float fx = hx - sh.pw[0][pcl], fy = hy - sh.pw[1][pcl], fz = hz - sh.pw[2][pcl];
@@ -1448,6 +1449,17 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
                         if (lane == j) val = tot[j];
                     if (h < hn) atomicAdd((unsigned long long*)&sh.g[h][c], (unsigned long long)to_fix(val));
                 }
+            };
+            if (SIDE_A) {
+                const int units = hn * nBlk;
+                const int u1 = (int)((long long)units * (wv + 1) / COORD_CB_WAVES);
+                for (int u = (int)((long long)units * wv / COORD_CB_WAVES); u < u1;) {
+                    const int grp = u / nBlk, b = u - grp * nBlk, e = min(nBlk, b + (u1 - u));
+                    run_task(grp, b, e);
+                    u += e - b;
+                }
+            } else {
+                for (int t = wv; t < nGroups; t += COORD_CB_WAVES) run_task(t, 0, nBlk);
             }
         }
         __syncthreads();
@@ -1507,12 +1519,10 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
 }
 
 template <int CB>
-__global__ __launch_bounds__(CB) __attribute__((amdgpu_waves_per_eu(CB == 256 ? 2 : 4))) void k_coord_cells(StepView v, CoordBuffers cb, const CoordItem* __restrict__ items,
-                                                    int item, int w0)
+__global__ __launch_bounds__(CB) __attribute__((amdgpu_waves_per_eu(CB == 256 ? 2 : 4))) void k_coord_cells(StepView v, CoordBuffers cb, CoordItem it, int item, int w0)
 {
     __shared__ CoordCellShared<CB> sh;
     const int w = w0 + blockIdx.y;
-    const CoordItem it = items[item];
     const int ncell = it.nc[0] * it.nc[1] * it.nc[2];
     if ((int)blockIdx.x < ncell) coord_cell<true, CB>(v, cb, it, item, w, blockIdx.x, sh);
     else coord_cell<false, CB>(v, cb, it, item, w, blockIdx.x - ncell, sh);
@@ -1532,9 +1542,8 @@ bool launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordI
                 ge = *extra;
                 carried = true;
             }
-            hipLaunchKernelGGL(k_coord_rank, dim3(nb, nw), dim3(CELL_BLOCK), 0, s, v, cb, dItems, item, w0);
-            hipLaunchKernelGGL(k_coord_place, dim3(nb + ge.n, nw), dim3(CELL_BLOCK), 0, s, cb, dItems, item, w0, v,
-                               ge);
+            hipLaunchKernelGGL(k_coord_rank, dim3(nb, nw), dim3(CELL_BLOCK), 0, s, v, cb, hIt, item, w0);
+            hipLaunchKernelGGL(k_coord_place, dim3(nb + ge.n, nw), dim3(CELL_BLOCK), 0, s, cb, hIt, item, w0, v, ge);
         } else {
             hipLaunchKernelGGL(k_coord_zero, dim3((ncell2 + 255) / 256, nw), dim3(256), 0, s, cb, hIt.cellBase,
                                ncell2, w0);
@@ -1549,11 +1558,11 @@ bool launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordI
         const int ncell = hIt.nc[0] * hIt.nc[1] * hIt.nc[2];
         const dim3 g(2 * ncell, nw);
         if (hIt.cellBlock == 256)
-            hipLaunchKernelGGL(k_coord_cells<256>, g, dim3(256), 0, s, v, cb, dItems, item, w0);
+            hipLaunchKernelGGL(k_coord_cells<256>, g, dim3(256), 0, s, v, cb, hIt, item, w0);
         else if (hIt.cellBlock == 1024)
-            hipLaunchKernelGGL(k_coord_cells<1024>, g, dim3(1024), 0, s, v, cb, dItems, item, w0);
+            hipLaunchKernelGGL(k_coord_cells<1024>, g, dim3(1024), 0, s, v, cb, hIt, item, w0);
         else
-            hipLaunchKernelGGL(k_coord_cells<512>, g, dim3(512), 0, s, v, cb, dItems, item, w0);
+            hipLaunchKernelGGL(k_coord_cells<512>, g, dim3(512), 0, s, v, cb, hIt, item, w0);
     } else {
         hipLaunchKernelGGL((k_coord_pairs<true>), gA, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
         hipLaunchKernelGGL((k_coord_pairs<false>), gB, dim3(COORD_BLOCK), 0, s, v, cb, dItems, item, w0);
