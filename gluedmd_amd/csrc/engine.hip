@@ -1461,7 +1461,7 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
     cp.cmPart = dCmPart_.as<double>();
     cp.cmCount = dCmCount_.as<unsigned int>();
     // batched form: the wide group CVs ride in the first coordination
-    // item's placement launch when it has one (no data between them)
+    // item's rank launch when it has one (no data between them)
     const GroupExtra ge{dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupPos_.as<int>(),
                         dGroupMass_.as<float>(), dGroupRef_.as<float>()};
     bool bigPending = false;

@@ -281,7 +281,7 @@ struct CoordBuffers {
 };
 
 // Wide group CVs launched as extra workgroups of another launch (the
-// coordination placement kernel): items, atom slots, masses, references.
+// coordination rank kernel): items, atom slots, masses, references.
 struct GroupExtra {
     const GroupCVItem* items;
     int n;

@@ -764,7 +764,8 @@ __global__ void k_coord_fill(StepView v, CoordBuffers cb, const CoordItem* items
     const int* st = cb.cellStart + (size_t)w * (cb.cellStride + cb.nItems) + it.cellBase + item;
     int pos = st[c] + atomicAdd(&cb.cellCount[(size_t)w * cb.cellStride + it.cellBase + c], 1);
     const int local = i < it.nA ? i : i - it.nA;
-    cb.sorted[(size_t)w * cb.sortStride + it.sortBase + pos] = make_float4(q.x, q.y, q.z, __int_as_float(local));
+    const int rp = v.rowPos[(i < it.nA ? it.rowA : it.rowB) + local];
+    cb.sorted[(size_t)w * cb.sortStride + it.sortBase + pos] = make_float4(q.x, q.y, q.z, __int_as_float(rp));
 }
 
 constexpr int CELL_BLOCK = 1024;
@@ -786,14 +787,25 @@ int coord_sort_fits(int ncellBoth, int n) { return ncellBoth <= CELL_LDS && ncel
 
 constexpr int CELL_WAVE_TAB = 2048;  // cells (both groups) up to which the waves' counts go to a per-wave table
 
+// The rank launch may carry the wide group CVs of the step as ge.n extra
+// workgroups after the ranking ones (they share no data with the sort, and so
+// take no launch of their own in the batched step).
+static_assert(CELL_BLOCK == GROUP_BIG_BLOCK, "wide group CVs ride in the rank launch");
+
 __global__ __launch_bounds__(CELL_BLOCK) void k_coord_rank(StepView v, CoordBuffers cb, CoordItem it, int item,
-                                                           int w0)
+                                                           int w0, GroupExtra ge)
 {
     constexpr int WAVES = CELL_BLOCK / 64;
     __shared__ int lcnt[CELL_LDS];
     __shared__ unsigned short wcnt[WAVES][CELL_WAVE_TAB];  // per wave, its atoms per cell
+    const int nb = gridDim.x - ge.n;
+    if ((int)blockIdx.x >= nb) {
+        group_item<GROUP_BIG_BLOCK, GROUP_BIG_REG>(v, ge.items[blockIdx.x - nb], ge.slots, ge.mass, ge.ref,
+                                                    w0 + blockIdx.y);
+        return;
+    }
     const int w = w0 + blockIdx.y, tid = threadIdx.x, b = blockIdx.x;
-    const int n = it.nA + it.nB, nb = gridDim.x;
+    const int n = it.nA + it.nB;
     const int ncell = it.nc[0] * it.nc[1] * it.nc[2], n2 = 2 * ncell;
     const bool table = n2 <= CELL_WAVE_TAB;  // uniform
     const int i = b * CELL_BLOCK + tid, lane = tid & 63, wv = tid >> 6;
@@ -809,10 +821,12 @@ __global__ __launch_bounds__(CELL_BLOCK) void k_coord_rank(StepView v, CoordBuff
     const size_t o = (size_t)w * cb.sortStride + it.sortBase + i;
     if (valid) {
         const bool a = i < it.nA;
+        const int rp = v.rowPos[(a ? it.rowA + i : it.rowB + (i - it.nA))];
         const float4 q = v.posq[cb.pos[(size_t)w * cb.posStride + (a ? it.aBegin + i : it.bBegin + (i - it.nA))]];
         c = cell_of(q.x, q.y, q.z, v.box + (size_t)w * 12, it.nc) + (a ? 0 : ncell);
         cb.cellOf[(size_t)w * cb.cellOfStride + it.sortBase + i] = c;
-        cb.unsorted[o] = make_float4(q.x, q.y, q.z, __int_as_float(a ? i : i - it.nA));
+        // w: the atom's Jacobian row position (the table load does not wait for the gather)
+        cb.unsorted[o] = make_float4(q.x, q.y, q.z, __int_as_float(rp));
     }
     // rank among the wave's lower lanes of the same cell, and the group size
     // (each lane compares its cell with every lane's; the lowest lane of a
@@ -868,22 +882,12 @@ __global__ __launch_bounds__(CELL_BLOCK) void k_coord_rank(StepView v, CoordBuff
 // This replaces a one-workgroup scan launch between rank and place.
 constexpr int CELL_SCAN_PER = 16;  // table entries per thread held in registers by the scan
 
-// The launch may carry the wide group CVs of the step as ge.n extra
-// workgroups after the placing ones (they share no data with the sort, and
-// so take no launch of their own in the batched step).
-static_assert(CELL_BLOCK == GROUP_BIG_BLOCK, "wide group CVs ride in the placement launch");
-
-__global__ __launch_bounds__(CELL_BLOCK) void k_coord_place(CoordBuffers cb, CoordItem it, int item, int w0,
-                                                            StepView v, GroupExtra ge)
+__global__ __launch_bounds__(CELL_BLOCK) void k_coord_place(CoordBuffers cb, CoordItem it, int item, int w0)
 {
     __shared__ int sOff[CELL_LDS];
+    __shared__ int sTab[CELL_BLOCK * CELL_SCAN_PER];
     __shared__ int waveTot[CELL_BLOCK / 64];
-    const int nb = gridDim.x - ge.n;
-    if ((int)blockIdx.x >= nb) {
-        group_item<GROUP_BIG_BLOCK, GROUP_BIG_REG>(v, ge.items[blockIdx.x - nb], ge.slots, ge.mass, ge.ref,
-                                                    w0 + blockIdx.y);
-        return;
-    }
+    const int nb = gridDim.x;
     const int w = w0 + blockIdx.y, b = blockIdx.x, tid = threadIdx.x;
     const int n = it.nA + it.nB;
     const int ncell = it.nc[0] * it.nc[1] * it.nc[2], n2 = 2 * ncell, len = n2 * nb;
@@ -899,9 +903,27 @@ __global__ __launch_bounds__(CELL_BLOCK) void k_coord_place(CoordBuffers cb, Coo
     int carry = 0;
     for (int r0 = 0; r0 < len; r0 += CELL_BLOCK * CELL_SCAN_PER) {
         const int e0 = r0 + tid * CELL_SCAN_PER;
+        // the round's entries staged through LDS with coalesced loads (each
+        // thread then takes CELL_SCAN_PER consecutive ones: read directly,
+        // every load of a wave would touch 64 different lines)
+        {
+            int y[CELL_SCAN_PER];
+#pragma unroll
+            for (int u = 0; u < CELL_SCAN_PER; u++) y[u] = tab[min(r0 + tid + u * CELL_BLOCK, len - 1)];
+            __syncthreads();  // the previous round's entries consumed
+#pragma unroll
+            for (int u = 0; u < CELL_SCAN_PER; u++) sTab[tid + u * CELL_BLOCK] = y[u];
+            __syncthreads();
+        }
         int x[CELL_SCAN_PER];
 #pragma unroll
-        for (int k = 0; k < CELL_SCAN_PER; k++) x[k] = tab[min(e0 + k, len - 1)];
+        for (int k = 0; k < CELL_SCAN_PER; k += 4) {
+            const int4 t4 = *reinterpret_cast<const int4*>(&sTab[tid * CELL_SCAN_PER + k]);
+            x[k] = t4.x;
+            x[k + 1] = t4.y;
+            x[k + 2] = t4.z;
+            x[k + 3] = t4.w;
+        }
         int tsum = 0;
 #pragma unroll
         for (int k = 0; k < CELL_SCAN_PER; k++)
@@ -920,14 +942,18 @@ __global__ __launch_bounds__(CELL_BLOCK) void k_coord_place(CoordBuffers cb, Coo
         for (int q = 0; q < wid; q++) run += waveTot[q];
         int roundTot = 0;
         for (int q = 0; q < CELL_BLOCK / 64; q++) roundTot += waveTot[q];
+        int c = e0 / nb, bb = e0 - c * nb;  // (cell, workgroup) of entry e0, advanced entry by entry
 #pragma unroll
         for (int k = 0; k < CELL_SCAN_PER; k++) {
             const int e = e0 + k;
             if (e < len) {
-                const int c = e / nb, bb = e - c * nb;
                 if (bb == b) sOff[c] = run;
                 if (b == 0 && bb == 0) st[c] = run;  // start of cell c
                 run += x[k];
+            }
+            if (++bb == nb) {
+                bb = 0;
+                c++;
             }
         }
         carry += roundTot;
@@ -1267,6 +1293,7 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int* st = cb.cellStart + (size_t)w * (cb.cellStride + cb.nItems) + it.cellBase + item;
     const float4* sorted = cb.sorted + (size_t)w * cb.sortStride + it.sortBase;
+    int cnt = 0;  // lanes 0..26 of wave 0: atoms in neighbour cell `lane`
     if (tid < 27) {
         int cx = hc % nx, cy = (hc / nx) % ny, cz = hc / (nx * ny);
         cx = (cx + tid % 3 - 1 + nx) % nx;
@@ -1275,7 +1302,7 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
         const int pc = (cz * ny + cy) * nx + cx + (SIDE_A ? ncell : 0);
         const int b = st[pc], e = st[pc + 1];
         sh.cellB[tid] = b;
-        sh.pre[tid + 1] = e - b;
+        cnt = e - b;
     } else if (tid == 32) {
         const int c = hc + (SIDE_A ? 0 : ncell);
         const int b = st[c], e = st[c + 1];
@@ -1298,34 +1325,39 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
         if (v.pbc) t -= bx[4 * d] * rint(t * bx[9 + d]);
         return (float)t;
     };
+    if (wv == 0) {  // inclusive scan of the 27 counts: pre[c + 1] = atoms in cells 0..c
+        int incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 32; d <<= 1) {
+            const int t = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += t;
+        }
+        if (lane < 27) sh.pre[lane + 1] = incl;
+    }
     const SwitchD sw = make_switch(it);
     const float rc2f = (float)(it.rc2 * (1.0 + 1e-5));
-    __syncthreads();
-    if (tid == 0)
-        for (int k = 0; k < 27; k++) sh.pre[k + 1] += sh.pre[k];
     __syncthreads();
     const int P = sh.pre[27], H = sh.homeN, homeB = sh.homeB;
     double cnLane = 0.0;  // this lane's CN terms, in its fixed evaluation order
     unsigned long long np = 0;
     short* Q = sh.q[wv];
+    static_assert(COORD_HCAP <= 256, "one staged home per thread");
     for (int h0 = 0; h0 < H; h0 += COORD_HCAP) {
         const int hn = min(COORD_HCAP, H - h0);
-        __syncthreads();
-        for (int i = tid; i < hn; i += COORD_CB) {
-            const float4 hq = sorted[homeB + h0 + i];
-            sh.home[i] = hq;
-            sh.hw[0][i] = rel(hq.x, 0);
-            sh.hw[1][i] = rel(hq.y, 1);
-            sh.hw[2][i] = rel(hq.z, 2);
-            sh.g[i][0] = sh.g[i][1] = sh.g[i][2] = 0;
-        }
-        for (int p0 = 0; p0 < P; p0 += COORD_PCAP) {
-            const int pn = min(COORD_PCAP, P - p0);
-            __syncthreads();
-            {
-                // every source index first (neighbour cell of flat index q:
-                // pre[c] <= q < pre[c + 1]), then all loads, then the stores
-                constexpr int PER = (COORD_PCAP + COORD_CB - 1) / COORD_CB;
+        for (int p0 = 0; p0 < max(P, 1); p0 += COORD_PCAP) {
+            const int pn = min(COORD_PCAP, P - p0);  // 0: no partners (the homes' rows are zero)
+            // first chunk: this thread's home too, loaded with the partners
+            // and written to LDS behind the same barrier (one round trip)
+            // (both defined on every path: nothing is carried across the tasks)
+            float4 hq = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (p0 == 0) hq = sorted[homeB + h0 + min(tid, hn - 1)];
+            // every source index first (neighbour cell of flat index q:
+            // pre[c] <= q < pre[c + 1]), then all loads, then the stores
+            constexpr int PER = (COORD_PCAP + COORD_CB - 1) / COORD_CB;
+            float4 pv[PER];
+#pragma unroll
+            for (int u = 0; u < PER; u++) pv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (pn > 0) {
                 int src[PER];
 #pragma unroll
                 for (int u = 0; u < PER; u++) {
@@ -1338,9 +1370,18 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
                     }
                     src[u] = sh.cellB[lo] + (q - sh.pre[lo]);
                 }
-                float4 pv[PER];
 #pragma unroll
                 for (int u = 0; u < PER; u++) pv[u] = sorted[src[u]];
+            }
+            __syncthreads();  // the previous chunk's tasks and row stores are done with LDS
+            if (p0 == 0 && tid < hn) {
+                sh.home[tid] = hq;
+                sh.hw[0][tid] = rel(hq.x, 0);
+                sh.hw[1][tid] = rel(hq.y, 1);
+                sh.hw[2][tid] = rel(hq.z, 2);
+                sh.g[tid][0] = sh.g[tid][1] = sh.g[tid][2] = 0;
+            }
+            if (pn > 0) {
 #pragma unroll
                 for (int u = 0; u < PER; u++) {
                     const int i = tid + u * COORD_CB;
@@ -1410,11 +1451,9 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
                         np++;
                     }
                 };
-                for (int pbk = pbBeg; pbk < pbEnd; pbk++) {
-                    const int pi = pbk * LPH + (lane % LPH);
-                    const int pcl = min(pi, pn - 1);
-                    const float fx = hx - sh.pw[0][pcl], fy = hy - sh.pw[1][pcl], fz = hz - sh.pw[2][pcl];
-                    const bool cand = myOk && pi < pn && fx * fx + fy * fy + fz * fz < rc2f;
+                // queue this lane's partner pi when it passes the pre-check;
+                // evaluate a full round as soon as 64 entries wait
+                auto push = [&](int pi, bool cand) {
                     const unsigned long long m = __ballot(cand);
                     if (cand)
                         Q[(tail + (unsigned)__popcll(m & ((1ull << lane) - 1ull))) & 127] =
@@ -1426,6 +1465,15 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
                         head += 64;
                         __builtin_amdgcn_wave_barrier();
                     }
+                };
+                auto precheck = [&](int pi, float px, float py, float pz) {
+                    const float fx = hx - px, fy = hy - py, fz = hz - pz;
+                    return myOk && pi < pn && fx * fx + fy * fy + fz * fz < rc2f;
+                };
+                for (int pbk = pbBeg; pbk < pbEnd; pbk++) {
+                    const int pi = pbk * LPH + (lane % LPH);
+                    const int pcl = min(pi, pn - 1);
+                    push(pi, precheck(pi, sh.pw[0][pcl], sh.pw[1][pcl], sh.pw[2][pcl]));
                 }
                 __builtin_amdgcn_wave_barrier();
                 if (lane < (int)(tail - head)) evaluate(head + lane);
@@ -1463,11 +1511,9 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
             }
         }
         __syncthreads();
-        for (int i = tid; i < hn; i += COORD_CB) {
-            const int homeIdx = __float_as_int(sh.home[i].w);
-            store_row(v, w, (SIDE_A ? it.rowA : it.rowB) + homeIdx, (double)sh.g[i][0] * COORD_UNFIX,
-                      (double)sh.g[i][1] * COORD_UNFIX, (double)sh.g[i][2] * COORD_UNFIX);
-        }
+        for (int i = tid; i < hn; i += COORD_CB)  // w of a staged atom: its Jacobian row position
+            store_row_at(v, (size_t)w * v.R + __float_as_int(sh.home[i].w), (double)sh.g[i][0] * COORD_UNFIX,
+                         (double)sh.g[i][1] * COORD_UNFIX, (double)sh.g[i][2] * COORD_UNFIX);
     }
     if (!SIDE_A) return;
     U128 cn{0ull, 0ll};
@@ -1542,8 +1588,8 @@ bool launch_coordination(const StepView& v, const CoordBuffers& cb, const CoordI
                 ge = *extra;
                 carried = true;
             }
-            hipLaunchKernelGGL(k_coord_rank, dim3(nb, nw), dim3(CELL_BLOCK), 0, s, v, cb, hIt, item, w0);
-            hipLaunchKernelGGL(k_coord_place, dim3(nb + ge.n, nw), dim3(CELL_BLOCK), 0, s, cb, hIt, item, w0, v, ge);
+            hipLaunchKernelGGL(k_coord_rank, dim3(nb + ge.n, nw), dim3(CELL_BLOCK), 0, s, v, cb, hIt, item, w0, ge);
+            hipLaunchKernelGGL(k_coord_place, dim3(nb, nw), dim3(CELL_BLOCK), 0, s, cb, hIt, item, w0);
         } else {
             hipLaunchKernelGGL(k_coord_zero, dim3((ncell2 + 255) / 256, nw), dim3(256), 0, s, cb, hIt.cellBase,
                                ncell2, w0);
