@@ -64,7 +64,7 @@ PMC_KIB = {
     "mixed-large": {"k_cv_fused": (11516.2, 2892.4), "k_cv_group_big": (2954.5, 815.5),
                     "k_path_rows<8>": (3869.4, 554.9), "k_step_bias_finalize<false>": (87.0, 2.0),
                     "k_scatter_dense_rows": (41542.5, 49685.2)},
-    # k_coord_place carries the Rg(5,000) workgroup
+    # k_coord_rank carries the Rg(5,000) workgroup
     "coordination-opes": {"k_coord_rank": (297.9, 792.4), "k_coord_place": (550.6, 576.8),
                           "k_coord_cells<512>": (1904.5, 705.3), "k_step_bias_chunks<2, 2>": (147.3, 3.0),
                           "k_step_bias_finalize<false>": (13.5, 0.1), "k_scatter": (379.7, 515.7)},
