@@ -59,17 +59,17 @@ FIXED_SCALE = 4294967296.0
 # are in the workload summaries under profiles/.
 PMC_FETCH_SCALE, PMC_WRITE_SCALE = 2.0, 1.0
 PMC_KIB = {
-    "multiwalker": {"k_cv_fused": (448.6, 512.5), "k_step_bias_chunks<1, 2>": (7585.5, 60.0),
-                    "k_step_bias_finalize<false>": (540.2, 769.0)},
-    "mixed-large": {"k_cv_fused": (11516.2, 2892.4), "k_cv_group_big": (2954.5, 815.5),
+    "multiwalker": {"k_cv_fused": (447.6, 512.5), "k_step_bias_chunks<1, 2>": (7585.6, 60.0),
+                    "k_step_bias_finalize<false>": (540.3, 769.0)},
+    "mixed-large": {"k_cv_fused": (11515.7, 2877.8), "k_cv_group_big": (2950.5, 815.5),
                     "k_path_rows<8>": (3869.4, 554.9), "k_step_bias_finalize<false>": (87.0, 2.0),
-                    "k_scatter_dense_rows": (41542.5, 49685.2)},
+                    "k_scatter_dense_rows": (41545.5, 49685.1)},
     # k_coord_rank carries the Rg(5,000) workgroup
-    "coordination-opes": {"k_coord_rank": (297.9, 792.4), "k_coord_place": (550.6, 576.8),
-                          "k_coord_cells<512>": (1904.5, 705.3), "k_step_bias_chunks<2, 2>": (147.3, 3.0),
+    "coordination-opes": {"k_coord_rank": (383.8, 948.7), "k_coord_place": (452.9, 420.6),
+                          "k_coord_cells<512>": (1596.7, 704.8), "k_step_bias_chunks<2, 2>": (147.2, 3.0),
                           "k_step_bias_finalize<false>": (13.5, 0.1), "k_scatter": (379.7, 515.7)},
-    "path-batch": {"k_cv_fused": (3470.8, 1132.3), "k_path_rows<4>": (3476.9, 2544.0),
-                   "k_step_hill_chunks_shared<2>": (448.9, 336.0), "k_step_bias_finalize<false>": (1749.0, 1823.3)},
+    "path-batch": {"k_cv_fused": (3468.8, 1132.3), "k_path_rows<4>": (3577.2, 2544.0),
+                   "k_step_hill_chunks_shared<2>": (307.7, 384.0), "k_step_bias_finalize<false>": (1775.8, 1823.3)},
     "small": {"k_step_tiny<1, 2, false>": (42.4, 0.9)},
 }
 # the kernel(s) each workload's roofline line is about
