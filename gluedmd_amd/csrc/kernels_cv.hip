@@ -1477,8 +1477,9 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
                 }
                 __builtin_amdgcn_wave_barrier();
                 if (lane < (int)(tail - head)) evaluate(head + lane);
-                // the task's row of each home: a deterministic FP64 wave sum
-                // of the lanes' sums, converted once to 2^40 fixed point;
+                // the task's row of each home: a deterministic FP64 wave
+                // reduce-scatter of the lanes' sums (lane j holds value j),
+                // converted once to 2^40 fixed point;
                 // the tasks' integers then add exactly in LDS
                 double tot[3 * HPT];
 #pragma unroll
@@ -1487,14 +1488,10 @@ __device__ void coord_cell(const StepView& v, const CoordBuffers& cb, const Coor
                     tot[3 * k + 1] = acc[k][1];
                     tot[3 * k + 2] = acc[k][2];
                 }
-                wave_sum_n<3 * HPT>(tot);
+                const double val = wave_reduce_scatter_small<3 * HPT>(tot);  // lane j: total of value j
                 if (lane < 3 * HPT) {
                     const int k = lane / 3, c = lane - 3 * k;
                     const int h = grp * HPT + k;
-                    double val = tot[0];
-#pragma unroll
-                    for (int j = 1; j < 3 * HPT; j++)
-                        if (lane == j) val = tot[j];
                     if (h < hn) atomicAdd((unsigned long long*)&sh.g[h][c], (unsigned long long)to_fix(val));
                 }
             };

@@ -87,6 +87,37 @@ __device__ inline double wave_reduce_scatter(const double (&x)[NV])
     return t;
 }
 
+// Reduce-scatter of up to 8 FP64 sums over a wave, for few values: lane l
+// (l < NV) ends with the wave total of value l (other lanes hold partial
+// sums of no use).  Within each group of W = 4 or 8 lanes the DPP steps of
+// wave_reduce_scatter halve the values a lane carries, so lane l holds value
+// (l mod W) summed over its group; the groups of a 16-lane row are then
+// added by row rotations (by 4 and 8 lanes), and the rows by two xor
+// shuffles.  Each
+// total is formed by the same additions whatever the data.
+constexpr int DPP_ROW_ROR4 = 0x124;  // row rotation by 4: lane i receives lane (i - 4) mod 16 of its row
+constexpr int DPP_ROW_ROR8 = 0x128;  // row rotation by 8: lane i receives lane i ^ 8
+
+template <int NV>
+__device__ inline double wave_reduce_scatter_small(const double (&x)[NV])
+{
+    static_assert(NV >= 1 && NV <= 8, "at most 8 values");
+    constexpr int W = NV <= 4 ? 4 : 8;
+    const int lane = threadIdx.x & 63;
+    double v[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) v[k] = k < NV ? x[k < NV ? k : 0] : 0.0;
+    if constexpr (W == 8) rs_step<DPP_ROW_HALF_MIRROR, 4>(v, lane & 4);
+    rs_step<DPP_QUAD_XOR2, 2>(v, lane & 2);
+    rs_step<DPP_QUAD_XOR1, 1>(v, lane & 1);
+    double t = v[0];
+    if constexpr (W == 4) t += dpp_f64<DPP_ROW_ROR4>(t);
+    t += dpp_f64<DPP_ROW_ROR8>(t);
+    t += __shfl_xor(t, 16, 64);
+    t += __shfl_xor(t, 32, 64);
+    return t;
+}
+
 __device__ inline double wave_sum1(double x)
 {
     double v[1] = {x};
