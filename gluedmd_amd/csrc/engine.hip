@@ -579,6 +579,30 @@ void Engine::buildPlan(int nMass, const double* masses)
             }
         }
     }
+    // Slots relabelled in atom order.  The force scatters walk the slots in
+    // order and read each slot's rows together, so atoms next to each other
+    // in the force buffer (in the usual identity GPU order) then also have
+    // their rows next to each other, and row reads share memory sectors.  A
+    // relabelling only: every CV keeps its atom lists and rows, and each
+    // slot's rows keep their order (the same sums, bit for bit).
+    {
+        const int S = (int)slotAtom_.size();
+        std::vector<int> order(S), newOf(S), atoms(S);
+        for (int k = 0; k < S; k++) order[k] = k;
+        std::sort(order.begin(), order.end(), [&](int a, int b) { return slotAtom_[a] < slotAtom_[b]; });
+        for (int k = 0; k < S; k++) {
+            newOf[order[k]] = k;
+            atoms[k] = slotAtom_[order[k]];
+        }
+        slotAtom_ = atoms;
+        for (auto& it : small_) {
+            const int na = it.type == CV_DISTANCE ? 2 : it.type == CV_ANGLE ? 3 : it.type == CV_DIHEDRAL ? 4
+                           : it.type == CV_POSITION ? 1 : 0;
+            for (int a = 0; a < na; a++) it.slot[a] = newOf[it.slot[a]];
+        }
+        for (auto* v : {&groupSlots, &coordSlots, &pathSlots, &rowSlot_})
+            for (int& x : *v) x = newOf[x];
+    }
     S_ = (int)slotAtom_.size();
     R_ = (int)rowSlot_.size();
     // rows grouped by slot: the rows of slot s occupy [segStart_[s], segStart_[s + 1])
