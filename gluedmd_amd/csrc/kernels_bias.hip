@@ -977,18 +977,22 @@ void launch_step_bias(const StepView& v, const FusedBiasPlan& fp, int nw, hipStr
 // ---------------------------------------------------------------------------
 
 // Grid deposit, CommonGluedKernels.cpp:3645-3679.  A workgroup covers 64
-// grid nodes; its four waves take every fourth walker of a shared grid (a
-// private grid is one y-block row per walker) and the four partial sums of a
-// node are added in wave order, then once to the grid (deterministic, no
-// atomics).  The walkers' heights h0 exp(-V hif) and centres are formed once
-// per workgroup in LDS.  A node is skipped for a walker when some
+// grid nodes; the walkers' heights h0 exp(-V hif) and centres are formed once
+// per workgroup in LDS, and only the walkers whose Gaussian can reach one of
+// the workgroup's nodes are kept (in walker order): a walker is dropped when,
+// in some dimension, its centre lies farther than 4 sigma (plus a 1e-6
+// relative margin) from the range of the nodes' coordinates, every image of
+// a periodic centre included.  The four waves take every fourth kept walker
+// of a shared grid (a private grid is one y-block row per walker) and the
+// four partial sums of a node are added in wave order, then once to the grid
+// (deterministic, no atomics).  A node is skipped for a walker when some
 // |dc_d| > 4 sigma_d (with a 1e-9 relative margin), which implies E > 8;
 // otherwise E is formed with the reference's expression, unfused, so the
 // E <= 8 predicate is the oracle's.
 constexpr int DEP_BLOCK = 256;
-constexpr int DEP_NODES = 64;
+constexpr int DEP_NODES = 64;  // one wave: wave 0 forms the nodes' coordinate ranges
 constexpr int DEP_GROUPS = DEP_BLOCK / DEP_NODES;
-constexpr int DEP_WALKERS = 256;  // walkers staged in LDS per pass
+constexpr int DEP_WALKERS = DEP_BLOCK;  // walkers staged in LDS per pass (one per thread)
 
 __global__ __launch_bounds__(DEP_BLOCK) void k_dep_grid(StepView v, GridBiasDev g, int w0, int nw)
 {
@@ -996,7 +1000,10 @@ __global__ __launch_bounds__(DEP_BLOCK) void k_dep_grid(StepView v, GridBiasDev 
     __shared__ double sC[DEP_WALKERS][3];
     __shared__ double sH[DEP_WALKERS];
     __shared__ double sAdd[DEP_GROUPS][DEP_NODES];
+    __shared__ double sLo[3], sHi[3];
+    __shared__ int sKeep[DEP_BLOCK / 64];
     const int nl = threadIdx.x % DEP_NODES, grp = threadIdx.x / DEP_NODES;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int idx = blockIdx.x * DEP_NODES + nl;
     const bool priv = g.walkerStride != 0;
     const int wFirst = priv ? w0 + blockIdx.y : w0;
@@ -1012,23 +1019,72 @@ __global__ __launch_bounds__(DEP_BLOCK) void k_dep_grid(StepView v, GridBiasDev 
         two_s2[d] = 2.0 * g.sigma[d] * g.sigma[d];
         far2[d] = 16.0 * g.sigma[d] * g.sigma[d] * (1.0 + 1e-9);
     }
+    if (wid == 0) {  // the range of the workgroup's node coordinates per dimension
+        for (int d = 0; d < g.D; d++) {
+            double lo = pos[d], hi = pos[d];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                lo = fmin(lo, __shfl_xor(lo, o, 64));
+                hi = fmax(hi, __shfl_xor(hi, o, 64));
+            }
+            if (lane == 0) {
+                sLo[d] = lo;
+                sHi[d] = hi;
+            }
+        }
+    }
+    // the node's grid value, requested now (its round trip overlaps the work below)
+    double* grid = g.grid + (size_t)wFirst * g.walkerStride;
+    const double old = grp == 0 ? grid[min(idx, g.total - 1)] : 0.0;
     double add = 0.0;
     for (int base = 0; base < wCount; base += DEP_WALKERS) {
         const int m = min(DEP_WALKERS, wCount - base);
-        __syncthreads();
-        for (int t = threadIdx.x; t < m; t += blockDim.x) {
+        __syncthreads();  // sLo / sHi written; the previous pass done with sC / sH
+        // thread t: walker base + t, kept when its Gaussian can reach a node
+        const int t = threadIdx.x;
+        double c[3] = {0.0, 0.0, 0.0}, h = 0.0;
+        bool keep = t < m;
+        if (keep) {
             const int w = wFirst + base + t;
             const double* cv = v.cvVal + (size_t)w * v.V;
             const double V = v.benergy[(size_t)w * v.B + g.eslot];
-            sH[t] = g.hif > 0.0 ? g.h0 * exp(-V * g.hif) : g.h0;
-            for (int d = 0; d < g.D; d++) sC[t][d] = cv[g.cv[d]];
+            h = g.hif > 0.0 ? g.h0 * exp(-V * g.hif) : g.h0;
+            for (int d = 0; d < g.D; d++) {
+                c[d] = cv[g.cv[d]];
+                const double lo = sLo[d], hi = sHi[d];
+                auto gap = [&](double x) { return fmax(0.0, fmax(lo - x, x - hi)); };
+                double dist = gap(c[d]);
+                if (g.periodic[d]) {
+                    // the centre's image nearest the range's midpoint and its two
+                    // neighbours: a node of the range is within one period of it
+                    const double L = per[d];
+                    const double cc = c[d] - L * rint((c[d] - 0.5 * (lo + hi)) / L);
+                    dist = fmin(gap(cc), fmin(gap(cc + L), gap(cc - L)));
+                }
+                if (dist * dist > far2[d] * (1.0 + 1e-6)) keep = false;
+            }
+        }
+        // compact the kept walkers in walker order
+        const unsigned long long bal = __ballot(keep);
+        if (lane == 0) sKeep[wid] = __popcll(bal);
+        __syncthreads();
+        int off = 0, mk = 0;
+#pragma unroll
+        for (int q = 0; q < DEP_BLOCK / 64; q++) {
+            off += q < wid ? sKeep[q] : 0;
+            mk += sKeep[q];
+        }
+        if (keep) {
+            const int k = off + (int)__popcll(bal & ((1ull << lane) - 1ull));
+            sH[k] = h;
+            for (int d = 0; d < g.D; d++) sC[k][d] = c[d];
         }
         __syncthreads();
-        for (int t = grp; t < m; t += DEP_GROUPS) {
+        for (int u = grp; u < mk; u += DEP_GROUPS) {
             double e = 0.0;
             bool near = true;
             for (int d = 0; d < g.D && near; d++) {
-                double dc = pos[d] - sC[t][d];
+                double dc = pos[d] - sC[u][d];
                 if (g.periodic[d]) dc -= round(dc / per[d]) * per[d];
                 if (dc * dc > far2[d]) {
                     near = false;
@@ -1036,7 +1092,7 @@ __global__ __launch_bounds__(DEP_BLOCK) void k_dep_grid(StepView v, GridBiasDev 
                     e += dc * dc / two_s2[d];
                 }
             }
-            if (near && e <= 8.0) add += sH[t] * exp(-e);
+            if (near && e <= 8.0) add += sH[u] * exp(-e);
         }
     }
     sAdd[grp][nl] = add;
@@ -1045,8 +1101,7 @@ __global__ __launch_bounds__(DEP_BLOCK) void k_dep_grid(StepView v, GridBiasDev 
         double tot = sAdd[0][nl];
 #pragma unroll
         for (int q = 1; q < DEP_GROUPS; q++) tot += sAdd[q][nl];
-        double* grid = g.grid + (size_t)wFirst * g.walkerStride;
-        grid[idx] += tot;
+        grid[idx] = old + tot;
     }
 }
 
