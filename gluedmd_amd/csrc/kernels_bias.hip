@@ -989,7 +989,7 @@ void launch_step_bias(const StepView& v, const FusedBiasPlan& fp, int nw, hipStr
 // |dc_d| > 4 sigma_d (with a 1e-9 relative margin), which implies E > 8;
 // otherwise E is formed with the reference's expression, unfused, so the
 // E <= 8 predicate is the oracle's.
-constexpr int DEP_BLOCK = 256;
+constexpr int DEP_BLOCK = 512;
 constexpr int DEP_NODES = 64;  // one wave: wave 0 forms the nodes' coordinate ranges
 constexpr int DEP_GROUPS = DEP_BLOCK / DEP_NODES;
 constexpr int DEP_WALKERS = DEP_BLOCK;  // walkers staged in LDS per pass (one per thread)
