@@ -19,9 +19,9 @@ void launch_cv_group(const StepView&, const GroupCVItem*, int, const int*, const
 void launch_cv_path(const StepView&, const PathItem*, int, int, int, int, const int*, const float*, int, int, int,
                     hipStream_t);
 void launch_cv_group_big(const StepView&, const GroupCVItem*, int, const int*, const float*, const float*, int, int,
-                         hipStream_t);
-void launch_path_rows(const StepView&, const PathItem*, int, int, int, const int*, const float*, int, int, int,
-                      hipStream_t);
+                         int, hipStream_t);
+bool launch_path_rows(const StepView&, const PathItem*, int, int, int, const int*, const float*, int, int, int,
+                      hipStream_t, const GroupExtra*);
 
 bool launch_coordination(const StepView&, const CoordBuffers&, const CoordItem*, const CoordItem&, int, int, int,
                          hipStream_t, const GroupExtra*);
@@ -673,6 +673,10 @@ void Engine::buildPlan(int nMass, const double* masses)
         }
         group_ = small;
         groupBig_ = big;
+        // their workgroup size: 512 threads when a path-row launch can carry
+        // them (a plan with a path CV and no coordination, whose rank launch
+        // carries 1,024-thread items), else 1,024
+        groupBigNT_ = !path_.empty() && coord_.empty() ? 512 : 1024;
     }
     dGroup_.upload(group_);
     dGroupBig_.upload(groupBig_);
@@ -1485,26 +1489,31 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
     cp.cmPart = dCmPart_.as<double>();
     cp.cmCount = dCmCount_.as<unsigned int>();
     // batched form: the wide group CVs ride in the first coordination
-    // item's rank launch when it has one (no data between them)
+    // item's rank launch (1,024-thread items) or in the path-row launch
+    // (512-thread items, 8 phases) when the plan has one (no data between
+    // them), else they take a launch of their own
     const GroupExtra ge{dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupPos_.as<int>(),
                         dGroupMass_.as<float>(), dGroupRef_.as<float>()};
     bool bigPending = false;
     if (fused && w0 == 0 && nw == W_) {
         launch_cv_fused(v, cp, nw, s);
         bigPending = !groupBig_.empty();
-        if (bigPending && coord_.empty()) {
-            launch_cv_group_big(v, ge.items, ge.n, ge.slots, ge.mass, ge.ref, w0, nw, s);
+        const bool pathCarries = groupBigNT_ == 512 && !path_.empty() && tune_.pathPhases == 8;
+        if (bigPending && coord_.empty() && !pathCarries) {
+            launch_cv_group_big(v, ge.items, ge.n, ge.slots, ge.mass, ge.ref, w0, nw, groupBigNT_, s);
             bigPending = false;
         }
-        launch_path_rows(v, dPath_.as<PathItem>(), (int)path_.size(), pathRowBlocks_, pathMaxN_, dPathPos_.as<int>(),
-                         dPathFrames_.as<float>(), w0, nw, tune_.pathPhases, s);
+        if (launch_path_rows(v, dPath_.as<PathItem>(), (int)path_.size(), pathRowBlocks_, pathMaxN_,
+                             dPathPos_.as<int>(), dPathFrames_.as<float>(), w0, nw, tune_.pathPhases, s,
+                             bigPending && pathCarries ? &ge : nullptr))
+            bigPending = false;
     } else {
         launch_cv_small(v, dSmall_.as<SmallCVItem>(), (int)small_.size(), w0, nw, s);
         launch_cv_group(v, dGroup_.as<GroupCVItem>(), (int)group_.size(), dGroupPos_.as<int>(),
                         dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
         launch_cv_cm_chunks(v, cp, w0, nw, s);
         launch_cv_group_big(v, dGroupBig_.as<GroupCVItem>(), (int)groupBig_.size(), dGroupPos_.as<int>(),
-                            dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, s);
+                            dGroupMass_.as<float>(), dGroupRef_.as<float>(), w0, nw, groupBigNT_, s);
         launch_cv_path(v, dPath_.as<PathItem>(), (int)path_.size(), pathMsdUnits_, pathRowBlocks_, pathMaxN_,
                        dPathPos_.as<int>(), dPathFrames_.as<float>(), w0, nw, tune_.pathPhases, s);
     }
@@ -1532,10 +1541,10 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
         cb.nItems = (int)coord_.size();
         for (size_t i = 0; i < coord_.size(); i++)
             if (launch_coordination(v, cb, dCoord_.as<CoordItem>(), coord_[i], (int)i, w0, nw, s,
-                                    bigPending ? &ge : nullptr))
+                                    bigPending && groupBigNT_ == 1024 ? &ge : nullptr))
                 bigPending = false;
     }
-    if (bigPending) launch_cv_group_big(v, ge.items, ge.n, ge.slots, ge.mass, ge.ref, w0, nw, s);
+    if (bigPending) launch_cv_group_big(v, ge.items, ge.n, ge.slots, ge.mass, ge.ref, w0, nw, groupBigNT_, s);
     }
     SmallBiasState st;
     st.grids = dGridTable_.as<GridBiasDev>();

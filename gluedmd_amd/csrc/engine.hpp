@@ -180,6 +180,7 @@ class Engine {
     std::vector<int> natoms_, padded_;
     std::vector<long long> posOff_, forceOff_;
     std::vector<int> slotAtom_;          // user atom of every slot
+    int groupBigNT_ = 1024;  // threads per workgroup of the wide group CVs (fixed per plan)
     std::vector<int> groupSlots_, pathSlots_, coordSlots_;  // slot of every group / path / coordination gather entry
     std::vector<std::vector<int>> userToGpu_;
     std::vector<double> boxHost_;        // [W][12]

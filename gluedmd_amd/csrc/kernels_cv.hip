@@ -645,23 +645,32 @@ void launch_cv_group(const StepView& v, const GroupCVItem* items, int n, const i
 }
 
 // Group CVs over many atoms or pairs (more than GROUP_BLOCK * GROUP_REG): one
-// 1,024-thread workgroup each, 8 atoms per thread in registers.
+// workgroup each, 8 atoms per thread in registers.  Its size is fixed per
+// plan (1,024 threads, or 512 when the batched step carries these items in
+// the 512-thread path-row launch), so that every form runs the same code and
+// gives the same bits.
 constexpr int GROUP_BIG_BLOCK = 1024;
+constexpr int GROUP_BIG_BLOCK_PATH = 512;
 constexpr int GROUP_BIG_REG = 8;
 
-__global__ __launch_bounds__(GROUP_BIG_BLOCK) void k_cv_group_big(StepView v, const GroupCVItem* __restrict__ items,
-                                                                  const int* __restrict__ slots,
-                                                                  const float* __restrict__ mass,
-                                                                  const float* __restrict__ ref, int w0)
+template <int NT>
+__global__ __launch_bounds__(NT) void k_cv_group_big(StepView v, const GroupCVItem* __restrict__ items,
+                                                     const int* __restrict__ slots, const float* __restrict__ mass,
+                                                     const float* __restrict__ ref, int w0)
 {
-    group_item<GROUP_BIG_BLOCK, GROUP_BIG_REG>(v, items[blockIdx.x], slots, mass, ref, w0 + blockIdx.y);
+    group_item<NT, GROUP_BIG_REG>(v, items[blockIdx.x], slots, mass, ref, w0 + blockIdx.y);
 }
 
 void launch_cv_group_big(const StepView& v, const GroupCVItem* items, int n, const int* slots, const float* mass,
-                         const float* ref, int w0, int nw, hipStream_t s)
+                         const float* ref, int w0, int nw, int nt, hipStream_t s)
 {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_cv_group_big, dim3(n, nw), dim3(GROUP_BIG_BLOCK), 0, s, v, items, slots, mass, ref, w0);
+    if (nt == GROUP_BIG_BLOCK_PATH)
+        hipLaunchKernelGGL(k_cv_group_big<GROUP_BIG_BLOCK_PATH>, dim3(n, nw), dim3(GROUP_BIG_BLOCK_PATH), 0, s, v,
+                           items, slots, mass, ref, w0);
+    else
+        hipLaunchKernelGGL(k_cv_group_big<GROUP_BIG_BLOCK>, dim3(n, nw), dim3(GROUP_BIG_BLOCK), 0, s, v, items,
+                           slots, mass, ref, w0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1776,14 +1785,27 @@ __global__ __launch_bounds__(PATH_MSD_THREADS) void k_path_msd(StepView v, const
     path_msd_block(v, items[p], slots, frames, blockIdx.x - items[p].msdBlk0, wt0, min(PATH_TW, w0 + nw - wt0));
 }
 
-template <int PH>
+// CARRY: the launch also runs the plan's wide group CVs as ge.n extra
+// workgroups per walker row (512 threads each; they share no data with the
+// rows).
+template <int PH, bool CARRY>
 __global__ __launch_bounds__(64 * PH) void k_path_rows(StepView v, const PathItem* __restrict__ items,
                                                                 int nItems, const int* __restrict__ slots,
-                                                                const float* __restrict__ frames, int w0)
+                                                                const float* __restrict__ frames, int w0,
+                                                                GroupExtra ge)
 {
     __shared__ double shW[PATH_MAX_FRAMES];
     __shared__ double shRed[6][64 * PH];
     __shared__ double sh[4 * 16];
+    if constexpr (CARRY) {
+        static_assert(64 * PH == GROUP_BIG_BLOCK_PATH, "the carried group CVs run on the launch's block size");
+        const int nb = gridDim.x - ge.n;
+        if ((int)blockIdx.x >= nb) {
+            group_item<GROUP_BIG_BLOCK_PATH, GROUP_BIG_REG>(v, ge.items[blockIdx.x - nb], ge.slots, ge.mass, ge.ref,
+                                                            w0 + blockIdx.y);
+            return;
+        }
+    }
     int p = 0;
     while (p + 1 < nItems && (int)blockIdx.x >= items[p + 1].rowBlk0) p++;
     const PathItem& it = items[p];
@@ -1984,16 +2006,28 @@ void launch_path_frames_adjoint(const StepView& v, const PathItem* items, int pi
 // zeros), so the adjoint's coefficients match either.
 // phases: 4 or 8 waves per row block (the autotuner's choice; 0 = 4 when the
 // blocks exceed one residency round and the paths have at most 256 frames)
-void launch_path_rows(const StepView& v, const PathItem* items, int n, int nRowBlocks, int maxN, const int* slots,
-                      const float* frames, int w0, int nw, int phases, hipStream_t s)
+// Returns true when the launch carried the wide group CVs of `extra` (their
+// workgroups are 512 threads: with 8 phases).
+bool launch_path_rows(const StepView& v, const PathItem* items, int n, int nRowBlocks, int maxN, const int* slots,
+                      const float* frames, int w0, int nw, int phases, hipStream_t s, const GroupExtra* extra)
 {
-    if (n <= 0) return;
+    if (n <= 0) return false;
     if (phases == 0) phases = (long long)nRowBlocks * nw > 512 && maxN <= 256 ? 4 : PATH_PHASES;
-    if (phases == 4)
-        hipLaunchKernelGGL(k_path_rows<4>, dim3(nRowBlocks, nw), dim3(64 * 4), 0, s, v, items, n, slots, frames, w0);
-    else
-        hipLaunchKernelGGL(k_path_rows<PATH_PHASES>, dim3(nRowBlocks, nw), dim3(64 * PATH_PHASES), 0, s, v, items, n,
-                           slots, frames, w0);
+    static_assert(64 * PATH_PHASES == GROUP_BIG_BLOCK_PATH, "8 phases: the carrying launch");
+    const GroupExtra none{};
+    if (phases == 4) {
+        hipLaunchKernelGGL((k_path_rows<4, false>), dim3(nRowBlocks, nw), dim3(64 * 4), 0, s, v, items, n, slots,
+                           frames, w0, none);
+        return false;
+    }
+    if (extra && extra->n > 0) {
+        hipLaunchKernelGGL((k_path_rows<PATH_PHASES, true>), dim3(nRowBlocks + extra->n, nw), dim3(64 * PATH_PHASES),
+                           0, s, v, items, n, slots, frames, w0, *extra);
+        return true;
+    }
+    hipLaunchKernelGGL((k_path_rows<PATH_PHASES, false>), dim3(nRowBlocks, nw), dim3(64 * PATH_PHASES), 0, s, v,
+                       items, n, slots, frames, w0, none);
+    return false;
 }
 
 void launch_cv_path(const StepView& v, const PathItem* items, int n, int nMsdBlocks, int nRowBlocks, int maxN,
@@ -2002,7 +2036,7 @@ void launch_cv_path(const StepView& v, const PathItem* items, int n, int nMsdBlo
     if (n <= 0) return;
     hipLaunchKernelGGL(k_path_msd, dim3(nMsdBlocks, (nw + PATH_TW - 1) / PATH_TW), dim3(PATH_MSD_THREADS), 0, s, v,
                        items, n, slots, frames, w0, nw);
-    launch_path_rows(v, items, n, nRowBlocks, maxN, slots, frames, w0, nw, phases, s);
+    launch_path_rows(v, items, n, nRowBlocks, maxN, slots, frames, w0, nw, phases, s, nullptr);
 }
 
 // ---------------------------------------------------------------------------
