@@ -45,10 +45,10 @@ def test_two_handles_share_tables_like_one_batched_handle():
     hB = Context(f, walkers=[N])
     for g in (one, hA, hB):
         g.set_box(4.0)
-    for b in range(3):
-        ptrs = hA.getMultiWalkerPtrs(b)
+    for b in range(3):  # the reference's calling form: force.getMultiWalkerPtrs(context, bias)
+        ptrs = f.getMultiWalkerPtrs(hA, b)
         assert len(ptrs) == (5, 3, 1)[b] and all(p != 0 for p in ptrs)
-        hB.setMultiWalkerPtrs(b, ptrs, primary=hA)
+        f.setMultiWalkerPtrs(hB, b, ptrs, primary=hA)
     assert hB.getMultiWalkerPtrs(0) == hA.getMultiWalkerPtrs(0)
     rng = np.random.default_rng(21)
     for step in range(14):
