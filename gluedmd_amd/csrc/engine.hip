@@ -1575,8 +1575,10 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
             fp.nSmall = 0;
         }
         fp.scatter = fplan_.scatter && forces;
+        const bool dense = energyValue_ >= 0 && dSlotMask_.bytes;
+        fp.finalizeScatter = forces && !fp.scatter && !dense;
         launch_step_bias(v, fp, nw, s);
-        scattered = fp.scatter != 0;
+        scattered = fp.scatter != 0 || fp.finalizeScatter != 0;
         if (forces && !scattered) {
             if (energyValue_ >= 0 && dSlotMask_.bytes) {
                 // slots and the dense energy CV in one pass (dU/dCV from the finalize)

@@ -335,6 +335,8 @@ struct FusedBiasPlan {
     int walkersPerChunk;
     int nWalkers;            // set at launch
     int extra;               // 1: some small bias is a restraint or PBMetaD (kernels with that code)
+    int finalizeScatter;     // 1: the finalize launch also performs the slot scatter (a workgroup
+                             // per BIAS_BLOCK slots, each forming dU/dCV itself)
 };
 
 constexpr int CHUNK_WB = 4;

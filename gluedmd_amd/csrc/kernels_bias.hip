@@ -798,8 +798,10 @@ constexpr int FIN_OWNERS = 3 * 64;     // slot-owning threads (waves 0, 2, 3)
 constexpr int FIN_LDS_G = 512;         // bias-gradient slots held in LDS
 constexpr int FIN_LDS_COMB = 1024;     // combine entries held in LDS
 
+// Returns the LDS copy of walker w's dU/dCV (the first MAX_LDS_VALUES
+// values; the rest are in global dUdCV, written by this workgroup).
 template <bool EXTRA>
-__device__ void finalize_walker(const StepView& v, const FusedBiasPlan& fp, int w)
+__device__ const double* finalize_walker(const StepView& v, const FusedBiasPlan& fp, int w)
 {
     __shared__ double sU[MAX_LDS_VALUES];
     __shared__ double sBg[FIN_LDS_G];
@@ -859,7 +861,7 @@ __device__ void finalize_walker(const StepView& v, const FusedBiasPlan& fp, int 
             scatter_walker_block(v, sp, w, sU, MAX_LDS_VALUES);
         else
             combine_dudcv(v, sp, w, sU, MAX_LDS_VALUES, true);
-        return;
+        return sU;
     }
     // dU/dCV from LDS (the combine order of combine_dudcv)
     for (int c = tid; c < v.V; c += BIAS_BLOCK) {
@@ -869,7 +871,7 @@ __device__ void finalize_walker(const StepView& v, const FusedBiasPlan& fp, int 
         v.dUdCV[(size_t)w * v.V + c] = u;
     }
     __syncthreads();
-    if (!doScatter) return;
+    if (!doScatter) return sU;
     const WalkerInfo wi = v.walkers[w];
     long long* F = v.force + wi.forceOff;
     if (ownsSlots) {
@@ -895,12 +897,27 @@ __device__ void finalize_walker(const StepView& v, const FusedBiasPlan& fp, int 
         }
     }
     for (int s = tid + FIN_SLOTS * FIN_OWNERS; s < v.S; s += BIAS_BLOCK) scatter_slot(v, sp, w, s, sU, MAX_LDS_VALUES);
+    return sU;
 }
 
 template <bool EXTRA>
 __global__ __launch_bounds__(BIAS_BLOCK) void k_step_bias_finalize(StepView v, FusedBiasPlan fp)
 {
     finalize_walker<EXTRA>(v, fp, blockIdx.x);
+}
+
+// The finalize and the slot scatter in one launch, for plans whose atom slots
+// are too many for one workgroup per walker: every workgroup of walker
+// blockIdx.y forms dU/dCV itself (the finalize's code, so the same bits; the
+// global outputs are written with the same values by each of them), then
+// scatters its BIAS_BLOCK slots.  Saves the scatter launch and its ramp.
+template <bool EXTRA>
+__global__ __launch_bounds__(BIAS_BLOCK) void k_step_finalize_scatter(StepView v, FusedBiasPlan fp)
+{
+    const int w = blockIdx.y;
+    const double* sU = finalize_walker<EXTRA>(v, fp, w);
+    const int s = blockIdx.x * BIAS_BLOCK + threadIdx.x;
+    if (s < v.S) scatter_slot(v, fp.sp, w, s, sU, MAX_LDS_VALUES);
 }
 
 // ---------------------------------------------------------------------------
@@ -966,6 +983,14 @@ void launch_step_bias_chunks(const StepView& v, const FusedBiasPlan& fp, int nw,
 void launch_step_bias(const StepView& v, const FusedBiasPlan& fp, int nw, hipStream_t s)
 {
     if (fp.totalChunks > 0) launch_chunks(v, fp, nw, s);
+    if (fp.finalizeScatter && !fp.scatter) {
+        const dim3 g((v.S + BIAS_BLOCK - 1) / BIAS_BLOCK, nw);
+        if (fp.extra)
+            hipLaunchKernelGGL(k_step_finalize_scatter<true>, g, dim3(BIAS_BLOCK), 0, s, v, fp);
+        else
+            hipLaunchKernelGGL(k_step_finalize_scatter<false>, g, dim3(BIAS_BLOCK), 0, s, v, fp);
+        return;
+    }
     if (fp.extra)
         hipLaunchKernelGGL(k_step_bias_finalize<true>, dim3(nw), dim3(BIAS_BLOCK), 0, s, v, fp);
     else
