@@ -149,13 +149,14 @@ __global__ __launch_bounds__(DENSE_ROWS_BLOCK) void k_scatter_dense_rows(StepVie
         // the list (clamped to this group's entries: unconditional loads)
         const int2* rl = rowList + (size_t)w * listStride + mw.y + __popc(mw.x & ((1u << (g0 & 31)) - 1u));
         const int nr = __popc(bits);
-        int2 rr[4];
-#pragma unroll
-        for (int t = 0; t < 4; t++) rr[t] = rl[min(t, nr - 1)];
+        // four named entries and selects (an indexed array here was kept on
+        // the scratch stack: a private-memory round trip per atom)
+        const int2 rr0 = rl[0], rr1 = rl[min(1, nr - 1)], rr2 = rl[min(2, nr - 1)], rr3 = rl[min(3, nr - 1)];
 #pragma unroll
         for (int t = 0; t < 4; t++) {
             const int e = __popc(bits & ((1u << t) - 1u));  // list entry of atom t
-            const int2 r = e == 0 ? rr[0] : e == 1 ? rr[1] : e == 2 ? rr[2] : rr[3];
+            const int2 lo = (e & 1) ? rr1 : rr0, hi = (e & 1) ? rr3 : rr2;
+            const int2 r = (e & 2) ? hi : lo;
             const int kb = (bits >> t) & 1u ? r.x : 0, ke = (bits >> t) & 1u ? r.y : 0;
             double fx = 0.0, fy = 0.0, fz = 0.0;
             for (int k = kb; k < ke; k++) {
