@@ -1488,6 +1488,7 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
     cp.nCmBlocks = cmBlocks_;
     cp.cmPart = dCmPart_.as<double>();
     cp.cmCount = dCmCount_.as<unsigned int>();
+    cp.cmDefer = 0;
     // batched form: the wide group CVs ride in the first coordination
     // item's rank launch (1,024-thread items) or in the path-row launch
     // (512-thread items, 8 phases) when the plan has one (no data between
@@ -1496,6 +1497,8 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
                         dGroupMass_.as<float>(), dGroupRef_.as<float>()};
     bool bigPending = false;
     if (fused && w0 == 0 && nw == W_) {
+        // a plan with a path CV: its row launch adds the contact-map partials
+        cp.cmDefer = !path_.empty() && !cm_.empty() ? 1 : 0;
         launch_cv_fused(v, cp, nw, s);
         bigPending = !groupBig_.empty();
         const bool pathCarries = groupBigNT_ == 512 && !path_.empty() && tune_.pathPhases == 8;
@@ -1503,9 +1506,17 @@ void Engine::launchStep(hipStream_t s, int w0, int nw, bool forces, int stageMas
             launch_cv_group_big(v, ge.items, ge.n, ge.slots, ge.mass, ge.ref, w0, nw, groupBigNT_, s);
             bigPending = false;
         }
+        GroupExtra px = ge;
+        if (!(bigPending && pathCarries)) px.n = 0;
+        if (cp.cmDefer) {
+            px.cm = cp.cm;
+            px.nCm = cp.nCm;
+            px.nCmBlocks = cp.nCmBlocks;
+            px.cmPart = cp.cmPart;
+        }
         if (launch_path_rows(v, dPath_.as<PathItem>(), (int)path_.size(), pathRowBlocks_, pathMaxN_,
                              dPathPos_.as<int>(), dPathFrames_.as<float>(), w0, nw, tune_.pathPhases, s,
-                             bigPending && pathCarries ? &ge : nullptr))
+                             px.n > 0 || px.nCm > 0 ? &px : nullptr))
             bigPending = false;
     } else {
         launch_cv_small(v, dSmall_.as<SmallCVItem>(), (int)small_.size(), w0, nw, s);

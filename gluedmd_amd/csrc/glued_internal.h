@@ -288,6 +288,12 @@ struct GroupExtra {
     const int* slots;
     const float* mass;
     const float* ref;
+    // contact maps whose chunk partials the carrying launch adds up (one
+    // more workgroup per walker row; nCm = 0: none)
+    const GroupCVItem* cm;
+    int nCm;
+    int nCmBlocks;
+    const double* cmPart;
 };
 
 // Scatter plan: Jacobian rows grouped by atom slot (CSR), CV of every row,
@@ -364,6 +370,8 @@ struct CVFusedPlan {
     int nCmBlocks;          // chunks over all of them
     double* cmPart;         // [W][nCmBlocks] chunk partial sums
     unsigned int* cmCount;  // [W][nCm] chunks finished (the last resets it)
+    int cmDefer;            // 1: the chunks only write their partials; the
+                            // path-row launch adds them (no ticket, no fence)
 };
 
 }  // namespace glued

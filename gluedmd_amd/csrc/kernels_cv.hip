@@ -1785,9 +1785,25 @@ __global__ __launch_bounds__(PATH_MSD_THREADS) void k_path_msd(StepView v, const
     path_msd_block(v, items[p], slots, frames, blockIdx.x - items[p].msdBlk0, wt0, min(PATH_TW, w0 + nw - wt0));
 }
 
+// Contact-map values from the chunk partials an earlier launch of the step
+// wrote: wave k adds maps k, k + waves, ... with the additions of the chunks'
+// own ticketed form (wave_sum_partials in chunk order), so either form gives
+// the same bits.
+__device__ void cm_finish(const StepView& v, const GroupExtra& ge, int w)
+{
+    const int nwv = blockDim.x >> 6;
+    for (int q = threadIdx.x >> 6; q < ge.nCm; q += nwv) {
+        const GroupCVItem& it = ge.cm[q];
+        const int nChunks = ((it.end - it.begin) / 2 + CM_CHUNK - 1) / CM_CHUNK;
+        const double tot = wave_sum_partials(ge.cmPart + (size_t)w * ge.nCmBlocks + it.chunk0, nChunks);
+        if ((threadIdx.x & 63) == 0) v.cvVal[(size_t)w * v.V + it.value] = tot;
+    }
+}
+
 // CARRY: the launch also runs the plan's wide group CVs as ge.n extra
 // workgroups per walker row (512 threads each; they share no data with the
-// rows).
+// rows).  ge.nCm > 0: one more workgroup per walker row, last, adds the
+// contact-map chunk partials.
 template <int PH, bool CARRY>
 __global__ __launch_bounds__(64 * PH) void k_path_rows(StepView v, const PathItem* __restrict__ items,
                                                                 int nItems, const int* __restrict__ slots,
@@ -1797,9 +1813,13 @@ __global__ __launch_bounds__(64 * PH) void k_path_rows(StepView v, const PathIte
     __shared__ double shW[PATH_MAX_FRAMES];
     __shared__ double shRed[6][64 * PH];
     __shared__ double sh[4 * 16];
+    if (ge.nCm > 0 && (int)blockIdx.x == (int)gridDim.x - 1) {
+        cm_finish(v, ge, w0 + blockIdx.y);
+        return;
+    }
     if constexpr (CARRY) {
         static_assert(64 * PH == GROUP_BIG_BLOCK_PATH, "the carried group CVs run on the launch's block size");
-        const int nb = gridDim.x - ge.n;
+        const int nb = gridDim.x - ge.n - (ge.nCm > 0);
         if ((int)blockIdx.x >= nb) {
             group_item<GROUP_BIG_BLOCK_PATH, GROUP_BIG_REG>(v, ge.items[blockIdx.x - nb], ge.slots, ge.mass, ge.ref,
                                                             w0 + blockIdx.y);
@@ -2007,26 +2027,34 @@ void launch_path_frames_adjoint(const StepView& v, const PathItem* items, int pi
 // phases: 4 or 8 waves per row block (the autotuner's choice; 0 = 4 when the
 // blocks exceed one residency round and the paths have at most 256 frames)
 // Returns true when the launch carried the wide group CVs of `extra` (their
-// workgroups are 512 threads: with 8 phases).
+// workgroups are 512 threads: with 8 phases).  The contact-map sums of
+// `extra` (extra->nCm > 0) run with any phase count.
 bool launch_path_rows(const StepView& v, const PathItem* items, int n, int nRowBlocks, int maxN, const int* slots,
                       const float* frames, int w0, int nw, int phases, hipStream_t s, const GroupExtra* extra)
 {
     if (n <= 0) return false;
     if (phases == 0) phases = (long long)nRowBlocks * nw > 512 && maxN <= 256 ? 4 : PATH_PHASES;
     static_assert(64 * PATH_PHASES == GROUP_BIG_BLOCK_PATH, "8 phases: the carrying launch");
-    const GroupExtra none{};
+    GroupExtra cmOnly{};
+    if (extra) {
+        cmOnly.cm = extra->cm;
+        cmOnly.nCm = extra->nCm;
+        cmOnly.nCmBlocks = extra->nCmBlocks;
+        cmOnly.cmPart = extra->cmPart;
+    }
+    const int cmb = cmOnly.nCm > 0 ? 1 : 0;
     if (phases == 4) {
-        hipLaunchKernelGGL((k_path_rows<4, false>), dim3(nRowBlocks, nw), dim3(64 * 4), 0, s, v, items, n, slots,
-                           frames, w0, none);
+        hipLaunchKernelGGL((k_path_rows<4, false>), dim3(nRowBlocks + cmb, nw), dim3(64 * 4), 0, s, v, items, n,
+                           slots, frames, w0, cmOnly);
         return false;
     }
     if (extra && extra->n > 0) {
-        hipLaunchKernelGGL((k_path_rows<PATH_PHASES, true>), dim3(nRowBlocks + extra->n, nw), dim3(64 * PATH_PHASES),
-                           0, s, v, items, n, slots, frames, w0, *extra);
+        hipLaunchKernelGGL((k_path_rows<PATH_PHASES, true>), dim3(nRowBlocks + extra->n + cmb, nw),
+                           dim3(64 * PATH_PHASES), 0, s, v, items, n, slots, frames, w0, *extra);
         return true;
     }
-    hipLaunchKernelGGL((k_path_rows<PATH_PHASES, false>), dim3(nRowBlocks, nw), dim3(64 * PATH_PHASES), 0, s, v,
-                       items, n, slots, frames, w0, none);
+    hipLaunchKernelGGL((k_path_rows<PATH_PHASES, false>), dim3(nRowBlocks + cmb, nw), dim3(64 * PATH_PHASES), 0, s,
+                       v, items, n, slots, frames, w0, cmOnly);
     return false;
 }
 
@@ -2045,7 +2073,10 @@ void launch_cv_path(const StepView& v, const PathItem* items, int n, int nMsdBlo
 // arithmetic), so a long map spreads over many CUs.  Each chunk writes its
 // rows and its partial sum; the last chunk of the map to finish adds the
 // partials in chunk order and writes the value, so the value does not depend
-// on which chunk finished last.
+// on which chunk finished last.  In the batched step of a plan with a path CV
+// the chunks stop at their partials and the path-row launch that follows adds
+// them (cm_finish): the ticket's agent-scope fence, which writes back the
+// L2 with every row stored so far, costs a chunk about 4 us.
 // ---------------------------------------------------------------------------
 constexpr int CM_THREADS = 256;
 constexpr int CM_PER_THREAD = CM_CHUNK / CM_THREADS;
@@ -2091,6 +2122,7 @@ __device__ void cm_chunk_block(const StepView& v, const CVFusedPlan& p, int lb, 
     double* part = p.cmPart + (size_t)w * p.nCmBlocks + it.chunk0;
     unsigned int* cnt = p.cmCount + (size_t)w * p.nCm + it.chunkItem;
     if (tid == 0) part[c] = s[0];
+    if (p.cmDefer) return;  // the path-row launch adds the partials
     if (!last_block(cnt, (unsigned int)nChunks)) return;
     if (tid >= 64) return;
     const double tot = wave_sum_partials(part, nChunks);

@@ -4,7 +4,10 @@ a path CV and no coordination, by the coordination rank launch (1,024-thread
 items) in a plan with coordination.  Either way the plain per-walker launches
 run the same group code on a launch of their own, so the two forms give the
 same bits (CV values and fixed-point forces), and both agree with the FP64
-oracle (CommonGluedKernels.cpp:556-639 gyration, :1130-1224 DRMSD)."""
+oracle (CommonGluedKernels.cpp:556-639 gyration, :1130-1224 DRMSD).  The plan
+also holds a contact map of three 512-pair chunks: in the batched step the
+path-row launch adds its chunk partials, in the plain form the last chunk
+does (CommonGluedKernels.cpp:1236-1323)."""
 import numpy as np
 import pytest
 
@@ -30,6 +33,9 @@ def _force(rng, n, M, N, L, coordination):
     pairs = pairs[pairs[:, 0] != pairs[:, 1]]
     f.add_drmsd(pairs.reshape(-1).tolist(), rng.uniform(0.3, 1.5, size=len(pairs)).tolist())
     f.add_gyration(list(range(M, M + 1500)))
+    cm = rng.choice(n, size=(1300, 2), replace=True)  # three chunks of up to 512 pairs
+    cm = cm[cm[:, 0] != cm[:, 1]]
+    f.add_contact_map(cm.reshape(-1).tolist(), r0=rng.uniform(0.4, 1.2, size=len(cm)).tolist())
     if coordination:
         f.add_coordination(list(range(0, 60)), list(range(200, 1400)), 0.3, cutoff=0.9)
     nv = f.getNumCollectiveVariables()
