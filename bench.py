@@ -59,18 +59,18 @@ FIXED_SCALE = 4294967296.0
 # are in the workload summaries under profiles/.
 PMC_FETCH_SCALE, PMC_WRITE_SCALE = 2.0, 1.0
 PMC_KIB = {
-    "multiwalker": {"k_cv_fused": (442.6, 512.5), "k_step_bias_chunks<1, 2>": (7590.1, 60.0),
-                    "k_step_bias_finalize<false>": (532.8, 769.0)},
-    # k_path_rows<8, true> carries the DRMSD(1,500 pairs) workgroups
-    "mixed-large": {"k_cv_fused": (11522.5, 4465.5), "k_path_rows<8, true>": (17086.3, 1866.4),
-                    "k_step_bias_finalize<false>": (94.0, 2.0), "k_scatter_dense_rows": (35455.2, 49686.9)},
+    "multiwalker": {"k_cv_fused": (442.8, 512.5), "k_step_bias_chunks<1, 2>": (7583.6, 60.0),
+                    "k_step_bias_finalize<false>": (535.3, 769.0)},
+    # k_path_rows<8, true> carries the DRMSD(1,500 pairs) workgroups and the contact-map sums
+    "mixed-large": {"k_cv_fused": (11485.8, 4285.3), "k_path_rows<8, true>": (11412.7, 1865.0),
+                    "k_step_bias_finalize<false>": (93.0, 2.0), "k_scatter_dense_rows": (31152.9, 37512.6)},
     # k_coord_rank carries the Rg(5,000) workgroup
-    "coordination-opes": {"k_coord_rank": (387.4, 948.7), "k_coord_place": (450.9, 420.6),
-                          "k_coord_cells<512>": (1596.7, 709.2), "k_step_bias_chunks<2, 2>": (150.1, 3.0),
-                          "k_step_bias_finalize<false>": (13.3, 0.1), "k_scatter": (379.7, 515.7)},
-    "path-batch": {"k_cv_fused": (3432.3, 648.1), "k_path_rows<4, false>": (3200.0, 2544.0),
-                   "k_step_hill_chunks_shared<2>": (313.2, 384.0), "k_step_bias_finalize<false>": (1766.7, 1823.3)},
-    "small": {"k_step_tiny<1, 2, false>": (43.3, 0.9)},
+    "coordination-opes": {"k_coord_rank": (387.9, 948.7), "k_coord_place": (450.9, 420.6),
+                          "k_coord_cells<512>": (1596.7, 709.4), "k_step_bias_chunks<2, 2>": (150.1, 3.0),
+                          "k_step_finalize_scatter<false>": (470.6, 516.4)},
+    "path-batch": {"k_cv_fused": (3476.3, 770.7), "k_path_rows<4, false>": (3484.8, 2544.0),
+                   "k_step_hill_chunks_shared<2>": (313.2, 384.0), "k_step_bias_finalize<false>": (1769.5, 1823.3)},
+    "small": {"k_step_tiny<1, 2, false>": (43.1, 0.9)},
 }
 # the kernel(s) each workload's roofline line is about
 ROOFLINE_KERNELS = {
