@@ -288,12 +288,13 @@ struct GroupExtra {
     const int* slots;
     const float* mass;
     const float* ref;
-    // contact maps whose chunk partials the carrying launch adds up (one
-    // more workgroup per walker row; nCm = 0: none)
+    // contact maps whose chunk partials the path-row launch adds up, in
+    // its workgroup cmBlock of every walker row (-1: none)
     const GroupCVItem* cm;
     int nCm;
     int nCmBlocks;
     const double* cmPart;
+    int cmBlock;
 };
 
 // Scatter plan: Jacobian rows grouped by atom slot (CSR), CV of every row,

@@ -1802,10 +1802,17 @@ __device__ void cm_finish(const StepView& v, const GroupExtra& ge, int w)
 
 // CARRY: the launch also runs the plan's wide group CVs as ge.n extra
 // workgroups per walker row (512 threads each; they share no data with the
-// rows).  ge.nCm > 0: one more workgroup per walker row, last, adds the
-// contact-map chunk partials.
+// rows).  ge.cmBlock >= 0: that workgroup of every walker row (the one after
+// the row blocks) adds the contact-map chunk partials instead.  8 phases: at
+// most 128 VGPRs, so two workgroups fit a CU and a launch of up to 512
+// workgroups is one residency round (the carrying form came to 129 with the
+// contact-map branch, and the 288 workgroups of the 16 x 100k-atom plan then
+// took two rounds: 15.0 -> 19.6 us).  At most 128 VGPRs: two 8-phase workgroups
+// per CU, so a launch of up to 512 workgroups is one residency round (at 129
+// the carrying launch of the 16 x 100k-atom plan, 288 workgroups, took a
+// second round: 15.0 -> 19.6 us).
 template <int PH, bool CARRY>
-__global__ __launch_bounds__(64 * PH) void k_path_rows(StepView v, const PathItem* __restrict__ items,
+__global__ __launch_bounds__(64 * PH) __attribute__((amdgpu_waves_per_eu(PH == 8 ? 4 : 1))) void k_path_rows(StepView v, const PathItem* __restrict__ items,
                                                                 int nItems, const int* __restrict__ slots,
                                                                 const float* __restrict__ frames, int w0,
                                                                 GroupExtra ge)
@@ -1813,18 +1820,18 @@ __global__ __launch_bounds__(64 * PH) void k_path_rows(StepView v, const PathIte
     __shared__ double shW[PATH_MAX_FRAMES];
     __shared__ double shRed[6][64 * PH];
     __shared__ double sh[4 * 16];
-    if (ge.nCm > 0 && (int)blockIdx.x == (int)gridDim.x - 1) {
-        cm_finish(v, ge, w0 + blockIdx.y);
-        return;
-    }
     if constexpr (CARRY) {
         static_assert(64 * PH == GROUP_BIG_BLOCK_PATH, "the carried group CVs run on the launch's block size");
-        const int nb = gridDim.x - ge.n - (ge.nCm > 0);
+        const int nb = gridDim.x - ge.n;
         if ((int)blockIdx.x >= nb) {
             group_item<GROUP_BIG_BLOCK_PATH, GROUP_BIG_REG>(v, ge.items[blockIdx.x - nb], ge.slots, ge.mass, ge.ref,
                                                             w0 + blockIdx.y);
             return;
         }
+    }
+    if ((int)blockIdx.x == ge.cmBlock) {
+        cm_finish(v, ge, w0 + blockIdx.y);
+        return;
     }
     int p = 0;
     while (p + 1 < nItems && (int)blockIdx.x >= items[p + 1].rowBlk0) p++;
@@ -2036,6 +2043,7 @@ bool launch_path_rows(const StepView& v, const PathItem* items, int n, int nRowB
     if (phases == 0) phases = (long long)nRowBlocks * nw > 512 && maxN <= 256 ? 4 : PATH_PHASES;
     static_assert(64 * PATH_PHASES == GROUP_BIG_BLOCK_PATH, "8 phases: the carrying launch");
     GroupExtra cmOnly{};
+    cmOnly.cmBlock = -1;
     if (extra) {
         cmOnly.cm = extra->cm;
         cmOnly.nCm = extra->nCm;
@@ -2043,14 +2051,17 @@ bool launch_path_rows(const StepView& v, const PathItem* items, int n, int nRowB
         cmOnly.cmPart = extra->cmPart;
     }
     const int cmb = cmOnly.nCm > 0 ? 1 : 0;
+    if (cmb) cmOnly.cmBlock = nRowBlocks;
     if (phases == 4) {
         hipLaunchKernelGGL((k_path_rows<4, false>), dim3(nRowBlocks + cmb, nw), dim3(64 * 4), 0, s, v, items, n,
                            slots, frames, w0, cmOnly);
         return false;
     }
     if (extra && extra->n > 0) {
-        hipLaunchKernelGGL((k_path_rows<PATH_PHASES, true>), dim3(nRowBlocks + extra->n + cmb, nw),
-                           dim3(64 * PATH_PHASES), 0, s, v, items, n, slots, frames, w0, *extra);
+        GroupExtra ge = *extra;
+        ge.cmBlock = cmOnly.cmBlock;
+        hipLaunchKernelGGL((k_path_rows<PATH_PHASES, true>), dim3(nRowBlocks + cmb + extra->n, nw),
+                           dim3(64 * PATH_PHASES), 0, s, v, items, n, slots, frames, w0, ge);
         return true;
     }
     hipLaunchKernelGGL((k_path_rows<PATH_PHASES, false>), dim3(nRowBlocks + cmb, nw), dim3(64 * PATH_PHASES), 0, s,
