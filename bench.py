@@ -62,8 +62,8 @@ PMC_KIB = {
     "multiwalker": {"k_cv_fused": (442.8, 512.5), "k_step_bias_chunks<1, 2>": (7583.6, 60.0),
                     "k_step_bias_finalize<false>": (535.3, 769.0)},
     # k_path_rows<8, true> carries the DRMSD(1,500 pairs) workgroups and the contact-map sums
-    "mixed-large": {"k_cv_fused": (11485.8, 4285.3), "k_path_rows<8, true>": (11412.7, 1865.0),
-                    "k_step_bias_finalize<false>": (93.0, 2.0), "k_scatter_dense_rows": (31152.9, 37512.6)},
+    "mixed-large": {"k_cv_fused": (11525.5, 4417.7), "k_path_rows<8, true>": (11817.1, 1865.8),
+                    "k_step_bias_finalize<false>": (93.0, 2.0), "k_scatter_dense_rows": (31155.6, 37512.5)},
     # k_coord_rank carries the Rg(5,000) workgroup
     "coordination-opes": {"k_coord_rank": (387.9, 948.7), "k_coord_place": (450.9, 420.6),
                           "k_coord_cells<512>": (1596.7, 709.4), "k_step_bias_chunks<2, 2>": (150.1, 3.0),
