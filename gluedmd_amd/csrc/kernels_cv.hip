@@ -1820,17 +1820,18 @@ __global__ __launch_bounds__(64 * PH) __attribute__((amdgpu_waves_per_eu(PH == 8
     __shared__ double shW[PATH_MAX_FRAMES];
     __shared__ double shRed[6][64 * PH];
     __shared__ double sh[4 * 16];
-    if constexpr (CARRY) {
-        static_assert(64 * PH == GROUP_BIG_BLOCK_PATH, "the carried group CVs run on the launch's block size");
-        const int nb = gridDim.x - ge.n;
-        if ((int)blockIdx.x >= nb) {
+    static_assert(!CARRY || 64 * PH == GROUP_BIG_BLOCK_PATH, "the carried group CVs run on the launch's block size");
+    // blocks past the rows: the contact-map sums (cmBlock), then the group
+    // CVs; one branch region for both (two separate branches cost a VGPR
+    // and a spill)
+    if ((int)blockIdx.x >= (int)gridDim.x - ge.n - (ge.cmBlock >= 0 ? 1 : 0)) {
+        if ((int)blockIdx.x == ge.cmBlock) {
+            cm_finish(v, ge, w0 + blockIdx.y);
+        } else if constexpr (CARRY) {
+            const int nb = gridDim.x - ge.n;
             group_item<GROUP_BIG_BLOCK_PATH, GROUP_BIG_REG>(v, ge.items[blockIdx.x - nb], ge.slots, ge.mass, ge.ref,
                                                             w0 + blockIdx.y);
-            return;
         }
-    }
-    if ((int)blockIdx.x == ge.cmBlock) {
-        cm_finish(v, ge, w0 + blockIdx.y);
         return;
     }
     int p = 0;
